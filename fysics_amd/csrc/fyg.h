@@ -1,0 +1,144 @@
+/* fyg.h -- the thin C ABI between the C host code (the files under host/, which
+ * implement the fysics.h entry points) and the HIP side (the .hip files: device
+ * runtime and kernels).
+ *
+ * Everything here is plain C: device buffers are plain device addresses and
+ * streams are opaque handles.  The host code never includes HIP
+ * headers; the HIP files never see fysics.h structs.  Every int-returning
+ * function returns 0 on success and nonzero on any HIP error.
+ */
+#ifndef FYG_H
+#define FYG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ runtime */
+typedef struct fyg_stream fyg_stream;
+
+/* the calling host thread's stream (created on first use, default priority) */
+fyg_stream *fyg_thread_stream(void);
+/* wrap an external hipStream_t (NULL -> the thread stream) */
+fyg_stream *fyg_stream_wrap(void *hip_stream);
+void *fyg_stream_handle(fyg_stream *s);          /* the hipStream_t */
+int   fyg_sync(fyg_stream *s);
+
+/* per-stream grow-only device workspace: slot in [0, FYG_WS_SLOTS) */
+#define FYG_WS_SLOTS 8
+void *fyg_ws(fyg_stream *s, int slot, size_t bytes);
+/* per-stream grow-only pinned host staging buffer */
+void *fyg_pinned(fyg_stream *s, int slot, size_t bytes);
+
+int  fyg_malloc(void **p, size_t bytes);
+void fyg_free(void *p);
+int  fyg_h2d(void *dst, const void *src, size_t bytes, fyg_stream *s);
+int  fyg_d2h(void *dst, const void *src, size_t bytes, fyg_stream *s);
+int  fyg_d2d(void *dst, const void *src, size_t bytes, fyg_stream *s);
+int  fyg_zero(void *dst, size_t bytes, fyg_stream *s);    /* fill kernel, graph-safe */
+int  fyg_last_error(void);                                /* sticky launch error check */
+const char *fyg_error_string(void);
+
+/* process-wide cache of small read-only device tables (twiddles, transfer
+ * LUTs).  Returns the device copy for key, uploading data on a miss.  Thread
+ * safe; entries live until process end (bounded LRU for transfer tables). */
+const void *fyg_table(const void *key, size_t keylen, const void *data, size_t bytes);
+const void *fyg_table_find(const void *key, size_t keylen);
+/* fp32 twiddles cos/sin(2 pi m / L), m < L, evaluated in double (float2[L]) */
+const void *fyg_twiddles(int L);
+
+/* per-kernel timing with HIP events recorded on the launch stream */
+void fyg_prof_enable(int on);
+int  fyg_prof_collect(char *names, int names_len, double *ms, long *count, int max);
+void fyg_prof_reset(void);
+
+/* ------------------------------------------------------------------ FFT / deconv */
+int fyg_fft_supported(int L);                   /* 2^k or 3*2^k, L <= FYG_FFT_MAX */
+#define FYG_FFT_MAX 4096
+
+/* tile prologue applied to u8 input voxels: f = u8/255, ring subtraction,
+ * z-drift factor, dewindow (pipeline.c:492-494,266-268) */
+typedef struct {
+    int dering; const float *ring; int nr, nslab, slab_z; double cy, cx; float scale;
+    int zdrift; const float *zfactor;
+    long gz0, gy0, gx0;              /* global origin of the buffer */
+    int window; float wlo, wspan;
+} fyg_prologue;
+
+/* epilogue applied to every deconvolved voxel before it is stored
+ * (pipeline.c:273-283): rewindow, dec-range rescale, clamp to [0,1] */
+typedef struct {
+    int window; float wlo, winv;
+    int dec; float dlo, dinv;
+    int clamp01;
+} fyg_epilogue;
+
+/* one real-input Wiener deconvolution on device buffers */
+typedef struct {
+    int nz, ny, nx;                  /* logical (cropped) volume */
+    int pz, py, px, hx, hxp;         /* padded FFT sizes, half-spectrum width and row pitch */
+    const float *lut;                /* 4097 fp32 weights over u = f^2 */
+    double us;                       /* LUT index scale: t = u*us */
+    const double *fz2, *fy2, *fx2;   /* squared frequencies (fz2 pre-scaled by sigma_z ratio) */
+    const void *in; int in_u8;       /* fp32 or u8 input (nz*ny*nx) */
+    fyg_prologue pro;                /* u8 input only */
+    float *out;                      /* fp32 nz*ny*nx, may alias a fp32 in */
+    fyg_epilogue epi;
+    void *spec;                      /* complex workspace pz*py*hxp float2 */
+    int *nonzero_flag;               /* optional: set to 1 if any input u8 is nonzero */
+} fyg_deconv_args;
+int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s);
+size_t fyg_deconv_spec_bytes(int pz, int py, int px);
+int fyg_deconv_hxp(int px);
+
+/* complex FFT of split re/im device arrays (fy_fft1d/fy_fft3d) */
+int fyg_fft3d_run(float *d_re, float *d_im, int nz, int ny, int nx, int sign,
+                  void *work /* 8*nz*ny*nx bytes */, fyg_stream *s);
+int fyg_scale_run(float *d, size_t n, float k, fyg_stream *s);
+
+/* ------------------------------------------------------------------ guided / box */
+typedef struct {
+    int quant;                       /* 0: fp32 out over the whole volume; 1: u8 of the inner tile */
+    float *out;                      /* fp32 nz*ny*nx (quant 0); may alias in */
+    unsigned char *q;                /* u8 tz*ty*tx (quant 1) */
+    long iz0, iy0, ix0, tz, ty, tx;  /* inner tile, local coordinates */
+    long gz0, gy0, gx0;              /* global origin of the buffer (dither coordinates) */
+    int no_dither;
+    int recenter; float rmid, rk;    /* pipeline.c:413-421 */
+    const int *nonzero_flag;         /* optional: if it reads 0, the tile is all air -> zeros */
+} fyg_emit;
+
+typedef struct {
+    int nz, ny, nx, r, s;            /* s <= 1: exact path */
+    double eps;
+    const float *in;
+    fyg_emit em;
+    void *ws;                        /* device workspace, fyg_guided_ws_bytes() */
+} fyg_guided_args;
+size_t fyg_guided_ws_bytes(int nz, int ny, int nx, int s);
+int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s);
+/* prologue-only path (deconv disabled): u8 -> f32 through the tile prologue */
+int fyg_prologue_run(const unsigned char *in, float *out, int nz, int ny, int nx,
+                     const fyg_prologue *pro, int *nonzero_flag, fyg_stream *s);
+/* emit without guided filtering: recenter + quantise the inner tile */
+int fyg_emit_run(const float *f, int nz, int ny, int nx, const fyg_emit *em, fyg_stream *s);
+int fyg_box_mean_run(const float *in, float *out, int nz, int ny, int nx, int r,
+                     void *ws, fyg_stream *s);
+
+/* ------------------------------------------------------------------ elementwise */
+int fyg_u8_to_f32_run(const unsigned char *in, float *out, size_t n, float lo, float scale, fyg_stream *s);
+int fyg_f32_to_u8_run(const float *in, unsigned char *out, size_t n, float lo, float k, fyg_stream *s);
+int fyg_zdrift_run(float *f, int nz, int ny, int nx, const float *factor_at_z0, fyg_stream *s);
+int fyg_dering_apply_run(float *f, long dz, long dy, long dx, const fyg_prologue *pro, fyg_stream *s);
+/* accumulate (sector,radius) sums/counts from a u8 region with a per-(y,x) bin map */
+int fyg_dering_accum_run(const unsigned char *buf, long dz, long dy, long dx, int ss,
+                         const int *map, int nbins, unsigned long long *sum,
+                         unsigned int *cnt, fyg_stream *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

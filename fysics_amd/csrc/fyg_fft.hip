@@ -1,0 +1,774 @@
+// fyg_fft.hip -- the real-input Wiener deconvolution of deconv_radial
+// (paganin.c:152-293) and the complex FFTs of fft.c, as CDNA4 kernels.
+//
+// Engine: batched Stockham FFT of length L (2^k or 3*2^k, L <= 4096).  Each
+// workgroup owns NL lines.  Stage s (radix R, span NS) reads x[j + r*L/R],
+// multiplies by w^(r*(j%NS)) with w = exp(sign*2*pi*i/(NS*R)), runs the R-point
+// DFT in registers and writes y[(j/NS)*NS*R + j%NS + r*NS]; the first stage
+// reads HBM, the last writes HBM, the ones between go through LDS (in place:
+// read, barrier, write, barrier).  Radices are 16 where possible (2 stages for
+// 256), so a 256-line makes a single LDS round trip.  Twiddles are fp32 copies
+// of double-evaluated cos/sin (the fft.c:47-61 scheme), staged in LDS.
+//
+// Passes of one deconvolution over the half spectrum S[z][y][k] (pitch hxp):
+//   k_rfft_x    rows (y, y+1) of a reflect-padded z plane packed re/im, one
+//               complex FFT, untangled to two half-spectrum rows; the u8 tile
+//               prologue (f = u8/255, rings, z-drift, dewindow) is fused into
+//               the gather                                   (paganin.c:166-194)
+//   k_cfft_cols Y forward over all z                         (paganin.c:197-203)
+//   k_zfused    Z forward, LUT weight, Z inverse in one kernel: the weight is
+//               pointwise, so the spectrum never returns to HBM between the
+//               forward and inverse Z transforms             (paganin.c:204-253)
+//   k_cfft_cols Y inverse for z < nz                         (paganin.c:254-260)
+//   k_irfft_x   Hermitian repack of rows (y, y+1), complex inverse, 1/(pz*py*px),
+//               crop, fused rewindow / dec-range / clamp     (paganin.c:263-289)
+#include <cstdio>
+
+#include "fyg_internal.hpp"
+
+namespace {
+
+// ------------------------------------------------------------------ plans
+struct Radices {
+    int n;
+    int r[10];
+};
+
+constexpr int ilog2c(int x) {
+    int l = 0;
+    while ((1 << l) < x) l++;
+    return l;
+}
+
+constexpr Radices make_plan(int L) {
+    Radices p{};
+    p.n = 0;
+    int m = L;
+    if (m % 3 == 0) {
+        p.r[p.n++] = 3;
+        m /= 3;
+    }
+    int lg = ilog2c(m);
+    int q = lg / 4, rem = lg % 4;
+    if (rem == 1 && q >= 1) {
+        p.r[p.n++] = 4;
+        p.r[p.n++] = 8;
+        q -= 1;
+    } else if (rem) {
+        p.r[p.n++] = 1 << rem;
+    }
+    for (int i = 0; i < q; i++) p.r[p.n++] = 16;
+    if (p.n == 0) p.r[p.n++] = 1;
+    return p;
+}
+
+// stage s of the forward plan (REV=false) or of the reversed plan (REV=true)
+template <int L, bool REV>
+struct Plan {
+    static constexpr Radices P = make_plan(L);
+    static constexpr int n = P.n;
+    static constexpr int R(int s) { return REV ? P.r[P.n - 1 - s] : P.r[s]; }
+    static constexpr int NS(int s) {
+        int v = 1;
+        for (int i = 0; i < s; i++) v *= R(i);
+        return v;
+    }
+};
+
+// lines per workgroup: column passes keep 16 contiguous columns (128-byte rows),
+// row passes fill ~4096 elements
+template <int L>
+struct Shape {
+    static constexpr int T = 256;
+    static constexpr int NL_CM = L <= 256 ? 16 : (4096 / L);
+    static constexpr int NL_RM = (4096 / L) > 1024 ? 1024 : (4096 / L) < 1 ? 1 : (4096 / L);
+    static constexpr int LD = L + (L >> 4) + 1;   // padded line pitch in LDS (float2)
+};
+
+__device__ __forceinline__ int lds_pos(int pos) { return pos + (pos >> 4); }
+
+// ------------------------------------------------------------------ small DFTs
+__device__ __forceinline__ float2 f2add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 f2sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// a * (sg * i)
+__device__ __forceinline__ float2 f2rot(float2 a, float sg) { return make_float2(-sg * a.y, sg * a.x); }
+// a * (c + i*sg*s)
+__device__ __forceinline__ float2 f2tw(float2 a, float c, float s, float sg) {
+    float ss = sg * s;
+    return make_float2(a.x * c - a.y * ss, a.x * ss + a.y * c);
+}
+
+template <int R>
+__device__ __forceinline__ void dft(float2 *v, float sg);
+
+template <>
+__device__ __forceinline__ void dft<1>(float2 *, float) {}
+
+template <>
+__device__ __forceinline__ void dft<2>(float2 *v, float) {
+    float2 a = v[0], b = v[1];
+    v[0] = f2add(a, b);
+    v[1] = f2sub(a, b);
+}
+
+template <>
+__device__ __forceinline__ void dft<3>(float2 *v, float sg) {
+    const float c3 = 0.86602540378443864676f;
+    float2 t = f2add(v[1], v[2]);
+    float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+    float2 d = f2sub(v[1], v[2]);
+    d = make_float2(-sg * c3 * d.y, sg * c3 * d.x);
+    v[0] = f2add(v[0], t);
+    v[1] = f2add(m, d);
+    v[2] = f2sub(m, d);
+}
+
+template <>
+__device__ __forceinline__ void dft<4>(float2 *v, float sg) {
+    float2 t0 = f2add(v[0], v[2]), t1 = f2sub(v[0], v[2]);
+    float2 t2 = f2add(v[1], v[3]), t3 = f2rot(f2sub(v[1], v[3]), sg);
+    v[0] = f2add(t0, t2);
+    v[2] = f2sub(t0, t2);
+    v[1] = f2add(t1, t3);
+    v[3] = f2sub(t1, t3);
+}
+
+template <>
+__device__ __forceinline__ void dft<8>(float2 *v, float sg) {
+    const float c = 0.70710678118654752440f;
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    dft<4>(e, sg);
+    dft<4>(o, sg);
+    o[1] = f2tw(o[1], c, c, sg);
+    o[2] = f2rot(o[2], sg);
+    o[3] = f2tw(o[3], -c, c, sg);
+    for (int k = 0; k < 4; k++) {
+        v[k] = f2add(e[k], o[k]);
+        v[k + 4] = f2sub(e[k], o[k]);
+    }
+}
+
+// cos/sin(2*pi*m/16)
+__device__ __constant__ float k_c16[16] = {
+    1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+    0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
+    -1.0f, -0.92387953251128675613f, -0.70710678118654752440f, -0.38268343236508977173f,
+    0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f};
+
+template <>
+__device__ __forceinline__ void dft<16>(float2 *v, float sg) {
+    constexpr float C[16] = {
+        1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+        0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
+        -1.0f, -0.92387953251128675613f, -0.70710678118654752440f, -0.38268343236508977173f,
+        0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f};
+    float2 y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+        y[n2][0] = v[n2];
+        y[n2][1] = v[n2 + 4];
+        y[n2][2] = v[n2 + 8];
+        y[n2][3] = v[n2 + 12];
+        dft<4>(y[n2], sg);
+    }
+#pragma unroll
+    for (int n2 = 1; n2 < 4; n2++)
+#pragma unroll
+        for (int k1 = 1; k1 < 4; k1++) {
+            const int m = n2 * k1;
+            // W16^m = cos(2 pi m/16) + i*sg*sin(2 pi m/16); sin(x) = cos(x - pi/2)
+            y[n2][k1] = f2tw(y[n2][k1], C[m & 15], C[(m + 12) & 15], sg);
+        }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        float2 z[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+        dft<4>(z, sg);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = z[k2];
+    }
+}
+
+// ------------------------------------------------------------------ stage engine
+// item i -> (line, butterfly j); CM: consecutive threads walk lines (columns),
+// RM: consecutive threads walk j within a line (contiguous rows)
+template <int NL, int NB, bool CM>
+__device__ __forceinline__ void item_of(int i, int &l, int &j) {
+    if (CM) {
+        l = i % NL;
+        j = i / NL;
+    } else {
+        l = i / NB;
+        j = i % NB;
+    }
+}
+
+// one Stockham stage; LDIN/STOUT read/write (line, pos).  SYNC: barrier between
+// the read and the write phase (needed when both touch the same LDS buffer).
+template <int L, int NL, int T, bool CM, int R, int NS, bool SYNC, class LdF, class StF>
+__device__ __forceinline__ void stage(float sg, const float2 *__restrict__ tw, LdF ld, StF st) {
+    constexpr int NB = L / R;
+    constexpr int TOT = NL * NB;
+    constexpr int BPT = (TOT + T - 1) / T;
+    constexpr int TWS = L / (NS * R);
+    float2 v[BPT][R];
+#pragma unroll
+    for (int b = 0; b < BPT; b++) {
+        int i = threadIdx.x + b * T;
+        if (TOT % T == 0 || i < TOT) {
+            int l, j;
+            item_of<NL, NB, CM>(i, l, j);
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b][r] = ld(l, j + r * NB);
+        }
+    }
+    if (SYNC) __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BPT; b++) {
+        int i = threadIdx.x + b * T;
+        if (TOT % T == 0 || i < TOT) {
+            int l, j;
+            item_of<NL, NB, CM>(i, l, j);
+            const int k = j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    float2 w = tw[r * k * TWS];
+                    v[b][r] = f2tw(v[b][r], w.x, w.y, sg);
+                }
+            }
+            dft<R>(v[b], sg);
+            const int ob = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) st(l, ob + r * NS, v[b][r]);
+        }
+    }
+}
+
+template <int L, int NL>
+struct LdsIO {
+    float2 *buf;
+    __device__ float2 operator()(int l, int pos) const { return buf[l * Shape<L>::LD + lds_pos(pos)]; }
+    __device__ void operator()(int l, int pos, float2 v) const { buf[l * Shape<L>::LD + lds_pos(pos)] = v; }
+};
+template <int L, int NL>
+struct LdsRd {
+    const float2 *buf;
+    __device__ float2 operator()(int l, int pos) const { return buf[l * Shape<L>::LD + lds_pos(pos)]; }
+};
+template <int L, int NL>
+struct LdsWr {
+    float2 *buf;
+    __device__ void operator()(int l, int pos, float2 v) const { buf[l * Shape<L>::LD + lds_pos(pos)] = v; }
+};
+
+// stages [S, n) of a plan; stage 0 reads `first`, the final stage writes `last`
+template <int L, int NL, bool CM, bool REV, int S, class First, class Last>
+__device__ __forceinline__ void run_from(float sg, const float2 *tw, float2 *buf, First first, Last last) {
+    using P = Plan<L, REV>;
+    constexpr int T = Shape<L>::T;
+    if constexpr (S < P::n) {
+        constexpr int R = P::R(S), NS = P::NS(S);
+        constexpr bool FIRST = S == 0, FINAL = S == P::n - 1;
+        if constexpr (FIRST && FINAL) {
+            stage<L, NL, T, CM, R, NS, false>(sg, tw, first, last);
+        } else if constexpr (FIRST) {
+            stage<L, NL, T, CM, R, NS, false>(sg, tw, first, LdsWr<L, NL>{buf});
+            __syncthreads();
+            run_from<L, NL, CM, REV, S + 1>(sg, tw, buf, first, last);
+        } else if constexpr (FINAL) {
+            stage<L, NL, T, CM, R, NS, false>(sg, tw, LdsRd<L, NL>{buf}, last);
+        } else {
+            stage<L, NL, T, CM, R, NS, true>(sg, tw, LdsRd<L, NL>{buf}, LdsWr<L, NL>{buf});
+            __syncthreads();
+            run_from<L, NL, CM, REV, S + 1>(sg, tw, buf, first, last);
+        }
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void load_twiddles(float2 *tw_s, const float2 *__restrict__ tw) {
+    for (int i = threadIdx.x; i < L; i += Shape<L>::T) tw_s[i] = tw[i];
+}
+
+__device__ __forceinline__ int reflect_idx(int i, int n) { return i < n ? i : 2 * n - 1 - i; }
+
+// u8 tile prologue: f = u8/255 -> ring subtraction -> z-drift -> dewindow
+__device__ __forceinline__ float prologue(const fyg_prologue &P, unsigned char u, long gz, long gy, long gx) {
+    float f = (float)u * (1.0f / 255.0f);
+    if (P.dering && f > 0.0f) {
+        double dy = (double)gy - P.cy, dx = (double)gx - P.cx;
+        int ri = (int)(sqrt(dy * dy + dx * dx) + 0.5);
+        if (ri < P.nr) {
+            long slab = gz / P.slab_z;
+            if (slab < 0) slab = 0;
+            if (slab >= P.nslab) slab = P.nslab - 1;
+            float v = f - P.scale * P.ring[slab * P.nr + ri];
+            f = v > 0.0f ? v : 0.0f;
+        }
+    }
+    if (P.zdrift) f *= P.zfactor[gz];
+    if (P.window) f = P.wlo + f * P.wspan;
+    return f;
+}
+
+__device__ __forceinline__ float epilogue(const fyg_epilogue &E, float v) {
+    if (E.window) v = (v - E.wlo) * E.winv;
+    if (E.dec) v = (v - E.dlo) * E.dinv;
+    if (E.clamp01) v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+    return v;
+}
+
+// ------------------------------------------------------------------ kernels
+struct XFwd {
+    const void *in;
+    int nz, ny, nx, pz, py, hx, hxp;
+    long nlines, npairs;
+    float2 *spec;
+    const float2 *tw;
+    fyg_prologue pro;
+    int *flag;
+};
+
+template <int L, bool U8>
+__global__ __launch_bounds__(256) void k_rfft_x(XFwd a) {
+    constexpr int NL = Shape<L>::NL_RM;
+    __shared__ float2 buf[NL * Shape<L>::LD];
+    __shared__ float2 tw_s[L];
+    load_twiddles<L>(tw_s, a.tw);
+    __syncthreads();
+    const long g0 = (long)blockIdx.x * NL;
+    int any = 0;
+    auto first = [&](int l, int x) -> float2 {
+        long g = g0 + l;
+        if (g >= a.nlines) return make_float2(0.f, 0.f);
+        int z = (int)(g / a.npairs);
+        int y0 = 2 * (int)(g - (long)z * a.npairs);
+        int rz = reflect_idx(z, a.nz), rx = reflect_idx(x, a.nx);
+        int ry0 = reflect_idx(y0, a.ny);
+        bool pair = y0 + 1 < a.py;
+        int ry1 = pair ? reflect_idx(y0 + 1, a.ny) : 0;
+        size_t i0 = ((size_t)rz * a.ny + ry0) * a.nx + rx;
+        size_t i1 = ((size_t)rz * a.ny + ry1) * a.nx + rx;
+        float va, vb = 0.0f;
+        if (U8) {
+            const unsigned char *u = (const unsigned char *)a.in;
+            unsigned char ua = u[i0];
+            any |= ua;
+            va = prologue(a.pro, ua, a.pro.gz0 + rz, a.pro.gy0 + ry0, a.pro.gx0 + rx);
+            if (pair) {
+                unsigned char ub = u[i1];
+                any |= ub;
+                vb = prologue(a.pro, ub, a.pro.gz0 + rz, a.pro.gy0 + ry1, a.pro.gx0 + rx);
+            }
+        } else {
+            const float *f = (const float *)a.in;
+            va = f[i0];
+            if (pair) vb = f[i1];
+        }
+        return make_float2(va, vb);
+    };
+    run_from<L, NL, false, false, 0>(-1.0f, tw_s, buf, first, LdsWr<L, NL>{buf});
+    __syncthreads();
+    if (U8 && a.flag) {
+        if (__syncthreads_or(any) && threadIdx.x == 0) *a.flag = 1;
+    }
+    // untangle A(k) = (X(k) + conj X(-k))/2, B(k) = (X(k) - conj X(-k))/(2i)
+    for (int i = threadIdx.x; i < NL * a.hxp; i += Shape<L>::T) {
+        int l = i / a.hxp, k = i - l * a.hxp;
+        long g = g0 + l;
+        if (g >= a.nlines) continue;
+        int z = (int)(g / a.npairs);
+        int y0 = 2 * (int)(g - (long)z * a.npairs);
+        bool pair = y0 + 1 < a.py;
+        float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
+        if (k < a.hx) {
+            int nk = k ? L - k : 0;
+            float2 p = buf[l * Shape<L>::LD + lds_pos(k)];
+            float2 q = buf[l * Shape<L>::LD + lds_pos(nk)];
+            A = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
+            B = make_float2(0.5f * (p.y + q.y), 0.5f * (q.x - p.x));
+        }
+        size_t row = ((size_t)z * a.py + y0) * a.hxp;
+        a.spec[row + k] = A;
+        if (pair) a.spec[row + a.hxp + k] = B;
+    }
+}
+
+struct Cols {
+    float2 *data;
+    long ncols, estride, bstride;
+    float sg;
+    const float2 *tw;
+};
+
+template <int L>
+__global__ __launch_bounds__(256) void k_cfft_cols(Cols a) {
+    constexpr int NL = Shape<L>::NL_CM;
+    __shared__ float2 buf[NL * Shape<L>::LD];
+    __shared__ float2 tw_s[L];
+    load_twiddles<L>(tw_s, a.tw);
+    __syncthreads();
+    const long c0 = (long)blockIdx.x * NL;
+    float2 *base = a.data + (size_t)blockIdx.y * a.bstride + c0;
+    const long nc = a.ncols - c0;
+    auto first = [&](int l, int pos) -> float2 {
+        return l < nc ? base[(size_t)pos * a.estride + l] : make_float2(0.f, 0.f);
+    };
+    auto last = [&](int l, int pos, float2 v) {
+        if (l < nc) base[(size_t)pos * a.estride + l] = v;
+    };
+    if (a.sg < 0)
+        run_from<L, NL, true, false, 0>(-1.0f, tw_s, buf, first, last);
+    else
+        run_from<L, NL, true, false, 0>(1.0f, tw_s, buf, first, last);
+}
+
+struct ZFused {
+    float2 *data;
+    int py, hx, hxp;
+    long estride;               // py*hxp
+    const float *lut;
+    double us;
+    const double *fz2, *fy2, *fx2;
+    const float2 *tw;
+};
+
+// forward Z (plan order), weight, inverse Z (reversed plan), all in one kernel
+template <int L>
+__global__ __launch_bounds__(256) void k_zfused(ZFused a) {
+    constexpr int NL = Shape<L>::NL_CM;
+    constexpr int T = Shape<L>::T;
+    using PF = Plan<L, false>;
+    using PI = Plan<L, true>;
+    constexpr int n = PF::n;
+    constexpr int RL = PF::R(n - 1);          // last forward radix == first inverse radix
+    constexpr int NSL = PF::NS(n - 1);
+    __shared__ float2 buf[NL * Shape<L>::LD];
+    __shared__ float2 tw_s[L];
+    __shared__ double s_fyx[NL];
+    load_twiddles<L>(tw_s, a.tw);
+    const long c0 = (long)blockIdx.x * NL;
+    if (threadIdx.x < NL) {
+        long c = c0 + threadIdx.x;
+        int y = (int)(c / a.hxp), k = (int)(c - (long)y * a.hxp);
+        // column validity is carried by a negative sentinel (k >= hx or past the end)
+        s_fyx[threadIdx.x] = (y < a.py && k < a.hx) ? (double)k : -1.0;
+    }
+    __syncthreads();
+    float2 *base = a.data + c0;
+    auto valid = [&](int l) { return s_fyx[l] >= 0.0; };
+    auto first = [&](int l, int pos) -> float2 {
+        return valid(l) ? base[(size_t)pos * a.estride + l] : make_float2(0.f, 0.f);
+    };
+    auto last = [&](int l, int pos, float2 v) {
+        if (valid(l)) base[(size_t)pos * a.estride + l] = v;
+    };
+    // LUT weight of deconv_radial (paganin.c:231-242); z = pos
+    auto weight = [&](int l, int pos) -> float {
+        long c = c0 + l;
+        int y = (int)(c / a.hxp), k = (int)(c - (long)y * a.hxp);
+        if (!valid(l)) return 0.0f;
+        double fzy = a.fz2[pos] + a.fy2[y];
+        double t = (fzy + a.fx2[k]) * a.us;
+        int ti = (int)t;
+        if (ti >= 4096) ti = 4095;
+        float l0 = a.lut[ti], l1 = a.lut[ti + 1];
+        return l0 + (l1 - l0) * (float)(t - (double)ti);
+    };
+    // forward stages 0 .. n-2
+    if constexpr (n > 1) {
+        run_from<L, NL, true, false, 0>(-1.0f, tw_s, buf, first, LdsWr<L, NL>{buf});
+        // run_from above executes every forward stage including the last one into
+        // LDS; the fused weight step below therefore needs only a radix-RL pass
+    }
+    // The fused step: for n == 1 it reads HBM directly; otherwise the forward
+    // transform is complete in LDS (natural order) and the weight + first inverse
+    // stage (radix RL, NS = 1) read it from there.
+    {
+        constexpr int NB = L / RL;
+        constexpr int TOT = NL * NB;
+        constexpr int BPT = (TOT + T - 1) / T;
+        float2 v[BPT][RL];
+        if constexpr (n > 1) __syncthreads();
+#pragma unroll
+        for (int b = 0; b < BPT; b++) {
+            int i = threadIdx.x + b * T;
+            if (TOT % T == 0 || i < TOT) {
+                int l, j;
+                item_of<NL, NB, true>(i, l, j);
+#pragma unroll
+                for (int r = 0; r < RL; r++) {
+                    int pos = j + r * NB;
+                    float2 x;
+                    if constexpr (n > 1) x = buf[l * Shape<L>::LD + lds_pos(pos)];
+                    else x = first(l, pos);
+                    v[b][r] = x;
+                }
+                if constexpr (n == 1) dft<RL>(v[b], -1.0f);
+#pragma unroll
+                for (int r = 0; r < RL; r++) {
+                    float w = weight(l, j + r * NB);
+                    v[b][r].x *= w;
+                    v[b][r].y *= w;
+                }
+                dft<RL>(v[b], 1.0f);
+            }
+        }
+        if constexpr (n > 1) {
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < BPT; b++) {
+                int i = threadIdx.x + b * T;
+                if (TOT % T == 0 || i < TOT) {
+                    int l, j;
+                    item_of<NL, NB, true>(i, l, j);
+#pragma unroll
+                    for (int r = 0; r < RL; r++) buf[l * Shape<L>::LD + lds_pos(j * RL + r)] = v[b][r];
+                }
+            }
+            __syncthreads();
+            run_from<L, NL, true, true, 1>(1.0f, tw_s, buf, first, last);
+        } else {
+#pragma unroll
+            for (int b = 0; b < BPT; b++) {
+                int i = threadIdx.x + b * T;
+                if (TOT % T == 0 || i < TOT) {
+                    int l, j;
+                    item_of<NL, NB, true>(i, l, j);
+#pragma unroll
+                    for (int r = 0; r < RL; r++) last(l, j * RL + r, v[b][r]);
+                }
+            }
+        }
+    }
+    (void)NSL;
+}
+
+struct XInv {
+    const float2 *spec;
+    float *out;
+    int nz, ny, nx, py, hx, hxp;
+    long nlines, npairs;
+    float invn;
+    fyg_epilogue epi;
+    const float2 *tw;
+};
+
+template <int L>
+__global__ __launch_bounds__(256) void k_irfft_x(XInv a) {
+    constexpr int NL = Shape<L>::NL_RM;
+    __shared__ float2 buf[NL * Shape<L>::LD];
+    __shared__ float2 tw_s[L];
+    load_twiddles<L>(tw_s, a.tw);
+    __syncthreads();
+    const long g0 = (long)blockIdx.x * NL;
+    auto rowsof = [&](int l, int &z, int &y0, bool &pair) -> bool {
+        long g = g0 + l;
+        if (g >= a.nlines) return false;
+        z = (int)(g / a.npairs);
+        y0 = 2 * (int)(g - (long)z * a.npairs);
+        pair = y0 + 1 < a.ny;
+        return true;
+    };
+    auto first = [&](int l, int x) -> float2 {
+        int z, y0;
+        bool pair;
+        if (!rowsof(l, z, y0, pair)) return make_float2(0.f, 0.f);
+        int m = x < a.hx ? x : L - x;
+        size_t row = ((size_t)z * a.py + y0) * a.hxp;
+        float2 A = a.spec[row + m];
+        float2 B = pair ? a.spec[row + a.hxp + m] : make_float2(0.f, 0.f);
+        return x < a.hx ? make_float2(A.x - B.y, A.y + B.x) : make_float2(A.x + B.y, B.x - A.y);
+    };
+    auto last = [&](int l, int x, float2 v) {
+        int z, y0;
+        bool pair;
+        if (x >= a.nx || !rowsof(l, z, y0, pair)) return;
+        size_t o = ((size_t)z * a.ny + y0) * a.nx + x;
+        a.out[o] = epilogue(a.epi, v.x * a.invn);
+        if (pair) a.out[o + a.nx] = epilogue(a.epi, v.y * a.invn);
+    };
+    run_from<L, NL, false, false, 0>(1.0f, tw_s, buf, first, last);
+}
+
+struct Rows {
+    float2 *data;
+    long nlines;
+    float sg;
+    const float2 *tw;
+};
+
+template <int L>
+__global__ __launch_bounds__(256) void k_cfft_rows(Rows a) {
+    constexpr int NL = Shape<L>::NL_RM;
+    __shared__ float2 buf[NL * Shape<L>::LD];
+    __shared__ float2 tw_s[L];
+    load_twiddles<L>(tw_s, a.tw);
+    __syncthreads();
+    const long g0 = (long)blockIdx.x * NL;
+    auto first = [&](int l, int x) -> float2 {
+        long g = g0 + l;
+        return g < a.nlines ? a.data[(size_t)g * L + x] : make_float2(0.f, 0.f);
+    };
+    auto last = [&](int l, int x, float2 v) {
+        long g = g0 + l;
+        if (g < a.nlines) a.data[(size_t)g * L + x] = v;
+    };
+    if (a.sg < 0)
+        run_from<L, NL, false, false, 0>(-1.0f, tw_s, buf, first, last);
+    else
+        run_from<L, NL, false, false, 0>(1.0f, tw_s, buf, first, last);
+}
+
+__global__ void k_interleave(const float *re, const float *im, float2 *c, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        c[i] = make_float2(re[i], im[i]);
+}
+__global__ void k_deinterleave(const float2 *c, float *re, float *im, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float2 v = c[i];
+        re[i] = v.x;
+        im[i] = v.y;
+    }
+}
+__global__ void k_scale(float *d, size_t n, float k) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] *= k;
+}
+
+unsigned grid1d(size_t n) {
+    size_t b = (n + 255) / 256;
+    return (unsigned)(b < 8192 ? (b ? b : 1) : 8192);
+}
+
+// ------------------------------------------------------------------ dispatch over L
+template <int L>
+struct Launch {
+    static void rfft_x(fyg_stream *s, const XFwd &a, bool u8) {
+        unsigned g = (unsigned)((a.nlines + Shape<L>::NL_RM - 1) / Shape<L>::NL_RM);
+        if (u8) fyg_launch(s, "rfft_x_u8", k_rfft_x<L, true>, dim3(g), dim3(256), 0, a);
+        else fyg_launch(s, "rfft_x", k_rfft_x<L, false>, dim3(g), dim3(256), 0, a);
+    }
+    static void cols(fyg_stream *s, const Cols &a, int nbatch, const char *name) {
+        unsigned g = (unsigned)((a.ncols + Shape<L>::NL_CM - 1) / Shape<L>::NL_CM);
+        fyg_launch(s, name, k_cfft_cols<L>, dim3(g, nbatch), dim3(256), 0, a);
+    }
+    static void zfused(fyg_stream *s, const ZFused &a) {
+        long nc = (long)a.py * a.hxp;
+        unsigned g = (unsigned)((nc + Shape<L>::NL_CM - 1) / Shape<L>::NL_CM);
+        fyg_launch(s, "zfused", k_zfused<L>, dim3(g), dim3(256), 0, a);
+    }
+    static void irfft_x(fyg_stream *s, const XInv &a) {
+        unsigned g = (unsigned)((a.nlines + Shape<L>::NL_RM - 1) / Shape<L>::NL_RM);
+        fyg_launch(s, "irfft_x", k_irfft_x<L>, dim3(g), dim3(256), 0, a);
+    }
+    static void rows(fyg_stream *s, const Rows &a) {
+        unsigned g = (unsigned)((a.nlines + Shape<L>::NL_RM - 1) / Shape<L>::NL_RM);
+        fyg_launch(s, "cfft_rows", k_cfft_rows<L>, dim3(g), dim3(256), 0, a);
+    }
+};
+
+template <template <int> class F, class... Args>
+bool dispatch(int L, Args &&...args) {
+    switch (L) {
+#define FYG_CASE(n) \
+    case n: F<n>::go(args...); return true;
+        FYG_CASE(1) FYG_CASE(2) FYG_CASE(3) FYG_CASE(4) FYG_CASE(6) FYG_CASE(8) FYG_CASE(12)
+        FYG_CASE(16) FYG_CASE(24) FYG_CASE(32) FYG_CASE(48) FYG_CASE(64) FYG_CASE(96)
+        FYG_CASE(128) FYG_CASE(192) FYG_CASE(256) FYG_CASE(384) FYG_CASE(512) FYG_CASE(768)
+        FYG_CASE(1024) FYG_CASE(1536) FYG_CASE(2048) FYG_CASE(3072) FYG_CASE(4096)
+#undef FYG_CASE
+        default: return false;
+    }
+}
+
+template <int L>
+struct DoRfftX {
+    static void go(fyg_stream *s, const XFwd &a, bool u8) { Launch<L>::rfft_x(s, a, u8); }
+};
+template <int L>
+struct DoCols {
+    static void go(fyg_stream *s, const Cols &a, int nb, const char *nm) { Launch<L>::cols(s, a, nb, nm); }
+};
+template <int L>
+struct DoZ {
+    static void go(fyg_stream *s, const ZFused &a) { Launch<L>::zfused(s, a); }
+};
+template <int L>
+struct DoIrfftX {
+    static void go(fyg_stream *s, const XInv &a) { Launch<L>::irfft_x(s, a); }
+};
+template <int L>
+struct DoRows {
+    static void go(fyg_stream *s, const Rows &a) { Launch<L>::rows(s, a); }
+};
+
+}  // namespace
+
+extern "C" int fyg_fft_supported(int L) {
+    if (L < 1 || L > FYG_FFT_MAX) return 0;
+    int m = L % 3 == 0 ? L / 3 : L;
+    return (m & (m - 1)) == 0;
+}
+
+extern "C" int fyg_deconv_hxp(int px) {
+    int hx = px / 2 + 1;
+    return (hx + 15) / 16 * 16;
+}
+
+extern "C" size_t fyg_deconv_spec_bytes(int pz, int py, int px) {
+    return sizeof(float2) * (size_t)pz * py * fyg_deconv_hxp(px);
+}
+
+extern "C" int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s) {
+    if (!fyg_fft_supported(a->px) || !fyg_fft_supported(a->py) || !fyg_fft_supported(a->pz)) return 1;
+    const float2 *twx = (const float2 *)fyg_twiddles(a->px);
+    const float2 *twy = (const float2 *)fyg_twiddles(a->py);
+    const float2 *twz = (const float2 *)fyg_twiddles(a->pz);
+    if (!twx || !twy || !twz) return 1;
+    float2 *spec = (float2 *)a->spec;
+    XFwd xf{a->in, a->nz, a->ny, a->nx, a->pz, a->py, a->hx, a->hxp,
+            (long)a->pz * ((a->py + 1) / 2), (long)(a->py + 1) / 2, spec, twx, a->pro, a->nonzero_flag};
+    dispatch<DoRfftX>(a->px, s, xf, a->in_u8 != 0);
+    long plane = (long)a->py * a->hxp;
+    Cols yf{spec, a->hxp, a->hxp, plane, -1.0f, twy};
+    dispatch<DoCols>(a->py, s, yf, a->pz, "cfft_y_fwd");
+    ZFused zf{spec, a->py, a->hx, a->hxp, plane, a->lut, a->us, a->fz2, a->fy2, a->fx2, twz};
+    dispatch<DoZ>(a->pz, s, zf);
+    Cols yi{spec, a->hxp, a->hxp, plane, 1.0f, twy};
+    dispatch<DoCols>(a->py, s, yi, a->nz, "cfft_y_inv");
+    XInv xi{spec, a->out, a->nz, a->ny, a->nx, a->py, a->hx, a->hxp,
+            (long)a->nz * ((a->ny + 1) / 2), (long)(a->ny + 1) / 2,
+            1.0f / ((float)a->pz * (float)a->py * (float)a->px), a->epi, twx};
+    dispatch<DoIrfftX>(a->px, s, xi);
+    return fyg_last_error();
+}
+
+extern "C" int fyg_fft3d_run(float *d_re, float *d_im, int nz, int ny, int nx, int sign, void *work,
+                             fyg_stream *s) {
+    if (!fyg_fft_supported(nx) || !fyg_fft_supported(ny) || !fyg_fft_supported(nz)) return 1;
+    size_t n = (size_t)nz * ny * nx;
+    float2 *c = (float2 *)work;
+    float sg = sign < 0 ? -1.0f : 1.0f;
+    fyg_launch(s, "interleave", k_interleave, dim3(grid1d(n)), dim3(256), 0, (const float *)d_re,
+               (const float *)d_im, c, n);
+    Rows rx{c, (long)nz * ny, sg, (const float2 *)fyg_twiddles(nx)};
+    dispatch<DoRows>(nx, s, rx);
+    if (ny > 1) {
+        Cols cy{c, nx, nx, (long)ny * nx, sg, (const float2 *)fyg_twiddles(ny)};
+        dispatch<DoCols>(ny, s, cy, nz, "cfft_cols");
+    }
+    if (nz > 1) {
+        Cols cz{c, (long)ny * nx, (long)ny * nx, 0, sg, (const float2 *)fyg_twiddles(nz)};
+        dispatch<DoCols>(nz, s, cz, 1, "cfft_cols");
+    }
+    fyg_launch(s, "deinterleave", k_deinterleave, dim3(grid1d(n)), dim3(256), 0, (const float2 *)c, d_re,
+               d_im, n);
+    return fyg_last_error();
+}
+
+extern "C" int fyg_scale_run(float *d, size_t n, float k, fyg_stream *s) {
+    fyg_launch(s, "scale", k_scale, dim3(grid1d(n)), dim3(256), 0, d, n, k);
+    return fyg_last_error();
+}

@@ -1,0 +1,35 @@
+// fyg_internal.hpp -- shared by the HIP translation units: the stream object,
+// the launch helper that brackets each launch with timing events when
+// profiling is on, and small device utilities.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fyg.h"
+
+struct fyg_stream {
+    hipStream_t s = nullptr;
+    bool owned = false;
+    void *ws[FYG_WS_SLOTS] = {};
+    size_t wcap[FYG_WS_SLOTS] = {};
+    void *pin[2] = {};
+    size_t pcap[2] = {};
+};
+
+namespace fyg_prof {
+void begin(fyg_stream *s, const char *name, void **tok);
+void end(fyg_stream *s, void *tok);
+}  // namespace fyg_prof
+
+// launch k<<<g,b,sh,stream>>>(args...) and time it when profiling is enabled
+template <typename... KArgs, typename... Args>
+inline void fyg_launch(fyg_stream *s, const char *name, void (*k)(KArgs...), dim3 g, dim3 b,
+                       size_t sh, Args... args) {
+    void *tok;
+    fyg_prof::begin(s, name, &tok);
+    k<<<g, b, sh, s->s>>>(static_cast<KArgs>(args)...);
+    fyg_prof::end(s, tok);
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}

@@ -1,0 +1,284 @@
+// fyg_runtime.hip -- device runtime behind the fysics drop-in: per-thread
+// streams, grow-only device workspaces and pinned staging, a process-wide cache
+// of small read-only tables (twiddles, transfer LUTs) and event-based kernel
+// timing.  The reference keeps thread-local host scratch that is never freed
+// (fft.c:45,109-112; guided.c:49,83; pipeline.c:481-488); the device analogue
+// is one stream + workspace set per calling host thread, so concurrent callers
+// (OpenMP tile loops, export worker pools) never serialise on a global sync.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <list>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "fyg_internal.hpp"
+
+static thread_local fyg_stream *tls_stream = nullptr;
+static thread_local char tls_err[256] = {0};
+static std::atomic<int> g_err{0};
+
+static int note(hipError_t e, const char *what) {
+    if (e == hipSuccess) return 0;
+    snprintf(tls_err, sizeof tls_err, "%s: %s", what, hipGetErrorString(e));
+    g_err.store(1);
+    return 1;
+}
+
+extern "C" const char *fyg_error_string(void) { return tls_err; }
+
+extern "C" fyg_stream *fyg_thread_stream(void) {
+    if (tls_stream) return tls_stream;
+    fyg_stream *st = new fyg_stream();
+    if (note(hipStreamCreateWithFlags(&st->s, hipStreamNonBlocking), "hipStreamCreate")) {
+        delete st;
+        return nullptr;
+    }
+    st->owned = true;
+    tls_stream = st;
+    return st;
+}
+
+// external streams: one wrapper (with its own workspaces) per hipStream_t
+static std::mutex g_wrap_mu;
+static std::unordered_map<void *, fyg_stream *> g_wrapped;
+
+extern "C" fyg_stream *fyg_stream_wrap(void *hip_stream) {
+    if (!hip_stream) return fyg_thread_stream();
+    std::lock_guard<std::mutex> lk(g_wrap_mu);
+    auto it = g_wrapped.find(hip_stream);
+    if (it != g_wrapped.end()) return it->second;
+    fyg_stream *st = new fyg_stream();
+    st->s = (hipStream_t)hip_stream;
+    g_wrapped[hip_stream] = st;
+    return st;
+}
+
+extern "C" void *fyg_stream_handle(fyg_stream *s) { return s ? (void *)s->s : nullptr; }
+
+extern "C" int fyg_sync(fyg_stream *s) {
+    if (!s) return 1;
+    return note(hipStreamSynchronize(s->s), "hipStreamSynchronize");
+}
+
+extern "C" void *fyg_ws(fyg_stream *s, int slot, size_t bytes) {
+    if (!s || slot < 0 || slot >= FYG_WS_SLOTS) return nullptr;
+    if (bytes == 0) bytes = 256;
+    if (s->wcap[slot] >= bytes) return s->ws[slot];
+    // the previous buffer may still be read by queued work on this stream
+    if (s->ws[slot]) {
+        if (note(hipStreamSynchronize(s->s), "hipStreamSynchronize")) return nullptr;
+        (void)hipFree(s->ws[slot]);
+        s->ws[slot] = nullptr;
+        s->wcap[slot] = 0;
+    }
+    size_t want = bytes + bytes / 8;      // slack so slowly growing tiles do not realloc each call
+    void *p = nullptr;
+    if (note(hipMalloc(&p, want), "hipMalloc(workspace)")) return nullptr;
+    s->ws[slot] = p;
+    s->wcap[slot] = want;
+    return p;
+}
+
+extern "C" void *fyg_pinned(fyg_stream *s, int slot, size_t bytes) {
+    if (!s || slot < 0 || slot > 1) return nullptr;
+    if (s->pcap[slot] >= bytes) return s->pin[slot];
+    if (s->pin[slot]) {
+        if (note(hipStreamSynchronize(s->s), "hipStreamSynchronize")) return nullptr;
+        (void)hipHostFree(s->pin[slot]);
+        s->pin[slot] = nullptr;
+        s->pcap[slot] = 0;
+    }
+    void *p = nullptr;
+    if (note(hipHostMalloc(&p, bytes, hipHostMallocDefault), "hipHostMalloc")) return nullptr;
+    s->pin[slot] = p;
+    s->pcap[slot] = bytes;
+    return p;
+}
+
+extern "C" int fyg_malloc(void **p, size_t bytes) {
+    return note(hipMalloc(p, bytes ? bytes : 256), "hipMalloc");
+}
+extern "C" void fyg_free(void *p) {
+    if (p) (void)hipFree(p);
+}
+extern "C" int fyg_h2d(void *dst, const void *src, size_t bytes, fyg_stream *s) {
+    if (!bytes) return 0;
+    return note(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->s), "hipMemcpyH2D");
+}
+extern "C" int fyg_d2h(void *dst, const void *src, size_t bytes, fyg_stream *s) {
+    if (!bytes) return 0;
+    return note(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->s), "hipMemcpyD2H");
+}
+extern "C" int fyg_d2d(void *dst, const void *src, size_t bytes, fyg_stream *s) {
+    if (!bytes || dst == src) return 0;
+    return note(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s->s), "hipMemcpyD2D");
+}
+
+__global__ void k_fill_zero(uint32_t *p, size_t nwords, unsigned char *tail, int ntail) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < nwords; i += stride) p[i] = 0u;
+    if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+extern "C" int fyg_zero(void *dst, size_t bytes, fyg_stream *s) {
+    if (!bytes) return 0;
+    size_t nw = bytes / 4;
+    int ntail = (int)(bytes - nw * 4);
+    unsigned blocks = (unsigned)std::min<size_t>((nw + 255) / 256 + 1, 4096);
+    fyg_launch(s, "fill_zero", k_fill_zero, dim3(blocks), dim3(256), 0,
+               (uint32_t *)dst, nw, (unsigned char *)dst + nw * 4, ntail);
+    return fyg_last_error();
+}
+
+extern "C" int fyg_last_error(void) {
+    return note(hipGetLastError(), "kernel launch");
+}
+
+// ------------------------------------------------------------------ table cache
+namespace {
+struct TableEntry {
+    void *dev = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_tab_mu;
+std::unordered_map<std::string, TableEntry> g_tab;
+std::list<std::string> g_tab_lru;       // transfer tables only (keys not starting with 'T')
+const size_t kMaxTables = 512;
+}  // namespace
+
+extern "C" const void *fyg_table_find(const void *key, size_t keylen) {
+    std::string k((const char *)key, keylen);
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto it = g_tab.find(k);
+    return it == g_tab.end() ? nullptr : it->second.dev;
+}
+
+extern "C" const void *fyg_table(const void *key, size_t keylen, const void *data, size_t bytes) {
+    std::string k((const char *)key, keylen);
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto it = g_tab.find(k);
+    if (it != g_tab.end()) return it->second.dev;
+    if (!data) return nullptr;
+    if (k[0] != 'T' && g_tab_lru.size() >= kMaxTables) {
+        // evict the oldest transfer table; wait for all queued users first
+        (void)hipDeviceSynchronize();
+        auto old = g_tab.find(g_tab_lru.front());
+        if (old != g_tab.end()) {
+            (void)hipFree(old->second.dev);
+            g_tab.erase(old);
+        }
+        g_tab_lru.pop_front();
+    }
+    void *dev = nullptr;
+    if (note(hipMalloc(&dev, bytes), "hipMalloc(table)")) return nullptr;
+    // synchronous upload: the table must be valid before any stream uses it
+    if (note(hipMemcpy(dev, data, bytes, hipMemcpyHostToDevice), "hipMemcpy(table)")) {
+        (void)hipFree(dev);
+        return nullptr;
+    }
+    g_tab[k] = TableEntry{dev, bytes};
+    if (k[0] != 'T') g_tab_lru.push_back(k);
+    return dev;
+}
+
+extern "C" const void *fyg_twiddles(int L) {
+    char key[16];
+    int kl = snprintf(key, sizeof key, "Ttw%d", L);
+    const void *d = fyg_table_find(key, (size_t)kl);
+    if (d) return d;
+    std::vector<float> tw(2 * (size_t)L);
+    for (int m = 0; m < L; m++) {
+        double a = 2.0 * M_PI * (double)m / (double)L;
+        tw[2 * m] = (float)cos(a);
+        tw[2 * m + 1] = (float)sin(a);
+    }
+    return fyg_table(key, (size_t)kl, tw.data(), sizeof(float) * tw.size());
+}
+
+// ------------------------------------------------------------------ profiling
+namespace fyg_prof {
+std::atomic<int> enabled{0};
+std::mutex mu;
+struct Rec {
+    const char *name;
+    hipEvent_t a, b;
+};
+std::vector<Rec> pending;
+struct Acc {
+    double ms = 0;
+    long n = 0;
+};
+std::vector<std::pair<std::string, Acc>> acc;
+
+void begin(fyg_stream *s, const char *name, void **tok) {
+    *tok = nullptr;
+    if (!enabled.load(std::memory_order_relaxed)) return;
+    Rec *r = new Rec{name, nullptr, nullptr};
+    (void)hipEventCreate(&r->a);
+    (void)hipEventCreate(&r->b);
+    (void)hipEventRecord(r->a, s->s);
+    *tok = r;
+}
+void end(fyg_stream *s, void *tok) {
+    if (!tok) return;
+    Rec *r = (Rec *)tok;
+    (void)hipEventRecord(r->b, s->s);
+    std::lock_guard<std::mutex> lk(mu);
+    pending.push_back(*r);
+    delete r;
+}
+}  // namespace fyg_prof
+
+extern "C" void fyg_prof_enable(int on) { fyg_prof::enabled.store(on ? 1 : 0); }
+
+static void prof_drain() {
+    std::lock_guard<std::mutex> lk(fyg_prof::mu);
+    for (auto &r : fyg_prof::pending) {
+        (void)hipEventSynchronize(r.b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        bool found = false;
+        for (auto &kv : fyg_prof::acc)
+            if (kv.first == r.name) {
+                kv.second.ms += ms;
+                kv.second.n += 1;
+                found = true;
+                break;
+            }
+        if (!found) fyg_prof::acc.push_back({std::string(r.name), fyg_prof::Acc{ms, 1}});
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    fyg_prof::pending.clear();
+}
+
+extern "C" int fyg_prof_collect(char *names, int names_len, double *ms, long *count, int max) {
+    prof_drain();
+    std::lock_guard<std::mutex> lk(fyg_prof::mu);
+    int n = 0;
+    size_t off = 0;
+    for (auto &kv : fyg_prof::acc) {
+        if (n >= max) break;
+        size_t l = kv.first.size() + 1;
+        if (off + l > (size_t)names_len) break;
+        memcpy(names + off, kv.first.c_str(), l);
+        off += l;
+        ms[n] = kv.second.ms;
+        count[n] = kv.second.n;
+        n++;
+    }
+    return n;
+}
+
+extern "C" void fyg_prof_reset(void) {
+    prof_drain();
+    std::lock_guard<std::mutex> lk(fyg_prof::mu);
+    fyg_prof::acc.clear();
+}

@@ -1,0 +1,259 @@
+/* fy_chain.c -- the per-tile chain of fy_process_buffer (pipeline.c:252-297,
+ * 413-421, 447-510) scheduled as device work:
+ *
+ *   X-forward FFT kernel with the tile prologue fused into its gather
+ *       (u8 -> f01, ring subtraction, z-drift, dewindow)
+ *   Y forward, Z forward*weight*inverse, Y inverse
+ *   X-inverse kernel with rewindow / dec-range rescale / clamp fused into its store
+ *   guided filter (fast s=2 by default) whose last kernel emits the inner tile:
+ *       recenter, dithered quantisation at global coordinates, all-air handling
+ *
+ * With deconvolution off, one prologue kernel produces the float tile instead.
+ * The all-zero test of the reference (pipeline.c:473-477) becomes a device flag
+ * set by whichever kernel reads the u8 input; the emit stage writes zeros when
+ * it is clear, so no host round trip sits between the stages.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "fyh.h"
+
+/* FNV-1a over the content: cache key for small per-volume arrays */
+static unsigned long long fnv1a(const void *p, size_t n) {
+    const unsigned char *b = (const unsigned char *)p;
+    unsigned long long h = 1469598103934665603ULL;
+    for (size_t i = 0; i < n; i++) { h ^= b[i]; h *= 1099511628211ULL; }
+    return h;
+}
+
+const void *fyh_const_blob(char tag, const void *data, size_t bytes) {
+    struct { char tag[8]; unsigned long long h; size_t n; } key;
+    memset(&key, 0, sizeof key);
+    key.tag[0] = 'B'; key.tag[1] = tag;
+    key.h = fnv1a(data, bytes);
+    key.n = bytes;
+    const void *d = fyg_table_find(&key, sizeof key);
+    return d ? d : fyg_table(&key, sizeof key, data, bytes);
+}
+
+void fyh_scaled_physics(const fy_pipeline_cfg *c, fy_physics *p) {
+    p->delta_beta = c->delta_beta; p->energy_kev = c->energy_kev; p->distance_mm = c->distance_mm;
+    p->pixel_um = c->pixel_um; p->unsharp_sigma = c->unsharp_sigma; p->unsharp_coeff = c->unsharp_coeff;
+    p->psf_sigma_vox = c->psf_sigma_vox;
+    p->psf_sigma_z_vox = (c->psf_z_ratio > 1.0 && c->psf_sigma_vox > 0) ? c->psf_z_ratio * c->psf_sigma_vox : 0.0;
+    p->delta_beta *= fy_auto_deltabeta_scale(p);
+}
+
+/* per-tile guided eps from the radial noise fit (pipeline.c:447-456) */
+static double eps_for_tile(const fy_pipeline_cfg *cfg, double ecy, double ecx, long y0, long x0, long ty,
+                           long tx) {
+    if (!cfg->have_eps_r || cfg->guided_eps <= 0 || cfg->eps_fn_med <= 0) return 0.0;
+    double ry = y0 + ty / 2.0 - ecy, rx = x0 + tx / 2.0 - ecx;
+    double fn = cfg->eps_fn_a + cfg->eps_fn_b * sqrt(ry * ry + rx * rx);
+    double sc = fn / cfg->eps_fn_med, s2 = sc * sc;
+    if (s2 < 0.4) s2 = 0.4;
+    if (s2 > 8.0) s2 = 8.0;
+    return cfg->guided_eps * s2;
+}
+
+int fy_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long ty, long tx, long vol_y,
+                            long vol_x, fy_cube_params *o) {
+    memset(o, 0, sizeof *o);
+    if (cfg->do_air_zero || cfg->do_musica) return 1;          /* stages not provided */
+    fyh_scaled_physics(cfg, &o->phys);
+    if (cfg->do_deconv) {
+        int matched = cfg->use_matched_deconv && cfg->psf_sigma_vox > 0;
+        o->deconv = matched ? 2 : 1;
+        o->deconv_param = matched ? cfg->deconv_tikhonov : -1.0;
+        o->deconv_cap = matched ? 0.0 : FYH_MAX_GAIN;
+        o->fz2_scale = matched ? fyh_matched_fz2_scale(&o->phys) : 1.0;
+        o->window_lo = cfg->window_lo;
+        o->window_hi = cfg->window_hi;
+        o->have_dec_range = cfg->have_dec_range;
+        o->dec_lo = cfg->dec_lo;
+        o->dec_hi = cfg->dec_hi;
+    }
+    double ecy = cfg->dering_cy > 0 ? cfg->dering_cy : (vol_y - 1) / 2.0;
+    double ecx = cfg->dering_cx > 0 ? cfg->dering_cx : (vol_x - 1) / 2.0;
+    double eps_t = eps_for_tile(cfg, ecy, ecx, y0, x0, ty, tx);
+    if (cfg->guided_eps > 0 || eps_t > 0) {
+        o->guided_eps = eps_t > 0 ? eps_t : cfg->guided_eps;
+        o->guided_radius = 2;
+        o->guided_s = cfg->guided_subsample == 0 ? 2 : cfg->guided_subsample;
+    }
+    o->no_dither = cfg->no_dither;
+    if (cfg->have_norm && cfg->norm_hi > cfg->norm_lo) {
+        o->recenter = 1;
+        o->norm_lo = cfg->norm_lo;
+        o->norm_hi = cfg->norm_hi;
+    }
+    if (cfg->have_dering && cfg->dering) o->dering = cfg->dering;
+    if (cfg->have_zdrift && cfg->zdrift_factor) {
+        o->zdrift_factor = cfg->zdrift_factor;
+        o->vol_z = cfg->vol_z;
+    }
+    return 0;
+}
+
+static int chain_cube(const fy_cube *c, int *d_flag, fyg_stream *s) {
+    const fy_cube_params *P = c->params;
+    long rz0 = c->rz0, ry0 = c->ry0, rx0 = c->rx0, hz = c->hz, hy = c->hy, hx = c->hx;
+    if (hz <= 0 || hy <= 0 || hx <= 0 || c->tz <= 0 || c->ty <= 0 || c->tx <= 0) return 0;
+    if (c->z0 < rz0 || c->y0 < ry0 || c->x0 < rx0 || c->z0 + c->tz > rz0 + hz || c->y0 + c->ty > ry0 + hy ||
+        c->x0 + c->tx > rx0 + hx)
+        return 1;                                               /* inner tile must lie in the region */
+    size_t hn = (size_t)hz * hy * hx;
+    float *f = (float *)fyg_ws(s, WS_TILE, sizeof(float) * hn);
+    if (!f || fyg_zero(d_flag, sizeof(int), s)) return 1;
+
+    /* prologue (pipeline.c:492-494 and the dewindow of :266-268) */
+    fyg_prologue pro;
+    memset(&pro, 0, sizeof pro);
+    pro.gz0 = rz0; pro.gy0 = ry0; pro.gx0 = rx0;
+    if (P->dering && P->dering->have_rings) {
+        const fy_dering *d = P->dering;
+        pro.ring = (const float *)fyh_const_blob('R', d->ring, sizeof(float) * (size_t)d->nslab * d->nr);
+        if (!pro.ring) return 1;
+        pro.dering = 1; pro.nr = d->nr; pro.nslab = d->nslab; pro.slab_z = d->slab_z;
+        pro.cy = d->cy; pro.cx = d->cx; pro.scale = (float)(1.0 / 255.0);
+    }
+    if (P->zdrift_factor) {
+        if (P->vol_z < rz0 + hz || rz0 < 0) return 1;           /* factor table must cover the region */
+        pro.zfactor = (const float *)fyh_const_blob('Z', P->zdrift_factor, sizeof(float) * (size_t)P->vol_z);
+        if (!pro.zfactor) return 1;
+        pro.zdrift = 1;
+    }
+    int windowed = P->window_hi > P->window_lo;
+    float wlo = (float)P->window_lo, wspan = (float)(P->window_hi - P->window_lo);
+
+    if (P->deconv) {                                            /* (b) */
+        int matched = P->deconv == 2;
+        double param = P->deconv_param;
+        if (param <= 0) param = matched ? 0.05 : fy_auto_reg(&P->phys, 0.015);
+        fyh_tables t;
+        if (fyh_tables_get((int)hz, (int)hy, (int)hx, &P->phys, param, matched, P->deconv_cap,
+                           P->fz2_scale > 0 ? P->fz2_scale : 1.0, &t))
+            return 1;
+        if (!fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px)) return 1;
+        void *spec = fyg_ws(s, WS_SPEC, fyg_deconv_spec_bytes(t.pz, t.py, t.px));
+        if (!spec) return 1;
+        fyg_deconv_args a;
+        memset(&a, 0, sizeof a);
+        a.nz = (int)hz; a.ny = (int)hy; a.nx = (int)hx;
+        a.pz = t.pz; a.py = t.py; a.px = t.px; a.hx = t.hx; a.hxp = t.hxp;
+        a.lut = t.lut; a.us = t.us; a.fz2 = t.fz2; a.fy2 = t.fy2; a.fx2 = t.fx2;
+        a.in = c->d_in; a.in_u8 = 1;
+        a.pro = pro;
+        if (windowed) { a.pro.window = 1; a.pro.wlo = wlo; a.pro.wspan = wspan; }
+        a.out = f;
+        if (windowed) { a.epi.window = 1; a.epi.wlo = wlo; a.epi.winv = 1.0f / wspan; }
+        if (P->have_dec_range && P->dec_hi - P->dec_lo > 1e-6) {
+            a.epi.dec = 1; a.epi.dlo = (float)P->dec_lo; a.epi.dinv = 1.0f / (float)(P->dec_hi - P->dec_lo);
+        }
+        a.epi.clamp01 = 1;
+        a.spec = spec;
+        a.nonzero_flag = d_flag;
+        if (fyg_deconv_run(&a, s)) return 1;
+    } else {
+        if (fyg_prologue_run(c->d_in, f, (int)hz, (int)hy, (int)hx, &pro, d_flag, s)) return 1;
+    }
+
+    fyg_emit em;
+    memset(&em, 0, sizeof em);
+    em.quant = 1;
+    em.q = c->d_out;
+    em.iz0 = c->z0 - rz0; em.iy0 = c->y0 - ry0; em.ix0 = c->x0 - rx0;
+    em.tz = c->tz; em.ty = c->ty; em.tx = c->tx;
+    em.gz0 = rz0; em.gy0 = ry0; em.gx0 = rx0;
+    em.no_dither = P->no_dither;
+    if (P->recenter && P->norm_hi > P->norm_lo) {               /* (d.5) */
+        em.recenter = 1;
+        em.rmid = 0.5f * (float)(P->norm_lo + P->norm_hi) / 255.0f;
+        em.rk = (2.0f * 0.375f) * 255.0f / (float)(P->norm_hi - P->norm_lo);
+    }
+    em.nonzero_flag = d_flag;
+
+    const float *src = f;
+    if (P->guided_eps > 0) {                                    /* (c) */
+        fyg_guided_args g;
+        memset(&g, 0, sizeof g);
+        g.nz = (int)hz; g.ny = (int)hy; g.nx = (int)hx;
+        g.r = P->guided_radius < 1 ? 2 : P->guided_radius;
+        g.s = P->guided_s == 0 ? 2 : P->guided_s;
+        g.eps = P->guided_eps;
+        g.in = f;
+        g.ws = fyg_ws(s, WS_GUIDED, fyg_guided_ws_bytes(g.nz, g.ny, g.nx, g.s));
+        if (!g.ws) return 1;
+        if (!c->d_float) {
+            g.em = em;
+            return fyg_guided_run(&g, s);                       /* filter + emit in one pass */
+        }
+        g.em.quant = 0;
+        g.em.out = c->d_float;
+        if (fyg_guided_run(&g, s)) return 1;
+        src = c->d_float;
+    } else if (c->d_float) {
+        if (fyg_d2d(c->d_float, f, sizeof(float) * hn, s)) return 1;
+    }
+    return fyg_emit_run(src, (int)hz, (int)hy, (int)hx, &em, s);
+}
+
+int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream) {
+    fyg_stream *s = fyg_stream_wrap(stream);
+    if (!s) return 1;
+    int *dflag = (int *)fyg_ws(s, WS_FLAG, 64);
+    if (!dflag) return 1;
+    for (int i = 0; i < n; i++)
+        if (chain_cube(&cubes[i], dflag, s)) return 1;
+    return 0;
+}
+
+int fyh_chain(const fy_pipeline_cfg *cfg, const unsigned char *d_u8, long rz0, long ry0, long rx0, long hz,
+              long hy, long hx, long z0, long y0, long x0, long tz, long ty, long tx, long vol_y, long vol_x,
+              unsigned char *d_out, int *d_flag, fyg_stream *s) {
+    fy_cube_params P;
+    if (fy_cube_params_from_cfg(cfg, y0, x0, ty, tx, vol_y, vol_x, &P)) return 1;
+    fy_cube c = {d_u8, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, d_out, NULL, &P};
+    return chain_cube(&c, d_flag, s);
+}
+
+int fy_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf, long rz0, long ry0, long rx0,
+                      long hz, long hy, long hx, long z0, long y0, long x0, long tz, long ty, long tx,
+                      long vol_y, long vol_x, unsigned char *out, int *all_air) {
+    size_t hn = (size_t)hz * hy * hx, tn = (size_t)tz * ty * tx;
+    if (!hn || !tn) {
+        if (all_air) *all_air = 1;
+        return 0;
+    }
+    fyg_stream *s = fyg_thread_stream();
+    if (!s) return 1;
+    unsigned char *du = (unsigned char *)fyg_ws(s, WS_U8, hn);
+    unsigned char *dq = (unsigned char *)fyg_ws(s, WS_OUT, tn);
+    int *dflag = (int *)fyg_ws(s, WS_FLAG, 64);
+    if (!du || !dq || !dflag) return 1;
+    if (fyg_h2d(du, u8buf, hn, s)) return 1;
+    if (fyh_chain(cfg, du, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, dq, dflag, s)) return 1;
+    int flag = 0;
+    if (fyg_d2h(out, dq, tn, s) || fyg_d2h(&flag, dflag, sizeof(int), s) || fyg_sync(s)) return 1;
+    if (all_air) *all_air = !flag;
+    return 0;
+}
+
+int fy_gpu_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *d_u8, long rz0, long ry0, long rx0,
+                          long hz, long hy, long hx, long z0, long y0, long x0, long tz, long ty, long tx,
+                          long vol_y, long vol_x, unsigned char *d_out, int *all_air, void *stream) {
+    fyg_stream *s = fyg_stream_wrap(stream);
+    if (!s) return 1;
+    int *dflag = (int *)fyg_ws(s, WS_FLAG, 64);
+    if (!dflag) return 1;
+    if (fyh_chain(cfg, d_u8, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, d_out, dflag, s))
+        return 1;
+    if (all_air) {
+        int flag = 0;
+        if (fyg_d2h(&flag, dflag, sizeof(int), s) || fyg_sync(s)) return 1;
+        *all_air = !flag;
+    }
+    return 0;
+}

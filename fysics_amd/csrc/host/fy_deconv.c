@@ -1,0 +1,177 @@
+/* fy_deconv.c -- Wiener deconvolution entry points (paganin.c:152-342).
+ *
+ * The host side builds, once per (weight, physics, padded shape), the transfer
+ * tables deconv_radial uses: the squared-frequency axes (fftfreq convention,
+ * fz^2 scaled by (sigma_z/sigma_xy)^2) and the 4097-entry fp32 LUT of the radial
+ * weight over u = f^2, evaluated in double (paganin.c:212-230).  They stay
+ * resident on the device (fyg_table cache); the GPU passes in fyg_fft.hip do
+ * the rest.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "fyh.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* H/(H^2+reg) of the recon transfer (paganin.c:296-299) */
+static double wiener_weight(double fr, const fy_physics *p, double reg) {
+    double h = fy_recon_transfer(fr, p);
+    return h / (h * h + reg);
+}
+
+/* G*U/((G*U)^2+gamma), G the Gaussian system PSF (paganin.c:321-328) */
+static double matched_weight(double fr, const fy_physics *p, double gamma) {
+    double sg = p->psf_sigma_vox > 0 ? p->psf_sigma_vox : 1.0;
+    double g = exp(-2.0 * M_PI * M_PI * sg * sg * (fr * fr));
+    double hf = g * fy_unsharp_transfer(fr, p);
+    return hf / (hf * hf + gamma);
+}
+
+double fyh_matched_fz2_scale(const fy_physics *p) {
+    double sxy = p->psf_sigma_vox > 0 ? p->psf_sigma_vox : 1.0;
+    if (p->psf_sigma_z_vox > 0 && sxy > 0) {
+        double q = p->psf_sigma_z_vox / sxy;
+        return q * q;
+    }
+    return 1.0;
+}
+
+typedef struct {
+    char tag[4];
+    int kind, pz, py, px;
+    double phys[8], param, cap, fz2_scale;
+} table_key;
+
+int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, int kind,
+                   double cap, double fz2_scale, fyh_tables *t) {
+    int pz = fy_next_fft_size(nz), py = fy_next_fft_size(ny), px = fy_next_fft_size(nx);
+    int hx = px / 2 + 1;
+    if (fz2_scale <= 0) fz2_scale = 1.0;
+    t->pz = pz; t->py = py; t->px = px; t->hx = hx; t->hxp = fyg_deconv_hxp(px);
+    /* layout of the device block: lut[4097] (padded to 4100 floats), fz2, fy2, fx2 */
+    size_t lut_bytes = sizeof(float) * 4100;
+    size_t bytes = lut_bytes + sizeof(double) * (size_t)(pz + py + hx);
+    double *fz2 = (double *)malloc(sizeof(double) * (size_t)(pz + py + hx));
+    if (!fz2) return 1;
+    double *fy2 = fz2 + pz, *fx2 = fy2 + py;
+    double mz = 0, my = 0, mx = 0;
+    for (int i = 0; i < pz; i++) {
+        double f = (i <= pz / 2) ? (double)i / pz : (double)(i - pz) / pz;
+        fz2[i] = f * f * fz2_scale;
+        if (fz2[i] > mz) mz = fz2[i];
+    }
+    for (int i = 0; i < py; i++) {
+        double f = (i <= py / 2) ? (double)i / py : (double)(i - py) / py;
+        fy2[i] = f * f;
+        if (fy2[i] > my) my = fy2[i];
+    }
+    for (int i = 0; i < hx; i++) {
+        double f = (double)i / px;
+        fx2[i] = f * f;
+        if (fx2[i] > mx) mx = fx2[i];
+    }
+    double us = FYH_LUT_N / (mz + my + mx);
+    t->us = us;
+
+    table_key key;
+    memset(&key, 0, sizeof key);
+    memcpy(key.tag, "LUT", 4);
+    key.kind = kind; key.pz = pz; key.py = py; key.px = px;
+    memcpy(key.phys, p, sizeof(double) * 8);
+    key.param = param; key.cap = cap; key.fz2_scale = fz2_scale;
+    const unsigned char *dev = (const unsigned char *)fyg_table_find(&key, sizeof key);
+    if (!dev) {
+        unsigned char *blk = (unsigned char *)malloc(bytes);
+        if (!blk) { free(fz2); return 1; }
+        float *lut = (float *)blk;
+        for (int i = 0; i <= FYH_LUT_N; i++) {
+            double fr = sqrt((double)i / us);
+            double w = kind == 0 ? wiener_weight(fr, p, param) : matched_weight(fr, p, param);
+            if (cap > 0 && w > cap) w = cap;
+            lut[i] = (float)w;
+        }
+        lut[4097] = lut[4098] = lut[4099] = 0.0f;
+        memcpy(blk + lut_bytes, fz2, sizeof(double) * (size_t)(pz + py + hx));
+        dev = (const unsigned char *)fyg_table(&key, sizeof key, blk, bytes);
+        free(blk);
+    }
+    free(fz2);
+    if (!dev) return 1;
+    t->lut = (const float *)dev;
+    t->fz2 = (const double *)(dev + lut_bytes);
+    t->fy2 = t->fz2 + pz;
+    t->fx2 = t->fy2 + py;
+    return 0;
+}
+
+int fyh_deconv_dev(const float *d_in, float *d_out, int nz, int ny, int nx, const fy_physics *p,
+                   double param, int kind, double cap, double fz2_scale, fyg_stream *s) {
+    if (nz <= 0 || ny <= 0 || nx <= 0) return 0;
+    fyh_tables t;
+    if (fyh_tables_get(nz, ny, nx, p, param, kind, cap, fz2_scale, &t)) return 1;
+    if (!fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px)) return 1;
+    void *spec = fyg_ws(s, WS_SPEC, fyg_deconv_spec_bytes(t.pz, t.py, t.px));
+    if (!spec) return 1;
+    fyg_deconv_args a;
+    memset(&a, 0, sizeof a);
+    a.nz = nz; a.ny = ny; a.nx = nx;
+    a.pz = t.pz; a.py = t.py; a.px = t.px; a.hx = t.hx; a.hxp = t.hxp;
+    a.lut = t.lut; a.us = t.us; a.fz2 = t.fz2; a.fy2 = t.fy2; a.fx2 = t.fx2;
+    a.in = d_in; a.in_u8 = 0;
+    a.out = d_out;
+    a.spec = spec;
+    return fyg_deconv_run(&a, s);
+}
+
+static int deconv_host(const float *in, float *out, int nz, int ny, int nx, const fy_physics *p,
+                       double param, int kind, double cap, double fz2_scale) {
+    if (nz <= 0 || ny <= 0 || nx <= 0) return 0;
+    fyg_stream *s = fyg_thread_stream();
+    if (!s) return 1;
+    size_t n = (size_t)nz * ny * nx;
+    float *d = (float *)fyg_ws(s, WS_IN, sizeof(float) * n);
+    if (!d) return 1;
+    if (fyg_h2d(d, in, sizeof(float) * n, s)) return 1;
+    if (fyh_deconv_dev(d, d, nz, ny, nx, p, param, kind, cap, fz2_scale, s)) return 1;
+    if (fyg_d2h(out, d, sizeof(float) * n, s)) return 1;
+    return fyg_sync(s);
+}
+
+int fy_deconvolve(const float *in, float *out, int nz, int ny, int nx, const fy_physics *p, double reg) {
+    if (reg <= 0.0) reg = fy_auto_reg(p, 0.015);
+    return deconv_host(in, out, nz, ny, nx, p, reg, 0, FYH_MAX_GAIN, 1.0);
+}
+
+int fy_deconvolve_matched(const float *in, float *out, int nz, int ny, int nx, const fy_physics *p,
+                          double tikhonov) {
+    if (tikhonov <= 0) tikhonov = 0.05;
+    return deconv_host(in, out, nz, ny, nx, p, tikhonov, 1, 0.0, fyh_matched_fz2_scale(p));
+}
+
+int fy_gpu_deconvolve(const float *d_in, float *d_out, int nz, int ny, int nx, const fy_physics *p,
+                      double reg, void *stream) {
+    fyg_stream *s = fyg_stream_wrap(stream);
+    if (!s) return 1;
+    if (reg <= 0.0) reg = fy_auto_reg(p, 0.015);
+    return fyh_deconv_dev(d_in, d_out, nz, ny, nx, p, reg, 0, FYH_MAX_GAIN, 1.0, s);
+}
+
+int fy_gpu_deconvolve_matched(const float *d_in, float *d_out, int nz, int ny, int nx, const fy_physics *p,
+                              double tikhonov, void *stream) {
+    fyg_stream *s = fyg_stream_wrap(stream);
+    if (!s) return 1;
+    if (tikhonov <= 0) tikhonov = 0.05;
+    return fyh_deconv_dev(d_in, d_out, nz, ny, nx, p, tikhonov, 1, 0.0, fyh_matched_fz2_scale(p), s);
+}
+
+int fy_gpu_deconv_radial(const float *d_in, float *d_out, int nz, int ny, int nx, const fy_physics *p,
+                         double param, int kind, double cap, double fz2_scale, void *stream) {
+    fyg_stream *s = fyg_stream_wrap(stream);
+    if (!s) return 1;
+    return fyh_deconv_dev(d_in, d_out, nz, ny, nx, p, param, kind, cap, fz2_scale, s);
+}

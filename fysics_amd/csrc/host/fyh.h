@@ -1,0 +1,45 @@
+/* fyh.h -- internal helpers shared by the C host files of libfysics_amd. */
+#ifndef FYH_H
+#define FYH_H
+
+#include <stddef.h>
+
+#include "../../../include/fysics.h"
+#include "../../../include/fysics_gpu.h"
+#include "../fyg.h"
+
+#define FYH_LUT_N 4096          /* 4097-entry weight table (paganin.c:138) */
+#define FYH_MAX_GAIN 8.0        /* Wiener gain cap (paganin.c:50) */
+
+/* workspace slots of a stream */
+enum { WS_IN = 0, WS_SPEC = 1, WS_TILE = 2, WS_GUIDED = 3, WS_U8 = 4, WS_FLAG = 5, WS_OUT = 6, WS_MISC = 7 };
+
+/* device-resident transfer tables of one (weight, physics, padded shape) */
+typedef struct {
+    int pz, py, px, hx, hxp;
+    const float *lut;
+    double us;
+    const double *fz2, *fy2, *fx2;
+} fyh_tables;
+
+/* kind 0: Wiener weight of the recon transfer (cap 8); kind 1: matched PSF*unsharp */
+int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, int kind,
+                   double cap, double fz2_scale, fyh_tables *t);
+double fyh_matched_fz2_scale(const fy_physics *p);
+/* the physics fy_process_buffer deconvolves with (pipeline.c:89-104) */
+void fyh_scaled_physics(const fy_pipeline_cfg *c, fy_physics *p);
+
+/* device copy of a small host array, cached by content */
+const void *fyh_const_blob(char tag, const void *data, size_t bytes);
+
+/* run the whole tile chain of fy_process_buffer on device buffers */
+int fyh_chain(const fy_pipeline_cfg *cfg, const unsigned char *d_u8,
+              long rz0, long ry0, long rx0, long hz, long hy, long hx,
+              long z0, long y0, long x0, long tz, long ty, long tx,
+              long vol_y, long vol_x, unsigned char *d_out, int *d_flag, fyg_stream *s);
+
+/* deconvolution on device buffers (fp32 in/out) */
+int fyh_deconv_dev(const float *d_in, float *d_out, int nz, int ny, int nx, const fy_physics *p,
+                   double param, int kind, double cap, double fz2_scale, fyg_stream *s);
+
+#endif

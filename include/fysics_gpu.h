@@ -1,0 +1,102 @@
+/* fysics_gpu.h -- entry points that the CPU reference has no counterpart for:
+ * device-resident forms of the hot path (inputs and outputs already in HBM,
+ * work queued on a caller-supplied HIP stream) and the kernel-timing hooks.
+ *
+ * Pointers named d_* are device addresses (hipMalloc or a torch tensor's
+ * data_ptr()).  `stream` is a hipStream_t; NULL selects the calling thread's
+ * own stream.  Calls only enqueue work: results are ready after
+ * fy_gpu_sync(stream) (or any later stream-ordered consumer).  Return 0 on
+ * success, 1 on a device error or an unsupported size, like fysics.h.
+ */
+#ifndef FYSICS_GPU_H
+#define FYSICS_GPU_H
+
+#include <stddef.h>
+#include "fysics.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* fy_deconvolve / fy_deconvolve_matched on device buffers (paganin.c:301-342).
+ * d_out may equal d_in. */
+int fy_gpu_deconvolve(const float *d_in, float *d_out, int nz, int ny, int nx,
+                      const fy_physics *p, double reg, void *stream);
+int fy_gpu_deconvolve_matched(const float *d_in, float *d_out, int nz, int ny, int nx,
+                              const fy_physics *p, double tikhonov, void *stream);
+/* the deconvolution core with an explicit weight (paganin.c:152-293):
+ * kind 0 = Wiener of the recon transfer, kind 1 = matched PSF*unsharp;
+ * cap <= 0 disables the gain clamp; fz2_scale = (sigma_z/sigma_xy)^2 */
+int fy_gpu_deconv_radial(const float *d_in, float *d_out, int nz, int ny, int nx,
+                         const fy_physics *p, double param, int kind, double cap,
+                         double fz2_scale, void *stream);
+
+/* guided filter on device buffers; s <= 1 exact (guided.c:113), else fast (guided.c:152) */
+int fy_gpu_guided(const float *d_in, float *d_out, int nz, int ny, int nx,
+                  int radius, double eps, int s, void *stream);
+
+/* fy_process_buffer on device buffers (pipeline.c:467-510): d_u8 is the halo'd
+ * hz*hy*hx tile, d_out receives tz*ty*tx u8.  all_air, if non-NULL, makes the
+ * call synchronous and reports whether the tile was all zero. */
+int fy_gpu_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *d_u8,
+                          long rz0, long ry0, long rx0, long hz, long hy, long hx,
+                          long z0, long y0, long x0, long tz, long ty, long tx,
+                          long vol_y, long vol_x, unsigned char *d_out, int *all_air,
+                          void *stream);
+
+/* ---- per-cube chain (the batched and served forms' unit of work) ----
+ * One cube = a halo'd u8 region and the inner tile to emit, processed in
+ * fy_process_buffer order: u8/255 -> ring subtraction -> z-drift -> dewindow ->
+ * deconvolution -> rewindow / dec-range / clamp -> guided filter -> recenter ->
+ * dithered quantisation (pipeline.c:252-297,413-421,467-510).  Unlike
+ * fy_pipeline_cfg, the deconvolution weight, its regularisation and the z
+ * frequency scale are explicit, so the Paganin+unsharp Wiener weight can be
+ * combined with an anisotropic sigma_z (deconv_radial, paganin.c:152, with
+ * wiener_weight and fz2_scale = (sigma_z/sigma_xy)^2).  The physics is used as
+ * given (no delta/beta back-off is applied here). */
+typedef struct {
+    fy_physics phys;
+    int    deconv;             /* 0 off, 1 Wiener of the recon transfer, 2 matched PSF*unsharp */
+    double deconv_param;       /* reg (<=0 -> 0.015) or tikhonov (<=0 -> 0.05) */
+    double deconv_cap;         /* gain clamp; <=0 none (fy_deconvolve uses 8) */
+    double fz2_scale;          /* (sigma_z/sigma_xy)^2 applied to f_z^2; <=0 -> 1 */
+    double window_lo, window_hi;          /* dewindow/rewindow around the deconvolution */
+    int    have_dec_range; double dec_lo, dec_hi;
+    double guided_eps;         /* <=0: no guided filter */
+    int    guided_radius;      /* <1 -> 2 */
+    int    guided_s;           /* 0 -> 2 (fast), 1 exact, else decimation */
+    int    no_dither;
+    int    recenter; int norm_lo, norm_hi;   /* pipeline.c:413-421 */
+    const fy_dering *dering;   /* finalized ring state, or NULL */
+    const float *zdrift_factor; long vol_z;  /* per-slice factors, or NULL */
+} fy_cube_params;
+
+typedef struct {
+    const unsigned char *d_in;            /* hz*hy*hx u8, device */
+    long rz0, ry0, rx0, hz, hy, hx;       /* global origin and shape of the region */
+    long z0, y0, x0, tz, ty, tx;          /* inner tile (global), inside the region */
+    unsigned char *d_out;                 /* tz*ty*tx u8, device */
+    float *d_float;                       /* optional: hz*hy*hx fp32 tile before quantisation */
+    const fy_cube_params *params;
+} fy_cube;
+
+/* process n cubes on one stream */
+int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream);
+/* the cube parameters fy_process_buffer derives from a pipeline cfg
+ * (delta/beta back-off, matched vs plain weight, per-tile eps) */
+int fy_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long ty, long tx,
+                            long vol_y, long vol_x, fy_cube_params *out);
+
+int fy_gpu_sync(void *stream);
+
+/* per-kernel timing: HIP events recorded around every launch on its own stream */
+void fy_gpu_profile(int on);
+/* names: '\0'-separated kernel names; ms/count per name; returns #entries */
+int  fy_gpu_profile_read(char *names, int names_len, double *ms, long *count, int max);
+void fy_gpu_profile_reset(void);
+const char *fy_gpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

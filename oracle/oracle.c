@@ -1,0 +1,651 @@
+/* oracle.c -- CPU restatement of the fysics hot path (TEST INFRASTRUCTURE ONLY).
+ * See oracle.h for what this is and what pins it.  Builds liboracle.so
+ * (oracle/Makefile); symbols are prefixed orc_ so it never collides with the
+ * reference or the product.
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+#define ORC_LUT_N 4096          /* FY_DECONV_LUT_N, paganin.c:138 */
+#define ORC_MAX_GAIN 8.0        /* FY_MAX_DECONV_GAIN, paganin.c:50 */
+#define ORC_HC_KEV_UM 1.23984199e-3
+
+/* ------------------------------------------------------------------ sizes (fft.c:25-38) */
+static int orc_is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+static int orc_next_pow2(int n) { int p = 1; while (p < n) p *= 2; return p; }
+int orc_next_fft_size(int n) {
+    int p2 = orc_next_pow2(n);
+    int p3 = 3 * orc_next_pow2((n + 2) / 3);
+    return (p3 >= n && p3 < p2) ? p3 : p2;
+}
+
+/* reflect-pad index (paganin.c:121-128) */
+static int orc_reflect(int i, int n) {
+    if (n == 1) return 0;
+    while (i < 0 || i >= n) {
+        if (i < 0) i = -i - 1;
+        if (i >= n) i = 2 * n - i - 1;
+    }
+    return i;
+}
+
+/* ------------------------------------------------------------------ FFT, both precisions */
+#define REAL float
+#define SFX(x) x##_f32
+#include "orc_fft_impl.h"
+#include "orc_deconv_impl.h"
+#include "orc_guided_impl.h"
+#undef REAL
+#undef SFX
+#define REAL double
+#define SFX(x) x##_f64
+#include "orc_fft_impl.h"
+#include "orc_deconv_impl.h"
+#include "orc_guided_impl.h"
+#undef REAL
+#undef SFX
+
+void orc_fft3d_normalize_f32(float *re, float *im, int nz, int ny, int nx) {
+    size_t n = (size_t)nz * ny * nx;
+    float s = 1.0f / (float)n;
+    for (size_t i = 0; i < n; i++) { re[i] *= s; im[i] *= s; }
+}
+
+/* ------------------------------------------------------------------ transfers (paganin.c:17-91) */
+double orc_paganin_transfer(double f, const fy_physics *p) {
+    double lam = ORC_HC_KEV_UM / p->energy_kev;
+    double dist = p->distance_mm * 1000.0;
+    double px = p->pixel_um;
+    if (px > 0.0 && px < 0.1) px *= 1000.0;
+    double fu = f / px;
+    return 1.0 / (1.0 + p->delta_beta * lam * dist * M_PI * (fu * fu));
+}
+double orc_unsharp_transfer(double f, const fy_physics *p) {
+    if (p->unsharp_coeff == 0.0 || p->unsharp_sigma == 0.0) return 1.0;
+    double sg = p->unsharp_sigma;
+    double g = exp(-2.0 * M_PI * M_PI * sg * sg * f * f);
+    return 1.0 + p->unsharp_coeff * (1.0 - g);
+}
+double orc_recon_transfer(double f, const fy_physics *p) {
+    double h = orc_paganin_transfer(f, p) * orc_unsharp_transfer(f, p);
+    return h < 1e-4 ? 1e-4 : h;
+}
+double orc_auto_reg(const fy_physics *p, double base) { (void)p; return base > 0.0 ? base : 0.015; }
+double orc_auto_deltabeta_scale(const fy_physics *p) {
+    double h = orc_paganin_transfer(0.5, p);
+    if (!isfinite(h) || h <= 0) return 1.0;
+    if (h <= 0.0025) return 0.35;
+    if (h >= 0.0070) return 1.0;
+    return 0.35 + (h - 0.0025) / (0.0070 - 0.0025) * (1.0 - 0.35);
+}
+/* impulse-response extent of H/(H^2+0.05) along one axis (paganin.c:93-118) */
+int orc_kernel_halo(const fy_physics *p) {
+    const int n = 256;
+    float re[256], im[256];
+    for (int i = 0; i < n; i++) {
+        double f = (i <= n / 2) ? (double)i / n : (double)(i - n) / n;
+        double h = orc_recon_transfer(fabs(f), p);
+        re[i] = (float)(h / (h * h + 0.05)); im[i] = 0.0f;
+    }
+    orc_fft1d_f32(re, im, n, +1);
+    float peak = 0.0f;
+    for (int i = 0; i < n; i++) if (fabsf(re[i]) > peak) peak = fabsf(re[i]);
+    int half = 8;
+    for (int d = 1; d < n / 2; d++) if (fabsf(re[d]) > 0.01f * peak) half = d;
+    return half < 8 ? 8 : (half > 96 ? 96 : half);
+}
+
+/* ------------------------------------------------------------------ deconv LUT (paganin.c:212-230,296-328) */
+static double orc_weight(double fr, const fy_physics *p, double param, int kind) {
+    if (kind == 0) {                                   /* wiener_weight */
+        double h = orc_recon_transfer(fr, p);
+        return h / (h * h + param);
+    }
+    double sg = p->psf_sigma_vox > 0 ? p->psf_sigma_vox : 1.0;   /* matched_transfer */
+    double g = exp(-2.0 * M_PI * M_PI * sg * sg * (fr * fr));
+    double hf = g * orc_unsharp_transfer(fr, p);
+    return hf / (hf * hf + param);
+}
+static void orc_freq_tables(int pz, int py, int px, double fz2_scale,
+                            double *fz2, double *fy2, double *fx2, double *umax) {
+    int hx = px / 2 + 1;
+    double mz = 0, my = 0, mx = 0;
+    if (fz2_scale <= 0) fz2_scale = 1.0;
+    for (int i = 0; i < pz; i++) {
+        double f = (i <= pz / 2) ? (double)i / pz : (double)(i - pz) / pz;
+        fz2[i] = f * f * fz2_scale; if (fz2[i] > mz) mz = fz2[i];
+    }
+    for (int i = 0; i < py; i++) {
+        double f = (i <= py / 2) ? (double)i / py : (double)(i - py) / py;
+        fy2[i] = f * f; if (fy2[i] > my) my = fy2[i];
+    }
+    for (int i = 0; i < hx; i++) { double f = (double)i / px; fx2[i] = f * f; if (fx2[i] > mx) mx = fx2[i]; }
+    *umax = mz + my + mx;
+}
+static void orc_lut_d(int pz, int py, int px, const fy_physics *p, double param, int kind,
+                      double cap, double umax, double *lutd, double *us) {
+    *us = ORC_LUT_N / umax;
+    for (int i = 0; i <= ORC_LUT_N; i++) {
+        double w = orc_weight(sqrt((double)i / *us), p, param, kind);
+        if (cap > 0 && w > cap) w = cap;
+        lutd[i] = w;
+    }
+    (void)pz; (void)py; (void)px;
+}
+void orc_deconv_lut(int pz, int py, int px, const fy_physics *p, double param, int kind,
+                    double cap, double fz2_scale, float *lut, double *us) {
+    double *fz2 = malloc(sizeof(double) * pz), *fy2 = malloc(sizeof(double) * py);
+    double *fx2 = malloc(sizeof(double) * (px / 2 + 1)), umax, lutd[ORC_LUT_N + 1];
+    orc_freq_tables(pz, py, px, fz2_scale, fz2, fy2, fx2, &umax);
+    orc_lut_d(pz, py, px, p, param, kind, cap, umax, lutd, us);
+    for (int i = 0; i <= ORC_LUT_N; i++) lut[i] = (float)lutd[i];
+    free(fz2); free(fy2); free(fx2);
+}
+
+static int orc_deconv_radial_any(const float *in, void *out, int f64, int nz, int ny, int nx,
+                                 const fy_physics *p, double param, int kind, double cap,
+                                 double fz2_scale) {
+    int pz = orc_next_fft_size(nz), py = orc_next_fft_size(ny), px = orc_next_fft_size(nx);
+    double *fz2 = malloc(sizeof(double) * pz), *fy2 = malloc(sizeof(double) * py);
+    double *fx2 = malloc(sizeof(double) * (px / 2 + 1)), umax, us, lutd[ORC_LUT_N + 1];
+    if (!fz2 || !fy2 || !fx2) { free(fz2); free(fy2); free(fx2); return 1; }
+    orc_freq_tables(pz, py, px, fz2_scale, fz2, fy2, fx2, &umax);
+    orc_lut_d(pz, py, px, p, param, kind, cap, umax, lutd, &us);
+    int rc = f64 ? orc_deconv_core_f64(in, (double *)out, nz, ny, nx, lutd, us, fz2, fy2, fx2)
+                 : orc_deconv_core_f32(in, (float *)out, nz, ny, nx, lutd, us, fz2, fy2, fx2);
+    free(fz2); free(fy2); free(fx2);
+    return rc;
+}
+int orc_deconv_radial_f32(const float *in, float *out, int nz, int ny, int nx, const fy_physics *p,
+                          double param, int kind, double cap, double fz2_scale) {
+    size_t n = (size_t)nz * ny * nx;
+    float *tmp = malloc(sizeof(float) * n);      /* out may alias in */
+    if (!tmp) return 1;
+    int rc = orc_deconv_radial_any(in, tmp, 0, nz, ny, nx, p, param, kind, cap, fz2_scale);
+    if (!rc) memcpy(out, tmp, sizeof(float) * n);
+    free(tmp);
+    return rc;
+}
+int orc_deconv_radial_f64(const float *in, double *out, int nz, int ny, int nx, const fy_physics *p,
+                          double param, int kind, double cap, double fz2_scale) {
+    return orc_deconv_radial_any(in, out, 1, nz, ny, nx, p, param, kind, cap, fz2_scale);
+}
+/* fy_deconvolve (paganin.c:301-308) */
+int orc_deconvolve(const float *in, float *out, int nz, int ny, int nx, const fy_physics *p, double reg) {
+    if (reg <= 0.0) reg = orc_auto_reg(p, 0.015);
+    return orc_deconv_radial_f32(in, out, nz, ny, nx, p, reg, 0, ORC_MAX_GAIN, 1.0);
+}
+/* fy_deconvolve_matched (paganin.c:330-342) */
+double orc_matched_fz2_scale(const fy_physics *p) {
+    double sxy = p->psf_sigma_vox > 0 ? p->psf_sigma_vox : 1.0;
+    if (p->psf_sigma_z_vox > 0 && sxy > 0) { double q = p->psf_sigma_z_vox / sxy; return q * q; }
+    return 1.0;
+}
+int orc_deconvolve_matched(const float *in, float *out, int nz, int ny, int nx, const fy_physics *p,
+                           double tikhonov) {
+    if (tikhonov <= 0) tikhonov = 0.05;
+    return orc_deconv_radial_f32(in, out, nz, ny, nx, p, tikhonov, 1, 0.0, orc_matched_fz2_scale(p));
+}
+
+/* ------------------------------------------------------------------ conversions (stream.c:27-63) */
+void orc_u8_to_float(const unsigned char *in, float *out, size_t n) {
+    const float k = 1.0f / 255.0f;
+    for (size_t i = 0; i < n; i++) out[i] = in[i] * k;
+}
+static unsigned char orc_clamp_u8(float v) { return v < 0 ? 0 : (v > 255 ? 255 : (unsigned char)v); }
+void orc_float_to_u8(const float *in, unsigned char *out, size_t n) {
+    for (size_t i = 0; i < n; i++) out[i] = orc_clamp_u8(in[i] * 255.0f + 0.5f);
+}
+void orc_u8_to_phys(const unsigned char *in, float *out, size_t n, double lo, double hi) {
+    float l = (float)lo, sc = (float)((hi - lo) / 255.0);
+    for (size_t i = 0; i < n; i++) out[i] = l + in[i] * sc;
+}
+void orc_phys_to_u8(const float *in, unsigned char *out, size_t n, double lo, double hi) {
+    double span = hi - lo;
+    float k = span != 0.0 ? (float)(255.0 / span) : 0.0f, l = (float)lo;
+    for (size_t i = 0; i < n; i++) out[i] = orc_clamp_u8((in[i] - l) * k + 0.5f);
+}
+float orc_phys_to_u8_level(double phys, double lo, double hi) {
+    double span = hi - lo;
+    return span == 0.0 ? 0.0f : (float)((phys - lo) / span * 255.0);
+}
+
+/* ------------------------------------------------------------------ quantiser (pipeline.c:53-65) */
+float orc_dither01(long z, long y, long x) {
+    unsigned int h = (unsigned int)(z * 73856093L) ^ (unsigned int)(y * 19349663L) ^ (unsigned int)(x * 83492791L);
+    h ^= h >> 16; h *= 0x7feb352dU;
+    h ^= h >> 15; h *= 0x846ca68bU;
+    h ^= h >> 16;
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+unsigned char orc_quant_u8(float v, long gz, long gy, long gx, int no_dither) {
+    float t = v * 255.0f + (no_dither ? 0.5f : orc_dither01(gz, gy, gx));
+    if (t <= 0.0f) return 0;
+    int q = (int)t;
+    return q > 255 ? 255 : (unsigned char)q;
+}
+
+/* ------------------------------------------------------------------ dering (dering.c) */
+int orc_dering_init(fy_dering *d, long Z, long Y, long X, double cy, double cx, int slab_z, int ns) {
+    memset(d, 0, sizeof(*d));
+    d->Z = Z; d->Y = Y; d->X = X;
+    d->cy = cy >= 0 ? cy : (Y - 1) / 2.0;
+    d->cx = cx >= 0 ? cx : (X - 1) / 2.0;
+    d->slab_z = slab_z > 0 ? slab_z : 512;
+    d->nslab = (int)((Z + d->slab_z - 1) / d->slab_z);
+    d->ns = ns > 0 ? ns : 8;
+    double ry = d->cy > Y - 1 - d->cy ? d->cy : Y - 1 - d->cy;
+    double rx = d->cx > X - 1 - d->cx ? d->cx : X - 1 - d->cx;
+    d->nr = (int)sqrt(ry * ry + rx * rx) + 2;
+    size_t nb = (size_t)d->nslab * d->ns * d->nr;
+    d->sum = calloc(nb, sizeof(float));
+    d->cnt = calloc(nb, sizeof(unsigned int));
+    d->ring = calloc((size_t)d->nslab * d->nr, sizeof(float));
+    if (!d->sum || !d->cnt || !d->ring) { orc_dering_free(d); return 1; }
+    return 0;
+}
+void orc_dering_free(fy_dering *d) {
+    free(d->sum); free(d->cnt); free(d->ring);
+    d->sum = NULL; d->cnt = NULL; d->ring = NULL;
+}
+int orc_dering_tile_init(fy_dering *t, const fy_dering *d) {
+    *t = *d;
+    t->nslab = 1;
+    t->sum = calloc((size_t)t->ns * t->nr, sizeof(float));
+    t->cnt = calloc((size_t)t->ns * t->nr, sizeof(unsigned int));
+    t->ring = NULL;
+    if (!t->sum || !t->cnt) { free(t->sum); free(t->cnt); return 1; }
+    return 0;
+}
+/* bin of (y,x): sector*nr + radius, -1 when the radius is out of range (dering.c:89-99) */
+static int orc_ring_bin(const fy_dering *t, long gy_i, long gx_i) {
+    double gy = (double)gy_i - t->cy, gx = (double)gx_i - t->cx;
+    int ri = (int)(sqrt(gy * gy + gx * gx) + 0.5);
+    if (ri >= t->nr) return -1;
+    int q = (int)((atan2(gy, gx) + M_PI) * (t->ns / (2.0 * M_PI)));
+    if (q >= t->ns) q = t->ns - 1;
+    return q * t->nr + ri;
+}
+void orc_dering_accumulate_u8(fy_dering *t, const unsigned char *buf,
+                              long y0, long x0, long dz, long dy, long dx, int ss) {
+    if (ss < 1) ss = 1;
+    for (long z = 0; z < dz; z++)
+        for (long y = 0; y < dy; y += ss)
+            for (long x = 0; x < dx; x += ss) {
+                unsigned char v = buf[((size_t)z * dy + y) * dx + x];
+                if (!v) continue;
+                int b = orc_ring_bin(t, y0 + y, x0 + x);
+                if (b < 0) continue;
+                t->sum[b] += v;
+                t->cnt[b]++;
+            }
+}
+void orc_dering_merge_tile(fy_dering *d, int slab, const fy_dering *t) {
+    if (slab < 0 || slab >= d->nslab) return;
+    size_t nb = (size_t)d->ns * d->nr, off = (size_t)slab * nb;
+    for (size_t i = 0; i < nb; i++) { d->sum[off + i] += t->sum[i]; d->cnt[off + i] += t->cnt[i]; }
+}
+static int orc_cmpf(const void *a, const void *b) {
+    float x = *(const float *)a, y = *(const float *)b;
+    return (x > y) - (x < y);
+}
+/* per slab: sector mean profiles, high-pass against the valid-bin box mean,
+ * all-sectors-agree sign vote, median, floor/clamp (dering.c:136-194) */
+long orc_dering_finalize(fy_dering *d, int hp_win, unsigned int min_cnt, double min_amp, double max_amp) {
+    if (hp_win < 3) hp_win = 15;
+    if (min_amp <= 0) min_amp = 0.5;
+    if (max_amp <= 0) max_amp = 6.0;
+    int ns = d->ns, nr = d->nr, hw = hp_win / 2;
+    float *prof = malloc(sizeof(float) * (size_t)ns * nr), *hp = malloc(sizeof(float) * (size_t)ns * nr);
+    unsigned char *ok = malloc((size_t)ns * nr);
+    float vals[64];
+    long found = 0;
+    for (int s = 0; s < d->nslab; s++) {
+        size_t so = (size_t)s * ns * nr;
+        for (int q = 0; q < ns; q++) {
+            float *pq = prof + (size_t)q * nr, *hq = hp + (size_t)q * nr;
+            unsigned char *vq = ok + (size_t)q * nr;
+            for (int r = 0; r < nr; r++) {
+                unsigned int c = d->cnt[so + (size_t)q * nr + r];
+                vq[r] = c >= min_cnt;
+                pq[r] = vq[r] ? d->sum[so + (size_t)q * nr + r] / c : 0.0f;
+            }
+            for (int r = 0; r < nr; r++) {
+                if (!vq[r]) { hq[r] = 0.0f; continue; }
+                double acc = 0; int c = 0;
+                int lo = r - hw < 0 ? 0 : r - hw, hi = r + hw >= nr ? nr - 1 : r + hw;
+                for (int j = lo; j <= hi; j++) if (vq[j]) { acc += pq[j]; c++; }
+                if (c >= hw) hq[r] = pq[r] - (float)(acc / c);
+                else { hq[r] = 0.0f; vq[r] = 0; }
+            }
+        }
+        float *ring = d->ring + (size_t)s * nr;
+        for (int r = 0; r < nr; r++) {
+            int nv = 0, pos = 0, neg = 0;
+            for (int q = 0; q < ns; q++) {
+                if (!ok[(size_t)q * nr + r]) continue;
+                float v = hp[(size_t)q * nr + r];
+                vals[nv++] = v;
+                pos += v > 0; neg += v < 0;
+            }
+            if (nv < ns - 2 || (pos && neg)) { ring[r] = 0; continue; }
+            qsort(vals, nv, sizeof(float), orc_cmpf);
+            float m = (nv & 1) ? vals[nv / 2] : 0.5f * (vals[nv / 2 - 1] + vals[nv / 2]);
+            if (fabsf(m) < (float)min_amp) { ring[r] = 0; continue; }
+            if (m > (float)max_amp) m = (float)max_amp;
+            if (m < (float)-max_amp) m = (float)-max_amp;
+            ring[r] = m;
+            found++;
+        }
+    }
+    free(prof); free(hp); free(ok);
+    d->have_rings = found >= 2;
+    return found;
+}
+void orc_dering_apply(const fy_dering *d, float *f, long z0, long y0, long x0,
+                      long dz, long dy, long dx, double scale) {
+    if (!d->have_rings) return;
+    float sc = (float)scale;
+    for (long z = 0; z < dz; z++) {
+        int slab = (int)((z0 + z) / d->slab_z);
+        if (slab < 0) slab = 0;
+        if (slab >= d->nslab) slab = d->nslab - 1;
+        const float *ring = d->ring + (size_t)slab * d->nr;
+        for (long y = 0; y < dy; y++) {
+            double gy = (double)(y0 + y) - d->cy;
+            for (long x = 0; x < dx; x++) {
+                float *v = f + ((size_t)z * dy + y) * dx + x;
+                double gx = (double)(x0 + x) - d->cx;
+                int ri = (int)(sqrt(gy * gy + gx * gx) + 0.5);
+                if (ri >= d->nr || *v <= 0.0f) continue;
+                float w = *v - sc * ring[ri];
+                *v = w > 0.0f ? w : 0.0f;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ noise (noise.c) */
+static void orc_box_sum_d(const float *in, double *out, double *tmp, int nz, int ny, int nx, int r) {
+    size_t plane = (size_t)ny * nx, n = plane * nz;
+    for (size_t row = 0; row < (size_t)nz * ny; row++) {      /* X (noise.c:24-34) */
+        const float *ir = in + row * nx; double *orow = out + row * nx, s = 0;
+        for (int x = 0; x <= r && x < nx; x++) s += ir[x];
+        for (int x = 0; x < nx; x++) {
+            orow[x] = s;
+            if (x + r + 1 < nx) s += ir[x + r + 1];
+            if (x - r >= 0) s -= ir[x - r];
+        }
+    }
+    for (int a = 1; a < 3; a++) {                              /* Y then Z (noise.c:36-58) */
+        memcpy(tmp, out, sizeof(double) * n);
+        int len = a == 1 ? ny : nz;
+        size_t st = a == 1 ? (size_t)nx : plane;
+        for (size_t line = 0; line < n / (size_t)len; line++) {
+            size_t b = a == 1 ? (line / nx) * plane + (line % nx) : line;
+            double s = 0;
+            for (int i = 0; i <= r && i < len; i++) s += tmp[b + (size_t)i * st];
+            for (int i = 0; i < len; i++) {
+                out[b + (size_t)i * st] = s;
+                if (i + r + 1 < len) s += tmp[b + (size_t)(i + r + 1) * st];
+                if (i - r >= 0) s -= tmp[b + (size_t)(i - r) * st];
+            }
+        }
+    }
+}
+static int orc_cmpd(const void *a, const void *b) {
+    double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+static double orc_pct(double *v, int n, double pct) {      /* noise.c:66-73 (sorts v) */
+    if (n <= 0) return 0.0;
+    qsort(v, n, sizeof(double), orc_cmpd);
+    double idx = pct / 100.0 * (n - 1);
+    int lo = (int)idx; double fr = idx - lo;
+    return lo + 1 < n ? v[lo] * (1 - fr) + v[lo + 1] * fr : v[lo];
+}
+int orc_estimate_noise(const float *in, int nz, int ny, int nx, int win, double flat_pct,
+                       double ref_intensity, fy_noise_model *out) {
+    if (!out) return 1;
+    if (win < 3) win = 5;
+    if (flat_pct <= 0) flat_pct = 10.0;
+    int r = win / 2;
+    size_t n = (size_t)nz * ny * nx;
+    double *sm = malloc(sizeof(double) * n), *sq = malloc(sizeof(double) * n), *tmp = malloc(sizeof(double) * n);
+    float *i2 = malloc(sizeof(float) * n);
+    double *cp = malloc(sizeof(double) * n);
+    if (!sm || !sq || !tmp || !i2 || !cp) { free(sm); free(sq); free(tmp); free(i2); free(cp); return 1; }
+    for (size_t i = 0; i < n; i++) i2[i] = in[i] * in[i];
+    orc_box_sum_d(in, sm, tmp, nz, ny, nx, r);
+    orc_box_sum_d(i2, sq, tmp, nz, ny, nx, r);
+    double wc = (double)win * win * win;
+    for (size_t i = 0; i < n; i++) {          /* tmp := local mean, sq := local var */
+        double m = sm[i] / wc, v = sq[i] / wc - m * m;
+        tmp[i] = m; sq[i] = v > 0 ? v : 0;
+    }
+    memcpy(cp, tmp, sizeof(double) * n);
+    double lo = orc_pct(cp, (int)n, 1.0);
+    double hi = orc_pct(cp, (int)n, 99.0);    /* cp is sorted; re-sorting is a no-op */
+    if (hi <= lo) hi = lo + 1e-6;
+    enum { NB = 24 };
+    double bi[NB], bf[NB]; long bc[NB];
+    int used = 0;
+    for (int k = 0; k < NB; k++) {            /* per-bin variance floor (noise.c:117-134) */
+        double b0 = lo + (hi - lo) * k / NB, b1 = lo + (hi - lo) * (k + 1) / NB;
+        long c = 0;
+        for (size_t i = 0; i < n; i++) if (tmp[i] >= b0 && tmp[i] < b1) cp[c++] = sq[i];
+        if (c < 200) continue;
+        bf[used] = orc_pct(cp, (int)c, flat_pct);
+        bi[used] = 0.5 * (b0 + b1);
+        bc[used] = c;
+        used++;
+    }
+    out->n_bins_used = used;
+    double g = 0, b = 0;
+    if (used >= 3) {                           /* IRLS (noise.c:139-161) */
+        double w[NB];
+        for (int i = 0; i < used; i++) w[i] = sqrt((double)bc[i]);
+        for (int it = 0; it < 4; it++) {
+            double s0 = 0, s1 = 0, sy = 0, s2 = 0, sxy = 0;
+            for (int i = 0; i < used; i++) {
+                s0 += w[i]; s1 += w[i] * bi[i]; sy += w[i] * bf[i];
+                s2 += w[i] * bi[i] * bi[i]; sxy += w[i] * bi[i] * bf[i];
+            }
+            double det = s0 * s2 - s1 * s1;
+            if (fabs(det) < 1e-20) break;
+            g = (s0 * sxy - s1 * sy) / det;
+            b = (s2 * sy - s1 * sxy) / det;
+            double res[NB], rs[NB];
+            for (int i = 0; i < used; i++) { res[i] = fabs(bf[i] - (g * bi[i] + b)); rs[i] = res[i]; }
+            double med = orc_pct(rs, used, 50.0) + 1e-9;
+            for (int i = 0; i < used; i++) {
+                double q = res[i] / (2.0 * med);
+                w[i] = sqrt((double)bc[i]) / (1.0 + q * q);
+            }
+        }
+    } else if (used > 0) {
+        double s = 0; for (int i = 0; i < used; i++) s += bf[i];
+        b = s / used;
+    } else {
+        memcpy(cp, sq, sizeof(double) * n);
+        b = orc_pct(cp, (int)n, flat_pct);
+    }
+    if (!isfinite(g)) g = 0;
+    if (!isfinite(b)) b = 0;
+    out->g = g; out->b = b;
+    double vr = g * ref_intensity + b;
+    if (vr < 0) vr = b > 0 ? b : 0;
+    out->noise_ref = sqrt(vr) * 1.15;
+    out->ref_intensity = ref_intensity;
+    free(sm); free(sq); free(tmp); free(i2); free(cp);
+    return 0;
+}
+double orc_guided_eps_for_noise(double noise_ref) {     /* noise.c:197-202 */
+    double e = (3.0 * noise_ref) * (3.0 * noise_ref);
+    return e > 1e-8 ? e : 1e-8;
+}
+/* lag-1 residual autocorrelation in flat 8^3 blocks (noise.c:210-269) */
+int orc_noise_aniso(const float *v, int nz, int ny, int nx, double *rho_z, double *rho_xy) {
+    const int B = 8, B3 = 512;
+    int bz = nz / B, by = ny / B, bx = nx / B;
+    long nb = (long)bz * by * bx;
+    if (nb < 64) return 1;
+    float *bv = malloc(sizeof(float) * nb);
+    if (!bv) return 1;
+    long nv = 0;
+    float blk[512];
+    for (int z = 0; z < bz; z++) for (int y = 0; y < by; y++) for (int x = 0; x < bx; x++) {
+        double s = 0, s2 = 0; int pos = 1;
+        for (int i = 0; i < B3; i++) {
+            float t = v[((size_t)(z * B + i / 64) * ny + (y * B + (i / 8) % 8)) * nx + (size_t)x * B + i % 8];
+            if (t <= 0) pos = 0;
+            s += t; s2 += (double)t * t;
+        }
+        if (!pos) continue;
+        double m = s / B3;
+        bv[nv++] = (float)(s2 / B3 - m * m);
+    }
+    if (nv < 64) { free(bv); return 1; }
+    /* the reference insertion-sorts bv ascending and takes bv[nv/5] */
+    qsort(bv, nv, sizeof(float), orc_cmpf);
+    float th = bv[nv / 5];
+    double nzs = 0, nys = 0, nxs = 0, den = 0;
+    long used = 0;
+    for (int z = 0; z < bz; z++) for (int y = 0; y < by; y++) for (int x = 0; x < bx; x++) {
+        double s = 0, s2 = 0; int pos = 1;
+        for (int i = 0; i < B3; i++) {
+            float t = v[((size_t)(z * B + i / 64) * ny + (y * B + (i / 8) % 8)) * nx + (size_t)x * B + i % 8];
+            if (t <= 0) pos = 0;
+            blk[i] = t; s += t; s2 += (double)t * t;
+        }
+        if (!pos) continue;
+        double m = s / B3;
+        if (s2 / B3 - m * m > th) continue;
+        for (int i = 0; i < B3; i++) blk[i] -= (float)m;
+        for (int i = 0; i < B3; i++) {
+            int dz = i / 64, dy = (i / 8) % 8, dx = i % 8;
+            double c = blk[i];
+            den += c * c;
+            if (dz + 1 < B) nzs += c * blk[i + 64];
+            if (dy + 1 < B) nys += c * blk[i + 8];
+            if (dx + 1 < B) nxs += c * blk[i + 1];
+        }
+        used++;
+    }
+    free(bv);
+    if (used < 32 || den <= 0) return 1;
+    double k = (double)B / (B - 1);
+    *rho_z = k * nzs / den;
+    *rho_xy = k * 0.5 * (nys + nxs) / den;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ chain pieces */
+void orc_zdrift_apply(float *chunk, int nz, int ny, int nx, int z0, const float *factor) {
+    size_t plane = (size_t)ny * nx;                              /* zdrift.c:64-71 */
+    for (int z = 0; z < nz; z++) for (size_t i = 0; i < plane; i++) chunk[(size_t)z * plane + i] *= factor[z0 + z];
+}
+double orc_eps_for_tile(const fy_pipeline_cfg *cfg, double ecy, double ecx,
+                        long y0, long x0, long ty, long tx) {  /* pipeline.c:447-456 */
+    if (!cfg->have_eps_r || cfg->guided_eps <= 0 || cfg->eps_fn_med <= 0) return 0.0;
+    double ry = y0 + ty / 2.0 - ecy, rx = x0 + tx / 2.0 - ecx;
+    double fn = cfg->eps_fn_a + cfg->eps_fn_b * sqrt(ry * ry + rx * rx);
+    double s = fn / cfg->eps_fn_med, s2 = s * s;
+    if (s2 < 0.4) s2 = 0.4;
+    if (s2 > 8.0) s2 = 8.0;
+    return cfg->guided_eps * s2;
+}
+static double orc_scaled_db(const fy_pipeline_cfg *c, fy_physics *p) {   /* pipeline.c:89-104 */
+    p->delta_beta = c->delta_beta; p->energy_kev = c->energy_kev; p->distance_mm = c->distance_mm;
+    p->pixel_um = c->pixel_um; p->unsharp_sigma = c->unsharp_sigma; p->unsharp_coeff = c->unsharp_coeff;
+    p->psf_sigma_vox = c->psf_sigma_vox;
+    p->psf_sigma_z_vox = (c->psf_z_ratio > 1.0 && c->psf_sigma_vox > 0) ? c->psf_z_ratio * c->psf_sigma_vox : 0.0;
+    p->delta_beta *= orc_auto_deltabeta_scale(p);
+    return p->delta_beta;
+}
+
+/* fy_process_buffer restricted to the hot-path stages: u8->f01, dering, z-drift,
+ * (b) deconv chain, (c) guided, (d.5) recenter, quantisation of the inner tile
+ * (pipeline.c:252-297,413-421,467-510).  Returns 1 for a cfg that enables the
+ * air-zero or MUSICA stages. */
+int orc_process_buffer_f(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
+                         long rz0, long ry0, long rx0, long hz, long hy, long hx,
+                         long z0, long y0, long x0, long tz, long ty, long tx,
+                         long vol_y, long vol_x, unsigned char *out, int *all_air, float *ftile) {
+    if (cfg->do_air_zero || cfg->do_musica) return 1;
+    size_t hn = (size_t)hz * hy * hx;
+    int any = 0;
+    for (size_t i = 0; i < hn && !any; i++) any = u8buf[i] != 0;
+    if (!any) {
+        memset(out, 0, (size_t)tz * ty * tx);
+        if (ftile) memset(ftile, 0, sizeof(float) * hn);
+        if (all_air) *all_air = 1;
+        return 0;
+    }
+    float *f = malloc(sizeof(float) * hn), *b1 = malloc(sizeof(float) * hn);
+    if (!f || !b1) { free(f); free(b1); return 1; }
+    orc_u8_to_float(u8buf, f, hn);
+    if (cfg->have_dering && cfg->dering) orc_dering_apply(cfg->dering, f, rz0, ry0, rx0, hz, hy, hx, 1.0 / 255.0);
+    if (cfg->have_zdrift && cfg->zdrift_factor) orc_zdrift_apply(f, (int)hz, (int)hy, (int)hx, (int)rz0, cfg->zdrift_factor);
+    double ecy = cfg->dering_cy > 0 ? cfg->dering_cy : (vol_y - 1) / 2.0;
+    double ecx = cfg->dering_cx > 0 ? cfg->dering_cx : (vol_x - 1) / 2.0;
+    double eps_t = orc_eps_for_tile(cfg, ecy, ecx, y0, x0, ty, tx);
+    int nz = (int)hz, ny = (int)hy, nx = (int)hx;
+    if (cfg->do_deconv) {                                      /* (b) */
+        fy_physics ph; orc_scaled_db(cfg, &ph);
+        int win = cfg->window_hi > cfg->window_lo;
+        float wlo = (float)cfg->window_lo, wspan = (float)(cfg->window_hi - cfg->window_lo);
+        if (win) for (size_t i = 0; i < hn; i++) f[i] = wlo + f[i] * wspan;
+        int rc = (cfg->use_matched_deconv && cfg->psf_sigma_vox > 0)
+            ? orc_deconvolve_matched(f, b1, nz, ny, nx, &ph, cfg->deconv_tikhonov)
+            : orc_deconvolve(f, b1, nz, ny, nx, &ph, -1.0);
+        if (rc == 0) {
+            if (win) { float k = 1.0f / wspan; for (size_t i = 0; i < hn; i++) b1[i] = (b1[i] - wlo) * k; }
+            if (cfg->have_dec_range && cfg->dec_hi - cfg->dec_lo > 1e-6) {
+                float lo = (float)cfg->dec_lo, k = 1.0f / (float)(cfg->dec_hi - cfg->dec_lo);
+                for (size_t i = 0; i < hn; i++) b1[i] = (b1[i] - lo) * k;
+            }
+            for (size_t i = 0; i < hn; i++) { float v = b1[i]; f[i] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+        } else if (win) {
+            float k = 1.0f / wspan;
+            for (size_t i = 0; i < hn; i++) f[i] = (f[i] - wlo) * k;
+        }
+    }
+    if (cfg->guided_eps > 0 || eps_t > 0) {                    /* (c) */
+        double eps = eps_t > 0 ? eps_t : cfg->guided_eps;
+        int gs = cfg->guided_subsample == 0 ? 2 : cfg->guided_subsample;
+        if (orc_guided_fast_f32(f, b1, nz, ny, nx, 2, eps, gs) == 0) memcpy(f, b1, sizeof(float) * hn);
+    }
+    if (cfg->have_norm && cfg->norm_hi > cfg->norm_lo) {        /* (d.5) */
+        float mid = 0.5f * (float)(cfg->norm_lo + cfg->norm_hi) / 255.0f;
+        float k = (2.0f * 0.375f) * 255.0f / (float)(cfg->norm_hi - cfg->norm_lo);
+        for (size_t i = 0; i < hn; i++) {
+            if (f[i] <= 0.0f) { f[i] = 0.0f; continue; }
+            float v = 0.5f + (f[i] - mid) * k;
+            f[i] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+        }
+    }
+    long iz = z0 - rz0, iy = y0 - ry0, ix = x0 - rx0, o = 0;
+    for (long z = iz; z < iz + tz; z++)
+        for (long y = iy; y < iy + ty; y++)
+            for (long x = ix; x < ix + tx; x++)
+                out[o++] = orc_quant_u8(f[((size_t)z * hy + y) * hx + x], rz0 + z, ry0 + y, rx0 + x, cfg->no_dither);
+    if (ftile) memcpy(ftile, f, sizeof(float) * hn);
+    if (all_air) *all_air = 0;
+    free(f); free(b1);
+    return 0;
+}
+int orc_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
+                       long rz0, long ry0, long rx0, long hz, long hy, long hx,
+                       long z0, long y0, long x0, long tz, long ty, long tx,
+                       long vol_y, long vol_x, unsigned char *out, int *all_air) {
+    return orc_process_buffer_f(cfg, u8buf, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx,
+                                vol_y, vol_x, out, all_air, NULL);
+}

@@ -1,0 +1,115 @@
+/* oracle.h -- CPU restatement of the fysics hot path, used ONLY as a checker.
+ *
+ * TEST INFRASTRUCTURE.  Nothing in the product (libfysics_amd.so) links, loads or
+ * calls this code: it is imported by tests/, by __graft_entry__.smoke() and by
+ * the cpu_baseline leg of bench.py, and there only to produce the expected
+ * answer.  Every function restates one reference routine (cited file:line into
+ * the fysics sources); the fp32 versions follow the reference's operation order,
+ * the fp64 twins (suffix _f64) evaluate the same algorithm -- including the
+ * 4097-entry LUT weight semantics of deconv_radial -- in double precision.
+ *
+ * Pinning (what stops this from being a retyped formula): tests/test_oracle.py
+ * checks each function against golden vectors produced by the compiled
+ * reference (oracle/_ref, tests/golden/) and against closed forms (naive DFT,
+ * Parseval, brute-force box means, the nabu Paganin formula, integer hashes).
+ * Functions without such a check are marked "parity unpinned" below.
+ */
+#ifndef FYSICS_ORACLE_H
+#define FYSICS_ORACLE_H
+
+#include <stddef.h>
+#include "../include/fysics.h"   /* struct layouts only */
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* fft.c */
+int  orc_next_fft_size(int n);
+void orc_fft1d_f32(float *re, float *im, int n, int sign);
+void orc_fft1d_f64(double *re, double *im, int n, int sign);
+void orc_fft3d_f32(float *re, float *im, int nz, int ny, int nx, int sign);
+void orc_fft3d_f64(double *re, double *im, int nz, int ny, int nx, int sign);
+void orc_fft3d_normalize_f32(float *re, float *im, int nz, int ny, int nx);
+
+/* paganin.c transfers */
+double orc_paganin_transfer(double f, const fy_physics *p);
+double orc_unsharp_transfer(double f, const fy_physics *p);
+double orc_recon_transfer(double f, const fy_physics *p);
+double orc_auto_reg(const fy_physics *p, double base);
+double orc_auto_deltabeta_scale(const fy_physics *p);
+int    orc_kernel_halo(const fy_physics *p);
+
+/* paganin.c deconv_radial. kind 0 = Wiener of the recon transfer (fy_deconvolve),
+ * kind 1 = matched PSF*unsharp (fy_deconvolve_matched). */
+int orc_deconv_radial_f32(const float *in, float *out, int nz, int ny, int nx,
+                          const fy_physics *p, double param, int kind,
+                          double cap, double fz2_scale);
+int orc_deconv_radial_f64(const float *in, double *out, int nz, int ny, int nx,
+                          const fy_physics *p, double param, int kind,
+                          double cap, double fz2_scale);
+/* the LUT that deconv_radial builds (4097 floats) and its scale us */
+void orc_deconv_lut(int pz, int py, int px, const fy_physics *p, double param, int kind,
+                    double cap, double fz2_scale, float *lut, double *us);
+int orc_deconvolve(const float *in, float *out, int nz, int ny, int nx,
+                   const fy_physics *p, double reg);
+int orc_deconvolve_matched(const float *in, float *out, int nz, int ny, int nx,
+                           const fy_physics *p, double tikhonov);
+/* parameters fy_deconvolve_matched passes to deconv_radial */
+double orc_matched_fz2_scale(const fy_physics *p);
+
+/* guided.c */
+void orc_box_mean_f32(const float *in, float *out, int nz, int ny, int nx, int r);
+void orc_box_mean_f64(const double *in, double *out, int nz, int ny, int nx, int r);
+int  orc_guided_exact_f32(const float *in, float *out, int nz, int ny, int nx, int r, double eps);
+int  orc_guided_fast_f32(const float *in, float *out, int nz, int ny, int nx, int r, double eps, int s);
+int  orc_guided_exact_f64(const double *in, double *out, int nz, int ny, int nx, int r, double eps);
+int  orc_guided_fast_f64(const double *in, double *out, int nz, int ny, int nx, int r, double eps, int s);
+
+/* stream.c conversions */
+void  orc_u8_to_float(const unsigned char *in, float *out, size_t n);
+void  orc_float_to_u8(const float *in, unsigned char *out, size_t n);
+void  orc_u8_to_phys(const unsigned char *in, float *out, size_t n, double lo, double hi);
+void  orc_phys_to_u8(const float *in, unsigned char *out, size_t n, double lo, double hi);
+float orc_phys_to_u8_level(double phys, double lo, double hi);
+
+/* pipeline.c quantiser */
+float         orc_dither01(long z, long y, long x);
+unsigned char orc_quant_u8(float v, long gz, long gy, long gx, int no_dither);
+
+/* dering.c */
+int  orc_dering_init(fy_dering *d, long Z, long Y, long X, double cy, double cx, int slab_z, int ns);
+void orc_dering_free(fy_dering *d);
+int  orc_dering_tile_init(fy_dering *t, const fy_dering *d);
+void orc_dering_accumulate_u8(fy_dering *t, const unsigned char *buf,
+                              long y0, long x0, long dz, long dy, long dx, int ss);
+void orc_dering_merge_tile(fy_dering *d, int slab, const fy_dering *t);
+long orc_dering_finalize(fy_dering *d, int hp_win, unsigned int min_cnt, double min_amp, double max_amp);
+void orc_dering_apply(const fy_dering *d, float *f, long z0, long y0, long x0,
+                      long dz, long dy, long dx, double scale);
+
+/* noise.c */
+int    orc_estimate_noise(const float *in, int nz, int ny, int nx, int win,
+                          double flat_pct, double ref_intensity, fy_noise_model *out);
+double orc_guided_eps_for_noise(double noise_ref);
+int    orc_noise_aniso(const float *v, int nz, int ny, int nx, double *rho_z, double *rho_xy);
+
+/* zdrift.c apply; pipeline.c per-tile eps and the tile chain */
+void   orc_zdrift_apply(float *chunk, int nz, int ny, int nx, int z0, const float *factor);
+double orc_eps_for_tile(const fy_pipeline_cfg *cfg, double ecy, double ecx,
+                        long y0, long x0, long ty, long tx);
+int    orc_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
+                          long rz0, long ry0, long rx0, long hz, long hy, long hx,
+                          long z0, long y0, long x0, long tz, long ty, long tx,
+                          long vol_y, long vol_x, unsigned char *out, int *all_air);
+/* same chain, also returning the float tile before quantisation (hz*hy*hx) */
+int    orc_process_buffer_f(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
+                            long rz0, long ry0, long rx0, long hz, long hy, long hx,
+                            long z0, long y0, long x0, long tz, long ty, long tx,
+                            long vol_y, long vol_x, unsigned char *out, int *all_air,
+                            float *ftile);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

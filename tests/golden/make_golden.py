@@ -1,0 +1,175 @@
+"""Write the golden vectors in tests/golden/*.npz from the reference itself.
+
+Runs the reference compiled from its own sources (oracle/_ref/libfysics_ref.so,
+built by oracle/Makefile where /root/reference exists) on small seeded inputs
+and stores inputs + outputs.  The fixtures pin the CPU restatement
+(oracle/liboracle.so) where the reference build is absent.
+
+    python tests/golden/make_golden.py
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import clibs  # noqa: E402
+
+PHYS_A = dict(delta_beta=1000.0, energy_kev=78.0, distance_mm=220.0, pixel_um=2.4,
+              unsharp_sigma=1.2, unsharp_coeff=4.0)                     # tests/test_fysics.c:229
+PHYS_B = dict(delta_beta=300.0, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
+              unsharp_sigma=0.8, unsharp_coeff=0.5)
+
+
+def lcg01(n, seed):
+    """((rng>>8)&0xFFFF)/65535 of rng = rng*1664525+1013904223 (tests/test_fysics.c:224-228)."""
+    out = np.empty(n, np.float32)
+    r = seed
+    for i in range(n):
+        r = (r * 1664525 + 1013904223) & 0xFFFFFFFF
+        out[i] = np.float32(((r >> 8) & 0xFFFF) / 65535.0)
+    return out
+
+
+def main():
+    ref = clibs.ref()
+    if ref is None:
+        sys.exit("oracle/_ref/libfysics_ref.so is missing (make -C oracle ref)")
+    out = {}
+
+    # ---- FFT lines (fft.c)
+    rng = np.random.default_rng(1)
+    for n in (16, 48, 96, 192, 256, 3, 6, 12):
+        re = rng.standard_normal(n).astype(np.float32)
+        im = rng.standard_normal(n).astype(np.float32)
+        for sign in (-1, 1):
+            a, b = re.copy(), im.copy()
+            ref.fy_fft1d(clibs.fp(a), clibs.fp(b), n, sign)
+            out[f"fft_{n}_{sign}_in"] = np.stack([re, im])
+            out[f"fft_{n}_{sign}_out"] = np.stack([a, b])
+
+    # ---- deconvolution (paganin.c)
+    shape = (22, 27, 30)                                      # tests/test_fysics.c:221
+    x = lcg01(int(np.prod(shape)), 99).reshape(shape)
+    p = clibs.physics(**PHYS_A)
+    y = np.empty_like(x)
+    assert ref.fy_deconvolve(clibs.fp(x), clibs.fp(y), *shape, C.byref(p), 0.015) == 0
+    out["deconv_in"], out["deconv_out"] = x, y
+    shape2 = (12, 16, 20)
+    x2 = rng.random(shape2).astype(np.float32)
+    pm = clibs.physics(**PHYS_B, psf_sigma_vox=1.1, psf_sigma_z_vox=1.3)
+    y2 = np.empty_like(x2)
+    assert ref.fy_deconvolve_matched(clibs.fp(x2), clibs.fp(y2), *shape2, C.byref(pm), 0.05) == 0
+    out["matched_in"], out["matched_out"] = x2, y2
+    shape3 = (10, 12, 9)
+    x3 = rng.random(shape3).astype(np.float32)
+    pb = clibs.physics(**PHYS_B)
+    y3 = np.empty_like(x3)
+    assert ref.ref_deconv_radial(clibs.fp(x3), clibs.fp(y3), *shape3, C.byref(pb), 0.015, 0, 8.0,
+                                 1.17 ** 2) == 0
+    out["aniso_in"], out["aniso_out"] = x3, y3
+    out["halo"] = np.array([ref.fy_kernel_halo(C.byref(clibs.physics(**PHYS_A))),
+                            ref.fy_kernel_halo(C.byref(clibs.physics(**PHYS_B)))])
+    out["transfer_f"] = np.linspace(0, 0.5, 11)
+    out["transfer_T"] = np.array([ref.fy_paganin_transfer(f, C.byref(p)) for f in out["transfer_f"]])
+    out["transfer_U"] = np.array([ref.fy_unsharp_transfer(f, C.byref(p)) for f in out["transfer_f"]])
+    out["transfer_H"] = np.array([ref.fy_recon_transfer(f, C.byref(p)) for f in out["transfer_f"]])
+    out["dbscale"] = np.array([ref.fy_auto_deltabeta_scale(C.byref(clibs.physics(**PHYS_A))),
+                               ref.fy_auto_deltabeta_scale(C.byref(clibs.physics(**PHYS_B)))])
+
+    # ---- guided filter and box mean (guided.c)
+    gs = (20, 18, 22)
+    g = (0.3 + 0.4 * (np.arange(gs[2]) > 11) + 0.05 * rng.standard_normal(gs)).astype(np.float32)
+    ws = np.empty(4 * g.size, np.float32)
+    ge = np.empty_like(g)
+    gf = np.empty_like(g)
+    ref.fy_guided_denoise_ws(clibs.fp(g), clibs.fp(ge), *gs, 2, 4e-4, clibs.fp(ws))
+    ref.fy_guided_denoise_fast_ws(clibs.fp(g), clibs.fp(gf), *gs, 2, 4e-4, 2, clibs.fp(ws))
+    out["guided_in"], out["guided_exact"], out["guided_fast"] = g, ge, gf
+    for r in (1, 2, 3):
+        b = np.empty_like(g)
+        tmp = np.empty_like(g)
+        ref.fy_box_smooth(clibs.fp(g), clibs.fp(b), clibs.fp(tmp), *gs, r)
+        out[f"box_r{r}"] = b
+
+    # ---- quantiser (pipeline.c:53-65)
+    qv = (rng.random(4000) * 1.4 - 0.2).astype(np.float32)
+    qc = rng.integers(0, 100000, size=(4000, 3))
+    out["quant_v"], out["quant_c"] = qv, qc
+    out["quant_q"] = np.array([ref.ref_quant_u8(float(v), int(a), int(b), int(c), 0) for v, (a, b, c) in zip(qv, qc)], np.uint8)
+    out["quant_q_nd"] = np.array([ref.ref_quant_u8(float(v), int(a), int(b), int(c), 1) for v, (a, b, c) in zip(qv, qc)], np.uint8)
+
+    # ---- rings (dering.c) on a small spiral+rings phantom
+    nz, ny, nx = 8, 96, 112
+    cy, cx = 47.5, 55.0
+    yy, xx = np.meshgrid(np.arange(ny) - cy, np.arange(nx) - cx, indexing="ij")
+    rr = np.sqrt(yy ** 2 + xx ** 2)
+    th = np.arctan2(yy, xx)
+    v = 128 + 30 * np.cos(2 * np.pi * (rr - 96 * th / (2 * np.pi)) / 12)
+    v = v + 9 * np.exp(-(rr - 20) ** 2 / 1.28) - 7 * np.exp(-(rr - 38) ** 2 / 1.28)
+    vol = np.clip(v[None] + rng.integers(0, 256, (nz, ny, nx)) / 25.5 - 5, 1, 255).astype(np.uint8)
+    vol[:, :5, :7] = 0
+    d = clibs.Dering()
+    t = clibs.Dering()
+    assert ref.fy_dering_init(C.byref(d), nz, ny, nx, cy, cx, 4, 8) == 0
+    assert ref.fy_dering_tile_init(C.byref(t), C.byref(d)) == 0
+    for slab in range(2):
+        ref.fy_dering_tile_reset(C.byref(t))
+        sub = np.ascontiguousarray(vol[4 * slab:4 * slab + 4])
+        ref.fy_dering_accumulate_u8(C.byref(t), clibs.up(sub), 0, 0, 4, ny, nx, 1)
+        ref.fy_dering_merge_tile(C.byref(d), slab, C.byref(t))
+    nb = d.nslab * d.ns * d.nr
+    out["dering_vol"] = vol
+    out["dering_sum"] = np.ctypeslib.as_array(d.sum, (nb,)).copy()
+    out["dering_cnt"] = np.ctypeslib.as_array(d.cnt, (nb,)).copy()
+    nfound = ref.fy_dering_finalize(C.byref(d), 9, 20, 1.0, 6.0)
+    out["dering_found"] = np.array([nfound, d.have_rings, d.nr, d.nslab])
+    out["dering_ring"] = np.ctypeslib.as_array(d.ring, (d.nslab * d.nr,)).copy()
+    f = (vol * np.float32(1 / 255)).astype(np.float32)
+    ref.fy_dering_apply(C.byref(d), clibs.fp(f), 0, 0, 0, nz, ny, nx, 1.0 / 255.0)
+    out["dering_applied"] = f
+    ref.fy_dering_free(C.byref(t))
+    ref.fy_dering_free(C.byref(d))
+
+    # ---- noise model (noise.c)
+    ns_ = (40, 40, 40)
+    zz, yy3, xx3 = np.meshgrid(*[np.arange(k) for k in ns_], indexing="ij")
+    sig = 0.5 + 0.3 * np.sin(xx3 * 0.10) * np.cos(yy3 * 0.09)
+    nv = (sig + (0.03 + 0.04 * sig) * rng.standard_normal(ns_)).astype(np.float32)
+    nm = clibs.NoiseModel()
+    assert ref.fy_estimate_noise(clibs.fp(nv), *ns_, 5, 10.0, 0.5, C.byref(nm)) == 0
+    out["noise_in"] = nv
+    out["noise_model"] = np.array([nm.g, nm.b, nm.noise_ref, nm.ref_intensity, nm.n_bins_used])
+    av = (0.5 + 0.05 * rng.standard_normal((64, 64, 64))).astype(np.float32)
+    bz = av.copy()
+    bz[1:-1] = 0.25 * av[:-2] + 0.5 * av[1:-1] + 0.25 * av[2:]
+    rz, rxy = C.c_double(), C.c_double()
+    assert ref.fy_noise_aniso(clibs.fp(bz), 64, 64, 64, C.byref(rz), C.byref(rxy)) == 0
+    out["aniso_vol"] = bz
+    out["aniso_rho"] = np.array([rz.value, rxy.value])
+
+    # ---- per-tile chain (pipeline.c:467-510)
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from fysics_amd import phantom
+    prng = np.random.default_rng(5)
+    pv = phantom.sheets((28, 30, 32), prng, n_sheets=6)
+    u8 = phantom.noisy_u8(phantom.forward(pv, **phantom.CONFIG1_PHYS), prng)
+    cfg = clibs.PipelineCfg()
+    for k, vv in dict(delta_beta=300.0, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
+                      unsharp_sigma=0.8, unsharp_coeff=0.5, window_lo=-0.1, window_hi=1.2,
+                      do_deconv=1, guided_eps=1.2e-3).items():
+        setattr(cfg, k, vv)
+    q = np.empty((20, 22, 24), np.uint8)
+    air = C.c_int(0)
+    assert ref.fy_process_buffer(C.byref(cfg), clibs.up(u8), 10, 20, 30, *u8.shape, 14, 24, 34, 20, 22, 24,
+                                 300, 400, clibs.up(q), C.byref(air)) == 0
+    out["chain_u8"], out["chain_out"] = u8, q
+
+    np.savez_compressed(os.path.join(HERE, "reference_vectors.npz"), **out)
+    print(f"wrote {len(out)} arrays")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,442 @@
+"""Pins the CPU restatement in oracle/ (CPU only).
+
+Each oracle function is checked against (a) the golden vectors written by the
+compiled reference (tests/golden/reference_vectors.npz), (b) the live reference
+build oracle/_ref when present, and (c) independent mathematics: numpy's FFT,
+brute-force box means, the nabu closed form, properties of the dithered
+quantiser.  The checks are chosen so that a dropped term, a wrong sign or index
+or a transposed axis in the restatement fails at least one of them.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import clibs
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz"))
+PHYS_A = dict(delta_beta=1000.0, energy_kev=78.0, distance_mm=220.0, pixel_um=2.4,
+              unsharp_sigma=1.2, unsharp_coeff=4.0)
+PHYS_B = dict(delta_beta=300.0, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
+              unsharp_sigma=0.8, unsharp_coeff=0.5)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# ------------------------------------------------------------------ FFT (fft.c)
+@pytest.mark.parametrize("n", [16, 48, 96, 192, 256, 3, 6, 12])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft1d_golden_and_numpy(orc, n, sign):
+    re, im = GOLD[f"fft_{n}_{sign}_in"]
+    a, b = re.copy(), im.copy()
+    orc.orc_fft1d_f32(clibs.fp(a), clibs.fp(b), n, sign)
+    gr, gi = GOLD[f"fft_{n}_{sign}_out"]
+    assert rel_l2(a + 1j * b, gr + 1j * gi) < 2e-6
+    x = re.astype(np.float64) + 1j * im
+    want = np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * n
+    a64, b64 = re.astype(np.float64), im.astype(np.float64)
+    orc.orc_fft1d_f64(clibs.dp(a64), clibs.dp(b64), n, sign)
+    assert rel_l2(a64 + 1j * b64, want) < 1e-14
+    assert rel_l2(a + 1j * b, want) < 2e-6
+
+
+def test_fft_naive_dft_small(orc):
+    """tests/test_fysics.c:26-40 and :56-72 (n = 16 and 48 against a direct sum)."""
+    for n in (16, 48):
+        t = np.arange(n)
+        re = (np.sin(t * 0.7) + 0.3 * np.cos(t * 1.3)).astype(np.float32)
+        im = (0.2 * np.sin(t * 0.31)).astype(np.float32)
+        k = t[:, None]
+        dft = (np.exp(-2j * np.pi * k * t[None] / n) * (re + 1j * im)[None]).sum(1)
+        a, b = re.copy(), im.copy()
+        orc.orc_fft1d_f32(clibs.fp(a), clibs.fp(b), n, -1)
+        assert np.max(np.abs(a - dft.real) + np.abs(b - dft.imag)) < 1e-3
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (12, 8, 24), (6, 4, 3)])
+def test_fft3d_roundtrip_and_numpy(orc, shape):
+    rng = np.random.default_rng(2)
+    re = rng.random(shape).astype(np.float32)
+    im = np.zeros(shape, np.float32)
+    a, b = re.copy(), im.copy()
+    orc.orc_fft3d_f32(clibs.fp(a), clibs.fp(b), *shape, -1)
+    assert rel_l2(a + 1j * b, np.fft.fftn(re.astype(np.float64))) < 2e-6
+    orc.orc_fft3d_f32(clibs.fp(a), clibs.fp(b), *shape, 1)
+    orc.orc_fft3d_normalize_f32(clibs.fp(a), clibs.fp(b), *shape)
+    assert np.abs(a - re).max() < 1e-4
+
+
+def test_next_fft_size(orc):
+    cases = {176: 192, 128: 128, 160: 192, 97: 128, 1: 1, 2: 2, 3: 3, 5: 6, 64: 64, 96: 96,
+             256: 256, 512: 512, 158: 192, 200: 256, 300: 384}
+    for n, want in cases.items():
+        assert orc.orc_next_fft_size(n) == want
+
+
+# ------------------------------------------------------------------ transfers (paganin.c)
+def test_transfers_golden_and_nabu(orc):
+    p = clibs.physics(**PHYS_A)
+    f = GOLD["transfer_f"]
+    for name, key in (("paganin_transfer", "transfer_T"), ("unsharp_transfer", "transfer_U"),
+                      ("recon_transfer", "transfer_H")):
+        got = np.array([getattr(orc, "orc_" + name)(x, C.byref(p)) for x in f])
+        assert np.allclose(got, GOLD[key], rtol=1e-14, atol=0)
+    # nabu formula (tests/test_fysics.c:327-330)
+    lam, dist, fu = 1.23984199e-3 / 78.0, 220000.0, 0.1 / 2.4
+    assert abs(orc.orc_paganin_transfer(0.1, C.byref(p)) - 1.0 / (1.0 + 1000.0 * lam * dist * np.pi * fu * fu)) < 1e-12
+    scales = [orc.orc_auto_deltabeta_scale(C.byref(clibs.physics(**k))) for k in (PHYS_A, PHYS_B)]
+    assert np.allclose(scales, GOLD["dbscale"], rtol=1e-14)
+    halos = [orc.orc_kernel_halo(C.byref(clibs.physics(**k))) for k in (PHYS_A, PHYS_B)]
+    assert halos == list(GOLD["halo"])
+
+
+# ------------------------------------------------------------------ deconvolution (paganin.c)
+def _lut_weight_numpy(fr2, us, lut):
+    t = fr2 * us
+    ti = np.minimum(t.astype(np.int64), 4095)
+    return lut[ti] + (lut[ti + 1] - lut[ti]) * (t - ti)
+
+
+def deconv_numpy_f64(x, weight_fn, cap, s, use_lut=True):
+    """Independent restatement through numpy's FFT: reflect-pad to the supported
+    size, full complex 3-D FFT, radial weight through the 4097-entry LUT over
+    u = s*fz^2 + fy^2 + fx^2 evaluated in float64, inverse, crop."""
+    nz, ny, nx = x.shape
+    nxt = lambda n: min([m for m in [2 ** k for k in range(14)] + [3 * 2 ** k for k in range(13)] if m >= n])
+    pz, py, px = nxt(nz), nxt(ny), nxt(nx)
+    refl = lambda n, p: np.array([i if i < n else 2 * n - 1 - i for i in range(p)]) if n > 1 else np.zeros(p, int)
+    xp = x.astype(np.float64)[np.ix_(refl(nz, pz), refl(ny, py), refl(nx, px))]
+    fz = np.fft.fftfreq(pz)[:, None, None] ** 2 * s
+    fy = np.fft.fftfreq(py)[None, :, None] ** 2
+    fx = (np.arange(px // 2 + 1) / px)[None, None, :] ** 2
+    umax = fz.max() + fy.max() + fx.max()
+    us = 4096 / umax
+    knots = np.array([weight_fn(np.sqrt(i / us)) for i in range(4097)])
+    if cap > 0:
+        knots = np.minimum(knots, cap)
+    u = fz + fy + fx
+    w = _lut_weight_numpy(u, us, knots) if use_lut else np.minimum(weight_fn(np.sqrt(u)), cap if cap > 0 else np.inf)
+    spec = np.fft.rfftn(xp) * w
+    return np.fft.irfftn(spec, s=(pz, py, px), axes=(0, 1, 2))[:nz, :ny, :nx]
+
+
+def wiener_np(p, reg):
+    lam = 1.23984199e-3 / p["energy_kev"]
+    def w(fr):
+        fu = fr / p["pixel_um"]
+        t = 1.0 / (1.0 + p["delta_beta"] * lam * p["distance_mm"] * 1000.0 * np.pi * fu * fu)
+        un = 1.0 + p["unsharp_coeff"] * (1.0 - np.exp(-2 * np.pi ** 2 * p["unsharp_sigma"] ** 2 * fr * fr))
+        h = np.maximum(t * un, 1e-4)
+        return h / (h * h + reg)
+    return w
+
+
+def test_deconv_golden(orc):
+    x, y = GOLD["deconv_in"], GOLD["deconv_out"]
+    p = clibs.physics(**PHYS_A)
+    a = np.empty_like(x)
+    assert orc.orc_deconvolve(clibs.fp(x), clibs.fp(a), *x.shape, C.byref(p), 0.015) == 0
+    assert rel_l2(a, y) < 1e-6
+    # tests/test_fysics.c:237: half-spectrum path == full-complex path within LUT tolerance
+    full = deconv_numpy_f64(x, wiener_np(PHYS_A, 0.015), 8.0, 1.0, use_lut=False)
+    assert np.abs(a - full).max() < 1e-3 * max(np.abs(full).max(), 1)
+
+
+@pytest.mark.parametrize("shape", [(22, 27, 30), (5, 7, 9), (1, 4, 6), (3, 1, 5), (12, 6, 1)])
+def test_deconv_f64_vs_numpy_lut(orc, shape):
+    rng = np.random.default_rng(sum(shape))
+    x = rng.random(shape).astype(np.float32)
+    p = clibs.physics(**PHYS_A)
+    got = np.empty(shape, np.float64)
+    assert orc.orc_deconv_radial_f64(clibs.fp(x), clibs.dp(got), *shape, C.byref(p), 0.015, 0, 8.0, 1.0) == 0
+    want = deconv_numpy_f64(x, wiener_np(PHYS_A, 0.015), 8.0, 1.0)
+    assert rel_l2(got, want) < 1e-12
+    f32 = np.empty(shape, np.float32)
+    assert orc.orc_deconvolve(clibs.fp(x), clibs.fp(f32), *shape, C.byref(p), 0.015) == 0
+    assert rel_l2(f32, want) < 2e-6
+
+
+def test_deconv_aniso_and_matched_golden(orc):
+    x, y = GOLD["aniso_in"], GOLD["aniso_out"]
+    p = clibs.physics(**PHYS_B)
+    a = np.empty_like(x)
+    assert orc.orc_deconv_radial_f32(clibs.fp(x), clibs.fp(a), *x.shape, C.byref(p), 0.015, 0, 8.0, 1.17 ** 2) == 0
+    assert rel_l2(a, y) < 1e-6
+    d = np.empty(x.shape, np.float64)
+    orc.orc_deconv_radial_f64(clibs.fp(x), clibs.dp(d), *x.shape, C.byref(p), 0.015, 0, 8.0, 1.17 ** 2)
+    assert rel_l2(d, deconv_numpy_f64(x, wiener_np(PHYS_B, 0.015), 8.0, 1.17 ** 2)) < 1e-12
+    x, y = GOLD["matched_in"], GOLD["matched_out"]
+    pm = clibs.physics(**PHYS_B, psf_sigma_vox=1.1, psf_sigma_z_vox=1.3)
+    a = np.empty_like(x)
+    assert orc.orc_deconvolve_matched(clibs.fp(x), clibs.fp(a), *x.shape, C.byref(pm), 0.05) == 0
+    assert rel_l2(a, y) < 1e-6
+    assert abs(orc.orc_matched_fz2_scale(C.byref(pm)) - (1.3 / 1.1) ** 2) < 1e-15
+
+
+def test_deconv_lut_knots(orc):
+    """LUT entries are the weight at sqrt(i/us), capped, rounded to fp32."""
+    p = clibs.physics(**PHYS_A)
+    lut = np.empty(4097, np.float32)
+    us = C.c_double()
+    orc.orc_deconv_lut(32, 24, 48, C.byref(p), 0.015, 0, 8.0, 1.0, clibs.fp(lut), C.byref(us))
+    umax = 0.25 + 0.25 + 0.25
+    assert abs(us.value - 4096 / umax) < 1e-9
+    w = wiener_np(PHYS_A, 0.015)
+    want = np.minimum([w(np.sqrt(i / us.value)) for i in range(4097)], 8.0).astype(np.float32)
+    assert np.array_equal(lut, want)
+
+
+# ------------------------------------------------------------------ guided / box (guided.c)
+def box_mean_numpy(x, r):
+    """Brute force: mean over the in-bounds part of each (2r+1)^3 window."""
+    nz, ny, nx = x.shape
+    c = np.pad(x.astype(np.float64), r)
+    o = np.pad(np.ones(x.shape), r)
+    s = np.zeros(x.shape)
+    cnt = np.zeros(x.shape)
+    for dz in range(2 * r + 1):
+        for dy in range(2 * r + 1):
+            for dx in range(2 * r + 1):
+                s += c[dz:dz + nz, dy:dy + ny, dx:dx + nx]
+                cnt += o[dz:dz + nz, dy:dy + ny, dx:dx + nx]
+    return s / cnt
+
+
+def guided_numpy(x, r, eps):
+    x = x.astype(np.float64)
+    m = box_mean_numpy(x, r)
+    v = np.maximum(box_mean_numpy(x * x, r) - m * m, 0)
+    a = v / (v + eps)
+    b = m * (1 - a)
+    return box_mean_numpy(a, r) * x + box_mean_numpy(b, r)
+
+
+@pytest.mark.parametrize("r", [1, 2, 3])
+def test_box_mean(orc, r):
+    g = GOLD["guided_in"]
+    a = np.empty_like(g)
+    orc.orc_box_mean_f32(clibs.fp(g), clibs.fp(a), *g.shape, r)
+    assert rel_l2(a, GOLD[f"box_r{r}"]) < 1e-6
+    d = np.empty(g.shape)
+    orc.orc_box_mean_f64(clibs.dp(g.astype(np.float64)), clibs.dp(d), *g.shape, r)
+    assert rel_l2(d, box_mean_numpy(g, r)) < 1e-13
+
+
+def test_guided_golden_and_bruteforce(orc):
+    g = GOLD["guided_in"]
+    a = np.empty_like(g)
+    orc.orc_guided_exact_f32(clibs.fp(g), clibs.fp(a), *g.shape, 2, 4e-4)
+    assert rel_l2(a, GOLD["guided_exact"]) < 1e-6
+    orc.orc_guided_fast_f32(clibs.fp(g), clibs.fp(a), *g.shape, 2, 4e-4, 2)
+    assert rel_l2(a, GOLD["guided_fast"]) < 1e-6
+    d = np.empty(g.shape)
+    orc.orc_guided_exact_f64(clibs.dp(g.astype(np.float64)), clibs.dp(d), *g.shape, 2, 4e-4)
+    assert rel_l2(d, guided_numpy(g, 2, 4e-4)) < 1e-12
+
+
+def test_guided_fast_f64_structure(orc):
+    """fast(s) == exact guided coefficients on the decimated grid, trilinear
+    upsampled (guided.c:152-228), re-derived with numpy."""
+    rng = np.random.default_rng(4)
+    x = rng.random((9, 10, 11))
+    s, r, eps = 2, 2, 1e-2
+    xl = x[::s, ::s, ::s]
+    m = box_mean_numpy(xl, 1)
+    v = np.maximum(box_mean_numpy(xl * xl, 1) - m * m, 0)
+    a = v / (v + eps)
+    b = m * (1 - a)
+    ma, mb = box_mean_numpy(a, 1), box_mean_numpy(b, 1)
+
+    def coords(n, ln):
+        u = np.arange(n, dtype=np.float32) / np.float32(s)
+        i0 = np.minimum(u.astype(int), ln - 2)
+        return i0, np.minimum(u - i0, 1.0).astype(np.float64)
+    (z0, fz), (y0, fy), (x0, fx) = coords(9, 5), coords(10, 5), coords(11, 6)
+
+    def tri(f):
+        out = np.zeros(x.shape)
+        for dz, wz in ((0, 1 - fz), (1, fz)):
+            for dy, wy in ((0, 1 - fy), (1, fy)):
+                for dx, wx in ((0, 1 - fx), (1, fx)):
+                    out += (wz[:, None, None] * wy[None, :, None] * wx[None, None, :] *
+                            f[np.ix_(z0 + dz, y0 + dy, x0 + dx)])
+        return out
+    want = tri(ma) * x + tri(mb)
+    got = np.empty(x.shape)
+    orc.orc_guided_fast_f64(clibs.dp(x), clibs.dp(got), *x.shape, r, eps, s)
+    assert rel_l2(got, want) < 1e-13
+
+
+# ------------------------------------------------------------------ quantiser, conversions
+def test_quant_golden(orc):
+    v, c = GOLD["quant_v"], GOLD["quant_c"]
+    q = np.array([orc.orc_quant_u8(float(a), int(z), int(y), int(x), 0) for a, (z, y, x) in zip(v, c)])
+    assert np.array_equal(q, GOLD["quant_q"])
+    q = np.array([orc.orc_quant_u8(float(a), int(z), int(y), int(x), 1) for a, (z, y, x) in zip(v, c)])
+    assert np.array_equal(q, GOLD["quant_q_nd"])
+
+
+def test_dither_is_unbiased(orc):
+    """floor(v*255 + d) with d ~ U[0,1) has mean v*255 (pipeline.c:49-52)."""
+    v = 0.4237
+    q = np.array([orc.orc_quant_u8(v, z, y, 7, 0) for z in range(60) for y in range(60)])
+    d = np.array([orc.orc_dither01(z, y, 7) for z in range(60) for y in range(60)])
+    assert abs(q.mean() - v * 255) < 0.02
+    assert d.min() >= 0 and d.max() < 1 and abs(d.mean() - 0.5) < 0.02
+
+
+def test_conversions_roundtrip(orc):
+    """tests/test_fysics.c:439-453."""
+    u = np.arange(256, dtype=np.uint8)
+    ph = np.empty(256, np.float32)
+    orc.orc_u8_to_phys(clibs.up(u), clibs.fp(ph), 256, -0.03, 0.21)
+    back = np.empty(256, np.uint8)
+    orc.orc_phys_to_u8(clibs.fp(ph), clibs.up(back), 256, -0.03, 0.21)
+    assert np.array_equal(back, u)
+    assert abs(ph[0] + 0.03) < 1e-6 and abs(ph[255] - 0.21) < 1e-6
+    assert abs(orc.orc_phys_to_u8_level(0.0, -0.03, 0.145) - orc.orc_phys_to_u8_level(0.0, -0.03, 0.21)) > 1.0
+
+
+# ------------------------------------------------------------------ rings (dering.c)
+def test_dering_golden(orc):
+    vol = GOLD["dering_vol"]
+    nz, ny, nx = vol.shape
+    d = clibs.Dering()
+    t = clibs.Dering()
+    assert orc.orc_dering_init(C.byref(d), nz, ny, nx, 47.5, 55.0, 4, 8) == 0
+    for slab in range(2):
+        assert orc.orc_dering_tile_init(C.byref(t), C.byref(d)) == 0
+        sub = np.ascontiguousarray(vol[4 * slab:4 * slab + 4])
+        orc.orc_dering_accumulate_u8(C.byref(t), clibs.up(sub), 0, 0, 4, ny, nx, 1)
+        orc.orc_dering_merge_tile(C.byref(d), slab, C.byref(t))
+        orc.orc_dering_free(C.byref(t))
+    nb = d.nslab * d.ns * d.nr
+    assert np.array_equal(np.ctypeslib.as_array(d.sum, (nb,)), GOLD["dering_sum"])
+    assert np.array_equal(np.ctypeslib.as_array(d.cnt, (nb,)), GOLD["dering_cnt"])
+    n = orc.orc_dering_finalize(C.byref(d), 9, 20, 1.0, 6.0)
+    assert [n, d.have_rings, d.nr, d.nslab] == list(GOLD["dering_found"])
+    ring = np.ctypeslib.as_array(d.ring, (d.nslab * d.nr,))
+    # votes and radii are integer decisions: identical; the amplitudes are fp32
+    # medians whose last bits depend on the reference build's fast-math division
+    assert np.array_equal(ring != 0, GOLD["dering_ring"] != 0)
+    assert np.allclose(ring, GOLD["dering_ring"], rtol=1e-5, atol=0)
+    f = (vol * np.float32(1 / 255)).astype(np.float32)
+    orc.orc_dering_apply(C.byref(d), clibs.fp(f), 0, 0, 0, nz, ny, nx, 1.0 / 255.0)
+    assert np.abs(f - GOLD["dering_applied"]).max() < 1e-7
+    orc.orc_dering_free(C.byref(d))
+
+
+def test_dering_counts_bruteforce(orc):
+    """Counts per (sector, radius) bin re-derived with numpy's sqrt/arctan2."""
+    vol = GOLD["dering_vol"][:2]
+    nz, ny, nx = vol.shape
+    d = clibs.Dering()
+    orc.orc_dering_init(C.byref(d), nz, ny, nx, 47.5, 55.0, 4, 8)
+    t = clibs.Dering()
+    orc.orc_dering_tile_init(C.byref(t), C.byref(d))
+    orc.orc_dering_accumulate_u8(C.byref(t), clibs.up(np.ascontiguousarray(vol)), 0, 0, nz, ny, nx, 2)
+    yy, xx = np.meshgrid(np.arange(0, ny, 2) - 47.5, np.arange(0, nx, 2) - 55.0, indexing="ij")
+    ri = np.floor(np.sqrt(yy ** 2 + xx ** 2) + 0.5).astype(int)
+    q = np.minimum(((np.arctan2(yy, xx) + np.pi) * (8 / (2 * np.pi))).astype(int), 7)
+    cnt = np.zeros(8 * d.nr, np.int64)
+    sm = np.zeros(8 * d.nr, np.int64)
+    for z in range(nz):
+        v = vol[z, ::2, ::2].astype(np.int64)
+        ok = (v > 0) & (ri < d.nr)
+        np.add.at(cnt, (q * d.nr + ri)[ok], 1)
+        np.add.at(sm, (q * d.nr + ri)[ok], v[ok])
+    assert np.array_equal(np.ctypeslib.as_array(t.cnt, (8 * d.nr,)), cnt)
+    assert np.array_equal(np.ctypeslib.as_array(t.sum, (8 * d.nr,)), sm.astype(np.float32))
+    orc.orc_dering_free(C.byref(t))
+    orc.orc_dering_free(C.byref(d))
+
+
+# ------------------------------------------------------------------ noise (noise.c)
+def test_noise_golden(orc):
+    v = GOLD["noise_in"]
+    nm = clibs.NoiseModel()
+    assert orc.orc_estimate_noise(clibs.fp(v), *v.shape, 5, 10.0, 0.5, C.byref(nm)) == 0
+    g = GOLD["noise_model"]
+    assert nm.n_bins_used == int(g[4])
+    assert np.allclose([nm.g, nm.b, nm.noise_ref], g[:3], rtol=1e-9, atol=1e-15)
+    a = GOLD["aniso_vol"]
+    rz, rxy = C.c_double(), C.c_double()
+    assert orc.orc_noise_aniso(clibs.fp(a), *a.shape, C.byref(rz), C.byref(rxy)) == 0
+    assert np.allclose([rz.value, rxy.value], GOLD["aniso_rho"], rtol=1e-9)
+    assert orc.orc_guided_eps_for_noise(0.01) == pytest.approx(9e-4)
+    assert orc.orc_guided_eps_for_noise(0.0) == 1e-8
+
+
+# ------------------------------------------------------------------ chain (pipeline.c)
+def _cfg(**kw):
+    c = clibs.PipelineCfg()
+    base = dict(delta_beta=300.0, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3, unsharp_sigma=0.8,
+                unsharp_coeff=0.5, window_lo=-0.1, window_hi=1.2, do_deconv=1, guided_eps=1.2e-3)
+    base.update(kw)
+    for k, v in base.items():
+        setattr(c, k, v)
+    return c
+
+
+def test_chain_golden(orc):
+    u8 = GOLD["chain_u8"]
+    q = np.empty((20, 22, 24), np.uint8)
+    air = C.c_int(0)
+    assert orc.orc_process_buffer(C.byref(_cfg()), clibs.up(u8), 10, 20, 30, *u8.shape, 14, 24, 34, 20, 22, 24,
+                                  300, 400, clibs.up(q), C.byref(air)) == 0
+    diff = np.abs(q.astype(int) - GOLD["chain_out"].astype(int))
+    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+
+
+CHAIN_CFGS = [dict(), dict(no_dither=1), dict(do_deconv=0), dict(guided_subsample=1),
+              dict(use_matched_deconv=1, psf_sigma_vox=1.0, psf_z_ratio=1.2),
+              dict(have_norm=1, norm_lo=30, norm_hi=210), dict(have_dec_range=1, dec_lo=0.05, dec_hi=0.8),
+              dict(have_eps_r=1, eps_fn_a=0.01, eps_fn_b=0.0001, eps_fn_med=0.012)]
+
+
+@pytest.mark.parametrize("i", range(len(CHAIN_CFGS)))
+def test_chain_vs_live_reference(orc, ref, i):
+    from fysics_amd import phantom
+    rng = np.random.default_rng(i)
+    u8 = phantom.noisy_u8(phantom.forward(phantom.sheets((26, 28, 30), rng, 5), **phantom.CONFIG1_PHYS), rng)
+    cfg = _cfg(**CHAIN_CFGS[i])
+    args = (10, 20, 30, *u8.shape, 13, 24, 34, 18, 20, 22, 300, 400)
+    a = np.empty((18, 20, 22), np.uint8)
+    b = np.empty((18, 20, 22), np.uint8)
+    air = C.c_int(0)
+    assert orc.orc_process_buffer(C.byref(cfg), clibs.up(u8), *args, clibs.up(a), C.byref(air)) == 0
+    assert ref.fy_process_buffer(C.byref(cfg), clibs.up(u8), *args, clibs.up(b), C.byref(air)) == 0
+    diff = np.abs(a.astype(int) - b.astype(int))
+    assert diff.max() <= 1 and (diff > 0).mean() < 2e-3
+    for y0, x0 in ((0, 0), (100, 350), (390, 10)):
+        assert orc.orc_eps_for_tile(C.byref(cfg), 150.0, 200.0, y0, x0, 32, 32) == \
+            ref.ref_eps_for_tile(C.byref(cfg), 150.0, 200.0, y0, x0, 32, 32)
+
+
+def test_chain_all_air(orc):
+    u8 = np.zeros((10, 10, 10), np.uint8)
+    q = np.full((8, 8, 8), 7, np.uint8)
+    air = C.c_int(0)
+    assert orc.orc_process_buffer(C.byref(_cfg()), clibs.up(u8), 0, 0, 0, 10, 10, 10, 1, 1, 1, 8, 8, 8,
+                                  10, 10, clibs.up(q), C.byref(air)) == 0
+    assert air.value == 1 and not q.any()
+
+
+@pytest.mark.parametrize("shape", [(17, 13, 29), (8, 32, 6)])
+def test_deconv_guided_vs_live_reference(orc, ref, shape):
+    rng = np.random.default_rng(9)
+    x = rng.random(shape).astype(np.float32)
+    p = clibs.physics(**PHYS_B)
+    a = np.empty_like(x); b = np.empty_like(x)
+    orc.orc_deconvolve(clibs.fp(x), clibs.fp(a), *shape, C.byref(p), 0.02)
+    ref.fy_deconvolve(clibs.fp(x), clibs.fp(b), *shape, C.byref(p), 0.02)
+    assert rel_l2(a, b) < 1e-6
+    ws = np.empty(4 * x.size, np.float32)
+    orc.orc_guided_fast_f32(clibs.fp(x), clibs.fp(a), *shape, 2, 1e-3, 3)
+    ref.fy_guided_denoise_fast_ws(clibs.fp(x), clibs.fp(b), *shape, 2, 1e-3, 3, clibs.fp(ws))
+    assert rel_l2(a, b) < 1e-6
