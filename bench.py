@@ -1,0 +1,272 @@
+#!/usr/bin/env python3
+"""Headline benchmark: Mvoxels/s of the deconvolution + guided-filter chain on a
+256^3 u8 cube, inputs and outputs resident in HBM, on one MI355X.
+
+Workload (the plain single-GPU configuration): a 256^3 synthetic scroll volume
+(fysics_amd.phantom.config2_volume: curved sheets, Paganin+unsharp forward blur
+with sigma_z/sigma_xy = 1.17, Poisson-Gaussian noise rising 3x to the rim, 20
+concentric rings, u8 through the [-0.1, 1.2] window).  One step = the whole
+cube through the device chain in fy_process_buffer order: u8 -> f01 -> ring
+subtraction -> dewindow -> Paganin+unsharp Wiener deconvolution (delta/beta
+back-off 0.35, reg 0.015, gain cap 8, anisotropic f_z scale 1.17^2) ->
+rewindow/clamp -> guided filter (r=2, s=2, radially scaled eps) -> dithered u8.
+Ring detection and the eps calibration are per-volume and run before timing.
+
+Prints one JSON line (see the keys below).  --gpus N > 1 is not supported:
+this build targets a single GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mvoxels/s deconv+guided on 256³ cubes; p50/p99 cube latency served vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(name, n, p, s=2):
+    """Algorithmic bytes one launch of `name` must move for an n^3 cube padded to
+    p^3: each distinct input element read once, each output element written once.
+    The half spectrum has H = p*p*(p/2+1) complex fp32 elements."""
+    hx = p // 2 + 1
+    N = n ** 3
+    H = p * p * hx
+    l = (n + s - 1) // s
+    Nl = l ** 3
+    table = {
+        "rfft_x_u8": N + 8 * H,                 # u8 in, half spectrum out
+        "cfft_y_fwd": 16 * H,                   # spectrum in and out
+        "zfused": 16 * H,                       # spectrum in and out (weight fused)
+        "cfft_y_inv": 16 * n * p * hx,          # planes z < n
+        "irfft_x": 8 * n * n * hx + 4 * N,      # rows y < n of planes z < n in, fp32 tile out
+        "guided_moments": 4 * Nl + 8 * Nl,      # decimated samples in, a/b out
+        "guided_box_ab": 8 * Nl + 8 * Nl,       # a/b in, mean_a/mean_b out
+        "guided_upsample": 4 * N + 8 * Nl + N,  # tile + coefficients in, u8 out
+    }
+    return table.get(name)
+
+
+def build_workload(n, seed):
+    from fysics_amd import phantom
+    t0 = time.time()
+    vol = phantom.config2_volume((n, n, n), seed=seed)
+    return vol, time.time() - t0
+
+
+def calibrate(F, vol):
+    """Per-volume calibration on the GPU: ring detection (pipeline pass 1:
+    accumulate with ss=2 over slabs of 512, finalize(15,100,0.5,6.0))."""
+    lib = F.lib()
+    nz, ny, nx = vol.shape
+    d = F.Dering()
+    t = F.Dering()
+    assert lib.fy_dering_init(C.byref(d), nz, ny, nx, -1.0, -1.0, 512, 8) == 0
+    assert lib.fy_dering_tile_init(C.byref(t), C.byref(d)) == 0
+    lib.fy_dering_accumulate_u8(C.byref(t), vol.ctypes.data_as(F.c_u8_p), 0, 0, nz, ny, nx, 2)
+    lib.fy_dering_merge_tile(C.byref(d), 0, C.byref(t))
+    found = lib.fy_dering_finalize(C.byref(d), 15, 100, 0.5, 6.0)
+    lib.fy_dering_free(C.byref(t))
+    return d, found
+
+
+def eps_for_cube(n):
+    """Radially scaled guided eps of the generator's noise model: flat-region
+    noise std fn(r) = fn0*(1 + 2 r/rmax) (rising 3x to the rim), guided_eps =
+    (4.2*fn_med)^2, per-tile scale clamp((fn(r_tile)/fn_med)^2, 0.4, 8)
+    (pipeline.c:447-456, 878-891)."""
+    fn0 = np.sqrt(0.02 * 110.0 + 4.0) / 255.0
+    rmax = np.sqrt(2) * (n - 1) / 2
+    fn = lambda r: fn0 * (1 + 2 * r / rmax)
+    fn_med = fn(0.5 * rmax)
+    guided_eps = (4.2 * fn_med) ** 2
+    s2 = min(max((fn(0.0) / fn_med) ** 2, 0.4), 8.0)      # the cube is centred on the axis
+    return guided_eps * s2
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
+              "(cubes are independent; see DESIGN.md for how the work would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import fysics_amd as F
+    torch.cuda.set_device(0)
+    lib = F.lib()
+
+    n = args.size
+    vol, gen_s = build_workload(n, args.seed)
+    ring, found = calibrate(F, vol)
+    eps = eps_for_cube(n)
+    phys = F.physics(delta_beta=300.0 * 0.35, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
+                     unsharp_sigma=0.8, unsharp_coeff=0.5)
+    prm = F.CubeParams()
+    prm.phys = phys
+    prm.deconv, prm.deconv_param, prm.deconv_cap, prm.fz2_scale = 1, 0.015, 8.0, 1.17 ** 2
+    prm.window_lo, prm.window_hi = -0.1, 1.2
+    prm.guided_eps, prm.guided_radius, prm.guided_s = eps, 2, 2
+    prm.dering = C.pointer(ring)
+
+    # a dedicated (non-null) stream: the library enqueues on it, events time it
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+    d_in = torch.from_numpy(vol).cuda()
+    d_out = torch.empty_like(d_in)
+    cube = F.Cube(d_in.data_ptr(), 0, 0, 0, n, n, n, 0, 0, 0, n, n, n, d_out.data_ptr(), None, C.pointer(prm))
+
+    def step():
+        rc = lib.fy_gpu_process_cubes(C.byref(cube), 1, sh)
+        if rc:
+            raise RuntimeError("fy_gpu_process_cubes: " + lib.fy_gpu_last_error().decode())
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # timed region: K steps, inputs resident in HBM
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1000.0 * wall / args.steps
+    mvox = n ** 3 / 1e6
+    value = mvox * args.steps / wall
+
+    # per-cube latency distribution (events around each step on the same stream)
+    lat = []
+    for _ in range(min(args.steps, 50)):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        step()
+        b.record(stream)
+        b.synchronize()
+        lat.append(a.elapsed_time(b))
+    lat.sort()
+    p50 = lat[len(lat) // 2]
+    p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]
+
+    # per-kernel HIP-event timing on the launch stream -> roofline of the dominant kernel
+    lib.fy_gpu_profile_reset()
+    lib.fy_gpu_profile(1)
+    for _ in range(min(args.steps, 20)):
+        step()
+    torch.cuda.synchronize()
+    lib.fy_gpu_profile(0)
+    prof = F.profile_read()
+    p = lib.fy_next_fft_size(n)
+    kernels = {}
+    for name, (ms, cnt) in prof.items():
+        avg_us = 1000.0 * ms / max(cnt, 1)
+        b = kernel_bytes(name, n, p)
+        kernels[name] = {"avg_us": round(avg_us, 2), "launches": cnt,
+                         "GBps": round(b / (avg_us * 1e-6) / 1e9, 1) if b else None}
+    dom = max(prof.items(), key=lambda kv: kv[1][0])[0]
+    dom_bytes = kernel_bytes(dom, n, p)
+    dom_us = kernels[dom]["avg_us"]
+    achieved = dom_bytes / (dom_us * 1e-6) / 1e9 if dom_bytes else None
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                "traffic": None, "algorithmic_bytes": dom_bytes, "avg_us": dom_us}
+
+    # plain drop-in with host buffers (PCIe-inclusive; reported beside, never `value`)
+    cfgc = F.pipeline_cfg(delta_beta=300.0, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
+                          unsharp_sigma=0.8, unsharp_coeff=0.5, window_lo=-0.1, window_hi=1.2,
+                          do_deconv=1, guided_eps=eps, have_dering=1)
+    cfgc.dering = C.pointer(ring)
+    host_out = np.empty_like(vol)
+    air = C.c_int()
+    host_t = []
+    for i in range(4):
+        t1 = time.perf_counter()
+        rc = lib.fy_process_buffer(C.byref(cfgc), vol.ctypes.data_as(F.c_u8_p), 0, 0, 0, n, n, n, 0, 0, 0,
+                                   n, n, n, n, n, host_out.ctypes.data_as(F.c_u8_p), C.byref(air))
+        host_t.append(time.perf_counter() - t1)
+        assert rc == 0
+    host_rate = mvox / statistics.median(host_t[1:])
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(vol, phys, eps, ring, args.cpu_threads)
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "Mvoxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 u8 cube, device chain: rings -> dewindow -> Paganin+unsharp Wiener "
+                               "deconv (sigma_z/sigma_xy 1.17, delta/beta x0.35, reg 0.015) -> guided r=2 s=2 "
+                               "radial eps -> dithered u8", "cube": [n, n, n], "fft": [p, p, p],
+                   "rings_detected": int(found), "guided_eps": eps, "parallelism": "single GPU"},
+        "latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4), "samples": len(lat)},
+        "roofline": roofline,
+        "kernels": kernels,
+        "host_api_mvox_s": round(host_rate, 1),
+        "cpu_baseline": cpu,
+        "workload_gen_s": round(gen_s, 1),
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline(vol, phys, eps, ring, threads):
+    """The reference's own chain (compiled from its sources, oracle/_ref) on the
+    host cores: one 256^3 cube per OpenMP thread, one warm-up round, median of 3."""
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libfysics_ref.so")
+    if not os.path.exists(ref_so):
+        return {"value": None, "note": "oracle/_ref not built on this host"}
+    import fysics_amd as F
+    ref = C.CDLL(ref_so)
+    ref.ref_chain_many.restype = C.c_double
+    ref.ref_chain_many.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_long, C.c_long,
+                                   C.c_long, C.c_long, C.POINTER(F.Physics), C.c_int, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                   C.POINTER(F.Dering), C.c_int]
+    if threads <= 0:
+        env = os.environ.get("OMP_NUM_THREADS")
+        threads = int(env) if env else len(os.sched_getaffinity(0))
+    n = vol.shape[0]
+    ins = [vol for _ in range(threads)]
+    outs = [np.empty_like(vol) for _ in range(threads)]
+    pin = (C.c_void_p * threads)(*[a.ctypes.data for a in ins])
+    pout = (C.c_void_p * threads)(*[a.ctypes.data for a in outs])
+    cpu_name = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_name = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    times = []
+    for _ in range(4):
+        t = ref.ref_chain_many(threads, pin, pout, n, n, n, 0, C.byref(phys), 1, 0.015, 8.0, 1.17 ** 2,
+                               -0.1, 1.2, eps, 2, C.byref(ring), threads)
+        if t < 0:
+            return {"value": None, "note": "reference chain failed"}
+        times.append(t)
+    med = statistics.median(times[1:])
+    return {"value": round(threads * n ** 3 / 1e6 / med, 1), "unit": "Mvoxels/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"{threads} x {n}^3 cubes per round (one per OpenMP thread), warm-up + median of 3 "
+                      f"rounds, {med:.2f} s/round; CPU: {cpu_name}"}
+
+
+if __name__ == "__main__":
+    main()

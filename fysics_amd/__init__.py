@@ -14,7 +14,8 @@ import os
 
 import numpy as np
 
-__all__ = ["lib", "Physics", "NoiseModel", "Dering", "PipelineCfg", "physics", "pipeline_cfg",
+__all__ = ["lib", "Physics", "NoiseModel", "Dering", "PipelineCfg", "CubeParams", "Cube", "physics",
+           "pipeline_cfg",
            "deconvolve", "deconvolve_matched", "guided", "guided_fast", "box_smooth", "fft1d",
            "fft3d", "process_buffer", "LIB_PATH"]
 
@@ -81,6 +82,27 @@ class PipelineCfg(C.Structure):
         ("occ_cz", C.c_long), ("occ_cy", C.c_long), ("occ_cx", C.c_long)]
 
 
+class CubeParams(C.Structure):
+    """fy_cube_params (include/fysics_gpu.h)."""
+    _fields_ = [
+        ("phys", Physics), ("deconv", C.c_int), ("deconv_param", C.c_double),
+        ("deconv_cap", C.c_double), ("fz2_scale", C.c_double), ("window_lo", C.c_double),
+        ("window_hi", C.c_double), ("have_dec_range", C.c_int), ("dec_lo", C.c_double),
+        ("dec_hi", C.c_double), ("guided_eps", C.c_double), ("guided_radius", C.c_int),
+        ("guided_s", C.c_int), ("no_dither", C.c_int), ("recenter", C.c_int), ("norm_lo", C.c_int),
+        ("norm_hi", C.c_int), ("dering", C.POINTER(Dering)), ("zdrift_factor", c_float_p),
+        ("vol_z", C.c_long)]
+
+
+class Cube(C.Structure):
+    """fy_cube (include/fysics_gpu.h): device u8 region, inner tile, outputs."""
+    _fields_ = [("d_in", C.c_void_p), ("rz0", C.c_long), ("ry0", C.c_long), ("rx0", C.c_long),
+                ("hz", C.c_long), ("hy", C.c_long), ("hx", C.c_long), ("z0", C.c_long),
+                ("y0", C.c_long), ("x0", C.c_long), ("tz", C.c_long), ("ty", C.c_long),
+                ("tx", C.c_long), ("d_out", C.c_void_p), ("d_float", C.c_void_p),
+                ("params", C.POINTER(CubeParams))]
+
+
 _P = C.POINTER(Physics)
 _VP = C.c_void_p
 _L = C.c_long
@@ -116,6 +138,9 @@ SIGNATURES = {
                                             C.c_int, C.c_double, C.c_int, c_float_p]),
     "fy_box_smooth": (None, [c_float_p, c_float_p, c_float_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "fy_guided_eps_for_noise": (C.c_double, [C.c_double]),
+    "fy_estimate_noise": (C.c_int, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                    C.c_double, C.POINTER(NoiseModel)]),
+    "fy_noise_aniso": (C.c_int, [c_float_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p]),
     "fy_zdrift_apply": (None, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int, c_float_p]),
     "fy_dering_init": (C.c_int, [C.POINTER(Dering), _L, _L, _L, C.c_double, C.c_double, C.c_int,
                                  C.c_int]),
@@ -138,6 +163,9 @@ SIGNATURES = {
                                 _VP]),
     "fy_gpu_process_buffer": (C.c_int, [C.POINTER(PipelineCfg), _VP] + [_L] * 14 +
                               [_VP, c_int_p, _VP]),
+    "fy_gpu_process_cubes": (C.c_int, [C.POINTER(Cube), C.c_int, _VP]),
+    "fy_cube_params_from_cfg": (C.c_int, [C.POINTER(PipelineCfg), _L, _L, _L, _L, _L, _L,
+                                          C.POINTER(CubeParams)]),
     "fy_gpu_sync": (C.c_int, [_VP]),
     "fy_gpu_profile": (None, [C.c_int]),
     "fy_gpu_profile_read": (C.c_int, [C.c_char_p, C.c_int, c_double_p, C.POINTER(C.c_long),

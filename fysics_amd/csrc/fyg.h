@@ -40,6 +40,10 @@ int  fyg_d2h(void *dst, const void *src, size_t bytes, fyg_stream *s);
 int  fyg_d2d(void *dst, const void *src, size_t bytes, fyg_stream *s);
 int  fyg_zero(void *dst, size_t bytes, fyg_stream *s);    /* fill kernel, graph-safe */
 int  fyg_last_error(void);                                /* sticky launch error check */
+/* per-stream all-air flag: returns a fresh generation number g >= 1 and the
+ * device flag; kernels that see a nonzero input voxel store g, consumers test
+ * (*flag == g), so no clearing launch is needed between calls */
+int  fyg_flag(fyg_stream *s, int **d_flag);
 const char *fyg_error_string(void);
 
 /* process-wide cache of small read-only device tables (twiddles, transfer
@@ -63,6 +67,7 @@ int fyg_fft_supported(int L);                   /* 2^k or 3*2^k, L <= FYG_FFT_MA
  * z-drift factor, dewindow (pipeline.c:492-494,266-268) */
 typedef struct {
     int dering; const float *ring; int nr, nslab, slab_z; double cy, cx; float scale;
+    int ring_fast;                   /* axis on the (half-)integer grid and radii < 512: float radius */
     int zdrift; const float *zfactor;
     long gz0, gy0, gx0;              /* global origin of the buffer */
     int window; float wlo, wspan;
@@ -88,7 +93,8 @@ typedef struct {
     float *out;                      /* fp32 nz*ny*nx, may alias a fp32 in */
     fyg_epilogue epi;
     void *spec;                      /* complex workspace pz*py*hxp float2 */
-    int *nonzero_flag;               /* optional: set to 1 if any input u8 is nonzero */
+    int *nonzero_flag;               /* optional: receives nonzero_gen if any input u8 is nonzero */
+    int nonzero_gen;
 } fyg_deconv_args;
 int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s);
 size_t fyg_deconv_spec_bytes(int pz, int py, int px);
@@ -108,7 +114,8 @@ typedef struct {
     long gz0, gy0, gx0;              /* global origin of the buffer (dither coordinates) */
     int no_dither;
     int recenter; float rmid, rk;    /* pipeline.c:413-421 */
-    const int *nonzero_flag;         /* optional: if it reads 0, the tile is all air -> zeros */
+    const int *nonzero_flag;         /* optional: unless it holds nonzero_gen the tile is all air -> zeros */
+    int nonzero_gen;
 } fyg_emit;
 
 typedef struct {
@@ -122,7 +129,7 @@ size_t fyg_guided_ws_bytes(int nz, int ny, int nx, int s);
 int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s);
 /* prologue-only path (deconv disabled): u8 -> f32 through the tile prologue */
 int fyg_prologue_run(const unsigned char *in, float *out, int nz, int ny, int nx,
-                     const fyg_prologue *pro, int *nonzero_flag, fyg_stream *s);
+                     const fyg_prologue *pro, int *nonzero_flag, int gen, fyg_stream *s);
 /* emit without guided filtering: recenter + quantise the inner tile */
 int fyg_emit_run(const float *f, int nz, int ny, int nx, const fyg_emit *em, fyg_stream *s);
 int fyg_box_mean_run(const float *in, float *out, int nz, int ny, int nx, int r,
@@ -137,6 +144,23 @@ int fyg_dering_apply_run(float *f, long dz, long dy, long dx, const fyg_prologue
 int fyg_dering_accum_run(const unsigned char *buf, long dz, long dy, long dx, int ss,
                          const int *map, int nbins, unsigned long long *sum,
                          unsigned int *cnt, fyg_stream *s);
+
+/* ------------------------------------------------------------------ noise model */
+/* local (2r+1)^3 box mean m and variance v (double) with sliding sums per line */
+int fyg_noise_local(const float *d_in, int nz, int ny, int nx, int r, double wc, double *m, double *v,
+                    double *t0, double *t1, double *t2, double *t3, fyg_stream *s);
+size_t fyg_sort_pairs_temp_bytes(size_t n);
+int fyg_sort_pairs_f64(const double *kin, double *kout, const double *vin, double *vout, size_t n,
+                       void *temp, size_t temp_bytes, fyg_stream *s);
+size_t fyg_segsort_temp_bytes(size_t n, int nseg);
+int fyg_segsort_f64(const double *in, double *out, size_t n, int nseg, const int *d_begin,
+                    const int *d_end, void *temp, size_t temp_bytes, fyg_stream *s);
+int fyg_lower_bounds_f64(const double *sorted, size_t n, const double *d_q, int nq, long *d_idx,
+                         fyg_stream *s);
+int fyg_gather_f64(const double *src, const long *d_idx, int n, double *d_out, fyg_stream *s);
+int fyg_aniso_blocks(const float *v, int nz, int ny, int nx, float *bvar, int *ok, fyg_stream *s);
+int fyg_aniso_lag(const float *v, int nz, int ny, int nx, float thresh, double *part, int *used,
+                  fyg_stream *s);
 
 #ifdef __cplusplus
 }

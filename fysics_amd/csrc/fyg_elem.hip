@@ -15,14 +15,16 @@ unsigned grid1d(size_t n) {
 }
 
 __global__ void k_u8_to_f32(const unsigned char *in, float *out, size_t n, float lo, float scale) {
+    #pragma clang fp contract(off)
     for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT)
-        out[i] = __fadd_rn(lo, __fmul_rn((float)in[i], scale));   // lo + u*scale, two roundings
+        out[i] = lo + (float)in[i] * scale;   // two roundings: contraction is off here
 }
 
 // (in - lo)*k + 0.5 clamped to [0,255] and truncated (stream.c:31-36,48-57)
 __global__ void k_f32_to_u8(const float *in, unsigned char *out, size_t n, float lo, float k) {
+    #pragma clang fp contract(off)
     for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
-        float v = __fadd_rn(__fmul_rn(in[i] - lo, k), 0.5f);
+        float v = (in[i] - lo) * k + 0.5f;
         out[i] = v < 0.0f ? 0 : (v > 255.0f ? 255 : (unsigned char)v);
     }
 }
@@ -33,9 +35,18 @@ __global__ void k_zdrift(float *f, size_t plane, size_t n, const float *factor) 
 }
 
 __device__ __forceinline__ float ring_sub(const fyg_prologue &P, float f, long gz, long gy, long gx) {
+    #pragma clang fp contract(off)
     if (f <= 0.0f) return f;
-    double dy = (double)gy - P.cy, dx = (double)gx - P.cx;
-    int ri = (int)(sqrt(dy * dy + dx * dx) + 0.5);
+    int ri;
+    if (P.ring_fast) {
+        float fy = (float)gy - (float)P.cy, fx = (float)gx - (float)P.cx;
+        float fy2 = fy * fy;
+        ri = (int)(sqrtf(fy2 + fx * fx) + 0.5f);
+    } else {
+        double dy = (double)gy - P.cy, dx = (double)gx - P.cx;
+        double dy2 = dy * dy;
+        ri = (int)(sqrt(dy2 + dx * dx) + 0.5);
+    }
     if (ri >= P.nr) return f;
     long slab = gz / P.slab_z;
     if (slab < 0) slab = 0;
@@ -55,7 +66,8 @@ __global__ void k_dering_apply(float *f, long dz, long dy, long dx, fyg_prologue
 }
 
 __global__ void k_prologue(const unsigned char *in, float *out, int nz, int ny, int nx, fyg_prologue P,
-                           int *flag) {
+                           int *flag, int gen) {
+    #pragma clang fp contract(off)
     size_t n = (size_t)nz * ny * nx;
     int any = 0;
     for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
@@ -70,7 +82,7 @@ __global__ void k_prologue(const unsigned char *in, float *out, int nz, int ny, 
         if (P.window) f = P.wlo + f * P.wspan;
         out[i] = f;
     }
-    if (flag && __syncthreads_or(any) && threadIdx.x == 0) *flag = 1;
+    if (flag && __syncthreads_or(any) && threadIdx.x == 0) *flag = gen;
 }
 
 // Sector/radius histogram of nonzero u8 samples on the ss-subsampled (y,x) grid.
@@ -151,10 +163,10 @@ extern "C" int fyg_dering_apply_run(float *f, long dz, long dy, long dx, const f
 }
 
 extern "C" int fyg_prologue_run(const unsigned char *in, float *out, int nz, int ny, int nx,
-                                const fyg_prologue *pro, int *flag, fyg_stream *s) {
+                                const fyg_prologue *pro, int *flag, int gen, fyg_stream *s) {
     size_t n = (size_t)nz * ny * nx;
     if (!n) return 0;
-    fyg_launch(s, "prologue", k_prologue, dim3(grid1d(n)), dim3(NT), 0, in, out, nz, ny, nx, *pro, flag);
+    fyg_launch(s, "prologue", k_prologue, dim3(grid1d(n)), dim3(NT), 0, in, out, nz, ny, nx, *pro, flag, gen);
     return fyg_last_error();
 }
 

@@ -81,7 +81,7 @@ template <int L>
 struct Shape {
     static constexpr int T = 256;
     static constexpr int NL_CM = L <= 256 ? 16 : (4096 / L);
-    static constexpr int NL_RM = (4096 / L) > 1024 ? 1024 : (4096 / L) < 1 ? 1 : (4096 / L);
+    static constexpr int NL_RM = (4096 / L) > 256 ? 256 : (4096 / L) < 1 ? 1 : (4096 / L);
     static constexpr int LD = L + (L >> 4) + 1;   // padded line pitch in LDS (float2)
 };
 
@@ -149,13 +149,7 @@ __device__ __forceinline__ void dft<8>(float2 *v, float sg) {
     }
 }
 
-// cos/sin(2*pi*m/16)
-__device__ __constant__ float k_c16[16] = {
-    1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
-    0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
-    -1.0f, -0.92387953251128675613f, -0.70710678118654752440f, -0.38268343236508977173f,
-    0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f};
-
+// W16 constants: C[m] = cos(2*pi*m/16)
 template <>
 __device__ __forceinline__ void dft<16>(float2 *v, float sg) {
     constexpr float C[16] = {
@@ -294,25 +288,42 @@ __device__ __forceinline__ void load_twiddles(float2 *tw_s, const float2 *__rest
 __device__ __forceinline__ int reflect_idx(int i, int n) { return i < n ? i : 2 * n - 1 - i; }
 
 // u8 tile prologue: f = u8/255 -> ring subtraction -> z-drift -> dewindow
-__device__ __forceinline__ float prologue(const fyg_prologue &P, unsigned char u, long gz, long gy, long gx) {
+// (pipeline.c:492-494,266-268).  ring_base = slab offset into the ring table,
+// dy2 = squared y distance of the row to the axis, zf = z-drift factor.
+__device__ __forceinline__ float prologue(const fyg_prologue &P, unsigned char u, int ring_base, double dy2,
+                                          float dy2f, float zf, long gx) {
+    #pragma clang fp contract(off)
     float f = (float)u * (1.0f / 255.0f);
     if (P.dering && f > 0.0f) {
-        double dy = (double)gy - P.cy, dx = (double)gx - P.cx;
-        int ri = (int)(sqrt(dy * dy + dx * dx) + 0.5);
+        int ri;
+        if (P.ring_fast) {
+            // exact squares of (half-)integer offsets and a radius < 512: the float
+            // result of floor(sqrt(r^2) + 0.5) equals the double one
+            float dx = (float)gx - (float)P.cx;
+            ri = (int)(sqrtf(dy2f + dx * dx) + 0.5f);
+        } else {
+            double dx = (double)gx - P.cx;
+            ri = (int)(sqrt(dy2 + dx * dx) + 0.5);
+        }
         if (ri < P.nr) {
-            long slab = gz / P.slab_z;
-            if (slab < 0) slab = 0;
-            if (slab >= P.nslab) slab = P.nslab - 1;
-            float v = f - P.scale * P.ring[slab * P.nr + ri];
+            float v = f - P.scale * P.ring[ring_base + ri];
             f = v > 0.0f ? v : 0.0f;
         }
     }
-    if (P.zdrift) f *= P.zfactor[gz];
+    f *= zf;
     if (P.window) f = P.wlo + f * P.wspan;
     return f;
 }
 
+__device__ __forceinline__ int ring_slab_base(const fyg_prologue &P, long gz) {
+    long slab = gz / P.slab_z;
+    if (slab < 0) slab = 0;
+    if (slab >= P.nslab) slab = P.nslab - 1;
+    return (int)(slab * P.nr);
+}
+
 __device__ __forceinline__ float epilogue(const fyg_epilogue &E, float v) {
+    #pragma clang fp contract(off)
     if (E.window) v = (v - E.wlo) * E.winv;
     if (E.dec) v = (v - E.dlo) * E.dinv;
     if (E.clamp01) v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
@@ -328,6 +339,18 @@ struct XFwd {
     const float2 *tw;
     fyg_prologue pro;
     int *flag;
+    int gen;
+};
+
+// per-line constants of a row-pair line, computed once per workgroup
+struct XLine {
+    long a, b;          // input offsets of rows y0 and y0+1 (b < 0: no second row)
+    long so;            // half-spectrum offset of row y0
+    double dya, dyb;    // squared distance of each row to the rotation axis
+    float dyaf, dybf;
+    float zf;           // z-drift factor of the plane
+    int ring;           // ring table offset of the plane's slab
+    int valid;
 };
 
 template <int L, bool U8>
@@ -335,52 +358,68 @@ __global__ __launch_bounds__(256) void k_rfft_x(XFwd a) {
     constexpr int NL = Shape<L>::NL_RM;
     __shared__ float2 buf[NL * Shape<L>::LD];
     __shared__ float2 tw_s[L];
+    __shared__ XLine meta[NL];
     load_twiddles<L>(tw_s, a.tw);
-    __syncthreads();
     const long g0 = (long)blockIdx.x * NL;
-    int any = 0;
-    auto first = [&](int l, int x) -> float2 {
+    for (int l = threadIdx.x; l < NL; l += Shape<L>::T) {
+        XLine m;
         long g = g0 + l;
-        if (g >= a.nlines) return make_float2(0.f, 0.f);
+        m.valid = g < a.nlines;
+        if (!m.valid) g = 0;
         int z = (int)(g / a.npairs);
         int y0 = 2 * (int)(g - (long)z * a.npairs);
-        int rz = reflect_idx(z, a.nz), rx = reflect_idx(x, a.nx);
-        int ry0 = reflect_idx(y0, a.ny);
+        int rz = reflect_idx(z, a.nz), ry0 = reflect_idx(y0, a.ny);
         bool pair = y0 + 1 < a.py;
         int ry1 = pair ? reflect_idx(y0 + 1, a.ny) : 0;
-        size_t i0 = ((size_t)rz * a.ny + ry0) * a.nx + rx;
-        size_t i1 = ((size_t)rz * a.ny + ry1) * a.nx + rx;
+        m.a = ((long)rz * a.ny + ry0) * a.nx;
+        m.b = pair ? ((long)rz * a.ny + ry1) * a.nx : -1;
+        m.so = ((long)z * a.py + y0) * a.hxp;
+        long gz = a.pro.gz0 + rz;
+        double da = (double)(a.pro.gy0 + ry0) - a.pro.cy, db = (double)(a.pro.gy0 + ry1) - a.pro.cy;
+        m.dya = da * da;
+        m.dyb = db * db;
+        float fa = (float)(a.pro.gy0 + ry0) - (float)a.pro.cy, fb = (float)(a.pro.gy0 + ry1) - (float)a.pro.cy;
+        m.dyaf = fa * fa;
+        m.dybf = fb * fb;
+        m.zf = (U8 && a.pro.zdrift) ? a.pro.zfactor[gz] : 1.0f;
+        m.ring = (U8 && a.pro.dering) ? ring_slab_base(a.pro, gz) : 0;
+        meta[l] = m;
+    }
+    __syncthreads();
+    int any = 0;
+    auto first = [&](int l, int x) -> float2 {
+        const XLine &m = meta[l];
+        if (!m.valid) return make_float2(0.f, 0.f);
+        int rx = reflect_idx(x, a.nx);
         float va, vb = 0.0f;
         if (U8) {
             const unsigned char *u = (const unsigned char *)a.in;
-            unsigned char ua = u[i0];
+            long gx = a.pro.gx0 + rx;
+            unsigned char ua = u[m.a + rx];
             any |= ua;
-            va = prologue(a.pro, ua, a.pro.gz0 + rz, a.pro.gy0 + ry0, a.pro.gx0 + rx);
-            if (pair) {
-                unsigned char ub = u[i1];
+            va = prologue(a.pro, ua, m.ring, m.dya, m.dyaf, m.zf, gx);
+            if (m.b >= 0) {
+                unsigned char ub = u[m.b + rx];
                 any |= ub;
-                vb = prologue(a.pro, ub, a.pro.gz0 + rz, a.pro.gy0 + ry1, a.pro.gx0 + rx);
+                vb = prologue(a.pro, ub, m.ring, m.dyb, m.dybf, m.zf, gx);
             }
         } else {
             const float *f = (const float *)a.in;
-            va = f[i0];
-            if (pair) vb = f[i1];
+            va = f[m.a + rx];
+            if (m.b >= 0) vb = f[m.b + rx];
         }
         return make_float2(va, vb);
     };
     run_from<L, NL, false, false, 0>(-1.0f, tw_s, buf, first, LdsWr<L, NL>{buf});
     __syncthreads();
     if (U8 && a.flag) {
-        if (__syncthreads_or(any) && threadIdx.x == 0) *a.flag = 1;
+        if (__syncthreads_or(any) && threadIdx.x == 0) *a.flag = a.gen;
     }
     // untangle A(k) = (X(k) + conj X(-k))/2, B(k) = (X(k) - conj X(-k))/(2i)
     for (int i = threadIdx.x; i < NL * a.hxp; i += Shape<L>::T) {
         int l = i / a.hxp, k = i - l * a.hxp;
-        long g = g0 + l;
-        if (g >= a.nlines) continue;
-        int z = (int)(g / a.npairs);
-        int y0 = 2 * (int)(g - (long)z * a.npairs);
-        bool pair = y0 + 1 < a.py;
+        const XLine &m = meta[l];
+        if (!m.valid) continue;
         float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
         if (k < a.hx) {
             int nk = k ? L - k : 0;
@@ -389,9 +428,8 @@ __global__ __launch_bounds__(256) void k_rfft_x(XFwd a) {
             A = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
             B = make_float2(0.5f * (p.y + q.y), 0.5f * (q.x - p.x));
         }
-        size_t row = ((size_t)z * a.py + y0) * a.hxp;
-        a.spec[row + k] = A;
-        if (pair) a.spec[row + a.hxp + k] = B;
+        a.spec[m.so + k] = A;
+        if (m.b >= 0) a.spec[m.so + a.hxp + k] = B;
     }
 }
 
@@ -446,31 +484,32 @@ __global__ __launch_bounds__(256) void k_zfused(ZFused a) {
     constexpr int NSL = PF::NS(n - 1);
     __shared__ float2 buf[NL * Shape<L>::LD];
     __shared__ float2 tw_s[L];
-    __shared__ double s_fyx[NL];
+    __shared__ double s_fy[NL], s_fx[NL];
+    __shared__ int s_ok[NL];
     load_twiddles<L>(tw_s, a.tw);
     const long c0 = (long)blockIdx.x * NL;
     if (threadIdx.x < NL) {
         long c = c0 + threadIdx.x;
         int y = (int)(c / a.hxp), k = (int)(c - (long)y * a.hxp);
-        // column validity is carried by a negative sentinel (k >= hx or past the end)
-        s_fyx[threadIdx.x] = (y < a.py && k < a.hx) ? (double)k : -1.0;
+        int ok = y < a.py && k < a.hx;     // padding columns k >= hx are neither read nor written
+        s_ok[threadIdx.x] = ok;
+        s_fy[threadIdx.x] = ok ? a.fy2[y] : 0.0;
+        s_fx[threadIdx.x] = ok ? a.fx2[k] : 0.0;
     }
     __syncthreads();
     float2 *base = a.data + c0;
-    auto valid = [&](int l) { return s_fyx[l] >= 0.0; };
+    auto valid = [&](int l) { return s_ok[l] != 0; };
     auto first = [&](int l, int pos) -> float2 {
         return valid(l) ? base[(size_t)pos * a.estride + l] : make_float2(0.f, 0.f);
     };
     auto last = [&](int l, int pos, float2 v) {
         if (valid(l)) base[(size_t)pos * a.estride + l] = v;
     };
-    // LUT weight of deconv_radial (paganin.c:231-242); z = pos
+    // LUT weight of deconv_radial (paganin.c:231-242) at z = pos:
+    // t = ((fz2[z] + fy2[y]) + fx2[k]) * us, linear interpolation between knots
     auto weight = [&](int l, int pos) -> float {
-        long c = c0 + l;
-        int y = (int)(c / a.hxp), k = (int)(c - (long)y * a.hxp);
-        if (!valid(l)) return 0.0f;
-        double fzy = a.fz2[pos] + a.fy2[y];
-        double t = (fzy + a.fx2[k]) * a.us;
+        double fzy = a.fz2[pos] + s_fy[l];
+        double t = (fzy + s_fx[l]) * a.us;
         int ti = (int)t;
         if (ti >= 4096) ti = 4095;
         float l0 = a.lut[ti], l1 = a.lut[ti + 1];
@@ -555,39 +594,48 @@ struct XInv {
     const float2 *tw;
 };
 
+struct XInvLine {
+    long so;        // half-spectrum offset of row y0
+    long oo;        // output offset of row y0
+    int pair;       // row y0+1 exists
+    int valid;
+};
+
 template <int L>
 __global__ __launch_bounds__(256) void k_irfft_x(XInv a) {
     constexpr int NL = Shape<L>::NL_RM;
     __shared__ float2 buf[NL * Shape<L>::LD];
     __shared__ float2 tw_s[L];
+    __shared__ XInvLine meta[NL];
     load_twiddles<L>(tw_s, a.tw);
-    __syncthreads();
     const long g0 = (long)blockIdx.x * NL;
-    auto rowsof = [&](int l, int &z, int &y0, bool &pair) -> bool {
+    for (int l = threadIdx.x; l < NL; l += Shape<L>::T) {
+        XInvLine m;
         long g = g0 + l;
-        if (g >= a.nlines) return false;
-        z = (int)(g / a.npairs);
-        y0 = 2 * (int)(g - (long)z * a.npairs);
-        pair = y0 + 1 < a.ny;
-        return true;
-    };
+        m.valid = g < a.nlines;
+        if (!m.valid) g = 0;
+        int z = (int)(g / a.npairs);
+        int y0 = 2 * (int)(g - (long)z * a.npairs);
+        m.pair = y0 + 1 < a.ny;
+        m.so = ((long)z * a.py + y0) * a.hxp;
+        m.oo = ((long)z * a.ny + y0) * a.nx;
+        meta[l] = m;
+    }
+    __syncthreads();
+    // L(x) = A(x) + i B(x) for x <= px/2, Hermitian extension above (paganin.c:271-280)
     auto first = [&](int l, int x) -> float2 {
-        int z, y0;
-        bool pair;
-        if (!rowsof(l, z, y0, pair)) return make_float2(0.f, 0.f);
-        int m = x < a.hx ? x : L - x;
-        size_t row = ((size_t)z * a.py + y0) * a.hxp;
-        float2 A = a.spec[row + m];
-        float2 B = pair ? a.spec[row + a.hxp + m] : make_float2(0.f, 0.f);
+        const XInvLine &m = meta[l];
+        if (!m.valid) return make_float2(0.f, 0.f);
+        int k = x < a.hx ? x : L - x;
+        float2 A = a.spec[m.so + k];
+        float2 B = m.pair ? a.spec[m.so + a.hxp + k] : make_float2(0.f, 0.f);
         return x < a.hx ? make_float2(A.x - B.y, A.y + B.x) : make_float2(A.x + B.y, B.x - A.y);
     };
     auto last = [&](int l, int x, float2 v) {
-        int z, y0;
-        bool pair;
-        if (x >= a.nx || !rowsof(l, z, y0, pair)) return;
-        size_t o = ((size_t)z * a.ny + y0) * a.nx + x;
-        a.out[o] = epilogue(a.epi, v.x * a.invn);
-        if (pair) a.out[o + a.nx] = epilogue(a.epi, v.y * a.invn);
+        const XInvLine &m = meta[l];
+        if (x >= a.nx || !m.valid) return;
+        a.out[m.oo + x] = epilogue(a.epi, v.x * a.invn);
+        if (m.pair) a.out[m.oo + a.nx + x] = epilogue(a.epi, v.y * a.invn);
     };
     run_from<L, NL, false, false, 0>(1.0f, tw_s, buf, first, last);
 }
@@ -729,7 +777,8 @@ extern "C" int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s) {
     if (!twx || !twy || !twz) return 1;
     float2 *spec = (float2 *)a->spec;
     XFwd xf{a->in, a->nz, a->ny, a->nx, a->pz, a->py, a->hx, a->hxp,
-            (long)a->pz * ((a->py + 1) / 2), (long)(a->py + 1) / 2, spec, twx, a->pro, a->nonzero_flag};
+            (long)a->pz * ((a->py + 1) / 2), (long)(a->py + 1) / 2, spec, twx, a->pro, a->nonzero_flag,
+            a->nonzero_gen};
     dispatch<DoRfftX>(a->px, s, xf, a->in_u8 != 0);
     long plane = (long)a->py * a->hxp;
     Cols yf{spec, a->hxp, a->hxp, plane, -1.0f, twy};

@@ -120,8 +120,9 @@ __device__ __forceinline__ float dither01(long z, long y, long x) {
 }
 
 __device__ __forceinline__ unsigned char quant_u8(float v, long gz, long gy, long gx, int no_dither) {
+    #pragma clang fp contract(off)
     // v*255 + d rounded as two separate operations, as written in pipeline.c:61
-    float t = __fadd_rn(__fmul_rn(v, 255.0f), no_dither ? 0.5f : dither01(gz, gy, gx));
+    float t = v * 255.0f + (no_dither ? 0.5f : dither01(gz, gy, gx));
     if (t <= 0.0f) return 0;
     int q = (int)t;
     return q > 255 ? 255 : (unsigned char)q;
@@ -129,6 +130,7 @@ __device__ __forceinline__ unsigned char quant_u8(float v, long gz, long gy, lon
 
 // write one filtered voxel (local coords z,y,x of an nz*ny*nx buffer)
 __device__ __forceinline__ void emit(const fyg_emit &E, int ny, int nx, long z, long y, long x, float v) {
+    #pragma clang fp contract(off)
     if (!E.quant) {
         E.out[((size_t)z * ny + y) * nx + x] = v;
         return;
@@ -136,7 +138,7 @@ __device__ __forceinline__ void emit(const fyg_emit &E, int ny, int nx, long z, 
     long lz = z - E.iz0, ly = y - E.iy0, lx = x - E.ix0;
     if (lz < 0 || lz >= E.tz || ly < 0 || ly >= E.ty || lx < 0 || lx >= E.tx) return;
     unsigned char q = 0;
-    if (!E.nonzero_flag || *E.nonzero_flag) {
+    if (!E.nonzero_flag || *E.nonzero_flag == E.nonzero_gen) {
         if (E.recenter) {
             if (v <= 0.0f) v = 0.0f;
             else {
@@ -234,58 +236,73 @@ __global__ __launch_bounds__(NT) void k_box1(Box1Args p) {
 struct UpArgs {
     int nz, ny, nx, lz, ly, lx, s;
     const float *ma, *mb, *in;
-    long z0, y0, x0, dz, dy, dx;   // region of output voxels to produce
+    int z0, y0, x0, dz, dy, dx;      // region of output voxels to produce
     fyg_emit em;
 };
 
+constexpr int UPX = 4;               // consecutive x outputs per thread
+
 // trilinear upsample of (mean_a, mean_b) from the decimated grid and apply
 // out = a*I + b (guided.c:188-228): u = c/s, i0 = min((int)u, l-2), f = min(u-i0, 1),
-// the four (z,y) rows blended first, then the x lerp.
+// the four (z,y) rows blended first, then the x lerp.  Block (64, 4): 4 output rows
+// of 256 x each; grid (x chunks, row groups, z).
 __global__ __launch_bounds__(NT) void k_upsample(UpArgs p) {
+    const int y = p.y0 + blockIdx.y * 4 + threadIdx.y;
+    const int z = p.z0 + blockIdx.z;
+    if (y >= p.y0 + p.dy) return;
     const float fs = (float)p.s;
-    size_t n = (size_t)p.dz * p.dy * p.dx;
-    for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
-        long x = p.x0 + (long)(i % p.dx);
-        size_t t = i / p.dx;
-        long y = p.y0 + (long)(t % p.dy);
-        long z = p.z0 + (long)(t / p.dy);
-        float uz = (float)z / fs, uy = (float)y / fs, ux = (float)x / fs;
-        int z0 = (int)uz; if (z0 > p.lz - 2) z0 = p.lz - 2;
-        int y0 = (int)uy; if (y0 > p.ly - 2) y0 = p.ly - 2;
+    float uz = (float)z / fs, uy = (float)y / fs;
+    int z0 = (int)uz; if (z0 > p.lz - 2) z0 = p.lz - 2;
+    int y0 = (int)uy; if (y0 > p.ly - 2) y0 = p.ly - 2;
+    float zf = uz - z0; if (zf > 1.0f) zf = 1.0f;
+    float yf = uy - y0; if (yf > 1.0f) yf = 1.0f;
+    const float w00 = (1 - zf) * (1 - yf), w01 = (1 - zf) * yf, w10 = zf * (1 - yf), w11 = zf * yf;
+    const size_t q00 = ((size_t)z0 * p.ly + y0) * p.lx, q01 = q00 + p.lx;
+    const size_t q10 = q00 + (size_t)p.ly * p.lx, q11 = q10 + p.lx;
+    const float *irow = p.in + ((size_t)z * p.ny + y) * p.nx;
+    const int xb = p.x0 + (blockIdx.x * 64 + threadIdx.x) * UPX;
+#pragma unroll
+    for (int j = 0; j < UPX; j++) {
+        int x = xb + j;
+        if (x >= p.x0 + p.dx) break;
+        float ux = (float)x / fs;
         int x0 = (int)ux; if (x0 > p.lx - 2) x0 = p.lx - 2;
-        float zf = uz - z0; if (zf > 1.0f) zf = 1.0f;
-        float yf = uy - y0; if (yf > 1.0f) yf = 1.0f;
         float fx = ux - x0; if (fx > 1.0f) fx = 1.0f;
         float gx = 1 - fx;
-        float w00 = (1 - zf) * (1 - yf), w01 = (1 - zf) * yf, w10 = zf * (1 - yf), w11 = zf * yf;
-        size_t q00 = ((size_t)z0 * p.ly + y0) * p.lx + x0, q01 = q00 + p.lx;
-        size_t q10 = q00 + (size_t)p.ly * p.lx, q11 = q10 + p.lx;
-        float a0 = w00 * p.ma[q00] + w01 * p.ma[q01] + w10 * p.ma[q10] + w11 * p.ma[q11];
-        float a1 = w00 * p.ma[q00 + 1] + w01 * p.ma[q01 + 1] + w10 * p.ma[q10 + 1] + w11 * p.ma[q11 + 1];
-        float b0 = w00 * p.mb[q00] + w01 * p.mb[q01] + w10 * p.mb[q10] + w11 * p.mb[q11];
-        float b1 = w00 * p.mb[q00 + 1] + w01 * p.mb[q01 + 1] + w10 * p.mb[q10 + 1] + w11 * p.mb[q11 + 1];
+        float a0 = w00 * p.ma[q00 + x0] + w01 * p.ma[q01 + x0] + w10 * p.ma[q10 + x0] + w11 * p.ma[q11 + x0];
+        float a1 = w00 * p.ma[q00 + x0 + 1] + w01 * p.ma[q01 + x0 + 1] + w10 * p.ma[q10 + x0 + 1] +
+                   w11 * p.ma[q11 + x0 + 1];
+        float b0 = w00 * p.mb[q00 + x0] + w01 * p.mb[q01 + x0] + w10 * p.mb[q10 + x0] + w11 * p.mb[q11 + x0];
+        float b1 = w00 * p.mb[q00 + x0 + 1] + w01 * p.mb[q01 + x0 + 1] + w10 * p.mb[q10 + x0 + 1] +
+                   w11 * p.mb[q11 + x0 + 1];
         float av = gx * a0 + fx * a1, bv = gx * b0 + fx * b1;
-        float I = p.in[((size_t)z * p.ny + y) * p.nx + x];
-        emit(p.em, p.ny, p.nx, z, y, x, av * I + bv);
+        emit(p.em, p.ny, p.nx, z, y, x, av * irow[x] + bv);
     }
 }
 
 struct EmitArgs {
     const float *f;
     int nz, ny, nx;
-    long z0, y0, x0, dz, dy, dx;
+    int z0, y0, x0, dz, dy, dx;
     fyg_emit em;
 };
 
 __global__ __launch_bounds__(NT) void k_emit(EmitArgs p) {
-    size_t n = (size_t)p.dz * p.dy * p.dx;
-    for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
-        long x = p.x0 + (long)(i % p.dx);
-        size_t t = i / p.dx;
-        long y = p.y0 + (long)(t % p.dy);
-        long z = p.z0 + (long)(t / p.dy);
-        emit(p.em, p.ny, p.nx, z, y, x, p.f[((size_t)z * p.ny + y) * p.nx + x]);
+    const int y = p.y0 + blockIdx.y * 4 + threadIdx.y;
+    const int z = p.z0 + blockIdx.z;
+    if (y >= p.y0 + p.dy) return;
+    const float *row = p.f + ((size_t)z * p.ny + y) * p.nx;
+    const int xb = p.x0 + (blockIdx.x * 64 + threadIdx.x) * UPX;
+#pragma unroll
+    for (int j = 0; j < UPX; j++) {
+        int x = xb + j;
+        if (x >= p.x0 + p.dx) break;
+        emit(p.em, p.ny, p.nx, z, y, x, row[x]);
     }
+}
+
+dim3 row_grid(int dx, int dy, int dz) {
+    return dim3((unsigned)((dx + 64 * UPX - 1) / (64 * UPX)), (unsigned)((dy + 3) / 4), (unsigned)dz);
 }
 
 // ---- global-memory separable fallback for radii the LDS tile does not cover
@@ -479,22 +496,21 @@ extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
     }
     UpArgs u{nz, ny, nx, lz, ly, lx, sd, MA, MB, a->in, 0, 0, 0, nz, ny, nx, a->em};
     if (a->em.quant) {
-        u.z0 = a->em.iz0; u.y0 = a->em.iy0; u.x0 = a->em.ix0;
-        u.dz = a->em.tz; u.dy = a->em.ty; u.dx = a->em.tx;
+        u.z0 = (int)a->em.iz0; u.y0 = (int)a->em.iy0; u.x0 = (int)a->em.ix0;
+        u.dz = (int)a->em.tz; u.dy = (int)a->em.ty; u.dx = (int)a->em.tx;
     }
-    size_t nout = (size_t)u.dz * u.dy * u.dx;
-    if (nout) fyg_launch(s, "guided_upsample", k_upsample, dim3(grid1d(nout)), dim3(NT), 0, u);
+    if ((size_t)u.dz * u.dy * u.dx)
+        fyg_launch(s, "guided_upsample", k_upsample, row_grid(u.dx, u.dy, u.dz), dim3(64, 4), 0, u);
     return fyg_last_error();
 }
 
 extern "C" int fyg_emit_run(const float *f, int nz, int ny, int nx, const fyg_emit *em, fyg_stream *s) {
     EmitArgs e{f, nz, ny, nx, 0, 0, 0, nz, ny, nx, *em};
     if (em->quant) {
-        e.z0 = em->iz0; e.y0 = em->iy0; e.x0 = em->ix0;
-        e.dz = em->tz; e.dy = em->ty; e.dx = em->tx;
+        e.z0 = (int)em->iz0; e.y0 = (int)em->iy0; e.x0 = (int)em->ix0;
+        e.dz = (int)em->tz; e.dy = (int)em->ty; e.dx = (int)em->tx;
     }
-    size_t n = (size_t)e.dz * e.dy * e.dx;
-    if (n) fyg_launch(s, "emit", k_emit, dim3(grid1d(n)), dim3(NT), 0, e);
+    if ((size_t)e.dz * e.dy * e.dx) fyg_launch(s, "emit", k_emit, row_grid(e.dx, e.dy, e.dz), dim3(64, 4), 0, e);
     return fyg_last_error();
 }
 

@@ -13,6 +13,8 @@ struct fyg_stream {
     size_t wcap[FYG_WS_SLOTS] = {};
     void *pin[2] = {};
     size_t pcap[2] = {};
+    int *flag = nullptr;     // all-air flag: holds the generation of the last call that saw data
+    int gen = 0;
 };
 
 namespace fyg_prof {

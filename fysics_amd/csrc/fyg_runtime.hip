@@ -137,6 +137,21 @@ extern "C" int fyg_zero(void *dst, size_t bytes, fyg_stream *s) {
     return fyg_last_error();
 }
 
+extern "C" int fyg_flag(fyg_stream *s, int **d_flag) {
+    if (!s) return 0;
+    if (!s->flag) {
+        if (note(hipMalloc((void **)&s->flag, 64), "hipMalloc(flag)")) return 0;
+        if (fyg_zero(s->flag, 64, s)) return 0;
+        s->gen = 0;
+    }
+    if (s->gen >= 0x7ffffff0) {              // wrap: clear so no stale generation can match
+        if (fyg_zero(s->flag, 64, s)) return 0;
+        s->gen = 0;
+    }
+    *d_flag = s->flag;
+    return ++s->gen;
+}
+
 extern "C" int fyg_last_error(void) {
     return note(hipGetLastError(), "kernel launch");
 }

@@ -97,7 +97,7 @@ int fy_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long t
     return 0;
 }
 
-static int chain_cube(const fy_cube *c, int *d_flag, fyg_stream *s) {
+static int chain_cube(const fy_cube *c, int *d_flag, int gen, fyg_stream *s) {
     const fy_cube_params *P = c->params;
     long rz0 = c->rz0, ry0 = c->ry0, rx0 = c->rx0, hz = c->hz, hy = c->hy, hx = c->hx;
     if (hz <= 0 || hy <= 0 || hx <= 0 || c->tz <= 0 || c->ty <= 0 || c->tx <= 0) return 0;
@@ -106,7 +106,7 @@ static int chain_cube(const fy_cube *c, int *d_flag, fyg_stream *s) {
         return 1;                                               /* inner tile must lie in the region */
     size_t hn = (size_t)hz * hy * hx;
     float *f = (float *)fyg_ws(s, WS_TILE, sizeof(float) * hn);
-    if (!f || fyg_zero(d_flag, sizeof(int), s)) return 1;
+    if (!f) return 1;
 
     /* prologue (pipeline.c:492-494 and the dewindow of :266-268) */
     fyg_prologue pro;
@@ -118,6 +118,7 @@ static int chain_cube(const fy_cube *c, int *d_flag, fyg_stream *s) {
         if (!pro.ring) return 1;
         pro.dering = 1; pro.nr = d->nr; pro.nslab = d->nslab; pro.slab_z = d->slab_z;
         pro.cy = d->cy; pro.cx = d->cx; pro.scale = (float)(1.0 / 255.0);
+        pro.ring_fast = fyh_ring_fast(d->cy, d->cx, ry0, rx0, hy, hx);
     }
     if (P->zdrift_factor) {
         if (P->vol_z < rz0 + hz || rz0 < 0) return 1;           /* factor table must cover the region */
@@ -155,9 +156,10 @@ static int chain_cube(const fy_cube *c, int *d_flag, fyg_stream *s) {
         a.epi.clamp01 = 1;
         a.spec = spec;
         a.nonzero_flag = d_flag;
+        a.nonzero_gen = gen;
         if (fyg_deconv_run(&a, s)) return 1;
     } else {
-        if (fyg_prologue_run(c->d_in, f, (int)hz, (int)hy, (int)hx, &pro, d_flag, s)) return 1;
+        if (fyg_prologue_run(c->d_in, f, (int)hz, (int)hy, (int)hx, &pro, d_flag, gen, s)) return 1;
     }
 
     fyg_emit em;
@@ -174,6 +176,7 @@ static int chain_cube(const fy_cube *c, int *d_flag, fyg_stream *s) {
         em.rk = (2.0f * 0.375f) * 255.0f / (float)(P->norm_hi - P->norm_lo);
     }
     em.nonzero_flag = d_flag;
+    em.nonzero_gen = gen;
 
     const float *src = f;
     if (P->guided_eps > 0) {                                    /* (c) */
@@ -203,20 +206,21 @@ static int chain_cube(const fy_cube *c, int *d_flag, fyg_stream *s) {
 int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream) {
     fyg_stream *s = fyg_stream_wrap(stream);
     if (!s) return 1;
-    int *dflag = (int *)fyg_ws(s, WS_FLAG, 64);
-    if (!dflag) return 1;
-    for (int i = 0; i < n; i++)
-        if (chain_cube(&cubes[i], dflag, s)) return 1;
+    for (int i = 0; i < n; i++) {
+        int *dflag = NULL;
+        int gen = fyg_flag(s, &dflag);
+        if (!gen || chain_cube(&cubes[i], dflag, gen, s)) return 1;
+    }
     return 0;
 }
 
 int fyh_chain(const fy_pipeline_cfg *cfg, const unsigned char *d_u8, long rz0, long ry0, long rx0, long hz,
               long hy, long hx, long z0, long y0, long x0, long tz, long ty, long tx, long vol_y, long vol_x,
-              unsigned char *d_out, int *d_flag, fyg_stream *s) {
+              unsigned char *d_out, int *d_flag, int gen, fyg_stream *s) {
     fy_cube_params P;
     if (fy_cube_params_from_cfg(cfg, y0, x0, ty, tx, vol_y, vol_x, &P)) return 1;
     fy_cube c = {d_u8, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, d_out, NULL, &P};
-    return chain_cube(&c, d_flag, s);
+    return chain_cube(&c, d_flag, gen, s);
 }
 
 int fy_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf, long rz0, long ry0, long rx0,
@@ -231,13 +235,15 @@ int fy_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf, lo
     if (!s) return 1;
     unsigned char *du = (unsigned char *)fyg_ws(s, WS_U8, hn);
     unsigned char *dq = (unsigned char *)fyg_ws(s, WS_OUT, tn);
-    int *dflag = (int *)fyg_ws(s, WS_FLAG, 64);
-    if (!du || !dq || !dflag) return 1;
+    int *dflag = NULL;
+    int gen = fyg_flag(s, &dflag);
+    if (!du || !dq || !gen) return 1;
     if (fyg_h2d(du, u8buf, hn, s)) return 1;
-    if (fyh_chain(cfg, du, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, dq, dflag, s)) return 1;
+    if (fyh_chain(cfg, du, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, dq, dflag, gen, s))
+        return 1;
     int flag = 0;
     if (fyg_d2h(out, dq, tn, s) || fyg_d2h(&flag, dflag, sizeof(int), s) || fyg_sync(s)) return 1;
-    if (all_air) *all_air = !flag;
+    if (all_air) *all_air = flag != gen;
     return 0;
 }
 
@@ -246,14 +252,15 @@ int fy_gpu_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *d_u8,
                           long vol_y, long vol_x, unsigned char *d_out, int *all_air, void *stream) {
     fyg_stream *s = fyg_stream_wrap(stream);
     if (!s) return 1;
-    int *dflag = (int *)fyg_ws(s, WS_FLAG, 64);
-    if (!dflag) return 1;
-    if (fyh_chain(cfg, d_u8, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, d_out, dflag, s))
+    int *dflag = NULL;
+    int gen = fyg_flag(s, &dflag);
+    if (!gen) return 1;
+    if (fyh_chain(cfg, d_u8, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, d_out, dflag, gen, s))
         return 1;
     if (all_air) {
         int flag = 0;
         if (fyg_d2h(&flag, dflag, sizeof(int), s) || fyg_sync(s)) return 1;
-        *all_air = !flag;
+        *all_air = flag != gen;
     }
     return 0;
 }

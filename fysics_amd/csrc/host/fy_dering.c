@@ -21,6 +21,21 @@
 #define M_PI 3.14159265358979323846
 #endif
 
+/* The ring radius floor(sqrt(dy^2 + dx^2) + 0.5) may be evaluated in fp32 when
+ * the axis sits on the half-integer grid (so every dy, dx and the squared radius
+ * d are exact multiples of 1/4) and every radius of the region is below 512.
+ * Then d either equals a boundary (k+1/2)^2 -- a perfect square, for which sqrt
+ * and the +0.5 are exact in both precisions -- or differs from every boundary by
+ * at least 1/4, i.e. sqrt(d) is at least 1/4/(2*512+1) away from k+1/2, far more
+ * than the fp32 rounding of sqrt(d) + 0.5 for d < 2^18. */
+int fyh_ring_fast(double cy, double cx, long y0, long x0, long dy, long dx) {
+    if (!(isfinite(cy) && isfinite(cx))) return 0;
+    if (2.0 * cy != floor(2.0 * cy) || 2.0 * cx != floor(2.0 * cx)) return 0;
+    double ay = fmax(fabs((double)y0 - cy), fabs((double)(y0 + dy - 1) - cy));
+    double ax = fmax(fabs((double)x0 - cx), fabs((double)(x0 + dx - 1) - cx));
+    return ay * ay + ax * ax < 512.0 * 512.0;
+}
+
 int fy_dering_init(fy_dering *d, long Z, long Y, long X, double cy, double cx, int slab_z, int ns) {
     memset(d, 0, sizeof(*d));
     d->Z = Z; d->Y = Y; d->X = X;
@@ -201,6 +216,7 @@ void fy_dering_apply(const fy_dering *d, float *f, long z0, long y0, long x0, lo
     pro.dering = 1; pro.ring = ring; pro.nr = d->nr; pro.nslab = d->nslab; pro.slab_z = d->slab_z;
     pro.cy = d->cy; pro.cx = d->cx; pro.scale = (float)scale;
     pro.gz0 = z0; pro.gy0 = y0; pro.gx0 = x0;
+    pro.ring_fast = fyh_ring_fast(d->cy, d->cx, y0, x0, dy, dx);
     if (fyg_h2d(df, f, sizeof(float) * n, s) || fyg_dering_apply_run(df, dz, dy, dx, &pro, s)) return;
     if (fyg_d2h(f, df, sizeof(float) * n, s)) return;
     fyg_sync(s);

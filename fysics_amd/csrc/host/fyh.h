@@ -29,6 +29,9 @@ double fyh_matched_fz2_scale(const fy_physics *p);
 /* the physics fy_process_buffer deconvolves with (pipeline.c:89-104) */
 void fyh_scaled_physics(const fy_pipeline_cfg *c, fy_physics *p);
 
+/* whether ring radii of a region may be evaluated in fp32 (see fy_dering.c) */
+int fyh_ring_fast(double cy, double cx, long y0, long x0, long dy, long dx);
+
 /* device copy of a small host array, cached by content */
 const void *fyh_const_blob(char tag, const void *data, size_t bytes);
 
@@ -36,7 +39,7 @@ const void *fyh_const_blob(char tag, const void *data, size_t bytes);
 int fyh_chain(const fy_pipeline_cfg *cfg, const unsigned char *d_u8,
               long rz0, long ry0, long rx0, long hz, long hy, long hx,
               long z0, long y0, long x0, long tz, long ty, long tx,
-              long vol_y, long vol_x, unsigned char *d_out, int *d_flag, fyg_stream *s);
+              long vol_y, long vol_x, unsigned char *d_out, int *d_flag, int gen, fyg_stream *s);
 
 /* deconvolution on device buffers (fp32 in/out) */
 int fyh_deconv_dev(const float *d_in, float *d_out, int nz, int ny, int nx, const fy_physics *p,

@@ -96,7 +96,16 @@ typedef struct {
     int    n_bins_used;
 } fy_noise_model;
 
-/* fy_estimate_noise / fy_noise_aniso: declared once their device kernels land */
+/* var = g*I + b from local (win^3) means/variances: 24 intensity bins, per-bin
+ * flat_pct percentile floor, 4-iteration IRLS fit; noise_ref = sqrt(g*ref+b)*1.15
+ * (noise.c:75-192).  0 on success. */
+int fy_estimate_noise(const float *in, int nz, int ny, int nx,
+                      int win, double flat_pct, double ref_intensity,
+                      fy_noise_model *out);
+/* lag-1 autocorrelation of the residual in flat 8^3 blocks (noise.c:210-269);
+ * 1 if fewer than 64 blocks / 32 flat blocks */
+int fy_noise_aniso(const float *v, int nz, int ny, int nx,
+                   double *rho_z, double *rho_xy);
 
 /* ---- z-drift apply (fysics.h:227; zdrift.c:64-71), used inside the tile chain ---- */
 void fy_zdrift_apply(float *chunk, int nz_slab, int ny, int nx, int z0, const float *factor);

@@ -571,15 +571,16 @@ static double orc_scaled_db(const fy_pipeline_cfg *c, fy_physics *p) {   /* pipe
     return p->delta_beta;
 }
 
-/* fy_process_buffer restricted to the hot-path stages: u8->f01, dering, z-drift,
- * (b) deconv chain, (c) guided, (d.5) recenter, quantisation of the inner tile
- * (pipeline.c:252-297,413-421,467-510).  Returns 1 for a cfg that enables the
- * air-zero or MUSICA stages. */
-int orc_process_buffer_f(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
-                         long rz0, long ry0, long rx0, long hz, long hy, long hx,
-                         long z0, long y0, long x0, long tz, long ty, long tx,
-                         long vol_y, long vol_x, unsigned char *out, int *all_air, float *ftile) {
-    if (cfg->do_air_zero || cfg->do_musica) return 1;
+/* The tile chain with explicit parameters (the fy_cube_params of
+ * fysics_gpu.h): u8->f01, rings, z-drift, dewindow, deconv_radial with the
+ * given weight/param/cap/fz2_scale, rewindow, dec-range, clamp, guided filter,
+ * recenter, dithered quantisation of the inner tile (pipeline.c:252-297,
+ * 413-421, 467-510).  ftile (optional) receives the float tile after the
+ * guided filter, before recenter and quantisation. */
+int orc_chain_cube(const fy_cube_params *P, const unsigned char *u8buf,
+                   long rz0, long ry0, long rx0, long hz, long hy, long hx,
+                   long z0, long y0, long x0, long tz, long ty, long tx,
+                   unsigned char *out, int *all_air, float *ftile) {
     size_t hn = (size_t)hz * hy * hx;
     int any = 0;
     for (size_t i = 0; i < hn && !any; i++) any = u8buf[i] != 0;
@@ -592,24 +593,22 @@ int orc_process_buffer_f(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
     float *f = malloc(sizeof(float) * hn), *b1 = malloc(sizeof(float) * hn);
     if (!f || !b1) { free(f); free(b1); return 1; }
     orc_u8_to_float(u8buf, f, hn);
-    if (cfg->have_dering && cfg->dering) orc_dering_apply(cfg->dering, f, rz0, ry0, rx0, hz, hy, hx, 1.0 / 255.0);
-    if (cfg->have_zdrift && cfg->zdrift_factor) orc_zdrift_apply(f, (int)hz, (int)hy, (int)hx, (int)rz0, cfg->zdrift_factor);
-    double ecy = cfg->dering_cy > 0 ? cfg->dering_cy : (vol_y - 1) / 2.0;
-    double ecx = cfg->dering_cx > 0 ? cfg->dering_cx : (vol_x - 1) / 2.0;
-    double eps_t = orc_eps_for_tile(cfg, ecy, ecx, y0, x0, ty, tx);
+    if (P->dering && P->dering->have_rings) orc_dering_apply(P->dering, f, rz0, ry0, rx0, hz, hy, hx, 1.0 / 255.0);
+    if (P->zdrift_factor) orc_zdrift_apply(f, (int)hz, (int)hy, (int)hx, (int)rz0, P->zdrift_factor);
     int nz = (int)hz, ny = (int)hy, nx = (int)hx;
-    if (cfg->do_deconv) {                                      /* (b) */
-        fy_physics ph; orc_scaled_db(cfg, &ph);
-        int win = cfg->window_hi > cfg->window_lo;
-        float wlo = (float)cfg->window_lo, wspan = (float)(cfg->window_hi - cfg->window_lo);
+    if (P->deconv) {                                           /* (b) */
+        int win = P->window_hi > P->window_lo;
+        float wlo = (float)P->window_lo, wspan = (float)(P->window_hi - P->window_lo);
         if (win) for (size_t i = 0; i < hn; i++) f[i] = wlo + f[i] * wspan;
-        int rc = (cfg->use_matched_deconv && cfg->psf_sigma_vox > 0)
-            ? orc_deconvolve_matched(f, b1, nz, ny, nx, &ph, cfg->deconv_tikhonov)
-            : orc_deconvolve(f, b1, nz, ny, nx, &ph, -1.0);
+        int matched = P->deconv == 2;
+        double param = P->deconv_param;
+        if (param <= 0) param = matched ? 0.05 : 0.015;
+        int rc = orc_deconv_radial_f32(f, b1, nz, ny, nx, &P->phys, param, matched, P->deconv_cap,
+                                       P->fz2_scale > 0 ? P->fz2_scale : 1.0);
         if (rc == 0) {
             if (win) { float k = 1.0f / wspan; for (size_t i = 0; i < hn; i++) b1[i] = (b1[i] - wlo) * k; }
-            if (cfg->have_dec_range && cfg->dec_hi - cfg->dec_lo > 1e-6) {
-                float lo = (float)cfg->dec_lo, k = 1.0f / (float)(cfg->dec_hi - cfg->dec_lo);
+            if (P->have_dec_range && P->dec_hi - P->dec_lo > 1e-6) {
+                float lo = (float)P->dec_lo, k = 1.0f / (float)(P->dec_hi - P->dec_lo);
                 for (size_t i = 0; i < hn; i++) b1[i] = (b1[i] - lo) * k;
             }
             for (size_t i = 0; i < hn; i++) { float v = b1[i]; f[i] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
@@ -618,14 +617,15 @@ int orc_process_buffer_f(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
             for (size_t i = 0; i < hn; i++) f[i] = (f[i] - wlo) * k;
         }
     }
-    if (cfg->guided_eps > 0 || eps_t > 0) {                    /* (c) */
-        double eps = eps_t > 0 ? eps_t : cfg->guided_eps;
-        int gs = cfg->guided_subsample == 0 ? 2 : cfg->guided_subsample;
-        if (orc_guided_fast_f32(f, b1, nz, ny, nx, 2, eps, gs) == 0) memcpy(f, b1, sizeof(float) * hn);
+    if (P->guided_eps > 0) {                                   /* (c) */
+        int gs = P->guided_s == 0 ? 2 : P->guided_s;
+        int r = P->guided_radius < 1 ? 2 : P->guided_radius;
+        if (orc_guided_fast_f32(f, b1, nz, ny, nx, r, P->guided_eps, gs) == 0) memcpy(f, b1, sizeof(float) * hn);
     }
-    if (cfg->have_norm && cfg->norm_hi > cfg->norm_lo) {        /* (d.5) */
-        float mid = 0.5f * (float)(cfg->norm_lo + cfg->norm_hi) / 255.0f;
-        float k = (2.0f * 0.375f) * 255.0f / (float)(cfg->norm_hi - cfg->norm_lo);
+    if (ftile) memcpy(ftile, f, sizeof(float) * hn);
+    if (P->recenter && P->norm_hi > P->norm_lo) {              /* (d.5) */
+        float mid = 0.5f * (float)(P->norm_lo + P->norm_hi) / 255.0f;
+        float k = (2.0f * 0.375f) * 255.0f / (float)(P->norm_hi - P->norm_lo);
         for (size_t i = 0; i < hn; i++) {
             if (f[i] <= 0.0f) { f[i] = 0.0f; continue; }
             float v = 0.5f + (f[i] - mid) * k;
@@ -636,11 +636,52 @@ int orc_process_buffer_f(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
     for (long z = iz; z < iz + tz; z++)
         for (long y = iy; y < iy + ty; y++)
             for (long x = ix; x < ix + tx; x++)
-                out[o++] = orc_quant_u8(f[((size_t)z * hy + y) * hx + x], rz0 + z, ry0 + y, rx0 + x, cfg->no_dither);
-    if (ftile) memcpy(ftile, f, sizeof(float) * hn);
+                out[o++] = orc_quant_u8(f[((size_t)z * hy + y) * hx + x], rz0 + z, ry0 + y, rx0 + x, P->no_dither);
     if (all_air) *all_air = 0;
     free(f); free(b1);
     return 0;
+}
+
+/* fy_cube_params derived from a pipeline cfg as fy_process_buffer does
+ * (pipeline.c:89-104,264-297,447-456) */
+int orc_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long ty, long tx,
+                             long vol_y, long vol_x, fy_cube_params *o) {
+    memset(o, 0, sizeof *o);
+    if (cfg->do_air_zero || cfg->do_musica) return 1;
+    orc_scaled_db(cfg, &o->phys);
+    if (cfg->do_deconv) {
+        int matched = cfg->use_matched_deconv && cfg->psf_sigma_vox > 0;
+        o->deconv = matched ? 2 : 1;
+        o->deconv_param = matched ? cfg->deconv_tikhonov : 0.015;
+        o->deconv_cap = matched ? 0.0 : ORC_MAX_GAIN;
+        o->fz2_scale = matched ? orc_matched_fz2_scale(&o->phys) : 1.0;
+        o->window_lo = cfg->window_lo; o->window_hi = cfg->window_hi;
+        o->have_dec_range = cfg->have_dec_range; o->dec_lo = cfg->dec_lo; o->dec_hi = cfg->dec_hi;
+    }
+    double ecy = cfg->dering_cy > 0 ? cfg->dering_cy : (vol_y - 1) / 2.0;
+    double ecx = cfg->dering_cx > 0 ? cfg->dering_cx : (vol_x - 1) / 2.0;
+    double eps_t = orc_eps_for_tile(cfg, ecy, ecx, y0, x0, ty, tx);
+    if (cfg->guided_eps > 0 || eps_t > 0) {
+        o->guided_eps = eps_t > 0 ? eps_t : cfg->guided_eps;
+        o->guided_radius = 2;
+        o->guided_s = cfg->guided_subsample == 0 ? 2 : cfg->guided_subsample;
+    }
+    o->no_dither = cfg->no_dither;
+    if (cfg->have_norm && cfg->norm_hi > cfg->norm_lo) { o->recenter = 1; o->norm_lo = cfg->norm_lo; o->norm_hi = cfg->norm_hi; }
+    if (cfg->have_dering && cfg->dering) o->dering = cfg->dering;
+    if (cfg->have_zdrift && cfg->zdrift_factor) { o->zdrift_factor = cfg->zdrift_factor; o->vol_z = cfg->vol_z; }
+    return 0;
+}
+
+/* fy_process_buffer restricted to the hot-path stages (pipeline.c:467-510).
+ * Returns 1 for a cfg that enables the air-zero or MUSICA stages. */
+int orc_process_buffer_f(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
+                         long rz0, long ry0, long rx0, long hz, long hy, long hx,
+                         long z0, long y0, long x0, long tz, long ty, long tx,
+                         long vol_y, long vol_x, unsigned char *out, int *all_air, float *ftile) {
+    fy_cube_params P;
+    if (orc_cube_params_from_cfg(cfg, y0, x0, ty, tx, vol_y, vol_x, &P)) return 1;
+    return orc_chain_cube(&P, u8buf, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, out, all_air, ftile);
 }
 int orc_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
                        long rz0, long ry0, long rx0, long hz, long hy, long hx,

@@ -18,7 +18,8 @@
 #define FYSICS_ORACLE_H
 
 #include <stddef.h>
-#include "../include/fysics.h"   /* struct layouts only */
+#include "../include/fysics.h"       /* struct layouts only */
+#include "../include/fysics_gpu.h"   /* fy_cube_params layout only */
 
 #ifdef __cplusplus
 extern "C" {
@@ -102,7 +103,15 @@ int    orc_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf
                           long rz0, long ry0, long rx0, long hz, long hy, long hx,
                           long z0, long y0, long x0, long tz, long ty, long tx,
                           long vol_y, long vol_x, unsigned char *out, int *all_air);
-/* same chain, also returning the float tile before quantisation (hz*hy*hx) */
+/* the chain with explicit cube parameters; ftile receives the float tile after
+ * the guided filter (before recenter and quantisation) */
+int    orc_chain_cube(const fy_cube_params *P, const unsigned char *u8buf,
+                      long rz0, long ry0, long rx0, long hz, long hy, long hx,
+                      long z0, long y0, long x0, long tz, long ty, long tx,
+                      unsigned char *out, int *all_air, float *ftile);
+int    orc_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long ty, long tx,
+                                long vol_y, long vol_x, fy_cube_params *o);
+/* same chain, also returning the float tile after the guided filter (hz*hy*hx) */
 int    orc_process_buffer_f(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
                             long rz0, long ry0, long rx0, long hz, long hy, long hx,
                             long z0, long y0, long x0, long tz, long ty, long tx,

@@ -1,0 +1,136 @@
+"""CPU-only checks of the product library: it loads, exports every function the
+headers declare with the declared struct layouts, and its host-side logic
+(sizes, transfer formulas, ring-vote finalize, cfg -> cube parameters) matches
+the oracle.  No device work is issued here."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import clibs
+import fysics_amd as F
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("fysics.h", "fysics_gpu.h")]
+
+
+def declared_functions():
+    names = []
+    for h in HEADERS:
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        src = re.sub(r"typedef struct \{.*?\}\s*\w+;", "", src, flags=re.S)
+        for m in re.finditer(r"\b(fy_\w+)\s*\(", src):
+            names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = F.lib()
+    names = declared_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python mirror binds every one of them
+    unbound = [n for n in names if n not in F.SIGNATURES]
+    assert not unbound, unbound
+
+
+def test_no_oracle_or_reference_symbols_in_product():
+    out = subprocess.run(["nm", "-D", "--defined-only", F.LIB_PATH], capture_output=True, text=True).stdout
+    assert "orc_" not in out and "ref_chain" not in out
+    deps = subprocess.run(["ldd", F.LIB_PATH], capture_output=True, text=True).stdout
+    assert "liboracle" not in deps and "libfysics_ref" not in deps
+
+
+def _c_sizeof(types, header_dir):
+    prog = "#include <stdio.h>\n#include \"fysics.h\"\nint main(void){" + "".join(
+        f'printf("%zu\\n", sizeof({t}));' for t in types) + "return 0;}"
+    exe = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"fy_sizeof_{os.getpid()}")
+    r = subprocess.run(["gcc", "-x", "c", "-I", header_dir, "-o", exe, "-"], input=prog, text=True,
+                       capture_output=True)
+    assert r.returncode == 0, r.stderr
+    vals = [int(v) for v in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
+    os.unlink(exe)
+    return vals
+
+
+STRUCTS = [("fy_physics", F.Physics), ("fy_noise_model", F.NoiseModel), ("fy_dering", F.Dering),
+           ("fy_pipeline_cfg", F.PipelineCfg)]
+
+
+def test_struct_layouts_match_ctypes():
+    sizes = _c_sizeof([s for s, _ in STRUCTS], os.path.join(ROOT, "include"))
+    assert sizes == [C.sizeof(t) for _, t in STRUCTS]
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference"), reason="reference header absent")
+def test_struct_layouts_match_reference_header():
+    ours = _c_sizeof([s for s, _ in STRUCTS], os.path.join(ROOT, "include"))
+    theirs = _c_sizeof([s for s, _ in STRUCTS], "/root/reference")
+    assert ours == theirs
+
+
+def test_sizes_and_transfers_match_oracle(fy, orc):
+    for n in list(range(1, 600)) + [1000, 2049, 4096]:
+        assert fy.fy_next_fft_size(n) == orc.orc_next_fft_size(n)
+    assert fy.fy_is_pow2(64) and not fy.fy_is_pow2(96) and fy.fy_next_pow2(97) == 128
+    for kw in (dict(), dict(delta_beta=300, energy_kev=110, distance_mm=1200, pixel_um=4.3,
+                            unsharp_sigma=0.8, unsharp_coeff=0.5),
+               dict(pixel_um=0.0024), dict(unsharp_coeff=0.0)):
+        p, q = F.physics(**kw), clibs.physics(**kw)
+        for f in np.linspace(0, 0.5, 23):
+            assert fy.fy_paganin_transfer(f, C.byref(p)) == orc.orc_paganin_transfer(f, C.byref(q))
+            assert fy.fy_unsharp_transfer(f, C.byref(p)) == orc.orc_unsharp_transfer(f, C.byref(q))
+            assert fy.fy_recon_transfer(f, C.byref(p)) == orc.orc_recon_transfer(f, C.byref(q))
+        assert fy.fy_auto_deltabeta_scale(C.byref(p)) == orc.orc_auto_deltabeta_scale(C.byref(q))
+    assert fy.fy_auto_reg(C.byref(F.physics()), -1.0) == 0.015
+    assert fy.fy_guided_eps_for_noise(0.01) == orc.orc_guided_eps_for_noise(0.01)
+    assert fy.fy_guided_eps_for_noise(float("nan")) == 1e-8
+    assert fy.fy_phys_to_u8_level(0.0, -0.03, 0.21) == orc.orc_phys_to_u8_level(0.0, -0.03, 0.21)
+    assert fy.fy_phys_to_u8_level(0.3, 0.1, 0.1) == 0.0
+
+
+def test_dering_finalize_host_logic_matches_oracle(fy, orc):
+    """Sector vote, median and clamps over the golden (sector, radius) sums."""
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "reference_vectors.npz"))
+    nz, ny, nx = gold["dering_vol"].shape
+    for args in ((9, 20, 1.0, 6.0), (15, 50, 2.0, 3.0), (3, 1, 0.2, 1.0)):
+        d = F.Dering()
+        o = clibs.Dering()
+        assert fy.fy_dering_init(C.byref(d), nz, ny, nx, 47.5, 55.0, 4, 8) == 0
+        assert orc.orc_dering_init(C.byref(o), nz, ny, nx, 47.5, 55.0, 4, 8) == 0
+        nb = d.nslab * d.ns * d.nr
+        for st in (d, o):
+            np.ctypeslib.as_array(st.sum, (nb,))[:] = gold["dering_sum"]
+            np.ctypeslib.as_array(st.cnt, (nb,))[:] = gold["dering_cnt"]
+        n1 = fy.fy_dering_finalize(C.byref(d), *args)
+        n2 = orc.orc_dering_finalize(C.byref(o), *args)
+        assert n1 == n2 and d.have_rings == o.have_rings
+        assert np.array_equal(np.ctypeslib.as_array(d.ring, (d.nslab * d.nr,)),
+                              np.ctypeslib.as_array(o.ring, (o.nslab * o.nr,)))
+        fy.fy_dering_free(C.byref(d))
+        orc.orc_dering_free(C.byref(o))
+
+
+def test_dering_tile_merge_host(fy):
+    d = F.Dering()
+    t = F.Dering()
+    assert fy.fy_dering_init(C.byref(d), 1000, 300, 200, -1, -1, 256, 6) == 0
+    assert d.nslab == 4 and d.ns == 6 and d.cy == 149.5 and d.cx == 99.5
+    assert d.nr == int(np.sqrt(149.5 ** 2 + 99.5 ** 2)) + 2
+    assert fy.fy_dering_tile_init(C.byref(t), C.byref(d)) == 0 and t.nslab == 1
+    nb = t.ns * t.nr
+    np.ctypeslib.as_array(t.sum, (nb,))[:] = 3.0
+    np.ctypeslib.as_array(t.cnt, (nb,))[:] = 2
+    fy.fy_dering_merge_tile(C.byref(d), 2, C.byref(t))
+    fy.fy_dering_merge_tile(C.byref(d), 7, C.byref(t))        # out of range: ignored
+    s = np.ctypeslib.as_array(d.sum, (d.nslab * nb,)).reshape(d.nslab, nb)
+    assert (s[2] == 3.0).all() and s[[0, 1, 3]].sum() == 0
+    fy.fy_dering_tile_reset(C.byref(t))
+    assert not np.ctypeslib.as_array(t.cnt, (nb,)).any()
+    fy.fy_dering_free(C.byref(t))
+    fy.fy_dering_free(C.byref(d))

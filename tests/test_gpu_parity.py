@@ -275,3 +275,47 @@ def test_process_buffer_rejects_unsupported_stages():
     u8 = np.full((12, 12, 12), 100, np.uint8)
     with pytest.raises(RuntimeError):
         F.process_buffer(cfg, u8, (0, 0, 0), (0, 0, 0), (12, 12, 12), (12, 12))
+
+
+@pytest.mark.parametrize("shape,sig", [((40, 40, 40), 0.05), ((48, 40, 56), 0.02), ((24, 24, 24), 0.1)])
+def test_estimate_noise_vs_oracle(orc, shape, sig):
+    """Local moments are formed in the reference's summation order and the
+    percentiles are exact order statistics, so the model matches the oracle to
+    the last bits (fp64 fit on identical inputs)."""
+    rng = np.random.default_rng(int(sig * 1000))
+    zz, yy, xx = np.meshgrid(*[np.arange(k) for k in shape], indexing="ij")
+    s = 0.5 + 0.3 * np.sin(xx * 0.10) * np.cos(yy * 0.09)
+    v = (s + sig * (0.5 + s) * rng.standard_normal(shape)).astype(np.float32)
+    a = F.NoiseModel()
+    b = clibs.NoiseModel()
+    assert F.lib().fy_estimate_noise(F._fp(v), *shape, 5, 10.0, 0.5, C.byref(a)) == 0
+    assert orc.orc_estimate_noise(clibs.fp(v), *shape, 5, 10.0, 0.5, C.byref(b)) == 0
+    assert a.n_bins_used == b.n_bins_used
+    assert np.allclose([a.g, a.b, a.noise_ref], [b.g, b.b, b.noise_ref], rtol=1e-12, atol=1e-18)
+
+
+def test_estimate_noise_golden():
+    gold = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                              "reference_vectors.npz"))
+    v = gold["noise_in"]
+    nm = F.NoiseModel()
+    assert F.lib().fy_estimate_noise(F._fp(v), *v.shape, 5, 10.0, 0.5, C.byref(nm)) == 0
+    g = gold["noise_model"]
+    assert nm.n_bins_used == int(g[4])
+    assert np.allclose([nm.g, nm.b, nm.noise_ref], g[:3], rtol=1e-9, atol=1e-15)
+
+
+@pytest.mark.parametrize("blur", [True, False])
+def test_noise_aniso_vs_oracle(orc, blur):
+    rng = np.random.default_rng(42)
+    v = (0.5 + 0.05 * (rng.random((64, 64, 64)) - 0.5)).astype(np.float32)
+    if blur:                                              # tests/test_fysics.c:257-278
+        b = v.copy()
+        b[1:-1] = 0.25 * v[:-2] + 0.5 * v[1:-1] + 0.25 * v[2:]
+        v = b
+    rz, rxy = C.c_double(), C.c_double()
+    oz, oxy = C.c_double(), C.c_double()
+    assert F.lib().fy_noise_aniso(F._fp(v), *v.shape, C.byref(rz), C.byref(rxy)) == 0
+    assert orc.orc_noise_aniso(clibs.fp(v), *v.shape, C.byref(oz), C.byref(oxy)) == 0
+    assert np.allclose([rz.value, rxy.value], [oz.value, oxy.value], rtol=1e-10)
+    assert (rz.value > rxy.value + 0.1) == blur
