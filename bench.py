@@ -49,6 +49,7 @@ def kernel_bytes(name, n, p, s=2):
         "irfft_x": 8 * n * n * hx + 4 * N,      # rows y < n of planes z < n in, fp32 tile out
         "guided_moments": 4 * Nl + 8 * Nl,      # decimated samples in, a/b out
         "guided_box_ab": 8 * Nl + 8 * Nl,       # a/b in, mean_a/mean_b out
+        "guided_coef": 4 * Nl + 8 * Nl,         # decimated samples in, mean_a/mean_b out
         "guided_upsample": 4 * N + 8 * Nl + N,  # tile + coefficients in, u8 out
     }
     return table.get(name)

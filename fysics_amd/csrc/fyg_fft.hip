@@ -256,26 +256,28 @@ struct LdsWr {
     __device__ void operator()(int l, int pos, float2 v) const { buf[l * Shape<L>::LD + lds_pos(pos)] = v; }
 };
 
-// stages [S, n) of a plan; stage 0 reads `first`, the final stage writes `last`
-template <int L, int NL, bool CM, bool REV, int S, class First, class Last>
+// stages [S, END) of a plan (END < 0: all); stage 0 reads `first`, the final
+// stage writes `last`, stages between go through LDS
+template <int L, int NL, bool CM, bool REV, int S, int END = -1, class First, class Last>
 __device__ __forceinline__ void run_from(float sg, const float2 *tw, float2 *buf, First first, Last last) {
     using P = Plan<L, REV>;
+    constexpr int E = END < 0 ? P::n : END;
     constexpr int T = Shape<L>::T;
-    if constexpr (S < P::n) {
+    if constexpr (S < E) {
         constexpr int R = P::R(S), NS = P::NS(S);
-        constexpr bool FIRST = S == 0, FINAL = S == P::n - 1;
+        constexpr bool FIRST = S == 0, FINAL = S == E - 1;
         if constexpr (FIRST && FINAL) {
             stage<L, NL, T, CM, R, NS, false>(sg, tw, first, last);
         } else if constexpr (FIRST) {
             stage<L, NL, T, CM, R, NS, false>(sg, tw, first, LdsWr<L, NL>{buf});
             __syncthreads();
-            run_from<L, NL, CM, REV, S + 1>(sg, tw, buf, first, last);
+            run_from<L, NL, CM, REV, S + 1, END>(sg, tw, buf, first, last);
         } else if constexpr (FINAL) {
             stage<L, NL, T, CM, R, NS, false>(sg, tw, LdsRd<L, NL>{buf}, last);
         } else {
             stage<L, NL, T, CM, R, NS, true>(sg, tw, LdsRd<L, NL>{buf}, LdsWr<L, NL>{buf});
             __syncthreads();
-            run_from<L, NL, CM, REV, S + 1>(sg, tw, buf, first, last);
+            run_from<L, NL, CM, REV, S + 1, END>(sg, tw, buf, first, last);
         }
     }
 }
@@ -287,20 +289,31 @@ __device__ __forceinline__ void load_twiddles(float2 *tw_s, const float2 *__rest
 
 __device__ __forceinline__ int reflect_idx(int i, int n) { return i < n ? i : 2 * n - 1 - i; }
 
+// nearest-integer radius floor(sqrt(d) + 0.5) of an exact fp32 squared radius
+// d < 2^18 that is a multiple of 1/4: a hardware sqrt estimate, then exact
+// corrections against the half-integer boundaries (k +- 1/2)^2, whose squares
+// are exact in fp32 for k < 2^11, so the result equals the double evaluation.
+__device__ __forceinline__ int ring_radius_fast(float d) {
+    #pragma clang fp contract(off)
+    int k = (int)(__builtin_amdgcn_sqrtf(d) + 0.5f);
+    float hi = (float)k + 0.5f, lo = (float)k - 0.5f;
+    if (hi * hi <= d) k++;
+    else if (k > 0 && lo * lo > d) k--;
+    return k;
+}
+
 // u8 tile prologue: f = u8/255 -> ring subtraction -> z-drift -> dewindow
 // (pipeline.c:492-494,266-268).  ring_base = slab offset into the ring table,
-// dy2 = squared y distance of the row to the axis, zf = z-drift factor.
+// dy2 = squared y distance of the row to the axis (dy2f its fp32 copy), dx2f the
+// fp32 squared x distance of the column, zf = z-drift factor.
 __device__ __forceinline__ float prologue(const fyg_prologue &P, unsigned char u, int ring_base, double dy2,
-                                          float dy2f, float zf, long gx) {
+                                          float dy2f, float dx2f, float zf, long gx) {
     #pragma clang fp contract(off)
     float f = (float)u * (1.0f / 255.0f);
     if (P.dering && f > 0.0f) {
         int ri;
         if (P.ring_fast) {
-            // exact squares of (half-)integer offsets and a radius < 512: the float
-            // result of floor(sqrt(r^2) + 0.5) equals the double one
-            float dx = (float)gx - (float)P.cx;
-            ri = (int)(sqrtf(dy2f + dx * dx) + 0.5f);
+            ri = ring_radius_fast(dy2f + dx2f);
         } else {
             double dx = (double)gx - P.cx;
             ri = (int)(sqrt(dy2 + dx * dx) + 0.5);
@@ -359,8 +372,15 @@ __global__ __launch_bounds__(256) void k_rfft_x(XFwd a) {
     __shared__ float2 buf[NL * Shape<L>::LD];
     __shared__ float2 tw_s[L];
     __shared__ XLine meta[NL];
+    __shared__ float s_dx2[U8 ? L : 1];
     load_twiddles<L>(tw_s, a.tw);
     const long g0 = (long)blockIdx.x * NL;
+    if (U8 && a.pro.dering && a.pro.ring_fast) {
+        for (int x = threadIdx.x; x < L; x += Shape<L>::T) {
+            float dx = (float)(a.pro.gx0 + reflect_idx(x, a.nx)) - (float)a.pro.cx;
+            s_dx2[x] = dx * dx;
+        }
+    }
     for (int l = threadIdx.x; l < NL; l += Shape<L>::T) {
         XLine m;
         long g = g0 + l;
@@ -395,13 +415,14 @@ __global__ __launch_bounds__(256) void k_rfft_x(XFwd a) {
         if (U8) {
             const unsigned char *u = (const unsigned char *)a.in;
             long gx = a.pro.gx0 + rx;
+            float dx2 = (a.pro.dering && a.pro.ring_fast) ? s_dx2[x] : 0.0f;
             unsigned char ua = u[m.a + rx];
             any |= ua;
-            va = prologue(a.pro, ua, m.ring, m.dya, m.dyaf, m.zf, gx);
+            va = prologue(a.pro, ua, m.ring, m.dya, m.dyaf, dx2, m.zf, gx);
             if (m.b >= 0) {
                 unsigned char ub = u[m.b + rx];
                 any |= ub;
-                vb = prologue(a.pro, ub, m.ring, m.dyb, m.dybf, m.zf, gx);
+                vb = prologue(a.pro, ub, m.ring, m.dyb, m.dybf, dx2, m.zf, gx);
             }
         } else {
             const float *f = (const float *)a.in;
@@ -416,8 +437,10 @@ __global__ __launch_bounds__(256) void k_rfft_x(XFwd a) {
         if (__syncthreads_or(any) && threadIdx.x == 0) *a.flag = a.gen;
     }
     // untangle A(k) = (X(k) + conj X(-k))/2, B(k) = (X(k) - conj X(-k))/(2i)
-    for (int i = threadIdx.x; i < NL * a.hxp; i += Shape<L>::T) {
-        int l = i / a.hxp, k = i - l * a.hxp;
+    const int dl = Shape<L>::T / a.hxp, dk = Shape<L>::T - dl * a.hxp;
+    int l = threadIdx.x / a.hxp, k = threadIdx.x - l * a.hxp;
+    for (; l < NL; l += dl, k += dk) {
+        if (k >= a.hxp) { k -= a.hxp; l++; if (l >= NL) break; }
         const XLine &m = meta[l];
         if (!m.valid) continue;
         float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
@@ -515,21 +538,52 @@ __global__ __launch_bounds__(256) void k_zfused(ZFused a) {
         float l0 = a.lut[ti], l1 = a.lut[ti + 1];
         return l0 + (l1 - l0) * (float)(t - (double)ti);
     };
-    // forward stages 0 .. n-2
+    // Forward stages 0 .. n-2 (HBM -> LDS), then one fused step: the last forward
+    // stage (radix RL, span NSL) leaves item j's outputs at positions j + r*L/RL --
+    // exactly the inputs of the first inverse stage (radix RL, span 1) for the same
+    // item -- so the weight multiply and that inverse stage run in registers and
+    // the spectrum makes a single LDS round trip around them.
+    constexpr int NB = L / RL;
+    constexpr int TOT = NL * NB;
+    constexpr int BPT = (TOT + T - 1) / T;
+    float2 v[BPT][RL];
     if constexpr (n > 1) {
-        run_from<L, NL, true, false, 0>(-1.0f, tw_s, buf, first, LdsWr<L, NL>{buf});
-        // run_from above executes every forward stage including the last one into
-        // LDS; the fused weight step below therefore needs only a radix-RL pass
+        run_from<L, NL, true, false, 0, n - 1>(-1.0f, tw_s, buf, first, LdsWr<L, NL>{buf});
+        __syncthreads();
     }
-    // The fused step: for n == 1 it reads HBM directly; otherwise the forward
-    // transform is complete in LDS (natural order) and the weight + first inverse
-    // stage (radix RL, NS = 1) read it from there.
-    {
-        constexpr int NB = L / RL;
-        constexpr int TOT = NL * NB;
-        constexpr int BPT = (TOT + T - 1) / T;
-        float2 v[BPT][RL];
-        if constexpr (n > 1) __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BPT; b++) {
+        int i = threadIdx.x + b * T;
+        if (TOT % T == 0 || i < TOT) {
+            int l, j;
+            item_of<NL, NB, true>(i, l, j);
+#pragma unroll
+            for (int r = 0; r < RL; r++) {
+                int pos = j + r * NB;
+                if constexpr (n > 1) v[b][r] = buf[l * Shape<L>::LD + lds_pos(pos)];
+                else v[b][r] = first(l, pos);
+            }
+            if constexpr (NSL > 1) {
+                constexpr int TWS = L / (NSL * RL);
+                const int k = j % NSL;
+#pragma unroll
+                for (int r = 1; r < RL; r++) {
+                    float2 w = tw_s[r * k * TWS];
+                    v[b][r] = f2tw(v[b][r], w.x, w.y, -1.0f);
+                }
+            }
+            dft<RL>(v[b], -1.0f);
+#pragma unroll
+            for (int r = 0; r < RL; r++) {
+                float w = weight(l, j + r * NB);
+                v[b][r].x *= w;
+                v[b][r].y *= w;
+            }
+            dft<RL>(v[b], 1.0f);
+        }
+    }
+    if constexpr (n > 1) {
+        __syncthreads();
 #pragma unroll
         for (int b = 0; b < BPT; b++) {
             int i = threadIdx.x + b * T;
@@ -537,51 +591,23 @@ __global__ __launch_bounds__(256) void k_zfused(ZFused a) {
                 int l, j;
                 item_of<NL, NB, true>(i, l, j);
 #pragma unroll
-                for (int r = 0; r < RL; r++) {
-                    int pos = j + r * NB;
-                    float2 x;
-                    if constexpr (n > 1) x = buf[l * Shape<L>::LD + lds_pos(pos)];
-                    else x = first(l, pos);
-                    v[b][r] = x;
-                }
-                if constexpr (n == 1) dft<RL>(v[b], -1.0f);
-#pragma unroll
-                for (int r = 0; r < RL; r++) {
-                    float w = weight(l, j + r * NB);
-                    v[b][r].x *= w;
-                    v[b][r].y *= w;
-                }
-                dft<RL>(v[b], 1.0f);
+                for (int r = 0; r < RL; r++) buf[l * Shape<L>::LD + lds_pos(j * RL + r)] = v[b][r];
             }
         }
-        if constexpr (n > 1) {
-            __syncthreads();
+        __syncthreads();
+        run_from<L, NL, true, true, 1>(1.0f, tw_s, buf, first, last);
+    } else {
 #pragma unroll
-            for (int b = 0; b < BPT; b++) {
-                int i = threadIdx.x + b * T;
-                if (TOT % T == 0 || i < TOT) {
-                    int l, j;
-                    item_of<NL, NB, true>(i, l, j);
+        for (int b = 0; b < BPT; b++) {
+            int i = threadIdx.x + b * T;
+            if (TOT % T == 0 || i < TOT) {
+                int l, j;
+                item_of<NL, NB, true>(i, l, j);
 #pragma unroll
-                    for (int r = 0; r < RL; r++) buf[l * Shape<L>::LD + lds_pos(j * RL + r)] = v[b][r];
-                }
-            }
-            __syncthreads();
-            run_from<L, NL, true, true, 1>(1.0f, tw_s, buf, first, last);
-        } else {
-#pragma unroll
-            for (int b = 0; b < BPT; b++) {
-                int i = threadIdx.x + b * T;
-                if (TOT % T == 0 || i < TOT) {
-                    int l, j;
-                    item_of<NL, NB, true>(i, l, j);
-#pragma unroll
-                    for (int r = 0; r < RL; r++) last(l, j * RL + r, v[b][r]);
-                }
+                for (int r = 0; r < RL; r++) last(l, j * RL + r, v[b][r]);
             }
         }
     }
-    (void)NSL;
 }
 
 struct XInv {

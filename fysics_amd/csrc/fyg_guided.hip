@@ -128,27 +128,50 @@ __device__ __forceinline__ unsigned char quant_u8(float v, long gz, long gy, lon
     return q > 255 ? 255 : (unsigned char)q;
 }
 
+// quantised output of one voxel at local (z,y,x): recenter, dither, all-air
+__device__ __forceinline__ unsigned char emit_q(const fyg_emit &E, bool keep, long z, long y, long x, float v) {
+    #pragma clang fp contract(off)
+    if (!keep) return 0;
+    if (E.recenter) {
+        if (v <= 0.0f) v = 0.0f;
+        else {
+            float w = 0.5f + (v - E.rmid) * E.rk;
+            v = w < 0.0f ? 0.0f : (w > 1.0f ? 1.0f : w);
+        }
+    }
+    return quant_u8(v, E.gz0 + z, E.gy0 + y, E.gx0 + x, E.no_dither);
+}
+
+__device__ __forceinline__ bool emit_keep(const fyg_emit &E) {
+    return !E.nonzero_flag || *E.nonzero_flag == E.nonzero_gen;
+}
+
 // write one filtered voxel (local coords z,y,x of an nz*ny*nx buffer)
 __device__ __forceinline__ void emit(const fyg_emit &E, int ny, int nx, long z, long y, long x, float v) {
-    #pragma clang fp contract(off)
     if (!E.quant) {
         E.out[((size_t)z * ny + y) * nx + x] = v;
         return;
     }
     long lz = z - E.iz0, ly = y - E.iy0, lx = x - E.ix0;
     if (lz < 0 || lz >= E.tz || ly < 0 || ly >= E.ty || lx < 0 || lx >= E.tx) return;
-    unsigned char q = 0;
-    if (!E.nonzero_flag || *E.nonzero_flag == E.nonzero_gen) {
-        if (E.recenter) {
-            if (v <= 0.0f) v = 0.0f;
-            else {
-                float w = 0.5f + (v - E.rmid) * E.rk;
-                v = w < 0.0f ? 0.0f : (w > 1.0f ? 1.0f : w);
-            }
-        }
-        q = quant_u8(v, E.gz0 + z, E.gy0 + y, E.gx0 + x, E.no_dither);
+    E.q[((size_t)lz * E.ty + ly) * E.tx + lx] = emit_q(E, emit_keep(E), z, y, x, v);
+}
+
+// write 4 consecutive voxels x..x+3 of one row (all inside the region)
+__device__ __forceinline__ void emit4(const fyg_emit &E, int ny, int nx, long z, long y, long x, const float *v,
+                                      bool vec, bool keep) {
+    if (!E.quant) {
+        float *o = E.out + ((size_t)z * ny + y) * nx + x;
+        if (vec) *(float4 *)o = make_float4(v[0], v[1], v[2], v[3]);
+        else for (int j = 0; j < 4; j++) o[j] = v[j];
+        return;
     }
-    E.q[((size_t)lz * E.ty + ly) * E.tx + lx] = q;
+    long lz = z - E.iz0, ly = y - E.iy0, lx = x - E.ix0;
+    unsigned char *o = E.q + ((size_t)lz * E.ty + ly) * E.tx + lx;
+    unsigned char q[4];
+    for (int j = 0; j < 4; j++) q[j] = emit_q(E, keep, z, y, x + j, v[j]);
+    if (vec) *(uchar4 *)o = make_uchar4(q[0], q[1], q[2], q[3]);
+    else for (int j = 0; j < 4; j++) o[j] = q[j];
 }
 
 // ------------------------------------------------------------------ kernels
@@ -216,6 +239,136 @@ __global__ __launch_bounds__(NT) void k_box2(Box2Args p) {
     box_tile<RHO, 2>(g, sm, load, out);
 }
 
+// ---- fused low-res coefficient field of the fast path (guided.c:163-182):
+// one workgroup produces mean_a, mean_b for a 32x8x4 tile of the decimated grid.
+// It loads the decimated, tile-centred samples with a 2*rho halo, forms
+// box(I-c0), box((I-c0)^2) -> a, b on the tile plus a rho halo, then box(a),
+// box(b) on the tile -- all in LDS, so a and b never reach HBM.
+template <int RHO>
+struct CoefTile {
+    static constexpr int X2 = TX + 4 * RHO, Y2 = TY + 4 * RHO, Z2 = TZ + 4 * RHO;   // samples
+    static constexpr int X1 = TX + 2 * RHO, Y1 = TY + 2 * RHO, Z1 = TZ + 2 * RHO;   // a, b
+    static constexpr int P0 = 2 * Z2 * Y2 * X1;     // largest intermediate
+    static constexpr int P1 = (2 * Z2 * Y1 * X1 > Z2 * Y2 * X2) ? 2 * Z2 * Y1 * X1 : Z2 * Y2 * X2;
+    static constexpr size_t bytes = sizeof(float) * (P0 + P1);
+};
+
+struct CoefArgs {
+    Grid g;                 // decimated grid; source rows are s*y, planes s*z, columns s*x
+    const float *in;
+    double eps;
+    float *ma, *mb;
+};
+
+template <int RHO>
+__global__ __launch_bounds__(NT) void k_coef(CoefArgs p) {
+    using CT = CoefTile<RHO>;
+    extern __shared__ float sm[];
+    float *P0 = sm, *P1 = sm + CT::P0;
+    const Grid &g = p.g;
+    const int bx = blockIdx.x * TX, by = blockIdx.y * TY, bz = blockIdx.z * TZ;
+    const size_t sz = (size_t)g.s * g.fny * g.fnx, sy = (size_t)g.s * g.fnx;
+    const float c0 = p.in[(size_t)bz * sz + (size_t)by * sy + (size_t)bx * g.s];
+    // 1. centred samples over the tile + 2*rho halo (zero outside the grid)
+    for (int i = threadIdx.x; i < CT::Z2 * CT::Y2 * CT::X2; i += NT) {
+        int ex = i % CT::X2, t = i / CT::X2, ey = t % CT::Y2, ez = t / CT::Y2;
+        int gx = bx + ex - 2 * RHO, gy = by + ey - 2 * RHO, gz = bz + ez - 2 * RHO;
+        float v = 0.0f;
+        if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz)
+            v = p.in[(size_t)gz * sz + (size_t)gy * sy + (size_t)gx * g.s] - c0;
+        P1[i] = v;
+    }
+    __syncthreads();
+    // 2. x sums of v and v^2: [Z2][Y2][X1] x 2
+    constexpr int NX = CT::Z2 * CT::Y2 * CT::X1;
+    for (int i = threadIdx.x; i < NX; i += NT) {
+        int x = i % CT::X1, row = i / CT::X1;
+        const float *a = P1 + row * CT::X2 + x;
+        float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d <= 2 * RHO; d++) { s1 += a[d]; s2 += a[d] * a[d]; }
+        P0[i] = s1;
+        P0[NX + i] = s2;
+    }
+    __syncthreads();
+    // 3. y sums: [Z2][Y1][X1] x 2
+    constexpr int NY = CT::Z2 * CT::Y1 * CT::X1;
+    for (int i = threadIdx.x; i < NY; i += NT) {
+        int x = i % CT::X1, t = i / CT::X1, y = t % CT::Y1, z = t / CT::Y1;
+        const float *a = P0 + (z * CT::Y2 + y) * CT::X1 + x;
+        float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d <= 2 * RHO; d++) { s1 += a[d * CT::X1]; s2 += a[NX + d * CT::X1]; }
+        P1[i] = s1;
+        P1[NY + i] = s2;
+    }
+    __syncthreads();
+    // 4. z sums -> a, b on the tile + rho halo (zero outside the grid): [Z1][Y1][X1] x 2
+    constexpr int NA = CT::Z1 * CT::Y1 * CT::X1;
+    const float epsf = (float)p.eps;
+    for (int i = threadIdx.x; i < NA; i += NT) {
+        int x = i % CT::X1, t = i / CT::X1, y = t % CT::Y1, z = t / CT::Y1;
+        int gx = bx + x - RHO, gy = by + y - RHO, gz = bz + z - RHO;
+        float av = 0.0f, bv = 0.0f;
+        if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz) {
+            const float *a = P1 + (z * CT::Y1 + y) * CT::X1 + x;
+            float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+            for (int d = 0; d <= 2 * RHO; d++) {
+                s1 += a[d * CT::Y1 * CT::X1];
+                s2 += a[NY + d * CT::Y1 * CT::X1];
+            }
+            float rc = count_recip(gx, g.nx, RHO) * count_recip(gy, g.ny, RHO) * count_recip(gz, g.nz, RHO);
+            float m = s1 * rc, q = s2 * rc;
+            float var = q - m * m;
+            if (var < 0.0f) var = 0.0f;
+            av = var / (var + epsf);
+            bv = (m + c0) * (1.0f - av);
+        }
+        P0[i] = av;
+        P0[NA + i] = bv;
+    }
+    __syncthreads();
+    // 5. x sums of a, b: [Z1][Y1][TX] x 2
+    constexpr int NB2 = CT::Z1 * CT::Y1 * TX;
+    for (int i = threadIdx.x; i < NB2; i += NT) {
+        int x = i % TX, row = i / TX;
+        const float *a = P0 + row * CT::X1 + x;
+        float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d <= 2 * RHO; d++) { s1 += a[d]; s2 += a[NA + d]; }
+        P1[i] = s1;
+        P1[NB2 + i] = s2;
+    }
+    __syncthreads();
+    // 6. y sums: [Z1][TY][TX] x 2
+    constexpr int NC = CT::Z1 * TY * TX;
+    for (int i = threadIdx.x; i < NC; i += NT) {
+        int x = i % TX, t = i / TX, y = t % TY, z = t / TY;
+        const float *a = P1 + (z * CT::Y1 + y) * TX + x;
+        float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d <= 2 * RHO; d++) { s1 += a[d * TX]; s2 += a[NB2 + d * TX]; }
+        P0[i] = s1;
+        P0[NC + i] = s2;
+    }
+    __syncthreads();
+    // 7. z sums -> mean_a, mean_b of the tile
+    for (int i = threadIdx.x; i < TZ * TY * TX; i += NT) {
+        int x = i % TX, t = i / TX, y = t % TY, z = t / TY;
+        int gx = bx + x, gy = by + y, gz = bz + z;
+        if (gx >= g.nx || gy >= g.ny || gz >= g.nz) continue;
+        const float *a = P0 + (z * TY + y) * TX + x;
+        float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d <= 2 * RHO; d++) { s1 += a[d * TY * TX]; s2 += a[NC + d * TY * TX]; }
+        float rc = count_recip(gx, g.nx, RHO) * count_recip(gy, g.ny, RHO) * count_recip(gz, g.nz, RHO);
+        size_t o = ((size_t)gz * g.ny + gy) * g.nx + gx;
+        p.ma[o] = s1 * rc;
+        p.mb[o] = s2 * rc;
+    }
+}
+
 struct Box1Args {
     Grid g;
     const float *in;
@@ -237,46 +390,73 @@ struct UpArgs {
     int nz, ny, nx, lz, ly, lx, s;
     const float *ma, *mb, *in;
     int z0, y0, x0, dz, dy, dx;      // region of output voxels to produce
+    int vin, vout;                   // 16-byte input rows / aligned 4-voxel output groups
     fyg_emit em;
 };
 
 constexpr int UPX = 4;               // consecutive x outputs per thread
+constexpr int UPK = 64 * UPX / 2 + 4;  // low-res columns one 256-wide chunk touches (s >= 2)
 
 // trilinear upsample of (mean_a, mean_b) from the decimated grid and apply
-// out = a*I + b (guided.c:188-228): u = c/s, i0 = min((int)u, l-2), f = min(u-i0, 1),
-// the four (z,y) rows blended first, then the x lerp.  Block (64, 4): 4 output rows
-// of 256 x each; grid (x chunks, row groups, z).
+// out = a*I + b (guided.c:188-228): u = c/s, i0 = min((int)u, l-2), f = min(u-i0, 1).
+// As in the reference, the four (z,y) low-res rows are blended once per output
+// row -- here into LDS, shared by the 64 threads of the row -- and each voxel
+// then needs only the x lerp.  Block (64, 4): 4 rows x 256 voxels.
 __global__ __launch_bounds__(NT) void k_upsample(UpArgs p) {
-    const int y = p.y0 + blockIdx.y * 4 + threadIdx.y;
+    __shared__ float sa[4][UPK], sb[4][UPK];
+    const int ty = threadIdx.y;
+    const int y = p.y0 + blockIdx.y * 4 + ty;
     const int z = p.z0 + blockIdx.z;
-    if (y >= p.y0 + p.dy) return;
+    const bool row_ok = y < p.y0 + p.dy;
     const float fs = (float)p.s;
-    float uz = (float)z / fs, uy = (float)y / fs;
-    int z0 = (int)uz; if (z0 > p.lz - 2) z0 = p.lz - 2;
-    int y0 = (int)uy; if (y0 > p.ly - 2) y0 = p.ly - 2;
-    float zf = uz - z0; if (zf > 1.0f) zf = 1.0f;
-    float yf = uy - y0; if (yf > 1.0f) yf = 1.0f;
-    const float w00 = (1 - zf) * (1 - yf), w01 = (1 - zf) * yf, w10 = zf * (1 - yf), w11 = zf * yf;
-    const size_t q00 = ((size_t)z0 * p.ly + y0) * p.lx, q01 = q00 + p.lx;
-    const size_t q10 = q00 + (size_t)p.ly * p.lx, q11 = q10 + p.lx;
+    const int xc0 = p.x0 + blockIdx.x * 64 * UPX;
+    const int xc1 = min(xc0 + 64 * UPX, p.x0 + p.dx) - 1;
+    int kb = (int)((float)xc0 / fs); if (kb > p.lx - 2) kb = p.lx - 2;
+    int ke = (int)((float)xc1 / fs); if (ke > p.lx - 2) ke = p.lx - 2;
+    const int nk = ke - kb + 2;
+    if (row_ok) {
+        float uz = (float)z / fs, uy = (float)y / fs;
+        int z0 = (int)uz; if (z0 > p.lz - 2) z0 = p.lz - 2;
+        int y0 = (int)uy; if (y0 > p.ly - 2) y0 = p.ly - 2;
+        float zf = uz - z0; if (zf > 1.0f) zf = 1.0f;
+        float yf = uy - y0; if (yf > 1.0f) yf = 1.0f;
+        const float w00 = (1 - zf) * (1 - yf), w01 = (1 - zf) * yf, w10 = zf * (1 - yf), w11 = zf * yf;
+        const size_t q00 = ((size_t)z0 * p.ly + y0) * p.lx + kb, q01 = q00 + p.lx;
+        const size_t q10 = q00 + (size_t)p.ly * p.lx, q11 = q10 + p.lx;
+        for (int k = threadIdx.x; k < nk; k += 64) {
+            sa[ty][k] = w00 * p.ma[q00 + k] + w01 * p.ma[q01 + k] + w10 * p.ma[q10 + k] + w11 * p.ma[q11 + k];
+            sb[ty][k] = w00 * p.mb[q00 + k] + w01 * p.mb[q01 + k] + w10 * p.mb[q10 + k] + w11 * p.mb[q11 + k];
+        }
+    }
+    __syncthreads();
+    if (!row_ok) return;
+    const int xb = xc0 + threadIdx.x * UPX;
+    if (xb > xc1) return;
     const float *irow = p.in + ((size_t)z * p.ny + y) * p.nx;
-    const int xb = p.x0 + (blockIdx.x * 64 + threadIdx.x) * UPX;
+    const bool full = xb + UPX - 1 <= xc1;
+    float iv[UPX], ov[UPX];
+    if (full && p.vin) {
+        float4 t = *(const float4 *)(irow + xb);
+        iv[0] = t.x; iv[1] = t.y; iv[2] = t.z; iv[3] = t.w;
+    } else {
+        for (int j = 0; j < UPX; j++) iv[j] = xb + j <= xc1 ? irow[xb + j] : 0.0f;
+    }
 #pragma unroll
     for (int j = 0; j < UPX; j++) {
         int x = xb + j;
-        if (x >= p.x0 + p.dx) break;
         float ux = (float)x / fs;
         int x0 = (int)ux; if (x0 > p.lx - 2) x0 = p.lx - 2;
         float fx = ux - x0; if (fx > 1.0f) fx = 1.0f;
         float gx = 1 - fx;
-        float a0 = w00 * p.ma[q00 + x0] + w01 * p.ma[q01 + x0] + w10 * p.ma[q10 + x0] + w11 * p.ma[q11 + x0];
-        float a1 = w00 * p.ma[q00 + x0 + 1] + w01 * p.ma[q01 + x0 + 1] + w10 * p.ma[q10 + x0 + 1] +
-                   w11 * p.ma[q11 + x0 + 1];
-        float b0 = w00 * p.mb[q00 + x0] + w01 * p.mb[q01 + x0] + w10 * p.mb[q10 + x0] + w11 * p.mb[q11 + x0];
-        float b1 = w00 * p.mb[q00 + x0 + 1] + w01 * p.mb[q01 + x0 + 1] + w10 * p.mb[q10 + x0 + 1] +
-                   w11 * p.mb[q11 + x0 + 1];
-        float av = gx * a0 + fx * a1, bv = gx * b0 + fx * b1;
-        emit(p.em, p.ny, p.nx, z, y, x, av * irow[x] + bv);
+        int k = x0 - kb;
+        float av = gx * sa[ty][k] + fx * sa[ty][k + 1], bv = gx * sb[ty][k] + fx * sb[ty][k + 1];
+        ov[j] = av * iv[j] + bv;
+    }
+    const bool keep = emit_keep(p.em);
+    if (full) {
+        emit4(p.em, p.ny, p.nx, z, y, xb, ov, p.vout != 0, keep);
+    } else {
+        for (int j = 0; j < UPX && xb + j <= xc1; j++) emit(p.em, p.ny, p.nx, z, y, xb + j, ov[j]);
     }
 }
 
@@ -477,7 +657,10 @@ extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
     if (rl < 1) rl = 1;
     float *A = ws, *B = ws + nl, *MA = ws + 2 * nl, *MB = ws + 3 * nl;
     Grid gl{lz, ly, lx, sd, ny, nx};
-    if (rl <= kMaxTileRho) {
+    if (rl == 1) {
+        CoefArgs c{gl, a->in, a->eps, MA, MB};
+        fyg_launch(s, "guided_coef", k_coef<1>, tile_grid(gl), dim3(NT), CoefTile<1>::bytes, c);
+    } else if (rl <= kMaxTileRho) {
         MomArgs m{gl, a->in, a->eps, A, B};
         moments(s, m, rl);
         Box2Args b{gl, A, B, 0, MA, MB, nullptr, a->em};
@@ -494,11 +677,14 @@ extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
         box_global(s, gl, rl, 2, A, B, O0, O1, T0, T1);
         if (fyg_d2d(MA, O0, sizeof(float) * nl, s) || fyg_d2d(MB, O1, sizeof(float) * nl, s)) return 1;
     }
-    UpArgs u{nz, ny, nx, lz, ly, lx, sd, MA, MB, a->in, 0, 0, 0, nz, ny, nx, a->em};
+    UpArgs u{nz, ny, nx, lz, ly, lx, sd, MA, MB, a->in, 0, 0, 0, nz, ny, nx, 0, 0, a->em};
     if (a->em.quant) {
         u.z0 = (int)a->em.iz0; u.y0 = (int)a->em.iy0; u.x0 = (int)a->em.ix0;
         u.dz = (int)a->em.tz; u.dy = (int)a->em.ty; u.dx = (int)a->em.tx;
     }
+    u.vin = (nx % 4 == 0) && (u.x0 % 4 == 0) && ((uintptr_t)a->in % 16 == 0);
+    u.vout = a->em.quant ? (a->em.tx % 4 == 0 && (uintptr_t)a->em.q % 4 == 0)
+                         : (u.vin && (uintptr_t)a->em.out % 16 == 0);
     if ((size_t)u.dz * u.dy * u.dx)
         fyg_launch(s, "guided_upsample", k_upsample, row_grid(u.dx, u.dy, u.dz), dim3(64, 4), 0, u);
     return fyg_last_error();
