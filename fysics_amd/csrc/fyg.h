@@ -71,7 +71,12 @@ typedef struct {
     int zdrift; const float *zfactor;
     long gz0, gy0, gx0;              /* global origin of the buffer */
     int window; float wlo, wspan;
+    /* deconvolution input only: workspace of fyg_ring_image_bytes() for the
+     * per-slab ring correction image scale*ring[slab][radius(y,x)] */
+    float *cimg;
 } fyg_prologue;
+
+size_t fyg_ring_image_bytes(const fyg_prologue *p, int nz, int ny, int nx);
 
 /* epilogue applied to every deconvolved voxel before it is stored
  * (pipeline.c:273-283): rewindow, dec-range rescale, clamp to [0,1] */

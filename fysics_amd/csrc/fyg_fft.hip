@@ -289,50 +289,44 @@ __device__ __forceinline__ void load_twiddles(float2 *tw_s, const float2 *__rest
 
 __device__ __forceinline__ int reflect_idx(int i, int n) { return i < n ? i : 2 * n - 1 - i; }
 
-// nearest-integer radius floor(sqrt(d) + 0.5) of an exact fp32 squared radius
-// d < 2^18 that is a multiple of 1/4: a hardware sqrt estimate, then exact
-// corrections against the half-integer boundaries (k +- 1/2)^2, whose squares
-// are exact in fp32 for k < 2^11, so the result equals the double evaluation.
-__device__ __forceinline__ int ring_radius_fast(float d) {
-    #pragma clang fp contract(off)
-    int k = (int)(__builtin_amdgcn_sqrtf(d) + 0.5f);
-    float hi = (float)k + 0.5f, lo = (float)k - 0.5f;
-    if (hi * hi <= d) k++;
-    else if (k > 0 && lo * lo > d) k--;
-    return k;
-}
-
 // u8 tile prologue: f = u8/255 -> ring subtraction -> z-drift -> dewindow
-// (pipeline.c:492-494,266-268).  ring_base = slab offset into the ring table,
-// dy2 = squared y distance of the row to the axis (dy2f its fp32 copy), dx2f the
-// fp32 squared x distance of the column, zf = z-drift factor.
-__device__ __forceinline__ float prologue(const fyg_prologue &P, unsigned char u, int ring_base, double dy2,
-                                          float dy2f, float dx2f, float zf, long gx) {
+// (pipeline.c:492-494,266-268).  c = scale*ring[slab][radius] comes from the
+// ring correction image (0 without rings); the subtraction applies to f > 0
+// only and clamps at 0 (dering.c:226-230).  Select form, so the unrolled gather
+// keeps all of its loads in flight.
+__device__ __forceinline__ float prologue(const fyg_prologue &P, unsigned char u, float c, float zf) {
     #pragma clang fp contract(off)
     float f = (float)u * (1.0f / 255.0f);
-    if (P.dering && f > 0.0f) {
-        int ri;
-        if (P.ring_fast) {
-            ri = ring_radius_fast(dy2f + dx2f);
-        } else {
-            double dx = (double)gx - P.cx;
-            ri = (int)(sqrt(dy2 + dx * dx) + 0.5);
-        }
-        if (ri < P.nr) {
-            float v = f - P.scale * P.ring[ring_base + ri];
-            f = v > 0.0f ? v : 0.0f;
-        }
-    }
+    float v = f - c;
+    v = v > 0.0f ? v : 0.0f;
+    f = f > 0.0f ? v : f;
     f *= zf;
-    if (P.window) f = P.wlo + f * P.wspan;
-    return f;
+    float w = P.wlo + f * P.wspan;
+    return P.window ? w : f;
 }
 
-__device__ __forceinline__ int ring_slab_base(const fyg_prologue &P, long gz) {
+__host__ __device__ __forceinline__ int ring_slab(const fyg_prologue &P, long gz) {
     long slab = gz / P.slab_z;
     if (slab < 0) slab = 0;
     if (slab >= P.nslab) slab = P.nslab - 1;
-    return (int)(slab * P.nr);
+    return (int)slab;
+}
+
+// ring correction image of a buffer region: img[s][y][x] = scale*ring[s0+s][ri]
+// for the slabs s0.. the region's planes fall in, ri = (int)(sqrt(dy^2+dx^2)+0.5)
+// in double as dering.c:207-213 evaluates it, 0 where ri >= nr.  The X pass then
+// subtracts one coalesced, cache-resident float per voxel instead of
+// evaluating a radius and gathering from the ring table.
+__global__ __launch_bounds__(256) void k_ring_image(fyg_prologue P, int ny, int nx, int s0) {
+    #pragma clang fp contract(off)
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, si = blockIdx.z;
+    if (x >= nx) return;
+    double dy = (double)(P.gy0 + y) - P.cy, dx = (double)(P.gx0 + x) - P.cx;
+    double dy2 = dy * dy;
+    int ri = (int)(sqrt(dy2 + dx * dx) + 0.5);
+    float c = 0.0f;
+    if (ri < P.nr) c = P.scale * P.ring[(long)(s0 + si) * P.nr + ri];
+    P.cimg[((long)si * ny + y) * nx + x] = c;
 }
 
 __device__ __forceinline__ float epilogue(const fyg_epilogue &E, float v) {
@@ -357,30 +351,25 @@ struct XFwd {
 
 // per-line constants of a row-pair line, computed once per workgroup
 struct XLine {
-    long a, b;          // input offsets of rows y0 and y0+1 (b < 0: no second row)
+    long a, b;          // input offsets of rows y0 and y0+1 (b == a: no second row)
+    long ca, cb;        // ring correction image offsets of the two rows
     long so;            // half-spectrum offset of row y0
-    double dya, dyb;    // squared distance of each row to the rotation axis
-    float dyaf, dybf;
     float zf;           // z-drift factor of the plane
-    int ring;           // ring table offset of the plane's slab
-    int valid;
+    int pair, valid;
 };
 
-template <int L, bool U8>
+// MODE 0: fp32 input; 1: u8 input with the prologue; 2: as 1 plus rings
+template <int L, int MODE>
 __global__ __launch_bounds__(256) void k_rfft_x(XFwd a) {
     constexpr int NL = Shape<L>::NL_RM;
+    constexpr bool U8 = MODE != 0, RING = MODE == 2;
     __shared__ float2 buf[NL * Shape<L>::LD];
     __shared__ float2 tw_s[L];
     __shared__ XLine meta[NL];
-    __shared__ float s_dx2[U8 ? L : 1];
     load_twiddles<L>(tw_s, a.tw);
     const long g0 = (long)blockIdx.x * NL;
-    if (U8 && a.pro.dering && a.pro.ring_fast) {
-        for (int x = threadIdx.x; x < L; x += Shape<L>::T) {
-            float dx = (float)(a.pro.gx0 + reflect_idx(x, a.nx)) - (float)a.pro.cx;
-            s_dx2[x] = dx * dx;
-        }
-    }
+    int s0 = 0;
+    if (RING) s0 = ring_slab(a.pro, a.pro.gz0);
     for (int l = threadIdx.x; l < NL; l += Shape<L>::T) {
         XLine m;
         long g = g0 + l;
@@ -389,46 +378,44 @@ __global__ __launch_bounds__(256) void k_rfft_x(XFwd a) {
         int z = (int)(g / a.npairs);
         int y0 = 2 * (int)(g - (long)z * a.npairs);
         int rz = reflect_idx(z, a.nz), ry0 = reflect_idx(y0, a.ny);
-        bool pair = y0 + 1 < a.py;
-        int ry1 = pair ? reflect_idx(y0 + 1, a.ny) : 0;
+        m.pair = y0 + 1 < a.py;
+        int ry1 = m.pair ? reflect_idx(y0 + 1, a.ny) : ry0;
         m.a = ((long)rz * a.ny + ry0) * a.nx;
-        m.b = pair ? ((long)rz * a.ny + ry1) * a.nx : -1;
+        m.b = ((long)rz * a.ny + ry1) * a.nx;
         m.so = ((long)z * a.py + y0) * a.hxp;
         long gz = a.pro.gz0 + rz;
-        double da = (double)(a.pro.gy0 + ry0) - a.pro.cy, db = (double)(a.pro.gy0 + ry1) - a.pro.cy;
-        m.dya = da * da;
-        m.dyb = db * db;
-        float fa = (float)(a.pro.gy0 + ry0) - (float)a.pro.cy, fb = (float)(a.pro.gy0 + ry1) - (float)a.pro.cy;
-        m.dyaf = fa * fa;
-        m.dybf = fb * fb;
         m.zf = (U8 && a.pro.zdrift) ? a.pro.zfactor[gz] : 1.0f;
-        m.ring = (U8 && a.pro.dering) ? ring_slab_base(a.pro, gz) : 0;
+        long cs = RING ? (long)(ring_slab(a.pro, gz) - s0) * a.ny : 0;
+        m.ca = (cs + ry0) * a.nx;
+        m.cb = (cs + ry1) * a.nx;
         meta[l] = m;
     }
     __syncthreads();
     int any = 0;
     auto first = [&](int l, int x) -> float2 {
         const XLine &m = meta[l];
-        if (!m.valid) return make_float2(0.f, 0.f);
         int rx = reflect_idx(x, a.nx);
-        float va, vb = 0.0f;
+        float va, vb;
         if (U8) {
             const unsigned char *u = (const unsigned char *)a.in;
-            long gx = a.pro.gx0 + rx;
-            float dx2 = (a.pro.dering && a.pro.ring_fast) ? s_dx2[x] : 0.0f;
-            unsigned char ua = u[m.a + rx];
-            any |= ua;
-            va = prologue(a.pro, ua, m.ring, m.dya, m.dyaf, dx2, m.zf, gx);
-            if (m.b >= 0) {
-                unsigned char ub = u[m.b + rx];
-                any |= ub;
-                vb = prologue(a.pro, ub, m.ring, m.dyb, m.dybf, dx2, m.zf, gx);
+            unsigned char ua = u[m.a + rx], ub = u[m.b + rx];
+            float ca = 0.0f, cb = 0.0f;
+            if (RING) {
+                ca = a.pro.cimg[m.ca + rx];
+                cb = a.pro.cimg[m.cb + rx];
             }
+            any |= m.valid ? (ua | ub) : 0;
+            va = prologue(a.pro, ua, ca, m.zf);
+            vb = prologue(a.pro, ub, cb, m.zf);
         } else {
             const float *f = (const float *)a.in;
             va = f[m.a + rx];
-            if (m.b >= 0) vb = f[m.b + rx];
+            vb = f[m.b + rx];
         }
+        // No selects on the loaded values: a line past the end transforms line 0
+        // again and is not stored; without a second row the imaginary part
+        // repeats row y0, which the untangle separates from the real part exactly
+        // as it would separate zeros, and B is not stored.
         return make_float2(va, vb);
     };
     run_from<L, NL, false, false, 0>(-1.0f, tw_s, buf, first, LdsWr<L, NL>{buf});
@@ -452,7 +439,7 @@ __global__ __launch_bounds__(256) void k_rfft_x(XFwd a) {
             B = make_float2(0.5f * (p.y + q.y), 0.5f * (q.x - p.x));
         }
         a.spec[m.so + k] = A;
-        if (m.b >= 0) a.spec[m.so + a.hxp + k] = B;
+        if (m.pair) a.spec[m.so + a.hxp + k] = B;
     }
 }
 
@@ -622,6 +609,7 @@ struct XInv {
 
 struct XInvLine {
     long so;        // half-spectrum offset of row y0
+    long sb;        // half-spectrum offset of row y0+1 (so when it does not exist)
     long oo;        // output offset of row y0
     int pair;       // row y0+1 exists
     int valid;
@@ -644,6 +632,7 @@ __global__ __launch_bounds__(256) void k_irfft_x(XInv a) {
         int y0 = 2 * (int)(g - (long)z * a.npairs);
         m.pair = y0 + 1 < a.ny;
         m.so = ((long)z * a.py + y0) * a.hxp;
+        m.sb = y0 + 1 < a.py ? m.so + a.hxp : m.so;
         m.oo = ((long)z * a.ny + y0) * a.nx;
         meta[l] = m;
     }
@@ -651,10 +640,13 @@ __global__ __launch_bounds__(256) void k_irfft_x(XInv a) {
     // L(x) = A(x) + i B(x) for x <= px/2, Hermitian extension above (paganin.c:271-280)
     auto first = [&](int l, int x) -> float2 {
         const XInvLine &m = meta[l];
-        if (!m.valid) return make_float2(0.f, 0.f);
         int k = x < a.hx ? x : L - x;
+        // Loads are unconditional so the unrolled gather keeps them all in flight:
+        // a line past the end recomputes line 0, and a row y0+1 >= ny still packs
+        // spectrum row y0+1 (or row y0 again) as the imaginary part -- a Hermitian
+        // half spectrum, so the real part stays row y0's -- and neither is stored.
         float2 A = a.spec[m.so + k];
-        float2 B = m.pair ? a.spec[m.so + a.hxp + k] : make_float2(0.f, 0.f);
+        float2 B = a.spec[m.sb + k];
         return x < a.hx ? make_float2(A.x - B.y, A.y + B.x) : make_float2(A.x + B.y, B.x - A.y);
     };
     auto last = [&](int l, int x, float2 v) {
@@ -721,8 +713,9 @@ template <int L>
 struct Launch {
     static void rfft_x(fyg_stream *s, const XFwd &a, bool u8) {
         unsigned g = (unsigned)((a.nlines + Shape<L>::NL_RM - 1) / Shape<L>::NL_RM);
-        if (u8) fyg_launch(s, "rfft_x_u8", k_rfft_x<L, true>, dim3(g), dim3(256), 0, a);
-        else fyg_launch(s, "rfft_x", k_rfft_x<L, false>, dim3(g), dim3(256), 0, a);
+        if (u8 && a.pro.dering) fyg_launch(s, "rfft_x_u8", k_rfft_x<L, 2>, dim3(g), dim3(256), 0, a);
+        else if (u8) fyg_launch(s, "rfft_x_u8", k_rfft_x<L, 1>, dim3(g), dim3(256), 0, a);
+        else fyg_launch(s, "rfft_x", k_rfft_x<L, 0>, dim3(g), dim3(256), 0, a);
     }
     static void cols(fyg_stream *s, const Cols &a, int nbatch, const char *name) {
         unsigned g = (unsigned)((a.ncols + Shape<L>::NL_CM - 1) / Shape<L>::NL_CM);
@@ -805,6 +798,12 @@ extern "C" int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s) {
     XFwd xf{a->in, a->nz, a->ny, a->nx, a->pz, a->py, a->hx, a->hxp,
             (long)a->pz * ((a->py + 1) / 2), (long)(a->py + 1) / 2, spec, twx, a->pro, a->nonzero_flag,
             a->nonzero_gen};
+    if (a->in_u8 && a->pro.dering) {
+        if (!a->pro.cimg) return 1;
+        int s0 = ring_slab(a->pro, a->pro.gz0), s1 = ring_slab(a->pro, a->pro.gz0 + a->nz - 1);
+        fyg_launch(s, "ring_image", k_ring_image, dim3((unsigned)((a->nx + 255) / 256), (unsigned)a->ny,
+                   (unsigned)(s1 - s0 + 1)), dim3(256), 0, a->pro, a->ny, a->nx, s0);
+    }
     dispatch<DoRfftX>(a->px, s, xf, a->in_u8 != 0);
     long plane = (long)a->py * a->hxp;
     Cols yf{spec, a->hxp, a->hxp, plane, -1.0f, twy};
@@ -818,6 +817,12 @@ extern "C" int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s) {
             1.0f / ((float)a->pz * (float)a->py * (float)a->px), a->epi, twx};
     dispatch<DoIrfftX>(a->px, s, xi);
     return fyg_last_error();
+}
+
+extern "C" size_t fyg_ring_image_bytes(const fyg_prologue *p, int nz, int ny, int nx) {
+    if (!p->dering) return 0;
+    int s0 = ring_slab(*p, p->gz0), s1 = ring_slab(*p, p->gz0 + nz - 1);
+    return sizeof(float) * (size_t)(s1 - s0 + 1) * ny * nx;
 }
 
 extern "C" int fyg_fft3d_run(float *d_re, float *d_im, int nz, int ny, int nx, int sign, void *work,

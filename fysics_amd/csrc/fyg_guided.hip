@@ -51,18 +51,29 @@ __device__ __forceinline__ void box_tile(const Grid &g, float *sm, Load load, Ou
     float *A = sm;                     // [NF][AREG]
     float *B = sm + NF * Tl::AREG;     // [NF][BREG]
     const int bx = blockIdx.x * TX, by = blockIdx.y * TY, bz = blockIdx.z * TZ;
-    for (int i = threadIdx.x; i < Tl::AREG; i += NT) {
-        int ex = i % Tl::EX, t = i / Tl::EX, ey = t % Tl::EY, ez = t / Tl::EY;
-        int gx = bx + ex - RHO, gy = by + ey - RHO, gz = bz + ez - RHO;
-        float v[NF];
-        if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz) {
-            load(gz, gy, gx, v);
-        } else {
+    {
+        // every load of a thread is issued before the first is used: cells
+        // outside the grid load the tile origin and are zeroed after
+        constexpr int K = (Tl::AREG + NT - 1) / NT;
+        float v[K][NF];
 #pragma unroll
-            for (int f = 0; f < NF; f++) v[f] = 0.0f;
+        for (int k = 0; k < K; k++) {
+            int i = threadIdx.x + k * NT;
+            int ex = i % Tl::EX, t = i / Tl::EX, ey = t % Tl::EY, ez = t / Tl::EY;
+            int gx = bx + ex - RHO, gy = by + ey - RHO, gz = bz + ez - RHO;
+            bool in = i < Tl::AREG && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz;
+            load(in ? gz : bz, in ? gy : by, in ? gx : bx, v[k]);
+#pragma unroll
+            for (int f = 0; f < NF; f++) v[k][f] = in ? v[k][f] : 0.0f;
         }
 #pragma unroll
-        for (int f = 0; f < NF; f++) A[f * Tl::AREG + i] = v[f];
+        for (int k = 0; k < K; k++) {
+            int i = threadIdx.x + k * NT;
+            if (i < Tl::AREG) {
+#pragma unroll
+                for (int f = 0; f < NF; f++) A[f * Tl::AREG + i] = v[k][f];
+            }
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < Tl::BREG; i += NT) {           // x
@@ -109,8 +120,10 @@ __device__ __forceinline__ void box_tile(const Grid &g, float *sm, Load load, Ou
 }
 
 // ------------------------------------------------------------------ emit
+// the low 32 bits of the 64-bit products (long)c * k are the 32-bit products
+// of the low words, so the hash runs in 32-bit arithmetic (pipeline.c:53-59)
 __device__ __forceinline__ float dither01(long z, long y, long x) {
-    unsigned int h = (unsigned int)(z * 73856093L) ^ (unsigned int)(y * 19349663L) ^ (unsigned int)(x * 83492791L);
+    unsigned int h = ((unsigned int)z * 73856093u) ^ ((unsigned int)y * 19349663u) ^ ((unsigned int)x * 83492791u);
     h ^= h >> 16;
     h *= 0x7feb352dU;
     h ^= h >> 15;
@@ -239,18 +252,28 @@ __global__ __launch_bounds__(NT) void k_box2(Box2Args p) {
     box_tile<RHO, 2>(g, sm, load, out);
 }
 
-// ---- fused low-res coefficient field of the fast path (guided.c:163-182):
-// one workgroup produces mean_a, mean_b for a 32x8x4 tile of the decimated grid.
-// It loads the decimated, tile-centred samples with a 2*rho halo, forms
-// box(I-c0), box((I-c0)^2) -> a, b on the tile plus a rho halo, then box(a),
-// box(b) on the tile -- all in LDS, so a and b never reach HBM.
+// ---- fused low-res coefficient field of the fast path (guided.c:163-182).
+// A workgroup owns a 16x16 column block of the decimated grid and marches
+// along z through ZC output planes (plus a 2*rho halo on each side).  Per
+// sample plane it
+//   1. stages the centred samples v = I - c0 of an (16+2rho)^2 patch in LDS,
+//   2. forms each column's x-then-y window sums of v and v^2 and keeps the last
+//      2*rho+1 of them in registers, so their sum is box(v), box(v^2) centred
+//      rho planes back -> a = var/(var+eps), b = (m+c0)(1-a) for that plane,
+//   3. stages a, b in LDS and sums them the same way for the (16-2*rho)^2 inner
+//      columns -> mean_a, mean_b one more rho plane back.
+// The halo costs 1.3x in samples instead of the 3.4x of a 3-D tile, a and b
+// never reach HBM, and each plane is two barriers deep.  Cells outside the
+// grid contribute 0 and windows shrink at the faces (guided.c:22-98).
 template <int RHO>
-struct CoefTile {
-    static constexpr int X2 = TX + 4 * RHO, Y2 = TY + 4 * RHO, Z2 = TZ + 4 * RHO;   // samples
-    static constexpr int X1 = TX + 2 * RHO, Y1 = TY + 2 * RHO, Z1 = TZ + 2 * RHO;   // a, b
-    static constexpr int P0 = 2 * Z2 * Y2 * X1;     // largest intermediate
-    static constexpr int P1 = (2 * Z2 * Y1 * X1 > Z2 * Y2 * X2) ? 2 * Z2 * Y1 * X1 : Z2 * Y2 * X2;
-    static constexpr size_t bytes = sizeof(float) * (P0 + P1);
+struct March {
+    static constexpr int AW = 16;                   // a, b columns per side (NT = AW*AW)
+    static constexpr int OW = AW - 2 * RHO;         // output columns per side
+    static constexpr int SW = AW + 2 * RHO;         // sample columns per side
+    static constexpr int NS = SW * SW;
+    static constexpr int KS = (NS + NT - 1) / NT;   // sample slots per thread
+    static constexpr int ZC = 16;                   // output planes per workgroup
+    static constexpr int Q = 2 * RHO + 1;           // window length
 };
 
 struct CoefArgs {
@@ -262,110 +285,124 @@ struct CoefArgs {
 
 template <int RHO>
 __global__ __launch_bounds__(NT) void k_coef(CoefArgs p) {
-    using CT = CoefTile<RHO>;
-    extern __shared__ float sm[];
-    float *P0 = sm, *P1 = sm + CT::P0;
+    using M = March<RHO>;
+    static_assert(M::AW * M::AW == NT, "one thread per a, b column");
+    __shared__ float sv[M::SW][M::SW + 1];
+    __shared__ float sa[M::AW][M::AW + 1], sb[M::AW][M::AW + 1];
     const Grid &g = p.g;
-    const int bx = blockIdx.x * TX, by = blockIdx.y * TY, bz = blockIdx.z * TZ;
+    const int bx = blockIdx.x * M::OW, by = blockIdx.y * M::OW, zs = blockIdx.z * M::ZC;
+    const int ze = min(zs + M::ZC, g.nz);
     const size_t sz = (size_t)g.s * g.fny * g.fnx, sy = (size_t)g.s * g.fnx;
-    const float c0 = p.in[(size_t)bz * sz + (size_t)by * sy + (size_t)bx * g.s];
-    // 1. centred samples over the tile + 2*rho halo (zero outside the grid)
-    for (int i = threadIdx.x; i < CT::Z2 * CT::Y2 * CT::X2; i += NT) {
-        int ex = i % CT::X2, t = i / CT::X2, ey = t % CT::Y2, ez = t / CT::Y2;
-        int gx = bx + ex - 2 * RHO, gy = by + ey - 2 * RHO, gz = bz + ez - 2 * RHO;
-        float v = 0.0f;
-        if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz)
-            v = p.in[(size_t)gz * sz + (size_t)gy * sy + (size_t)gx * g.s] - c0;
-        P1[i] = v;
-    }
-    __syncthreads();
-    // 2. x sums of v and v^2: [Z2][Y2][X1] x 2
-    constexpr int NX = CT::Z2 * CT::Y2 * CT::X1;
-    for (int i = threadIdx.x; i < NX; i += NT) {
-        int x = i % CT::X1, row = i / CT::X1;
-        const float *a = P1 + row * CT::X2 + x;
-        float s1 = 0.0f, s2 = 0.0f;
-#pragma unroll
-        for (int d = 0; d <= 2 * RHO; d++) { s1 += a[d]; s2 += a[d] * a[d]; }
-        P0[i] = s1;
-        P0[NX + i] = s2;
-    }
-    __syncthreads();
-    // 3. y sums: [Z2][Y1][X1] x 2
-    constexpr int NY = CT::Z2 * CT::Y1 * CT::X1;
-    for (int i = threadIdx.x; i < NY; i += NT) {
-        int x = i % CT::X1, t = i / CT::X1, y = t % CT::Y1, z = t / CT::Y1;
-        const float *a = P0 + (z * CT::Y2 + y) * CT::X1 + x;
-        float s1 = 0.0f, s2 = 0.0f;
-#pragma unroll
-        for (int d = 0; d <= 2 * RHO; d++) { s1 += a[d * CT::X1]; s2 += a[NX + d * CT::X1]; }
-        P1[i] = s1;
-        P1[NY + i] = s2;
-    }
-    __syncthreads();
-    // 4. z sums -> a, b on the tile + rho halo (zero outside the grid): [Z1][Y1][X1] x 2
-    constexpr int NA = CT::Z1 * CT::Y1 * CT::X1;
+    const float c0 = p.in[(size_t)zs * sz + (size_t)by * sy + (size_t)bx * g.s];
     const float epsf = (float)p.eps;
-    for (int i = threadIdx.x; i < NA; i += NT) {
-        int x = i % CT::X1, t = i / CT::X1, y = t % CT::Y1, z = t / CT::Y1;
-        int gx = bx + x - RHO, gy = by + y - RHO, gz = bz + z - RHO;
-        float av = 0.0f, bv = 0.0f;
-        if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz) {
-            const float *a = P1 + (z * CT::Y1 + y) * CT::X1 + x;
-            float s1 = 0.0f, s2 = 0.0f;
+
+    // sample slots: in-plane source offset (0 outside the grid) and LDS cell
+    size_t soff[M::KS];
+    bool sok[M::KS];
+    int scell[M::KS];
 #pragma unroll
-            for (int d = 0; d <= 2 * RHO; d++) {
-                s1 += a[d * CT::Y1 * CT::X1];
-                s2 += a[NY + d * CT::Y1 * CT::X1];
+    for (int k = 0; k < M::KS; k++) {
+        int i = threadIdx.x + k * NT;
+        int ex = i % M::SW, ey = i / M::SW;
+        int gx = bx - 2 * RHO + ex, gy = by - 2 * RHO + ey;
+        sok[k] = i < M::NS && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny;
+        soff[k] = sok[k] ? (size_t)gy * sy + (size_t)gx * g.s : 0;
+        scell[k] = i < M::NS ? ey * (M::SW + 1) + ex : -1;
+    }
+    // this thread's a, b column (and output column when inner)
+    const int ax = threadIdx.x % M::AW, ay = threadIdx.x / M::AW;
+    const int gx = bx - RHO + ax, gy = by - RHO + ay;
+    const bool colin = gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny;
+    const float rcxy = colin ? count_recip(gx, g.nx, RHO) * count_recip(gy, g.ny, RHO) : 0.0f;
+    const bool inner = ax >= RHO && ax < M::AW - RHO && ay >= RHO && ay < M::AW - RHO && colin;
+
+    float q1[M::Q], q2[M::Q], qa[M::Q], qb[M::Q];
+#pragma unroll
+    for (int j = 0; j < M::Q; j++) q1[j] = q2[j] = qa[j] = qb[j] = 0.0f;
+
+    auto fetch = [&](int zp, float *dst) {
+        size_t zo = (size_t)(zp >= 0 && zp < g.nz ? zp : zs) * sz;
+#pragma unroll
+        for (int k = 0; k < M::KS; k++) dst[k] = p.in[zo + soff[k]];
+    };
+    float cur[M::KS], nxt[M::KS];
+    const int zfirst = zs - 2 * RHO, zlast = ze - 1 + 2 * RHO;
+    fetch(zfirst, cur);
+    for (int zp = zfirst; zp <= zlast; zp++) {
+        fetch(zp + 1, nxt);                         // next plane in flight meanwhile
+        const bool zin = zp >= 0 && zp < g.nz;
+#pragma unroll
+        for (int k = 0; k < M::KS; k++)
+            if (scell[k] >= 0) (&sv[0][0])[scell[k]] = (zin && sok[k]) ? cur[k] - c0 : 0.0f;
+        __syncthreads();
+        // x-then-y window sums of v and v^2 at plane zp
+        float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int dy = 0; dy < M::Q; dy++) {
+            float r1 = 0.0f, r2 = 0.0f;
+#pragma unroll
+            for (int dx = 0; dx < M::Q; dx++) {
+                float v = sv[ay + dy][ax + dx];
+                r1 += v;
+                r2 += v * v;
             }
-            float rc = count_recip(gx, g.nx, RHO) * count_recip(gy, g.ny, RHO) * count_recip(gz, g.nz, RHO);
-            float m = s1 * rc, q = s2 * rc;
-            float var = q - m * m;
-            if (var < 0.0f) var = 0.0f;
-            av = var / (var + epsf);
-            bv = (m + c0) * (1.0f - av);
+            s1 += r1;
+            s2 += r2;
         }
-        P0[i] = av;
-        P0[NA + i] = bv;
-    }
-    __syncthreads();
-    // 5. x sums of a, b: [Z1][Y1][TX] x 2
-    constexpr int NB2 = CT::Z1 * CT::Y1 * TX;
-    for (int i = threadIdx.x; i < NB2; i += NT) {
-        int x = i % TX, row = i / TX;
-        const float *a = P0 + row * CT::X1 + x;
-        float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
-        for (int d = 0; d <= 2 * RHO; d++) { s1 += a[d]; s2 += a[NA + d]; }
-        P1[i] = s1;
-        P1[NB2 + i] = s2;
-    }
-    __syncthreads();
-    // 6. y sums: [Z1][TY][TX] x 2
-    constexpr int NC = CT::Z1 * TY * TX;
-    for (int i = threadIdx.x; i < NC; i += NT) {
-        int x = i % TX, t = i / TX, y = t % TY, z = t / TY;
-        const float *a = P1 + (z * CT::Y1 + y) * TX + x;
-        float s1 = 0.0f, s2 = 0.0f;
+        for (int j = 0; j < M::Q - 1; j++) { q1[j] = q1[j + 1]; q2[j] = q2[j + 1]; }
+        q1[M::Q - 1] = s1;
+        q2[M::Q - 1] = s2;
+        if (zp >= zs) {                             // a, b at plane za = zp - rho
+            const int za = zp - RHO;
+            float av = 0.0f, bv = 0.0f;
+            if (colin && za >= 0 && za < g.nz) {
+                float t1 = 0.0f, t2 = 0.0f;
 #pragma unroll
-        for (int d = 0; d <= 2 * RHO; d++) { s1 += a[d * TX]; s2 += a[NB2 + d * TX]; }
-        P0[i] = s1;
-        P0[NC + i] = s2;
-    }
-    __syncthreads();
-    // 7. z sums -> mean_a, mean_b of the tile
-    for (int i = threadIdx.x; i < TZ * TY * TX; i += NT) {
-        int x = i % TX, t = i / TX, y = t % TY, z = t / TY;
-        int gx = bx + x, gy = by + y, gz = bz + z;
-        if (gx >= g.nx || gy >= g.ny || gz >= g.nz) continue;
-        const float *a = P0 + (z * TY + y) * TX + x;
-        float s1 = 0.0f, s2 = 0.0f;
+                for (int j = 0; j < M::Q; j++) { t1 += q1[j]; t2 += q2[j]; }
+                float rc = rcxy * count_recip(za, g.nz, RHO);
+                float m = t1 * rc, q = t2 * rc;
+                float var = q - m * m;
+                if (var < 0.0f) var = 0.0f;
+                av = var / (var + epsf);
+                bv = (m + c0) * (1.0f - av);
+            }
+            sa[ay][ax] = av;
+            sb[ay][ax] = bv;
+        }
+        __syncthreads();
+        if (zp >= zs) {                             // window sums of a, b; output plane zp - 2 rho
+            float u1 = 0.0f, u2 = 0.0f;
+            if (inner) {
 #pragma unroll
-        for (int d = 0; d <= 2 * RHO; d++) { s1 += a[d * TY * TX]; s2 += a[NC + d * TY * TX]; }
-        float rc = count_recip(gx, g.nx, RHO) * count_recip(gy, g.ny, RHO) * count_recip(gz, g.nz, RHO);
-        size_t o = ((size_t)gz * g.ny + gy) * g.nx + gx;
-        p.ma[o] = s1 * rc;
-        p.mb[o] = s2 * rc;
+                for (int dy = -RHO; dy <= RHO; dy++) {
+                    float r1 = 0.0f, r2 = 0.0f;
+#pragma unroll
+                    for (int dx = -RHO; dx <= RHO; dx++) {
+                        r1 += sa[ay + dy][ax + dx];
+                        r2 += sb[ay + dy][ax + dx];
+                    }
+                    u1 += r1;
+                    u2 += r2;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < M::Q - 1; j++) { qa[j] = qa[j + 1]; qb[j] = qb[j + 1]; }
+            qa[M::Q - 1] = u1;
+            qb[M::Q - 1] = u2;
+            const int zo = zp - 2 * RHO;
+            if (inner && zo >= zs) {
+                float t1 = 0.0f, t2 = 0.0f;
+#pragma unroll
+                for (int j = 0; j < M::Q; j++) { t1 += qa[j]; t2 += qb[j]; }
+                float rc = rcxy * count_recip(zo, g.nz, RHO);
+                size_t o = ((size_t)zo * g.ny + gy) * g.nx + gx;
+                p.ma[o] = t1 * rc;
+                p.mb[o] = t2 * rc;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < M::KS; k++) cur[k] = nxt[k];
     }
 }
 
@@ -396,67 +433,133 @@ struct UpArgs {
 
 constexpr int UPX = 4;               // consecutive x outputs per thread
 constexpr int UPK = 64 * UPX / 2 + 4;  // low-res columns one 256-wide chunk touches (s >= 2)
+constexpr int URZ = 2, URY = 8;      // output planes x rows of one workgroup
+constexpr int UPZ = 3, UPY = 6;      // low-res planes x rows they touch (s >= 2)
 
 // trilinear upsample of (mean_a, mean_b) from the decimated grid and apply
 // out = a*I + b (guided.c:188-228): u = c/s, i0 = min((int)u, l-2), f = min(u-i0, 1).
-// As in the reference, the four (z,y) low-res rows are blended once per output
-// row -- here into LDS, shared by the 64 threads of the row -- and each voxel
-// then needs only the x lerp.  Block (64, 4): 4 rows x 256 voxels.
+// A workgroup produces 2 planes x 8 rows x 256 voxels.  Those 16 rows share a
+// patch of at most 3 x 6 low-res rows, which is loaded into LDS once.  As in
+// the reference, each output row blends its four (z,y) low-res rows first and
+// then lerps along x; a thread blends only the three low-res columns its four
+// consecutive voxels need.  Thread (q, g) owns x quad q of rows g, g+4, g+8, g+12.
+// P2: s is a power of two, so c/s is exactly c*(1/s) and needs no division.
+template <bool P2>
+struct UpCoord {
+    float fs, inv;
+    __device__ float u(int c) const { return P2 ? (float)c * inv : (float)c / fs; }
+    __device__ int i0(int c, int l) const {
+        int i = (int)u(c);
+        return i > l - 2 ? l - 2 : i;
+    }
+};
+
+template <bool P2>
 __global__ __launch_bounds__(NT) void k_upsample(UpArgs p) {
-    __shared__ float sa[4][UPK], sb[4][UPK];
-    const int ty = threadIdx.y;
-    const int y = p.y0 + blockIdx.y * 4 + ty;
-    const int z = p.z0 + blockIdx.z;
-    const bool row_ok = y < p.y0 + p.dy;
-    const float fs = (float)p.s;
+    __shared__ float pa[UPZ][UPY][UPK], pb[UPZ][UPY][UPK];
+    const UpCoord<P2> U{(float)p.s, 1.0f / (float)p.s};
+    const int Z0 = p.z0 + blockIdx.z * URZ, Y0 = p.y0 + blockIdx.y * URY;
+    const int zl = min(Z0 + URZ, p.z0 + p.dz) - 1, yl = min(Y0 + URY, p.y0 + p.dy) - 1;
     const int xc0 = p.x0 + blockIdx.x * 64 * UPX;
     const int xc1 = min(xc0 + 64 * UPX, p.x0 + p.dx) - 1;
-    int kb = (int)((float)xc0 / fs); if (kb > p.lx - 2) kb = p.lx - 2;
-    int ke = (int)((float)xc1 / fs); if (ke > p.lx - 2) ke = p.lx - 2;
-    const int nk = ke - kb + 2;
-    if (row_ok) {
-        float uz = (float)z / fs, uy = (float)y / fs;
-        int z0 = (int)uz; if (z0 > p.lz - 2) z0 = p.lz - 2;
-        int y0 = (int)uy; if (y0 > p.ly - 2) y0 = p.ly - 2;
-        float zf = uz - z0; if (zf > 1.0f) zf = 1.0f;
-        float yf = uy - y0; if (yf > 1.0f) yf = 1.0f;
-        const float w00 = (1 - zf) * (1 - yf), w01 = (1 - zf) * yf, w10 = zf * (1 - yf), w11 = zf * yf;
-        const size_t q00 = ((size_t)z0 * p.ly + y0) * p.lx + kb, q01 = q00 + p.lx;
-        const size_t q10 = q00 + (size_t)p.ly * p.lx, q11 = q10 + p.lx;
-        for (int k = threadIdx.x; k < nk; k += 64) {
-            sa[ty][k] = w00 * p.ma[q00 + k] + w01 * p.ma[q01 + k] + w10 * p.ma[q10 + k] + w11 * p.ma[q11 + k];
-            sb[ty][k] = w00 * p.mb[q00 + k] + w01 * p.mb[q01 + k] + w10 * p.mb[q10 + k] + w11 * p.mb[q11 + k];
+    const int kb = U.i0(xc0, p.lx), nk = U.i0(xc1, p.lx) - kb + 2;
+    const int zlo = U.i0(Z0, p.lz), nzp = U.i0(zl, p.lz) - zlo + 2;
+    const int ylo = U.i0(Y0, p.ly), nyp = U.i0(yl, p.ly) - ylo + 2;
+    const int q = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int xb = xc0 + q * UPX;
+    const bool xin = xb <= xc1, full = xb + UPX - 1 <= xc1;
+
+    // input voxels of the thread's four rows: issued first, used last
+    float iv[4][UPX];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int r = g + 4 * j, z = Z0 + r / URY, y = Y0 + r % URY;
+        bool ok = xin && z <= zl && y <= yl;
+        const float *irow = p.in + ((size_t)(ok ? z : Z0) * p.ny + (ok ? y : Y0)) * p.nx;
+        if (ok && full && p.vin) {
+            float4 t = *(const float4 *)(irow + xb);
+            iv[j][0] = t.x; iv[j][1] = t.y; iv[j][2] = t.z; iv[j][3] = t.w;
+        } else {
+#pragma unroll
+            for (int e = 0; e < UPX; e++) iv[j][e] = (ok && xb + e <= xc1) ? irow[xb + e] : 0.0f;
+        }
+    }
+    // low-res patch: wave g loads patch rows g, g+4, ..; lanes split the columns
+    {
+        constexpr int RW = (UPZ * UPY + 3) / 4, CW = (UPK + 63) / 64;
+        const int nrow = nzp * nyp;
+        float va[RW][CW], vb[RW][CW];
+#pragma unroll
+        for (int i = 0; i < RW; i++) {
+            int rr = g + 4 * i;
+            int pz = rr >= nyp ? (rr >= 2 * nyp ? 2 : 1) : 0, py = rr - pz * nyp;
+            size_t ro = ((size_t)(zlo + pz) * p.ly + (ylo + py)) * p.lx + kb;
+#pragma unroll
+            for (int c = 0; c < CW; c++) {
+                int col = q + 64 * c;
+                bool ok = rr < nrow && col < nk;
+                size_t o = ok ? ro + col : (size_t)zlo * p.ly * p.lx;
+                va[i][c] = p.ma[o];
+                vb[i][c] = p.mb[o];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < RW; i++) {
+            int rr = g + 4 * i;
+            int pz = rr >= nyp ? (rr >= 2 * nyp ? 2 : 1) : 0, py = rr - pz * nyp;
+#pragma unroll
+            for (int c = 0; c < CW; c++) {
+                int col = q + 64 * c;
+                if (rr < nrow && col < nk) {
+                    pa[pz][py][col] = va[i][c];
+                    pb[pz][py][col] = vb[i][c];
+                }
+            }
         }
     }
     __syncthreads();
-    if (!row_ok) return;
-    const int xb = xc0 + threadIdx.x * UPX;
-    if (xb > xc1) return;
-    const float *irow = p.in + ((size_t)z * p.ny + y) * p.nx;
-    const bool full = xb + UPX - 1 <= xc1;
-    float iv[UPX], ov[UPX];
-    if (full && p.vin) {
-        float4 t = *(const float4 *)(irow + xb);
-        iv[0] = t.x; iv[1] = t.y; iv[2] = t.z; iv[3] = t.w;
-    } else {
-        for (int j = 0; j < UPX; j++) iv[j] = xb + j <= xc1 ? irow[xb + j] : 0.0f;
-    }
+    if (!xin) return;
+    // x-dependent weights, shared by the four rows
+    const int k0 = U.i0(xb, p.lx) - kb;
+    float fxs[UPX];
+    bool dsel[UPX];
 #pragma unroll
-    for (int j = 0; j < UPX; j++) {
-        int x = xb + j;
-        float ux = (float)x / fs;
-        int x0 = (int)ux; if (x0 > p.lx - 2) x0 = p.lx - 2;
-        float fx = ux - x0; if (fx > 1.0f) fx = 1.0f;
-        float gx = 1 - fx;
-        int k = x0 - kb;
-        float av = gx * sa[ty][k] + fx * sa[ty][k + 1], bv = gx * sb[ty][k] + fx * sb[ty][k + 1];
-        ov[j] = av * iv[j] + bv;
+    for (int e = 0; e < UPX; e++) {
+        int x = xb + e, x0 = U.i0(x, p.lx);
+        float fx = U.u(x) - x0;
+        fxs[e] = fx > 1.0f ? 1.0f : fx;
+        dsel[e] = x0 - kb - k0 != 0;            // 0 or 1
     }
+    const int c0 = min(k0, nk - 1), c1 = min(k0 + 1, nk - 1), c2 = min(k0 + 2, nk - 1);
     const bool keep = emit_keep(p.em);
-    if (full) {
-        emit4(p.em, p.ny, p.nx, z, y, xb, ov, p.vout != 0, keep);
-    } else {
-        for (int j = 0; j < UPX && xb + j <= xc1; j++) emit(p.em, p.ny, p.nx, z, y, xb + j, ov[j]);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int r = g + 4 * j, z = Z0 + r / URY, y = Y0 + r % URY;
+        if (z > zl || y > yl) continue;
+        int z0 = U.i0(z, p.lz), y0 = U.i0(y, p.ly);
+        float zf = U.u(z) - z0; if (zf > 1.0f) zf = 1.0f;
+        float yf = U.u(y) - y0; if (yf > 1.0f) yf = 1.0f;
+        const float w00 = (1 - zf) * (1 - yf), w01 = (1 - zf) * yf, w10 = zf * (1 - yf), w11 = zf * yf;
+        const int pz = z0 - zlo, py = y0 - ylo;
+        auto blend = [&](const float (*P)[UPY][UPK], int c) {
+            return w00 * P[pz][py][c] + w01 * P[pz][py + 1][c] + w10 * P[pz + 1][py][c] + w11 * P[pz + 1][py + 1][c];
+        };
+        const float A0 = blend(pa, c0), A1 = blend(pa, c1), A2 = blend(pa, c2);
+        const float B0 = blend(pb, c0), B1 = blend(pb, c1), B2 = blend(pb, c2);
+        float ov[UPX];
+#pragma unroll
+        for (int e = 0; e < UPX; e++) {
+            float fx = fxs[e], gx = 1 - fx;
+            float a0 = dsel[e] ? A1 : A0, a1 = dsel[e] ? A2 : A1;
+            float b0 = dsel[e] ? B1 : B0, b1 = dsel[e] ? B2 : B1;
+            float av = gx * a0 + fx * a1, bv = gx * b0 + fx * b1;
+            ov[e] = av * iv[j][e] + bv;
+        }
+        if (full) {
+            emit4(p.em, p.ny, p.nx, z, y, xb, ov, p.vout != 0, keep);
+        } else {
+            for (int e = 0; e < UPX && xb + e <= xc1; e++) emit(p.em, p.ny, p.nx, z, y, xb + e, ov[e]);
+        }
     }
 }
 
@@ -479,6 +582,13 @@ __global__ __launch_bounds__(NT) void k_emit(EmitArgs p) {
         if (x >= p.x0 + p.dx) break;
         emit(p.em, p.ny, p.nx, z, y, x, row[x]);
     }
+}
+
+template <int RHO>
+dim3 march_grid(const Grid &g) {
+    using M = March<RHO>;
+    return dim3((unsigned)((g.nx + M::OW - 1) / M::OW), (unsigned)((g.ny + M::OW - 1) / M::OW),
+                (unsigned)((g.nz + M::ZC - 1) / M::ZC));
 }
 
 dim3 row_grid(int dx, int dy, int dz) {
@@ -657,14 +767,10 @@ extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
     if (rl < 1) rl = 1;
     float *A = ws, *B = ws + nl, *MA = ws + 2 * nl, *MB = ws + 3 * nl;
     Grid gl{lz, ly, lx, sd, ny, nx};
-    if (rl == 1) {
+    if (rl <= 2) {
         CoefArgs c{gl, a->in, a->eps, MA, MB};
-        fyg_launch(s, "guided_coef", k_coef<1>, tile_grid(gl), dim3(NT), CoefTile<1>::bytes, c);
-    } else if (rl <= kMaxTileRho) {
-        MomArgs m{gl, a->in, a->eps, A, B};
-        moments(s, m, rl);
-        Box2Args b{gl, A, B, 0, MA, MB, nullptr, a->em};
-        box2(s, b, rl);
+        if (rl == 1) fyg_launch(s, "guided_coef", k_coef<1>, march_grid<1>(gl), dim3(NT), 0, c);
+        else fyg_launch(s, "guided_coef", k_coef<2>, march_grid<2>(gl), dim3(NT), 0, c);
     } else {
         float *T0 = ws + 4 * nl, *T1 = ws + 5 * nl, *O0 = ws + 6 * nl, *O1 = ws + 7 * nl;
         float c0 = 0.0f;
@@ -685,8 +791,12 @@ extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
     u.vin = (nx % 4 == 0) && (u.x0 % 4 == 0) && ((uintptr_t)a->in % 16 == 0);
     u.vout = a->em.quant ? (a->em.tx % 4 == 0 && (uintptr_t)a->em.q % 4 == 0)
                          : (u.vin && (uintptr_t)a->em.out % 16 == 0);
-    if ((size_t)u.dz * u.dy * u.dx)
-        fyg_launch(s, "guided_upsample", k_upsample, row_grid(u.dx, u.dy, u.dz), dim3(64, 4), 0, u);
+    if ((size_t)u.dz * u.dy * u.dx) {
+        dim3 grid((unsigned)((u.dx + 64 * UPX - 1) / (64 * UPX)), (unsigned)((u.dy + URY - 1) / URY),
+                  (unsigned)((u.dz + URZ - 1) / URZ));
+        if ((sd & (sd - 1)) == 0) fyg_launch(s, "guided_upsample", k_upsample<true>, grid, dim3(NT), 0, u);
+        else fyg_launch(s, "guided_upsample", k_upsample<false>, grid, dim3(NT), 0, u);
+    }
     return fyg_last_error();
 }
 

@@ -147,6 +147,10 @@ static int chain_cube(const fy_cube *c, int *d_flag, int gen, fyg_stream *s) {
         a.lut = t.lut; a.us = t.us; a.fz2 = t.fz2; a.fy2 = t.fy2; a.fx2 = t.fx2;
         a.in = c->d_in; a.in_u8 = 1;
         a.pro = pro;
+        if (pro.dering) {
+            a.pro.cimg = (float *)fyg_ws(s, WS_MISC, fyg_ring_image_bytes(&pro, (int)hz, (int)hy, (int)hx));
+            if (!a.pro.cimg) return 1;
+        }
         if (windowed) { a.pro.window = 1; a.pro.wlo = wlo; a.pro.wspan = wspan; }
         a.out = f;
         if (windowed) { a.epi.window = 1; a.epi.wlo = wlo; a.epi.winv = 1.0f / wspan; }
