@@ -101,7 +101,13 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (the reference point that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters into HBM bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
               "(cubes are independent; see DESIGN.md for how the work would shard)", file=sys.stderr)
@@ -226,6 +232,19 @@ def main():
         "workload_gen_s": round(gen_s, 1),
     }
     print(json.dumps(line))
+
+
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """Plain device-to-device copies of a known size: a profiler's FETCH_SIZE and
+    WRITE_SIZE for this kernel, divided by nbytes, is the factor that turns the
+    same counters of the chain's kernels into HBM bytes."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
 
 
 def cpu_baseline(vol, phys, eps, ring, threads):

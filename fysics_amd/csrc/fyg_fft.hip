@@ -429,15 +429,13 @@ __global__ __launch_bounds__(256) void k_rfft_x(XFwd a) {
     for (; l < NL; l += dl, k += dk) {
         if (k >= a.hxp) { k -= a.hxp; l++; if (l >= NL) break; }
         const XLine &m = meta[l];
-        if (!m.valid) continue;
-        float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
-        if (k < a.hx) {
-            int nk = k ? L - k : 0;
-            float2 p = buf[l * Shape<L>::LD + lds_pos(k)];
-            float2 q = buf[l * Shape<L>::LD + lds_pos(nk)];
-            A = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
-            B = make_float2(0.5f * (p.y + q.y), 0.5f * (q.x - p.x));
-        }
+        // pitch padding k >= hx is never read by the later passes: not written
+        if (!m.valid || k >= a.hx) continue;
+        int nk = k ? L - k : 0;
+        float2 p = buf[l * Shape<L>::LD + lds_pos(k)];
+        float2 q = buf[l * Shape<L>::LD + lds_pos(nk)];
+        float2 A = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
+        float2 B = make_float2(0.5f * (p.y + q.y), 0.5f * (q.x - p.x));
         a.spec[m.so + k] = A;
         if (m.pair) a.spec[m.so + a.hxp + k] = B;
     }
@@ -806,11 +804,11 @@ extern "C" int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s) {
     }
     dispatch<DoRfftX>(a->px, s, xf, a->in_u8 != 0);
     long plane = (long)a->py * a->hxp;
-    Cols yf{spec, a->hxp, a->hxp, plane, -1.0f, twy};
+    Cols yf{spec, a->hx, a->hxp, plane, -1.0f, twy};        // columns k < hx: the pitch padding stays idle
     dispatch<DoCols>(a->py, s, yf, a->pz, "cfft_y_fwd");
     ZFused zf{spec, a->py, a->hx, a->hxp, plane, a->lut, a->us, a->fz2, a->fy2, a->fx2, twz};
     dispatch<DoZ>(a->pz, s, zf);
-    Cols yi{spec, a->hxp, a->hxp, plane, 1.0f, twy};
+    Cols yi{spec, a->hx, a->hxp, plane, 1.0f, twy};
     dispatch<DoCols>(a->py, s, yi, a->nz, "cfft_y_inv");
     XInv xi{spec, a->out, a->nz, a->ny, a->nx, a->py, a->hx, a->hxp,
             (long)a->nz * ((a->ny + 1) / 2), (long)(a->ny + 1) / 2,

@@ -120,10 +120,13 @@ __device__ __forceinline__ void box_tile(const Grid &g, float *sm, Load load, Ou
 }
 
 // ------------------------------------------------------------------ emit
-// the low 32 bits of the 64-bit products (long)c * k are the 32-bit products
-// of the low words, so the hash runs in 32-bit arithmetic (pipeline.c:53-59)
-__device__ __forceinline__ float dither01(long z, long y, long x) {
-    unsigned int h = ((unsigned int)z * 73856093u) ^ ((unsigned int)y * 19349663u) ^ ((unsigned int)x * 83492791u);
+// Dither hash of pipeline.c:53-59.  The low 32 bits of the 64-bit products
+// (long)c * k are the 32-bit products of the low words, so it runs in 32-bit
+// arithmetic; the coordinate terms are split so that a row computes its (z,y)
+// part once and steps the x part by its constant.
+constexpr unsigned kHz = 73856093u, kHy = 19349663u, kHx = 83492791u;
+
+__device__ __forceinline__ float dither_mix(unsigned int h) {
     h ^= h >> 16;
     h *= 0x7feb352dU;
     h ^= h >> 15;
@@ -132,26 +135,35 @@ __device__ __forceinline__ float dither01(long z, long y, long x) {
     return (float)(h >> 8) * (1.0f / 16777216.0f);
 }
 
-__device__ __forceinline__ unsigned char quant_u8(float v, long gz, long gy, long gx, int no_dither) {
+__device__ __forceinline__ float dither01(long z, long y, long x) {
+    return dither_mix(((unsigned int)z * kHz) ^ ((unsigned int)y * kHy) ^ ((unsigned int)x * kHx));
+}
+
+// v*255 + d rounded as two separate operations, as written in pipeline.c:61
+__device__ __forceinline__ unsigned char quant_d(float v, float d) {
     #pragma clang fp contract(off)
-    // v*255 + d rounded as two separate operations, as written in pipeline.c:61
-    float t = v * 255.0f + (no_dither ? 0.5f : dither01(gz, gy, gx));
+    float t = v * 255.0f + d;
     if (t <= 0.0f) return 0;
     int q = (int)t;
     return q > 255 ? 255 : (unsigned char)q;
 }
 
+__device__ __forceinline__ unsigned char quant_u8(float v, long gz, long gy, long gx, int no_dither) {
+    return quant_d(v, no_dither ? 0.5f : dither01(gz, gy, gx));
+}
+
+// recenter of pipeline.c:413-421
+__device__ __forceinline__ float recenter(const fyg_emit &E, float v) {
+    #pragma clang fp contract(off)
+    if (v <= 0.0f) return 0.0f;
+    float w = 0.5f + (v - E.rmid) * E.rk;
+    return w < 0.0f ? 0.0f : (w > 1.0f ? 1.0f : w);
+}
+
 // quantised output of one voxel at local (z,y,x): recenter, dither, all-air
 __device__ __forceinline__ unsigned char emit_q(const fyg_emit &E, bool keep, long z, long y, long x, float v) {
-    #pragma clang fp contract(off)
     if (!keep) return 0;
-    if (E.recenter) {
-        if (v <= 0.0f) v = 0.0f;
-        else {
-            float w = 0.5f + (v - E.rmid) * E.rk;
-            v = w < 0.0f ? 0.0f : (w > 1.0f ? 1.0f : w);
-        }
-    }
+    if (E.recenter) v = recenter(E, v);
     return quant_u8(v, E.gz0 + z, E.gy0 + y, E.gx0 + x, E.no_dither);
 }
 
@@ -181,8 +193,15 @@ __device__ __forceinline__ void emit4(const fyg_emit &E, int ny, int nx, long z,
     }
     long lz = z - E.iz0, ly = y - E.iy0, lx = x - E.ix0;
     unsigned char *o = E.q + ((size_t)lz * E.ty + ly) * E.tx + lx;
+    const unsigned hzy = ((unsigned)(E.gz0 + z) * kHz) ^ ((unsigned)(E.gy0 + y) * kHy);
+    unsigned hx = (unsigned)(E.gx0 + x) * kHx;
     unsigned char q[4];
-    for (int j = 0; j < 4; j++) q[j] = emit_q(E, keep, z, y, x + j, v[j]);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        float w = E.recenter ? recenter(E, v[j]) : v[j];
+        q[j] = keep ? quant_d(w, E.no_dither ? 0.5f : dither_mix(hzy ^ hx)) : 0;
+        hx += kHx;
+    }
     if (vec) *(uchar4 *)o = make_uchar4(q[0], q[1], q[2], q[3]);
     else for (int j = 0; j < 4; j++) o[j] = q[j];
 }
@@ -262,9 +281,9 @@ __global__ __launch_bounds__(NT) void k_box2(Box2Args p) {
 //      rho planes back -> a = var/(var+eps), b = (m+c0)(1-a) for that plane,
 //   3. stages a, b in LDS and sums them the same way for the (16-2*rho)^2 inner
 //      columns -> mean_a, mean_b one more rho plane back.
-// The halo costs 1.3x in samples instead of the 3.4x of a 3-D tile, a and b
-// never reach HBM, and each plane is two barriers deep.  Cells outside the
-// grid contribute 0 and windows shrink at the faces (guided.c:22-98).
+// The halo costs 1.3x in samples instead of the 3.4x of a 3-D tile and a and b
+// never reach HBM.  Cells outside the grid contribute 0 and windows shrink at
+// the faces (guided.c:22-98).
 template <int RHO>
 struct March {
     static constexpr int AW = 16;                   // a, b columns per side (NT = AW*AW)
@@ -272,8 +291,18 @@ struct March {
     static constexpr int SW = AW + 2 * RHO;         // sample columns per side
     static constexpr int NS = SW * SW;
     static constexpr int KS = (NS + NT - 1) / NT;   // sample slots per thread
-    static constexpr int ZC = 16;                   // output planes per workgroup
+#ifndef FYG_COEF_ZC
+#define FYG_COEF_ZC 16
+#endif
+    static constexpr int ZC = FYG_COEF_ZC;          // output planes per workgroup
     static constexpr int Q = 2 * RHO + 1;           // window length
+    static constexpr int STEPS = ZC + 4 * RHO;      // sample planes of a full chunk
+    // sample planes in flight + 1; divides STEPS so the buffer ring rotates by
+    // full unrolling (a short last chunk runs idle steps)
+    static constexpr int PD = STEPS % 5 == 0 ? 5 : STEPS % 6 == 0 ? 6 : STEPS % 4 == 0 ? 4 : 1;
+    // LDS row pitches: the two rows of 16 columns a half-wave reads land in
+    // disjoint banks (pitch 48 = 16 mod 32 for samples, 16 for a and b)
+    static constexpr int SP = 48, AP = 16;
 };
 
 struct CoefArgs {
@@ -287,8 +316,9 @@ template <int RHO>
 __global__ __launch_bounds__(NT) void k_coef(CoefArgs p) {
     using M = March<RHO>;
     static_assert(M::AW * M::AW == NT, "one thread per a, b column");
-    __shared__ float sv[M::SW][M::SW + 1];
-    __shared__ float sa[M::AW][M::AW + 1], sb[M::AW][M::AW + 1];
+    static_assert(M::SW <= M::SP, "sample row fits its pitch");
+    __shared__ float sv[M::SW][M::SP];
+    __shared__ float sa[M::AW][M::AP], sb[M::AW][M::AP];
     const Grid &g = p.g;
     const int bx = blockIdx.x * M::OW, by = blockIdx.y * M::OW, zs = blockIdx.z * M::ZC;
     const int ze = min(zs + M::ZC, g.nz);
@@ -307,7 +337,7 @@ __global__ __launch_bounds__(NT) void k_coef(CoefArgs p) {
         int gx = bx - 2 * RHO + ex, gy = by - 2 * RHO + ey;
         sok[k] = i < M::NS && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny;
         soff[k] = sok[k] ? (size_t)gy * sy + (size_t)gx * g.s : 0;
-        scell[k] = i < M::NS ? ey * (M::SW + 1) + ex : -1;
+        scell[k] = i < M::NS ? ey * M::SP + ex : -1;
     }
     // this thread's a, b column (and output column when inner)
     const int ax = threadIdx.x % M::AW, ay = threadIdx.x / M::AW;
@@ -325,11 +355,7 @@ __global__ __launch_bounds__(NT) void k_coef(CoefArgs p) {
 #pragma unroll
         for (int k = 0; k < M::KS; k++) dst[k] = p.in[zo + soff[k]];
     };
-    float cur[M::KS], nxt[M::KS];
-    const int zfirst = zs - 2 * RHO, zlast = ze - 1 + 2 * RHO;
-    fetch(zfirst, cur);
-    for (int zp = zfirst; zp <= zlast; zp++) {
-        fetch(zp + 1, nxt);                         // next plane in flight meanwhile
+    auto step = [&](int zp, const float *cur) {
         const bool zin = zp >= 0 && zp < g.nz;
 #pragma unroll
         for (int k = 0; k < M::KS; k++)
@@ -391,7 +417,7 @@ __global__ __launch_bounds__(NT) void k_coef(CoefArgs p) {
             qa[M::Q - 1] = u1;
             qb[M::Q - 1] = u2;
             const int zo = zp - 2 * RHO;
-            if (inner && zo >= zs) {
+            if (inner && zo >= zs && zo < ze) {
                 float t1 = 0.0f, t2 = 0.0f;
 #pragma unroll
                 for (int j = 0; j < M::Q; j++) { t1 += qa[j]; t2 += qb[j]; }
@@ -401,8 +427,20 @@ __global__ __launch_bounds__(NT) void k_coef(CoefArgs p) {
                 p.mb[o] = t2 * rc;
             }
         }
+    };
+    // sample planes PD-1 ahead are in flight while a plane is processed; the
+    // buffer ring rotates by full unrolling, so no register copy waits on a load
+    constexpr int PD = M::PD;
+    const int zfirst = zs - 2 * RHO, zlast = ze - 1 + 2 * RHO;
+    float buf[PD][M::KS];
 #pragma unroll
-        for (int k = 0; k < M::KS; k++) cur[k] = nxt[k];
+    for (int d = 0; d < PD - 1; d++) fetch(zfirst + d, buf[d]);
+    for (int z0 = zfirst; z0 <= zlast; z0 += PD) {
+#pragma unroll
+        for (int u = 0; u < PD; u++) {
+            fetch(z0 + u + PD - 1, buf[(u + PD - 1) % PD]);
+            step(z0 + u, buf[u]);
+        }
     }
 }
 
@@ -433,16 +471,24 @@ struct UpArgs {
 
 constexpr int UPX = 4;               // consecutive x outputs per thread
 constexpr int UPK = 64 * UPX / 2 + 4;  // low-res columns one 256-wide chunk touches (s >= 2)
-constexpr int URZ = 2, URY = 8;      // output planes x rows of one workgroup
-constexpr int UPZ = 3, UPY = 6;      // low-res planes x rows they touch (s >= 2)
+#ifndef FYG_UP_RZ
+#define FYG_UP_RZ 2
+#endif
+constexpr int URZ = FYG_UP_RZ, URY = 8;  // output planes x rows of one workgroup
+constexpr int RPT = URZ * URY / 4;       // rows per thread (4 row groups)
+constexpr int UPZ = URZ / 2 + 2, UPY = 6;  // low-res planes x rows they touch (s >= 2)
+// patch rows store even columns first, then odd ones: a wave's reads of
+// columns 2q+e (s = 2) then hit consecutive words instead of every other one
+constexpr int UPH = UPK / 2;
+__device__ __forceinline__ int up_col(int c) { return (c & 1) * UPH + (c >> 1); }
 
 // trilinear upsample of (mean_a, mean_b) from the decimated grid and apply
 // out = a*I + b (guided.c:188-228): u = c/s, i0 = min((int)u, l-2), f = min(u-i0, 1).
-// A workgroup produces 2 planes x 8 rows x 256 voxels.  Those 16 rows share a
-// patch of at most 3 x 6 low-res rows, which is loaded into LDS once.  As in
+// A workgroup produces URZ planes x 8 rows x 256 voxels.  Those rows share a
+// patch of at most UPZ x 6 low-res rows, which is loaded into LDS once.  As in
 // the reference, each output row blends its four (z,y) low-res rows first and
 // then lerps along x; a thread blends only the three low-res columns its four
-// consecutive voxels need.  Thread (q, g) owns x quad q of rows g, g+4, g+8, g+12.
+// consecutive voxels need.  Thread (q, g) owns x quad q of rows g, g+4, g+8, ...
 // P2: s is a power of two, so c/s is exactly c*(1/s) and needs no division.
 template <bool P2>
 struct UpCoord {
@@ -469,10 +515,10 @@ __global__ __launch_bounds__(NT) void k_upsample(UpArgs p) {
     const int xb = xc0 + q * UPX;
     const bool xin = xb <= xc1, full = xb + UPX - 1 <= xc1;
 
-    // input voxels of the thread's four rows: issued first, used last
-    float iv[4][UPX];
+    // input voxels of the thread's rows: issued first, used last
+    float iv[RPT][UPX];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < RPT; j++) {
         int r = g + 4 * j, z = Z0 + r / URY, y = Y0 + r % URY;
         bool ok = xin && z <= zl && y <= yl;
         const float *irow = p.in + ((size_t)(ok ? z : Z0) * p.ny + (ok ? y : Y0)) * p.nx;
@@ -492,7 +538,10 @@ __global__ __launch_bounds__(NT) void k_upsample(UpArgs p) {
 #pragma unroll
         for (int i = 0; i < RW; i++) {
             int rr = g + 4 * i;
-            int pz = rr >= nyp ? (rr >= 2 * nyp ? 2 : 1) : 0, py = rr - pz * nyp;
+            int pz = 0;
+#pragma unroll
+            for (int k = 1; k < UPZ; k++) pz += rr >= k * nyp;
+            int py = rr - pz * nyp;
             size_t ro = ((size_t)(zlo + pz) * p.ly + (ylo + py)) * p.lx + kb;
 #pragma unroll
             for (int c = 0; c < CW; c++) {
@@ -506,13 +555,16 @@ __global__ __launch_bounds__(NT) void k_upsample(UpArgs p) {
 #pragma unroll
         for (int i = 0; i < RW; i++) {
             int rr = g + 4 * i;
-            int pz = rr >= nyp ? (rr >= 2 * nyp ? 2 : 1) : 0, py = rr - pz * nyp;
+            int pz = 0;
+#pragma unroll
+            for (int k = 1; k < UPZ; k++) pz += rr >= k * nyp;
+            int py = rr - pz * nyp;
 #pragma unroll
             for (int c = 0; c < CW; c++) {
                 int col = q + 64 * c;
                 if (rr < nrow && col < nk) {
-                    pa[pz][py][col] = va[i][c];
-                    pb[pz][py][col] = vb[i][c];
+                    pa[pz][py][up_col(col)] = va[i][c];
+                    pb[pz][py][up_col(col)] = vb[i][c];
                 }
             }
         }
@@ -530,10 +582,10 @@ __global__ __launch_bounds__(NT) void k_upsample(UpArgs p) {
         fxs[e] = fx > 1.0f ? 1.0f : fx;
         dsel[e] = x0 - kb - k0 != 0;            // 0 or 1
     }
-    const int c0 = min(k0, nk - 1), c1 = min(k0 + 1, nk - 1), c2 = min(k0 + 2, nk - 1);
+    const int c0 = up_col(min(k0, nk - 1)), c1 = up_col(min(k0 + 1, nk - 1)), c2 = up_col(min(k0 + 2, nk - 1));
     const bool keep = emit_keep(p.em);
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < RPT; j++) {
         int r = g + 4 * j, z = Z0 + r / URY, y = Y0 + r % URY;
         if (z > zl || y > yl) continue;
         int z0 = U.i0(z, p.lz), y0 = U.i0(y, p.ly);
