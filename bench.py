@@ -55,6 +55,23 @@ def kernel_bytes(name, n, p, s=2):
     return table.get(name)
 
 
+# HBM bytes per launch of each kernel on the 256^3 workload: rocprofv3
+# FETCH_SIZE and WRITE_SIZE of the same bench.py command, scaled by the same
+# counters read over a plain 1 GiB device copy (fetch reports 0.5 of the bytes
+# moved, write 1.0), from the run summarised in
+# profiles/chain_256cube_kernel_trace.md.  Used for roofline.traffic only.
+MEASURED_TRAFFIC_256 = {
+    "ring_image": 296045,
+    "rfft_x_u8": 86636153,
+    "cfft_y_fwd": 144839817,
+    "zfused": 145100598,
+    "cfft_y_inv": 144839817,
+    "irfft_x": 142886317,
+    "guided_coef": 92672173,
+    "guided_upsample": 105242624,
+}
+
+
 def build_workload(n, seed):
     from fysics_amd import phantom
     t0 = time.time()
@@ -193,7 +210,8 @@ def main():
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                "traffic": None, "algorithmic_bytes": dom_bytes, "avg_us": dom_us}
+                "traffic": MEASURED_TRAFFIC_256.get(dom) if n == 256 else None,
+                "algorithmic_bytes": dom_bytes, "avg_us": dom_us}
 
     # plain drop-in with host buffers (PCIe-inclusive; reported beside, never `value`)
     cfgc = F.pipeline_cfg(delta_beta=300.0, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
