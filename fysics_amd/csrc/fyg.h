@@ -44,6 +44,8 @@ int  fyg_last_error(void);                                /* sticky launch error
  * device flag; kernels that see a nonzero input voxel store g, consumers test
  * (*flag == g), so no clearing launch is needed between calls */
 int  fyg_flag(fyg_stream *s, int **d_flag);
+/* the same with n flags (one per cube of a batch), all tested against one g */
+int  fyg_flags(fyg_stream *s, int n, int **d_flags);
 const char *fyg_error_string(void);
 
 /* process-wide cache of small read-only device tables (twiddles, transfer
@@ -58,6 +60,11 @@ const void *fyg_twiddles(int L);
 void fyg_prof_enable(int on);
 int  fyg_prof_collect(char *names, int names_len, double *ms, long *count, int max);
 void fyg_prof_reset(void);
+
+/* Cubes of one shape run as one launch per stage: up to FYG_BATCH cubes'
+ * descriptors travel in the kernel arguments (no descriptor upload, so the
+ * launches can be captured), and larger groups take several launches. */
+#define FYG_BATCH 8
 
 /* ------------------------------------------------------------------ FFT / deconv */
 int fyg_fft_supported(int L);                   /* 2^k or 3*2^k, L <= FYG_FFT_MAX */
@@ -102,6 +109,8 @@ typedef struct {
     int nonzero_gen;
 } fyg_deconv_args;
 int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s);
+/* n deconvolutions of one shape (same nz..hxp, in_u8 and pro.dering) */
+int fyg_deconv_run_batch(const fyg_deconv_args *a, int n, fyg_stream *s);
 size_t fyg_deconv_spec_bytes(int pz, int py, int px);
 int fyg_deconv_hxp(int px);
 
@@ -132,6 +141,8 @@ typedef struct {
 } fyg_guided_args;
 size_t fyg_guided_ws_bytes(int nz, int ny, int nx, int s);
 int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s);
+/* n guided filters; cubes of one shape on the fast path share launches */
+int fyg_guided_run_batch(const fyg_guided_args *a, int n, fyg_stream *s);
 /* prologue-only path (deconv disabled): u8 -> f32 through the tile prologue */
 int fyg_prologue_run(const unsigned char *in, float *out, int nz, int ny, int nx,
                      const fyg_prologue *pro, int *nonzero_flag, int gen, fyg_stream *s);

@@ -23,6 +23,7 @@
 //   k_irfft_x   Hermitian repack of rows (y, y+1), complex inverse, 1/(pz*py*px),
 //               crop, fused rewindow / dec-range / clamp     (paganin.c:263-289)
 #include <cstdio>
+#include <vector>
 
 #include "fyg_internal.hpp"
 
@@ -317,16 +318,24 @@ __host__ __device__ __forceinline__ int ring_slab(const fyg_prologue &P, long gz
 // in double as dering.c:207-213 evaluates it, 0 where ri >= nr.  The X pass then
 // subtracts one coalesced, cache-resident float per voxel instead of
 // evaluating a radius and gathering from the ring table.
-__global__ __launch_bounds__(256) void k_ring_image(fyg_prologue P, int ny, int nx, int s0) {
+struct RingImg {
+    fyg_prologue P;
+    int ny, nx, s0, nimg;
+};
+
+// grid: (x / 256, y, cube * mslab + slab)
+__global__ __launch_bounds__(256) void k_ring_image(Batch<RingImg> b, int mslab) {
     #pragma clang fp contract(off)
-    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, si = blockIdx.z;
-    if (x >= nx) return;
+    const RingImg &r = b.a[blockIdx.z / mslab];
+    const fyg_prologue &P = r.P;
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, si = blockIdx.z % mslab;
+    if (x >= r.nx || y >= r.ny || si >= r.nimg) return;
     double dy = (double)(P.gy0 + y) - P.cy, dx = (double)(P.gx0 + x) - P.cx;
     double dy2 = dy * dy;
     int ri = (int)(sqrt(dy2 + dx * dx) + 0.5);
     float c = 0.0f;
-    if (ri < P.nr) c = P.scale * P.ring[(long)(s0 + si) * P.nr + ri];
-    P.cimg[((long)si * ny + y) * nx + x] = c;
+    if (ri < P.nr) c = P.scale * P.ring[(long)(r.s0 + si) * P.nr + ri];
+    P.cimg[((long)si * r.ny + y) * r.nx + x] = c;
 }
 
 __device__ __forceinline__ float epilogue(const fyg_epilogue &E, float v) {
@@ -360,7 +369,8 @@ struct XLine {
 
 // MODE 0: fp32 input; 1: u8 input with the prologue; 2: as 1 plus rings
 template <int L, int MODE>
-__global__ __launch_bounds__(256) void k_rfft_x(XFwd a) {
+__global__ __launch_bounds__(256) void k_rfft_x(Batch<XFwd> bt) {
+    const XFwd &a = bt.a[blockIdx.y];
     constexpr int NL = Shape<L>::NL_RM;
     constexpr bool U8 = MODE != 0, RING = MODE == 2;
     __shared__ float2 buf[NL * Shape<L>::LD];
@@ -449,7 +459,8 @@ struct Cols {
 };
 
 template <int L>
-__global__ __launch_bounds__(256) void k_cfft_cols(Cols a) {
+__global__ __launch_bounds__(256) void k_cfft_cols(Batch<Cols> bt) {
+    const Cols &a = bt.a[blockIdx.z];
     constexpr int NL = Shape<L>::NL_CM;
     __shared__ float2 buf[NL * Shape<L>::LD];
     __shared__ float2 tw_s[L];
@@ -482,7 +493,8 @@ struct ZFused {
 
 // forward Z (plan order), weight, inverse Z (reversed plan), all in one kernel
 template <int L>
-__global__ __launch_bounds__(256) void k_zfused(ZFused a) {
+__global__ __launch_bounds__(256) void k_zfused(Batch<ZFused> bt) {
+    const ZFused &a = bt.a[blockIdx.y];
     constexpr int NL = Shape<L>::NL_CM;
     constexpr int T = Shape<L>::T;
     using PF = Plan<L, false>;
@@ -614,7 +626,8 @@ struct XInvLine {
 };
 
 template <int L>
-__global__ __launch_bounds__(256) void k_irfft_x(XInv a) {
+__global__ __launch_bounds__(256) void k_irfft_x(Batch<XInv> bt) {
+    const XInv &a = bt.a[blockIdx.y];
     constexpr int NL = Shape<L>::NL_RM;
     __shared__ float2 buf[NL * Shape<L>::LD];
     __shared__ float2 tw_s[L];
@@ -709,24 +722,47 @@ unsigned grid1d(size_t n) {
 // ------------------------------------------------------------------ dispatch over L
 template <int L>
 struct Launch {
-    static void rfft_x(fyg_stream *s, const XFwd &a, bool u8) {
-        unsigned g = (unsigned)((a.nlines + Shape<L>::NL_RM - 1) / Shape<L>::NL_RM);
-        if (u8 && a.pro.dering) fyg_launch(s, "rfft_x_u8", k_rfft_x<L, 2>, dim3(g), dim3(256), 0, a);
-        else if (u8) fyg_launch(s, "rfft_x_u8", k_rfft_x<L, 1>, dim3(g), dim3(256), 0, a);
-        else fyg_launch(s, "rfft_x", k_rfft_x<L, 0>, dim3(g), dim3(256), 0, a);
+    static void rfft_x(fyg_stream *s, const XFwd *a, int n, bool u8) {
+        unsigned g = (unsigned)((a[0].nlines + Shape<L>::NL_RM - 1) / Shape<L>::NL_RM);
+        for (int i = 0; i < n; i += FYG_BATCH) {
+            int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+            Batch<XFwd> b{};
+            for (int j = 0; j < m; j++) b.a[j] = a[i + j];
+            if (u8 && a[0].pro.dering)
+                fyg_launch(s, "rfft_x_u8", k_rfft_x<L, 2>, dim3(g, m), dim3(256), 0, b);
+            else if (u8)
+                fyg_launch(s, "rfft_x_u8", k_rfft_x<L, 1>, dim3(g, m), dim3(256), 0, b);
+            else
+                fyg_launch(s, "rfft_x", k_rfft_x<L, 0>, dim3(g, m), dim3(256), 0, b);
+        }
     }
-    static void cols(fyg_stream *s, const Cols &a, int nbatch, const char *name) {
-        unsigned g = (unsigned)((a.ncols + Shape<L>::NL_CM - 1) / Shape<L>::NL_CM);
-        fyg_launch(s, name, k_cfft_cols<L>, dim3(g, nbatch), dim3(256), 0, a);
+    static void cols(fyg_stream *s, const Cols *a, int n, int nbatch, const char *name) {
+        unsigned g = (unsigned)((a[0].ncols + Shape<L>::NL_CM - 1) / Shape<L>::NL_CM);
+        for (int i = 0; i < n; i += FYG_BATCH) {
+            int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+            Batch<Cols> b{};
+            for (int j = 0; j < m; j++) b.a[j] = a[i + j];
+            fyg_launch(s, name, k_cfft_cols<L>, dim3(g, nbatch, m), dim3(256), 0, b);
+        }
     }
-    static void zfused(fyg_stream *s, const ZFused &a) {
-        long nc = (long)a.py * a.hxp;
+    static void zfused(fyg_stream *s, const ZFused *a, int n) {
+        long nc = (long)a[0].py * a[0].hxp;
         unsigned g = (unsigned)((nc + Shape<L>::NL_CM - 1) / Shape<L>::NL_CM);
-        fyg_launch(s, "zfused", k_zfused<L>, dim3(g), dim3(256), 0, a);
+        for (int i = 0; i < n; i += FYG_BATCH) {
+            int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+            Batch<ZFused> b{};
+            for (int j = 0; j < m; j++) b.a[j] = a[i + j];
+            fyg_launch(s, "zfused", k_zfused<L>, dim3(g, m), dim3(256), 0, b);
+        }
     }
-    static void irfft_x(fyg_stream *s, const XInv &a) {
-        unsigned g = (unsigned)((a.nlines + Shape<L>::NL_RM - 1) / Shape<L>::NL_RM);
-        fyg_launch(s, "irfft_x", k_irfft_x<L>, dim3(g), dim3(256), 0, a);
+    static void irfft_x(fyg_stream *s, const XInv *a, int n) {
+        unsigned g = (unsigned)((a[0].nlines + Shape<L>::NL_RM - 1) / Shape<L>::NL_RM);
+        for (int i = 0; i < n; i += FYG_BATCH) {
+            int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+            Batch<XInv> b{};
+            for (int j = 0; j < m; j++) b.a[j] = a[i + j];
+            fyg_launch(s, "irfft_x", k_irfft_x<L>, dim3(g, m), dim3(256), 0, b);
+        }
     }
     static void rows(fyg_stream *s, const Rows &a) {
         unsigned g = (unsigned)((a.nlines + Shape<L>::NL_RM - 1) / Shape<L>::NL_RM);
@@ -750,19 +786,19 @@ bool dispatch(int L, Args &&...args) {
 
 template <int L>
 struct DoRfftX {
-    static void go(fyg_stream *s, const XFwd &a, bool u8) { Launch<L>::rfft_x(s, a, u8); }
+    static void go(fyg_stream *s, const XFwd *a, int n, bool u8) { Launch<L>::rfft_x(s, a, n, u8); }
 };
 template <int L>
 struct DoCols {
-    static void go(fyg_stream *s, const Cols &a, int nb, const char *nm) { Launch<L>::cols(s, a, nb, nm); }
+    static void go(fyg_stream *s, const Cols *a, int n, int nb, const char *nm) { Launch<L>::cols(s, a, n, nb, nm); }
 };
 template <int L>
 struct DoZ {
-    static void go(fyg_stream *s, const ZFused &a) { Launch<L>::zfused(s, a); }
+    static void go(fyg_stream *s, const ZFused *a, int n) { Launch<L>::zfused(s, a, n); }
 };
 template <int L>
 struct DoIrfftX {
-    static void go(fyg_stream *s, const XInv &a) { Launch<L>::irfft_x(s, a); }
+    static void go(fyg_stream *s, const XInv *a, int n) { Launch<L>::irfft_x(s, a, n); }
 };
 template <int L>
 struct DoRows {
@@ -786,36 +822,67 @@ extern "C" size_t fyg_deconv_spec_bytes(int pz, int py, int px) {
     return sizeof(float2) * (size_t)pz * py * fyg_deconv_hxp(px);
 }
 
-extern "C" int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s) {
-    if (!fyg_fft_supported(a->px) || !fyg_fft_supported(a->py) || !fyg_fft_supported(a->pz)) return 1;
-    const float2 *twx = (const float2 *)fyg_twiddles(a->px);
-    const float2 *twy = (const float2 *)fyg_twiddles(a->py);
-    const float2 *twz = (const float2 *)fyg_twiddles(a->pz);
-    if (!twx || !twy || !twz) return 1;
-    float2 *spec = (float2 *)a->spec;
-    XFwd xf{a->in, a->nz, a->ny, a->nx, a->pz, a->py, a->hx, a->hxp,
-            (long)a->pz * ((a->py + 1) / 2), (long)(a->py + 1) / 2, spec, twx, a->pro, a->nonzero_flag,
-            a->nonzero_gen};
-    if (a->in_u8 && a->pro.dering) {
-        if (!a->pro.cimg) return 1;
-        int s0 = ring_slab(a->pro, a->pro.gz0), s1 = ring_slab(a->pro, a->pro.gz0 + a->nz - 1);
-        fyg_launch(s, "ring_image", k_ring_image, dim3((unsigned)((a->nx + 255) / 256), (unsigned)a->ny,
-                   (unsigned)(s1 - s0 + 1)), dim3(256), 0, a->pro, a->ny, a->nx, s0);
+extern "C" int fyg_deconv_run_batch(const fyg_deconv_args *a, int n, fyg_stream *s) {
+    if (n < 1) return 0;
+    const fyg_deconv_args &h = a[0];
+    if (!fyg_fft_supported(h.px) || !fyg_fft_supported(h.py) || !fyg_fft_supported(h.pz)) return 1;
+    for (int i = 1; i < n; i++) {                   // one shape and one kernel variant per batch
+        const fyg_deconv_args &b = a[i];
+        if (b.nz != h.nz || b.ny != h.ny || b.nx != h.nx || b.pz != h.pz || b.py != h.py || b.px != h.px ||
+            b.hx != h.hx || b.hxp != h.hxp || (b.in_u8 != 0) != (h.in_u8 != 0) ||
+            (b.in_u8 && (b.pro.dering != 0) != (h.pro.dering != 0)))
+            return 1;
     }
-    dispatch<DoRfftX>(a->px, s, xf, a->in_u8 != 0);
-    long plane = (long)a->py * a->hxp;
-    Cols yf{spec, a->hx, a->hxp, plane, -1.0f, twy};        // columns k < hx: the pitch padding stays idle
-    dispatch<DoCols>(a->py, s, yf, a->pz, "cfft_y_fwd");
-    ZFused zf{spec, a->py, a->hx, a->hxp, plane, a->lut, a->us, a->fz2, a->fy2, a->fx2, twz};
-    dispatch<DoZ>(a->pz, s, zf);
-    Cols yi{spec, a->hx, a->hxp, plane, 1.0f, twy};
-    dispatch<DoCols>(a->py, s, yi, a->nz, "cfft_y_inv");
-    XInv xi{spec, a->out, a->nz, a->ny, a->nx, a->py, a->hx, a->hxp,
-            (long)a->nz * ((a->ny + 1) / 2), (long)(a->ny + 1) / 2,
-            1.0f / ((float)a->pz * (float)a->py * (float)a->px), a->epi, twx};
-    dispatch<DoIrfftX>(a->px, s, xi);
+    const float2 *twx = (const float2 *)fyg_twiddles(h.px);
+    const float2 *twy = (const float2 *)fyg_twiddles(h.py);
+    const float2 *twz = (const float2 *)fyg_twiddles(h.pz);
+    if (!twx || !twy || !twz) return 1;
+    const long plane = (long)h.py * h.hxp;
+    std::vector<XFwd> xf(n);
+    std::vector<Cols> yf(n), yi(n);
+    std::vector<ZFused> zf(n);
+    std::vector<XInv> xi(n);
+    for (int i = 0; i < n; i++) {
+        const fyg_deconv_args *d = &a[i];
+        float2 *spec = (float2 *)d->spec;
+        xf[i] = XFwd{d->in, d->nz, d->ny, d->nx, d->pz, d->py, d->hx, d->hxp,
+                     (long)d->pz * ((d->py + 1) / 2), (long)(d->py + 1) / 2, spec, twx, d->pro, d->nonzero_flag,
+                     d->nonzero_gen};
+        // columns k < hx: the pitch padding stays idle
+        yf[i] = Cols{spec, d->hx, d->hxp, plane, -1.0f, twy};
+        zf[i] = ZFused{spec, d->py, d->hx, d->hxp, plane, d->lut, d->us, d->fz2, d->fy2, d->fx2, twz};
+        yi[i] = Cols{spec, d->hx, d->hxp, plane, 1.0f, twy};
+        xi[i] = XInv{spec, d->out, d->nz, d->ny, d->nx, d->py, d->hx, d->hxp,
+                     (long)d->nz * ((d->ny + 1) / 2), (long)(d->ny + 1) / 2,
+                     1.0f / ((float)d->pz * (float)d->py * (float)d->px), d->epi, twx};
+    }
+    if (h.in_u8 && h.pro.dering) {
+        std::vector<RingImg> ri(n);
+        int mslab = 1;
+        for (int i = 0; i < n; i++) {
+            const fyg_prologue &P = a[i].pro;
+            if (!P.cimg) return 1;
+            int s0 = ring_slab(P, P.gz0), s1 = ring_slab(P, P.gz0 + a[i].nz - 1);
+            ri[i] = RingImg{P, a[i].ny, a[i].nx, s0, s1 - s0 + 1};
+            if (s1 - s0 + 1 > mslab) mslab = s1 - s0 + 1;
+        }
+        for (int i = 0; i < n; i += FYG_BATCH) {
+            int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+            Batch<RingImg> b{};
+            for (int j = 0; j < m; j++) b.a[j] = ri[i + j];
+            fyg_launch(s, "ring_image", k_ring_image, dim3((unsigned)((h.nx + 255) / 256), (unsigned)h.ny,
+                       (unsigned)(m * mslab)), dim3(256), 0, b, mslab);
+        }
+    }
+    dispatch<DoRfftX>(h.px, s, xf.data(), n, h.in_u8 != 0);
+    dispatch<DoCols>(h.py, s, yf.data(), n, h.pz, "cfft_y_fwd");
+    dispatch<DoZ>(h.pz, s, zf.data(), n);
+    dispatch<DoCols>(h.py, s, yi.data(), n, h.nz, "cfft_y_inv");
+    dispatch<DoIrfftX>(h.px, s, xi.data(), n);
     return fyg_last_error();
 }
+
+extern "C" int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s) { return fyg_deconv_run_batch(a, 1, s); }
 
 extern "C" size_t fyg_ring_image_bytes(const fyg_prologue *p, int nz, int ny, int nx) {
     if (!p->dering) return 0;
@@ -835,11 +902,11 @@ extern "C" int fyg_fft3d_run(float *d_re, float *d_im, int nz, int ny, int nx, i
     dispatch<DoRows>(nx, s, rx);
     if (ny > 1) {
         Cols cy{c, nx, nx, (long)ny * nx, sg, (const float2 *)fyg_twiddles(ny)};
-        dispatch<DoCols>(ny, s, cy, nz, "cfft_cols");
+        dispatch<DoCols>(ny, s, &cy, 1, nz, "cfft_cols");
     }
     if (nz > 1) {
         Cols cz{c, (long)ny * nx, (long)ny * nx, 0, sg, (const float2 *)fyg_twiddles(nz)};
-        dispatch<DoCols>(nz, s, cz, 1, "cfft_cols");
+        dispatch<DoCols>(nz, s, &cz, 1, 1, "cfft_cols");
     }
     fyg_launch(s, "deinterleave", k_deinterleave, dim3(grid1d(n)), dim3(256), 0, (const float2 *)c, d_re,
                d_im, n);

@@ -17,6 +17,8 @@
 //                                  k_box2 -> ma, mb (low-res); k_upsample -> out
 #include <cstdio>
 
+#include <vector>
+
 #include "fyg_internal.hpp"
 
 namespace {
@@ -312,15 +314,18 @@ struct CoefArgs {
     float *ma, *mb;
 };
 
+// grid: (x blocks, y blocks, cube * nzc + z chunk)
 template <int RHO>
-__global__ __launch_bounds__(NT) void k_coef(CoefArgs p) {
+__global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs> bt, int nzc) {
     using M = March<RHO>;
+    const CoefArgs &p = bt.a[blockIdx.z / nzc];
+    const int zchunk = blockIdx.z % nzc;
     static_assert(M::AW * M::AW == NT, "one thread per a, b column");
     static_assert(M::SW <= M::SP, "sample row fits its pitch");
     __shared__ float sv[M::SW][M::SP];
     __shared__ float sa[M::AW][M::AP], sb[M::AW][M::AP];
     const Grid &g = p.g;
-    const int bx = blockIdx.x * M::OW, by = blockIdx.y * M::OW, zs = blockIdx.z * M::ZC;
+    const int bx = blockIdx.x * M::OW, by = blockIdx.y * M::OW, zs = zchunk * M::ZC;
     const int ze = min(zs + M::ZC, g.nz);
     const size_t sz = (size_t)g.s * g.fny * g.fnx, sy = (size_t)g.s * g.fnx;
     const float c0 = p.in[(size_t)zs * sz + (size_t)by * sy + (size_t)bx * g.s];
@@ -500,11 +505,13 @@ struct UpCoord {
     }
 };
 
+// grid: (x chunks, y blocks, cube * nzb + z block)
 template <bool P2>
-__global__ __launch_bounds__(NT) void k_upsample(UpArgs p) {
+__global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs> bt, int nzb) {
+    const UpArgs &p = bt.a[blockIdx.z / nzb];
     __shared__ float pa[UPZ][UPY][UPK], pb[UPZ][UPY][UPK];
     const UpCoord<P2> U{(float)p.s, 1.0f / (float)p.s};
-    const int Z0 = p.z0 + blockIdx.z * URZ, Y0 = p.y0 + blockIdx.y * URY;
+    const int Z0 = p.z0 + (int)(blockIdx.z % nzb) * URZ, Y0 = p.y0 + blockIdx.y * URY;
     const int zl = min(Z0 + URZ, p.z0 + p.dz) - 1, yl = min(Y0 + URY, p.y0 + p.dy) - 1;
     const int xc0 = p.x0 + blockIdx.x * 64 * UPX;
     const int xc1 = min(xc0 + 64 * UPX, p.x0 + p.dx) - 1;
@@ -774,6 +781,38 @@ extern "C" size_t fyg_guided_ws_bytes(int nz, int ny, int nx, int s) {
     return sizeof(float) * 6 * n;       // a, b (+ 4 temporaries for the wide-radius fallback)
 }
 
+// upsample + emit arguments of one fast-path filter (region = inner tile when quantising)
+static UpArgs up_args(const fyg_guided_args *a, const float *MA, const float *MB) {
+    const int sd = a->s, nz = a->nz, ny = a->ny, nx = a->nx;
+    const int lz = (nz + sd - 1) / sd, ly = (ny + sd - 1) / sd, lx = (nx + sd - 1) / sd;
+    UpArgs u{nz, ny, nx, lz, ly, lx, sd, MA, MB, a->in, 0, 0, 0, nz, ny, nx, 0, 0, a->em};
+    if (a->em.quant) {
+        u.z0 = (int)a->em.iz0; u.y0 = (int)a->em.iy0; u.x0 = (int)a->em.ix0;
+        u.dz = (int)a->em.tz; u.dy = (int)a->em.ty; u.dx = (int)a->em.tx;
+    }
+    u.vin = (nx % 4 == 0) && (u.x0 % 4 == 0) && ((uintptr_t)a->in % 16 == 0);
+    u.vout = a->em.quant ? (a->em.tx % 4 == 0 && (uintptr_t)a->em.q % 4 == 0)
+                         : (u.vin && (uintptr_t)a->em.out % 16 == 0);
+    return u;
+}
+
+// upsample launches for n filters whose output regions have one shape
+static void launch_upsample(fyg_stream *s, const UpArgs *u, int n, int sd) {
+    if (n < 1 || (size_t)u[0].dz * u[0].dy * u[0].dx == 0) return;
+    const int nzb = (u[0].dz + URZ - 1) / URZ;
+    for (int i = 0; i < n; i += FYG_BATCH) {
+        int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+        Batch<UpArgs> b{};
+        for (int j = 0; j < m; j++) b.a[j] = u[i + j];
+        dim3 grid((unsigned)((u[0].dx + 64 * UPX - 1) / (64 * UPX)), (unsigned)((u[0].dy + URY - 1) / URY),
+                  (unsigned)(nzb * m));
+        if ((sd & (sd - 1)) == 0) fyg_launch(s, "guided_upsample", k_upsample<true>, grid, dim3(NT), 0, b, nzb);
+        else fyg_launch(s, "guided_upsample", k_upsample<false>, grid, dim3(NT), 0, b, nzb);
+    }
+}
+
+static int fast_batch(const fyg_guided_args *a, int n, fyg_stream *s);
+
 extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
     int r = a->r < 1 ? 2 : a->r;
     int sd = a->s;
@@ -817,13 +856,10 @@ extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
     size_t nl = (size_t)lz * ly * lx;
     int rl = r / sd;
     if (rl < 1) rl = 1;
+    if (rl <= 2) return fast_batch(a, 1, s);
     float *A = ws, *B = ws + nl, *MA = ws + 2 * nl, *MB = ws + 3 * nl;
     Grid gl{lz, ly, lx, sd, ny, nx};
-    if (rl <= 2) {
-        CoefArgs c{gl, a->in, a->eps, MA, MB};
-        if (rl == 1) fyg_launch(s, "guided_coef", k_coef<1>, march_grid<1>(gl), dim3(NT), 0, c);
-        else fyg_launch(s, "guided_coef", k_coef<2>, march_grid<2>(gl), dim3(NT), 0, c);
-    } else {
+    {
         float *T0 = ws + 4 * nl, *T1 = ws + 5 * nl, *O0 = ws + 6 * nl, *O1 = ws + 7 * nl;
         float c0 = 0.0f;
         if (fyg_d2h(&c0, a->in, sizeof(float), s) || fyg_sync(s)) return 1;
@@ -835,21 +871,65 @@ extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
         box_global(s, gl, rl, 2, A, B, O0, O1, T0, T1);
         if (fyg_d2d(MA, O0, sizeof(float) * nl, s) || fyg_d2d(MB, O1, sizeof(float) * nl, s)) return 1;
     }
-    UpArgs u{nz, ny, nx, lz, ly, lx, sd, MA, MB, a->in, 0, 0, 0, nz, ny, nx, 0, 0, a->em};
-    if (a->em.quant) {
-        u.z0 = (int)a->em.iz0; u.y0 = (int)a->em.iy0; u.x0 = (int)a->em.ix0;
-        u.dz = (int)a->em.tz; u.dy = (int)a->em.ty; u.dx = (int)a->em.tx;
-    }
-    u.vin = (nx % 4 == 0) && (u.x0 % 4 == 0) && ((uintptr_t)a->in % 16 == 0);
-    u.vout = a->em.quant ? (a->em.tx % 4 == 0 && (uintptr_t)a->em.q % 4 == 0)
-                         : (u.vin && (uintptr_t)a->em.out % 16 == 0);
-    if ((size_t)u.dz * u.dy * u.dx) {
-        dim3 grid((unsigned)((u.dx + 64 * UPX - 1) / (64 * UPX)), (unsigned)((u.dy + URY - 1) / URY),
-                  (unsigned)((u.dz + URZ - 1) / URZ));
-        if ((sd & (sd - 1)) == 0) fyg_launch(s, "guided_upsample", k_upsample<true>, grid, dim3(NT), 0, u);
-        else fyg_launch(s, "guided_upsample", k_upsample<false>, grid, dim3(NT), 0, u);
-    }
+    UpArgs u = up_args(a, MA, MB);
+    launch_upsample(s, &u, 1, sd);
     return fyg_last_error();
+}
+
+// fast path with rl = r/s <= 2 for n filters of one shape: per stage one launch
+// per FYG_BATCH cubes (coefficients, then upsample + emit)
+static int fast_batch(const fyg_guided_args *a, int n, fyg_stream *s) {
+    const fyg_guided_args &h = a[0];
+    const int sd = h.s, r = h.r < 1 ? 2 : h.r;
+    const int lz = (h.nz + sd - 1) / sd, ly = (h.ny + sd - 1) / sd, lx = (h.nx + sd - 1) / sd;
+    const size_t nl = (size_t)lz * ly * lx;
+    int rl = r / sd;
+    if (rl < 1) rl = 1;
+    const Grid gl{lz, ly, lx, sd, h.ny, h.nx};
+    std::vector<CoefArgs> c(n);
+    std::vector<UpArgs> u(n);
+    for (int i = 0; i < n; i++) {
+        float *ws = (float *)a[i].ws;
+        float *MA = ws + 2 * nl, *MB = ws + 3 * nl;
+        c[i] = CoefArgs{gl, a[i].in, a[i].eps, MA, MB};
+        u[i] = up_args(&a[i], MA, MB);
+    }
+    for (int i = 0; i < n; i += FYG_BATCH) {
+        int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+        Batch<CoefArgs> b{};
+        for (int j = 0; j < m; j++) b.a[j] = c[i + j];
+        if (rl == 1) {
+            dim3 g = march_grid<1>(gl);
+            int nzc = (int)g.z;
+            g.z *= m;
+            fyg_launch(s, "guided_coef", k_coef<1>, g, dim3(NT), 0, b, nzc);
+        } else {
+            dim3 g = march_grid<2>(gl);
+            int nzc = (int)g.z;
+            g.z *= m;
+            fyg_launch(s, "guided_coef", k_coef<2>, g, dim3(NT), 0, b, nzc);
+        }
+    }
+    launch_upsample(s, u.data(), n, sd);
+    return fyg_last_error();
+}
+
+extern "C" int fyg_guided_run_batch(const fyg_guided_args *a, int n, fyg_stream *s) {
+    if (n < 1) return 0;
+    const fyg_guided_args &h = a[0];
+    const int r = h.r < 1 ? 2 : h.r;
+    bool batch = h.s > 1 && (h.nz + h.s - 1) / h.s >= 2 && (h.ny + h.s - 1) / h.s >= 2 &&
+                 (h.nx + h.s - 1) / h.s >= 2 && r / h.s <= 2;
+    for (int i = 1; i < n && batch; i++) {
+        const fyg_guided_args &b = a[i];
+        batch = b.nz == h.nz && b.ny == h.ny && b.nx == h.nx && b.s == h.s && (b.r < 1 ? 2 : b.r) == r &&
+                (b.em.quant != 0) == (h.em.quant != 0) &&
+                (!h.em.quant || (b.em.tz == h.em.tz && b.em.ty == h.em.ty && b.em.tx == h.em.tx));
+    }
+    if (batch) return fast_batch(a, n, s);
+    for (int i = 0; i < n; i++)
+        if (fyg_guided_run(&a[i], s)) return 1;
+    return 0;
 }
 
 extern "C" int fyg_emit_run(const float *f, int nz, int ny, int nx, const fyg_emit *em, fyg_stream *s) {

@@ -13,7 +13,8 @@ struct fyg_stream {
     size_t wcap[FYG_WS_SLOTS] = {};
     void *pin[2] = {};
     size_t pcap[2] = {};
-    int *flag = nullptr;     // all-air flag: holds the generation of the last call that saw data
+    int *flag = nullptr;     // all-air flags: hold the generation of the last call that saw data
+    int nflag = 0;
     int gen = 0;
 };
 
@@ -31,6 +32,13 @@ inline void fyg_launch(fyg_stream *s, const char *name, void (*k)(KArgs...), dim
     k<<<g, b, sh, s->s>>>(static_cast<KArgs>(args)...);
     fyg_prof::end(s, tok);
 }
+
+// per-cube descriptors of one batched launch, passed by value in the kernel
+// arguments; block index -> cube
+template <class T>
+struct Batch {
+    T a[FYG_BATCH];
+};
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);

@@ -137,20 +137,30 @@ extern "C" int fyg_zero(void *dst, size_t bytes, fyg_stream *s) {
     return fyg_last_error();
 }
 
-extern "C" int fyg_flag(fyg_stream *s, int **d_flag) {
-    if (!s) return 0;
-    if (!s->flag) {
-        if (note(hipMalloc((void **)&s->flag, 64), "hipMalloc(flag)")) return 0;
-        if (fyg_zero(s->flag, 64, s)) return 0;
-        s->gen = 0;
+extern "C" int fyg_flags(fyg_stream *s, int n, int **d_flags) {
+    if (!s || n < 1) return 0;
+    if (n > s->nflag) {
+        // earlier generations may still be tested by queued work on this stream
+        if (s->flag) {
+            if (note(hipStreamSynchronize(s->s), "hipStreamSynchronize")) return 0;
+            (void)hipFree(s->flag);
+            s->flag = nullptr;
+            s->nflag = 0;
+        }
+        int cap = n < 16 ? 16 : n;
+        if (note(hipMalloc((void **)&s->flag, sizeof(int) * cap), "hipMalloc(flag)")) return 0;
+        if (fyg_zero(s->flag, sizeof(int) * cap, s)) return 0;   // zero never equals a generation
+        s->nflag = cap;
     }
     if (s->gen >= 0x7ffffff0) {              // wrap: clear so no stale generation can match
-        if (fyg_zero(s->flag, 64, s)) return 0;
+        if (fyg_zero(s->flag, sizeof(int) * s->nflag, s)) return 0;
         s->gen = 0;
     }
-    *d_flag = s->flag;
+    *d_flags = s->flag;
     return ++s->gen;
 }
+
+extern "C" int fyg_flag(fyg_stream *s, int **d_flag) { return fyg_flags(s, 1, d_flag); }
 
 extern "C" int fyg_last_error(void) {
     return note(hipGetLastError(), "kernel launch");

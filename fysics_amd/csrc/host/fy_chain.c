@@ -97,39 +97,56 @@ int fy_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long t
     return 0;
 }
 
-static int chain_cube(const fy_cube *c, int *d_flag, int gen, fyg_stream *s) {
+/* One cube's chain, planned on the host: every argument of its device stages
+ * except the workspace addresses, which run_group() hands out per group. */
+typedef struct {
+    const fy_cube *c;
+    int skip;                   /* empty region: nothing to do */
+    size_t hn;                  /* voxels of the halo'd region */
+    int deconv, guided;
+    fyg_prologue pro;           /* u8 prologue (both paths) */
+    fyg_deconv_args d;          /* deconv = 1 */
+    fyg_guided_args g;          /* guided = 1 */
+    fyg_emit em;
+    size_t spec_bytes, img_bytes, gws_bytes;
+} cube_plan;
+
+static int plan_cube(const fy_cube *c, cube_plan *pl) {
+    memset(pl, 0, sizeof *pl);
+    pl->c = c;
     const fy_cube_params *P = c->params;
     long rz0 = c->rz0, ry0 = c->ry0, rx0 = c->rx0, hz = c->hz, hy = c->hy, hx = c->hx;
-    if (hz <= 0 || hy <= 0 || hx <= 0 || c->tz <= 0 || c->ty <= 0 || c->tx <= 0) return 0;
-    if (c->z0 < rz0 || c->y0 < ry0 || c->x0 < rx0 || c->z0 + c->tz > rz0 + hz || c->y0 + c->ty > ry0 + hy ||
-        c->x0 + c->tx > rx0 + hx)
+    if (hz <= 0 || hy <= 0 || hx <= 0 || c->tz <= 0 || c->ty <= 0 || c->tx <= 0) {
+        pl->skip = 1;
+        return 0;
+    }
+    if (!P || c->z0 < rz0 || c->y0 < ry0 || c->x0 < rx0 || c->z0 + c->tz > rz0 + hz ||
+        c->y0 + c->ty > ry0 + hy || c->x0 + c->tx > rx0 + hx)
         return 1;                                               /* inner tile must lie in the region */
-    size_t hn = (size_t)hz * hy * hx;
-    float *f = (float *)fyg_ws(s, WS_TILE, sizeof(float) * hn);
-    if (!f) return 1;
+    pl->hn = (size_t)hz * hy * hx;
 
     /* prologue (pipeline.c:492-494 and the dewindow of :266-268) */
-    fyg_prologue pro;
-    memset(&pro, 0, sizeof pro);
-    pro.gz0 = rz0; pro.gy0 = ry0; pro.gx0 = rx0;
+    fyg_prologue *pro = &pl->pro;
+    pro->gz0 = rz0; pro->gy0 = ry0; pro->gx0 = rx0;
     if (P->dering && P->dering->have_rings) {
         const fy_dering *d = P->dering;
-        pro.ring = (const float *)fyh_const_blob('R', d->ring, sizeof(float) * (size_t)d->nslab * d->nr);
-        if (!pro.ring) return 1;
-        pro.dering = 1; pro.nr = d->nr; pro.nslab = d->nslab; pro.slab_z = d->slab_z;
-        pro.cy = d->cy; pro.cx = d->cx; pro.scale = (float)(1.0 / 255.0);
-        pro.ring_fast = fyh_ring_fast(d->cy, d->cx, ry0, rx0, hy, hx);
+        pro->ring = (const float *)fyh_const_blob('R', d->ring, sizeof(float) * (size_t)d->nslab * d->nr);
+        if (!pro->ring) return 1;
+        pro->dering = 1; pro->nr = d->nr; pro->nslab = d->nslab; pro->slab_z = d->slab_z;
+        pro->cy = d->cy; pro->cx = d->cx; pro->scale = (float)(1.0 / 255.0);
+        pro->ring_fast = fyh_ring_fast(d->cy, d->cx, ry0, rx0, hy, hx);
     }
     if (P->zdrift_factor) {
         if (P->vol_z < rz0 + hz || rz0 < 0) return 1;           /* factor table must cover the region */
-        pro.zfactor = (const float *)fyh_const_blob('Z', P->zdrift_factor, sizeof(float) * (size_t)P->vol_z);
-        if (!pro.zfactor) return 1;
-        pro.zdrift = 1;
+        pro->zfactor = (const float *)fyh_const_blob('Z', P->zdrift_factor, sizeof(float) * (size_t)P->vol_z);
+        if (!pro->zfactor) return 1;
+        pro->zdrift = 1;
     }
     int windowed = P->window_hi > P->window_lo;
     float wlo = (float)P->window_lo, wspan = (float)(P->window_hi - P->window_lo);
 
     if (P->deconv) {                                            /* (b) */
+        pl->deconv = 1;
         int matched = P->deconv == 2;
         double param = P->deconv_param;
         if (param <= 0) param = matched ? 0.05 : fy_auto_reg(&P->phys, 0.015);
@@ -138,84 +155,181 @@ static int chain_cube(const fy_cube *c, int *d_flag, int gen, fyg_stream *s) {
                            P->fz2_scale > 0 ? P->fz2_scale : 1.0, &t))
             return 1;
         if (!fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px)) return 1;
-        void *spec = fyg_ws(s, WS_SPEC, fyg_deconv_spec_bytes(t.pz, t.py, t.px));
-        if (!spec) return 1;
-        fyg_deconv_args a;
-        memset(&a, 0, sizeof a);
-        a.nz = (int)hz; a.ny = (int)hy; a.nx = (int)hx;
-        a.pz = t.pz; a.py = t.py; a.px = t.px; a.hx = t.hx; a.hxp = t.hxp;
-        a.lut = t.lut; a.us = t.us; a.fz2 = t.fz2; a.fy2 = t.fy2; a.fx2 = t.fx2;
-        a.in = c->d_in; a.in_u8 = 1;
-        a.pro = pro;
-        if (pro.dering) {
-            a.pro.cimg = (float *)fyg_ws(s, WS_MISC, fyg_ring_image_bytes(&pro, (int)hz, (int)hy, (int)hx));
-            if (!a.pro.cimg) return 1;
-        }
-        if (windowed) { a.pro.window = 1; a.pro.wlo = wlo; a.pro.wspan = wspan; }
-        a.out = f;
-        if (windowed) { a.epi.window = 1; a.epi.wlo = wlo; a.epi.winv = 1.0f / wspan; }
+        fyg_deconv_args *a = &pl->d;
+        a->nz = (int)hz; a->ny = (int)hy; a->nx = (int)hx;
+        a->pz = t.pz; a->py = t.py; a->px = t.px; a->hx = t.hx; a->hxp = t.hxp;
+        a->lut = t.lut; a->us = t.us; a->fz2 = t.fz2; a->fy2 = t.fy2; a->fx2 = t.fx2;
+        a->in = c->d_in; a->in_u8 = 1;
+        a->pro = *pro;
+        if (windowed) { a->pro.window = 1; a->pro.wlo = wlo; a->pro.wspan = wspan; }
+        if (windowed) { a->epi.window = 1; a->epi.wlo = wlo; a->epi.winv = 1.0f / wspan; }
         if (P->have_dec_range && P->dec_hi - P->dec_lo > 1e-6) {
-            a.epi.dec = 1; a.epi.dlo = (float)P->dec_lo; a.epi.dinv = 1.0f / (float)(P->dec_hi - P->dec_lo);
+            a->epi.dec = 1; a->epi.dlo = (float)P->dec_lo; a->epi.dinv = 1.0f / (float)(P->dec_hi - P->dec_lo);
         }
-        a.epi.clamp01 = 1;
-        a.spec = spec;
-        a.nonzero_flag = d_flag;
-        a.nonzero_gen = gen;
-        if (fyg_deconv_run(&a, s)) return 1;
-    } else {
-        if (fyg_prologue_run(c->d_in, f, (int)hz, (int)hy, (int)hx, &pro, d_flag, gen, s)) return 1;
+        a->epi.clamp01 = 1;
+        pl->spec_bytes = fyg_deconv_spec_bytes(t.pz, t.py, t.px);
+        if (pro->dering) pl->img_bytes = fyg_ring_image_bytes(pro, (int)hz, (int)hy, (int)hx);
     }
 
-    fyg_emit em;
-    memset(&em, 0, sizeof em);
-    em.quant = 1;
-    em.q = c->d_out;
-    em.iz0 = c->z0 - rz0; em.iy0 = c->y0 - ry0; em.ix0 = c->x0 - rx0;
-    em.tz = c->tz; em.ty = c->ty; em.tx = c->tx;
-    em.gz0 = rz0; em.gy0 = ry0; em.gx0 = rx0;
-    em.no_dither = P->no_dither;
+    fyg_emit *em = &pl->em;
+    em->quant = 1;
+    em->q = c->d_out;
+    em->iz0 = c->z0 - rz0; em->iy0 = c->y0 - ry0; em->ix0 = c->x0 - rx0;
+    em->tz = c->tz; em->ty = c->ty; em->tx = c->tx;
+    em->gz0 = rz0; em->gy0 = ry0; em->gx0 = rx0;
+    em->no_dither = P->no_dither;
     if (P->recenter && P->norm_hi > P->norm_lo) {               /* (d.5) */
-        em.recenter = 1;
-        em.rmid = 0.5f * (float)(P->norm_lo + P->norm_hi) / 255.0f;
-        em.rk = (2.0f * 0.375f) * 255.0f / (float)(P->norm_hi - P->norm_lo);
+        em->recenter = 1;
+        em->rmid = 0.5f * (float)(P->norm_lo + P->norm_hi) / 255.0f;
+        em->rk = (2.0f * 0.375f) * 255.0f / (float)(P->norm_hi - P->norm_lo);
     }
-    em.nonzero_flag = d_flag;
-    em.nonzero_gen = gen;
 
-    const float *src = f;
     if (P->guided_eps > 0) {                                    /* (c) */
-        fyg_guided_args g;
-        memset(&g, 0, sizeof g);
-        g.nz = (int)hz; g.ny = (int)hy; g.nx = (int)hx;
-        g.r = P->guided_radius < 1 ? 2 : P->guided_radius;
-        g.s = P->guided_s == 0 ? 2 : P->guided_s;
-        g.eps = P->guided_eps;
-        g.in = f;
-        g.ws = fyg_ws(s, WS_GUIDED, fyg_guided_ws_bytes(g.nz, g.ny, g.nx, g.s));
-        if (!g.ws) return 1;
-        if (!c->d_float) {
-            g.em = em;
-            return fyg_guided_run(&g, s);                       /* filter + emit in one pass */
-        }
-        g.em.quant = 0;
-        g.em.out = c->d_float;
-        if (fyg_guided_run(&g, s)) return 1;
-        src = c->d_float;
-    } else if (c->d_float) {
-        if (fyg_d2d(c->d_float, f, sizeof(float) * hn, s)) return 1;
+        pl->guided = 1;
+        fyg_guided_args *g = &pl->g;
+        g->nz = (int)hz; g->ny = (int)hy; g->nx = (int)hx;
+        g->r = P->guided_radius < 1 ? 2 : P->guided_radius;
+        g->s = P->guided_s == 0 ? 2 : P->guided_s;
+        g->eps = P->guided_eps;
+        pl->gws_bytes = fyg_guided_ws_bytes(g->nz, g->ny, g->nx, g->s);
     }
-    return fyg_emit_run(src, (int)hz, (int)hy, (int)hx, &em, s);
+    return 0;
+}
+
+/* cubes that can share every launch of the chain */
+static int same_group(const cube_plan *a, const cube_plan *b) {
+    const fy_cube *x = a->c, *y = b->c;
+    if (x->hz != y->hz || x->hy != y->hy || x->hx != y->hx || x->tz != y->tz || x->ty != y->ty || x->tx != y->tx)
+        return 0;
+    if (a->deconv != b->deconv || a->guided != b->guided || a->pro.dering != b->pro.dering) return 0;
+    if ((x->d_float != NULL) != (y->d_float != NULL)) return 0;
+    if (a->guided && (a->g.s != b->g.s || a->g.r != b->g.r)) return 0;
+    return 1;
+}
+
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+/* the chain of n planned cubes of one group; flags[i] is cube i's all-air flag */
+static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) {
+    const cube_plan *h = &pl[0];
+    size_t tile = al256(sizeof(float) * h->hn), spec = al256(h->spec_bytes), img = 0, gws = al256(h->gws_bytes);
+    for (int i = 0; i < n; i++)                                 /* ring images may span more slabs */
+        if (al256(pl[i].img_bytes) > img) img = al256(pl[i].img_bytes);
+    char *T = (char *)fyg_ws(s, WS_TILE, tile * n);
+    char *S = spec ? (char *)fyg_ws(s, WS_SPEC, spec * n) : NULL;
+    char *I = img ? (char *)fyg_ws(s, WS_MISC, img * n) : NULL;
+    char *G = gws ? (char *)fyg_ws(s, WS_GUIDED, gws * n) : NULL;
+    if (!T || (spec && !S) || (img && !I) || (gws && !G)) return 1;
+
+    if (h->deconv) {
+        fyg_deconv_args *da = (fyg_deconv_args *)malloc(sizeof *da * n);
+        if (!da) return 1;
+        for (int i = 0; i < n; i++) {
+            da[i] = pl[i].d;
+            da[i].out = (float *)(T + tile * i);
+            da[i].spec = S + spec * i;
+            if (pl[i].pro.dering) da[i].pro.cimg = (float *)(I + img * i);
+            da[i].nonzero_flag = flags[i];
+            da[i].nonzero_gen = gen;
+        }
+        int rc = fyg_deconv_run_batch(da, n, s);
+        free(da);
+        if (rc) return 1;
+    } else {
+        for (int i = 0; i < n; i++)
+            if (fyg_prologue_run(pl[i].c->d_in, (float *)(T + tile * i), (int)pl[i].c->hz, (int)pl[i].c->hy,
+                                 (int)pl[i].c->hx, &pl[i].pro, flags[i], gen, s))
+                return 1;
+    }
+
+    for (int i = 0; i < n; i++) {
+        pl[i].em.nonzero_flag = flags[i];
+        pl[i].em.nonzero_gen = gen;
+    }
+    if (h->guided && !h->c->d_float) {                          /* filter + emit in one pass */
+        fyg_guided_args *ga = (fyg_guided_args *)malloc(sizeof *ga * n);
+        if (!ga) return 1;
+        for (int i = 0; i < n; i++) {
+            ga[i] = pl[i].g;
+            ga[i].in = (const float *)(T + tile * i);
+            ga[i].ws = G + gws * i;
+            ga[i].em = pl[i].em;
+        }
+        int rc = fyg_guided_run_batch(ga, n, s);
+        free(ga);
+        return rc;
+    }
+    for (int i = 0; i < n; i++) {                               /* fp32 copy requested, or no filter */
+        const fy_cube *c = pl[i].c;
+        const float *src = (const float *)(T + tile * i);
+        if (h->guided) {
+            fyg_guided_args g = pl[i].g;
+            g.in = src;
+            g.ws = G + gws * i;
+            g.em = pl[i].em;
+            g.em.quant = 0;
+            g.em.out = c->d_float;
+            if (fyg_guided_run(&g, s)) return 1;
+            src = c->d_float;
+        } else if (c->d_float) {
+            if (fyg_d2d(c->d_float, src, sizeof(float) * pl[i].hn, s)) return 1;
+        }
+        if (fyg_emit_run(src, (int)c->hz, (int)c->hy, (int)c->hx, &pl[i].em, s)) return 1;
+    }
+    return 0;
+}
+
+/* device workspace one group may hold at once; larger groups run in parts */
+#define FYH_GROUP_BYTES ((size_t)8 << 30)
+
+/* Plan every cube, then run the cubes group by group (one shape and stage set
+ * per group, in order of first appearance), each group in as few launches per
+ * stage as FYG_BATCH and FYH_GROUP_BYTES allow.  Results do not depend on the
+ * grouping: every cube has its own workspace slice and flag. */
+static int run_cubes(const fy_cube *cubes, int n, int *d_flags, int gen, fyg_stream *s) {
+    cube_plan *pl = (cube_plan *)malloc(sizeof *pl * (size_t)n);
+    cube_plan *grp = (cube_plan *)malloc(sizeof *grp * (size_t)n);
+    int *gi = (int *)malloc(sizeof(int) * (size_t)n);
+    int **gf = (int **)malloc(sizeof(int *) * (size_t)n);
+    int rc = (!pl || !grp || !gi || !gf) ? 1 : 0;
+    for (int i = 0; i < n && !rc; i++) {
+        rc = plan_cube(&cubes[i], &pl[i]);
+        gi[i] = -1;
+    }
+    for (int i = 0; i < n && !rc; i++) {
+        if (gi[i] >= 0 || pl[i].skip) continue;
+        int m = 0;
+        for (int j = i; j < n; j++) {
+            if (gi[j] < 0 && !pl[j].skip && same_group(&pl[i], &pl[j])) {
+                gi[j] = i;
+                grp[m] = pl[j];
+                gf[m] = d_flags + j;
+                m++;
+            }
+        }
+        size_t per = al256(sizeof(float) * pl[i].hn) + al256(pl[i].spec_bytes) + al256(pl[i].gws_bytes) +
+                     2 * al256(pl[i].img_bytes);
+        int part = (int)(FYH_GROUP_BYTES / (per ? per : 1));
+        if (part < 1) part = 1;
+        for (int k = 0; k < m && !rc; k += part)
+            rc = run_group(grp + k, m - k < part ? m - k : part, gf + k, gen, s);
+    }
+    free(pl); free(grp); free(gi); free(gf);
+    return rc;
+}
+
+static int chain_cube(const fy_cube *c, int *d_flag, int gen, fyg_stream *s) {
+    return run_cubes(c, 1, d_flag, gen, s);
 }
 
 int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream) {
+    if (n <= 0) return 0;
     fyg_stream *s = fyg_stream_wrap(stream);
     if (!s) return 1;
-    for (int i = 0; i < n; i++) {
-        int *dflag = NULL;
-        int gen = fyg_flag(s, &dflag);
-        if (!gen || chain_cube(&cubes[i], dflag, gen, s)) return 1;
-    }
-    return 0;
+    int *dflags = NULL;
+    int gen = fyg_flags(s, n, &dflags);
+    if (!gen) return 1;
+    return run_cubes(cubes, n, dflags, gen, s);
 }
 
 int fyh_chain(const fy_pipeline_cfg *cfg, const unsigned char *d_u8, long rz0, long ry0, long rx0, long hz,

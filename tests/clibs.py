@@ -75,6 +75,18 @@ class PipelineCfg(C.Structure):
         ("occ_cz", C.c_long), ("occ_cy", C.c_long), ("occ_cx", C.c_long)]
 
 
+class CubeParams(C.Structure):
+    """fy_cube_params (include/fysics_gpu.h), as orc_chain_cube reads it."""
+    _fields_ = [
+        ("phys", Physics), ("deconv", C.c_int), ("deconv_param", C.c_double),
+        ("deconv_cap", C.c_double), ("fz2_scale", C.c_double), ("window_lo", C.c_double),
+        ("window_hi", C.c_double), ("have_dec_range", C.c_int), ("dec_lo", C.c_double),
+        ("dec_hi", C.c_double), ("guided_eps", C.c_double), ("guided_radius", C.c_int),
+        ("guided_s", C.c_int), ("no_dither", C.c_int), ("recenter", C.c_int), ("norm_lo", C.c_int),
+        ("norm_hi", C.c_int), ("dering", C.POINTER(Dering)), ("zdrift_factor", c_float_p),
+        ("vol_z", C.c_long)]
+
+
 def fp(a):
     return a.ctypes.data_as(c_float_p)
 
@@ -156,6 +168,8 @@ _ORC_ONLY = {
     "orc_quant_u8": (C.c_ubyte, [C.c_float, C.c_long, C.c_long, C.c_long, C.c_int]),
     "orc_eps_for_tile": (C.c_double, [C.POINTER(PipelineCfg), C.c_double, C.c_double,
                                       C.c_long, C.c_long, C.c_long, C.c_long]),
+    "orc_chain_cube": (C.c_int, [C.POINTER(CubeParams), c_u8_p] + [C.c_long] * 12 +
+                       [c_u8_p, c_int_p, c_float_p]),
     "orc_process_buffer_f": (C.c_int, [C.POINTER(PipelineCfg), c_u8_p] + [C.c_long] * 14 +
                              [c_u8_p, c_int_p, c_float_p]),
 }

@@ -1,0 +1,160 @@
+"""Batched chain (fy_gpu_process_cubes over many cubes of mixed shapes and
+parameters, one launch per stage per shape group and FYG_BATCH cubes).
+
+Bars: a cube's u8 output is bit-identical whether it runs alone or inside a
+mixed batch (result invariance: no state is shared between cubes); against the
+oracle's chain (orc_chain_cube) the u8 output is within +-1 LSB and the fp32
+tile after the guided filter within relative L2 1e-5.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import clibs
+import fysics_amd as F
+from fysics_amd import phantom
+
+pytestmark = pytest.mark.gpu
+
+HALO = 4
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def ring_model(shape, rng):
+    """fy_dering detected on a ringed phantom (pass-1 accumulate + finalize)."""
+    lib = F.lib()
+    vol = phantom.sheets(shape, rng, n_sheets=10)
+    u8 = phantom.noisy_u8(phantom.forward(vol, **phantom.CONFIG1_PHYS), rng)
+    u8, _ = phantom.add_rings(u8, rng, n_rings=8, amp=(5.0, 8.0))
+    nz, ny, nx = shape
+    d, t = F.Dering(), F.Dering()
+    assert lib.fy_dering_init(C.byref(d), nz, ny, nx, -1.0, -1.0, 512, 8) == 0
+    assert lib.fy_dering_tile_init(C.byref(t), C.byref(d)) == 0
+    lib.fy_dering_accumulate_u8(C.byref(t), u8.ctypes.data_as(F.c_u8_p), 0, 0, nz, ny, nx, 1)
+    lib.fy_dering_merge_tile(C.byref(d), 0, C.byref(t))
+    lib.fy_dering_finalize(C.byref(d), 15, 20, 0.5, 6.0)
+    lib.fy_dering_free(C.byref(t))
+    return d, u8
+
+
+def cube_params(rng, dering=None):
+    """Per-cube physics of the batched workload: delta/beta U[100,1000], pixel in
+    {2.4, 4.3, 7.9} um, sigma_z/sigma_xy U[1.0, 1.3] (Wiener, cap 8)."""
+    P = F.CubeParams()
+    P.phys = F.physics(delta_beta=float(rng.uniform(100, 1000)), energy_kev=110.0, distance_mm=1200.0,
+                       pixel_um=float(rng.choice([2.4, 4.3, 7.9])), unsharp_sigma=0.8, unsharp_coeff=0.5)
+    P.deconv, P.deconv_param, P.deconv_cap = 1, 0.015, 8.0
+    P.fz2_scale = float(rng.uniform(1.0, 1.3)) ** 2
+    P.window_lo, P.window_hi = -0.1, 1.2
+    P.guided_eps, P.guided_radius, P.guided_s = float(rng.uniform(5e-4, 3e-3)), 2, 2
+    if dering is not None:
+        P.dering = C.pointer(dering)
+    return P
+
+
+def as_oracle(P):
+    o = clibs.CubeParams()
+    C.memmove(C.byref(o), C.byref(P), C.sizeof(o))
+    return o
+
+
+class Case:
+    def __init__(self, u8, P, origin, want_float=False):
+        self.u8, self.P, self.origin = u8, P, origin
+        self.inner_o = tuple(o + HALO for o in origin)
+        self.inner = tuple(n - 2 * HALO for n in u8.shape)
+        self.d_in = torch.from_numpy(u8).cuda()
+        self.d_out = torch.zeros(self.inner, dtype=torch.uint8, device="cuda")
+        self.d_float = torch.zeros(u8.shape, dtype=torch.float32, device="cuda") if want_float else None
+
+    def cube(self):
+        return F.Cube(self.d_in.data_ptr(), *self.origin, *self.u8.shape, *self.inner_o, *self.inner,
+                      self.d_out.data_ptr(), self.d_float.data_ptr() if self.d_float is not None else None,
+                      C.pointer(self.P))
+
+    def oracle(self, orc):
+        want = np.empty(self.inner, np.uint8)
+        ftile = np.empty(self.u8.shape, np.float32)
+        air = C.c_int(0)
+        op = as_oracle(self.P)
+        assert orc.orc_chain_cube(C.byref(op), clibs.up(self.u8), *self.origin, *self.u8.shape, *self.inner_o,
+                                  *self.inner, clibs.up(want), C.byref(air), clibs.fp(ftile)) == 0
+        return want, ftile, bool(air.value)
+
+
+def run(cases):
+    arr = (F.Cube * len(cases))(*[c.cube() for c in cases])
+    stream = torch.cuda.Stream()
+    assert F.lib().fy_gpu_process_cubes(arr, len(cases), C.c_void_p(stream.cuda_stream)) == 0, \
+        F.lib().fy_gpu_last_error()
+    stream.synchronize()
+    return [c.d_out.cpu().numpy().copy() for c in cases]
+
+
+def make_cases(rng, want_float=False):
+    """13 cubes of three shapes, interleaved: shape A (9 cubes, more than one
+    launch of FYG_BATCH), shape B (2), shape C with ring removal (2); one cube
+    of shape A is all air."""
+    ring, _ = ring_model((24, 48, 56), np.random.default_rng(7))
+    assert ring.have_rings, "the ringed phantom must exercise the ring-removal group"
+    shapes = ["A", "B", "A", "C", "A", "A", "B", "A", "A", "C", "A", "A", "A"]
+    dims = {"A": (20, 24, 28), "B": (24, 32, 40), "C": (16, 36, 40)}
+    cases = []
+    for i, k in enumerate(shapes):
+        shp = dims[k]
+        vol = phantom.sheets(shp, rng, n_sheets=6)
+        u8 = phantom.noisy_u8(phantom.forward(vol, **phantom.CONFIG1_PHYS), rng)
+        if i == 5:
+            u8[:] = 0
+        P = cube_params(rng, ring if k == "C" else None)
+        origin = (int(rng.integers(0, 8)), int(rng.integers(0, 12)), int(rng.integers(0, 16)))
+        cases.append(Case(u8, P, origin, want_float))
+    cases[0].ring = ring
+    return cases
+
+
+def test_batch_matches_alone_and_oracle(orc):
+    rng = np.random.default_rng(11)
+    cases = make_cases(rng)
+    got = run(cases)
+    for i, c in enumerate(cases):
+        alone = run([c])[0]
+        assert np.array_equal(got[i], alone), f"cube {i}: batched and alone differ"
+        want, _, air = c.oracle(orc)
+        if i == 5:
+            assert air and not got[i].any()
+            continue
+        diff = np.abs(got[i].astype(int) - want.astype(int))
+        assert diff.max() <= 1, f"cube {i}"
+        assert (diff > 0).mean() < 2e-3, f"cube {i}"
+
+
+def test_batch_float_tiles_vs_oracle(orc):
+    rng = np.random.default_rng(12)
+    cases = make_cases(rng, want_float=True)
+    got = run(cases)
+    for i, c in enumerate(cases):
+        want, ftile, air = c.oracle(orc)
+        if i == 5:
+            continue
+        assert rel_l2(c.d_float.cpu().numpy(), ftile) <= 1e-5, f"cube {i}"
+        diff = np.abs(got[i].astype(int) - want.astype(int))
+        assert diff.max() <= 1, f"cube {i}"
+
+
+def test_batch_order_invariance():
+    """Reversing the submission order (which regroups the launches) leaves every
+    cube's output bit-identical."""
+    rng = np.random.default_rng(13)
+    cases = make_cases(rng)
+    a = run(cases)
+    b = run(cases[::-1])[::-1]
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
