@@ -164,6 +164,12 @@ SIGNATURES = {
     "fy_gpu_process_buffer": (C.c_int, [C.POINTER(PipelineCfg), _VP] + [_L] * 14 +
                               [_VP, c_int_p, _VP]),
     "fy_gpu_process_cubes": (C.c_int, [C.POINTER(Cube), C.c_int, _VP]),
+    "fy_cube_workspace_bytes": (C.c_size_t, [C.POINTER(Cube)]),
+    "fy_serve_create": (_VP, [C.c_size_t, C.c_int, C.c_int, _VP]),
+    "fy_serve_submit": (C.c_int, [_VP, C.POINTER(Cube), _L]),
+    "fy_serve_poll": (C.c_int, [_VP, C.POINTER(C.c_long), C.c_int]),
+    "fy_serve_outstanding": (C.c_int, [_VP]),
+    "fy_serve_destroy": (None, [_VP]),
     "fy_cube_params_from_cfg": (C.c_int, [C.POINTER(PipelineCfg), _L, _L, _L, _L, _L, _L,
                                           C.POINTER(CubeParams)]),
     "fy_gpu_sync": (C.c_int, [_VP]),

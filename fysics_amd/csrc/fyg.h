@@ -53,6 +53,23 @@ const char *fyg_error_string(void);
  * safe; entries live until process end (bounded LRU for transfer tables). */
 const void *fyg_table(const void *key, size_t keylen, const void *data, size_t bytes);
 const void *fyg_table_find(const void *key, size_t keylen);
+/* per-stream table slots for tables that change from call to call (per-request
+ * transfer LUTs): on a miss the host data is staged into pinned memory and
+ * copied on the stream, so no call waits on the device.  FYG_ARENA_SLOTS slots
+ * of FYG_ARENA_SLOT bytes; the least recently used one is recycled, which is
+ * safe in stream order as long as a caller holds fewer than FYG_ARENA_SLOTS/2
+ * tables between planning and launching.  data == NULL: lookup only.  Returns
+ * NULL if bytes exceeds a slot. */
+#define FYG_ARENA_SLOTS 256
+#define FYG_ARENA_SLOT ((size_t)128 << 10)
+const void *fyg_stream_table(fyg_stream *s, const void *key, size_t keylen, const void *data, size_t bytes);
+
+/* completion events on a stream */
+int  fyg_event_record(fyg_stream *s, void **ev);     /* creates *ev on first use */
+int  fyg_event_done(void *ev);                       /* 1 done, 0 pending, -1 error */
+int  fyg_event_wait(void *ev);
+void fyg_event_destroy(void *ev);
+
 /* fp32 twiddles cos/sin(2 pi m / L), m < L, evaluated in double (float2[L]) */
 const void *fyg_twiddles(int L);
 

@@ -16,6 +16,7 @@ struct fyg_stream {
     int *flag = nullptr;     // all-air flags: hold the generation of the last call that saw data
     int nflag = 0;
     int gen = 0;
+    struct Arena *arena = nullptr;   // per-stream table slots (fyg_stream_table)
 };
 
 namespace fyg_prof {

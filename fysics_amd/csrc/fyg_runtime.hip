@@ -213,6 +213,84 @@ extern "C" const void *fyg_table(const void *key, size_t keylen, const void *dat
     return dev;
 }
 
+// ------------------------------------------------------------------ per-stream table slots
+struct Arena {
+    char *dev = nullptr;
+    char *pin = nullptr;
+    std::vector<std::string> key;            // key held by each slot ("" = free)
+    std::vector<hipEvent_t> ev;              // recorded after the slot's last upload
+    std::vector<long> stamp;                 // last use; the least recent slot is recycled
+    std::unordered_map<std::string, int> where;
+    long clock = 0;
+};
+
+extern "C" const void *fyg_stream_table(fyg_stream *s, const void *key, size_t keylen, const void *data,
+                                        size_t bytes) {
+    if (!s || bytes > FYG_ARENA_SLOT) return nullptr;
+    if (!s->arena) {
+        if (!data) return nullptr;
+        Arena *a = new Arena();
+        if (note(hipMalloc((void **)&a->dev, FYG_ARENA_SLOT * FYG_ARENA_SLOTS), "hipMalloc(arena)") ||
+            note(hipHostMalloc((void **)&a->pin, FYG_ARENA_SLOT * FYG_ARENA_SLOTS, hipHostMallocDefault),
+                 "hipHostMalloc(arena)")) {
+            if (a->dev) (void)hipFree(a->dev);
+            delete a;
+            return nullptr;
+        }
+        a->key.assign(FYG_ARENA_SLOTS, std::string());
+        a->ev.assign(FYG_ARENA_SLOTS, nullptr);
+        a->stamp.assign(FYG_ARENA_SLOTS, -1);
+        s->arena = a;
+    }
+    Arena *a = s->arena;
+    std::string k((const char *)key, keylen);
+    auto it = a->where.find(k);
+    if (it != a->where.end()) {
+        a->stamp[it->second] = ++a->clock;
+        return a->dev + FYG_ARENA_SLOT * it->second;
+    }
+    if (!data) return nullptr;
+    int i = 0;
+    for (int j = 1; j < FYG_ARENA_SLOTS; j++)
+        if (a->stamp[j] < a->stamp[i]) i = j;
+    if (!a->key[i].empty()) {
+        // the slot's staging memory may still feed its previous upload; its
+        // device memory is rewritten in stream order, after earlier readers
+        if (note(hipEventSynchronize(a->ev[i]), "hipEventSynchronize(arena)")) return nullptr;
+        a->where.erase(a->key[i]);
+        a->key[i].clear();
+    }
+    char *pin = a->pin + FYG_ARENA_SLOT * i, *dev = a->dev + FYG_ARENA_SLOT * i;
+    memcpy(pin, data, bytes);
+    if (note(hipMemcpyAsync(dev, pin, bytes, hipMemcpyHostToDevice, s->s), "hipMemcpyAsync(arena)")) return nullptr;
+    if (!a->ev[i] && note(hipEventCreateWithFlags(&a->ev[i], hipEventDisableTiming), "hipEventCreate")) return nullptr;
+    if (note(hipEventRecord(a->ev[i], s->s), "hipEventRecord(arena)")) return nullptr;
+    a->key[i] = k;
+    a->where[k] = i;
+    a->stamp[i] = ++a->clock;
+    return dev;
+}
+
+extern "C" int fyg_event_record(fyg_stream *s, void **ev) {
+    if (!s || !ev) return 1;
+    if (!*ev && note(hipEventCreateWithFlags((hipEvent_t *)ev, hipEventDisableTiming), "hipEventCreate")) return 1;
+    return note(hipEventRecord((hipEvent_t)*ev, s->s), "hipEventRecord");
+}
+
+extern "C" int fyg_event_done(void *ev) {
+    hipError_t e = hipEventQuery((hipEvent_t)ev);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    note(e, "hipEventQuery");
+    return -1;
+}
+
+extern "C" int fyg_event_wait(void *ev) { return note(hipEventSynchronize((hipEvent_t)ev), "hipEventSynchronize"); }
+
+extern "C" void fyg_event_destroy(void *ev) {
+    if (ev) (void)hipEventDestroy((hipEvent_t)ev);
+}
+
 extern "C" const void *fyg_twiddles(int L) {
     char key[16];
     int kl = snprintf(key, sizeof key, "Ttw%d", L);

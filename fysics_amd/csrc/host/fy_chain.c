@@ -111,7 +111,7 @@ typedef struct {
     size_t spec_bytes, img_bytes, gws_bytes;
 } cube_plan;
 
-static int plan_cube(const fy_cube *c, cube_plan *pl) {
+static int plan_cube(const fy_cube *c, cube_plan *pl, fyg_stream *s) {
     memset(pl, 0, sizeof *pl);
     pl->c = c;
     const fy_cube_params *P = c->params;
@@ -152,7 +152,7 @@ static int plan_cube(const fy_cube *c, cube_plan *pl) {
         if (param <= 0) param = matched ? 0.05 : fy_auto_reg(&P->phys, 0.015);
         fyh_tables t;
         if (fyh_tables_get((int)hz, (int)hy, (int)hx, &P->phys, param, matched, P->deconv_cap,
-                           P->fz2_scale > 0 ? P->fz2_scale : 1.0, &t))
+                           P->fz2_scale > 0 ? P->fz2_scale : 1.0, s, &t))
             return 1;
         if (!fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px)) return 1;
         fyg_deconv_args *a = &pl->d;
@@ -282,18 +282,22 @@ static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) 
 /* device workspace one group may hold at once; larger groups run in parts */
 #define FYH_GROUP_BYTES ((size_t)8 << 30)
 
-/* Plan every cube, then run the cubes group by group (one shape and stage set
- * per group, in order of first appearance), each group in as few launches per
- * stage as FYG_BATCH and FYH_GROUP_BYTES allow.  Results do not depend on the
- * grouping: every cube has its own workspace slice and flag. */
-static int run_cubes(const fy_cube *cubes, int n, int *d_flags, int gen, fyg_stream *s) {
+/* cubes planned before their launches are queued: bounds the per-stream
+ * transfer tables held at once (FYG_ARENA_SLOTS/2) */
+#define FYH_PLAN_CHUNK (FYG_ARENA_SLOTS / 2)
+
+/* Plan every cube of a chunk, then run the chunk group by group (one shape and
+ * stage set per group, in order of first appearance), each group in as few
+ * launches per stage as FYG_BATCH and FYH_GROUP_BYTES allow.  Results do not
+ * depend on the grouping: every cube has its own workspace slice and flag. */
+static int run_chunk(const fy_cube *cubes, int n, int *d_flags, int gen, fyg_stream *s) {
     cube_plan *pl = (cube_plan *)malloc(sizeof *pl * (size_t)n);
     cube_plan *grp = (cube_plan *)malloc(sizeof *grp * (size_t)n);
     int *gi = (int *)malloc(sizeof(int) * (size_t)n);
     int **gf = (int **)malloc(sizeof(int *) * (size_t)n);
     int rc = (!pl || !grp || !gi || !gf) ? 1 : 0;
     for (int i = 0; i < n && !rc; i++) {
-        rc = plan_cube(&cubes[i], &pl[i]);
+        rc = plan_cube(&cubes[i], &pl[i], s);
         gi[i] = -1;
     }
     for (int i = 0; i < n && !rc; i++) {
@@ -318,8 +322,34 @@ static int run_cubes(const fy_cube *cubes, int n, int *d_flags, int gen, fyg_str
     return rc;
 }
 
+static int run_cubes(const fy_cube *cubes, int n, int *d_flags, int gen, fyg_stream *s) {
+    for (int i = 0; i < n; i += FYH_PLAN_CHUNK)
+        if (run_chunk(cubes + i, n - i < FYH_PLAN_CHUNK ? n - i : FYH_PLAN_CHUNK, d_flags + i, gen, s)) return 1;
+    return 0;
+}
+
 static int chain_cube(const fy_cube *c, int *d_flag, int gen, fyg_stream *s) {
     return run_cubes(c, 1, d_flag, gen, s);
+}
+
+size_t fy_cube_workspace_bytes(const fy_cube *c) {
+    if (!c || !c->params || c->hz <= 0 || c->hy <= 0 || c->hx <= 0) return 0;
+    const fy_cube_params *P = c->params;
+    size_t hn = (size_t)c->hz * c->hy * c->hx;
+    size_t b = al256(sizeof(float) * hn);
+    if (P->deconv) {
+        int pz = fy_next_fft_size((int)c->hz), py = fy_next_fft_size((int)c->hy), px = fy_next_fft_size((int)c->hx);
+        b += al256(fyg_deconv_spec_bytes(pz, py, px));
+        if (P->dering && P->dering->have_rings) {
+            fyg_prologue pro;
+            memset(&pro, 0, sizeof pro);
+            pro.dering = 1; pro.nslab = P->dering->nslab; pro.slab_z = P->dering->slab_z; pro.gz0 = c->rz0;
+            b += al256(fyg_ring_image_bytes(&pro, (int)c->hz, (int)c->hy, (int)c->hx));
+        }
+    }
+    if (P->guided_eps > 0)
+        b += al256(fyg_guided_ws_bytes((int)c->hz, (int)c->hy, (int)c->hx, P->guided_s == 0 ? 2 : P->guided_s));
+    return b;
 }
 
 int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream) {

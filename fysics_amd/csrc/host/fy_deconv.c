@@ -3,9 +3,10 @@
  * The host side builds, once per (weight, physics, padded shape), the transfer
  * tables deconv_radial uses: the squared-frequency axes (fftfreq convention,
  * fz^2 scaled by (sigma_z/sigma_xy)^2) and the 4097-entry fp32 LUT of the radial
- * weight over u = f^2, evaluated in double (paganin.c:212-230).  They stay
- * resident on the device (fyg_table cache); the GPU passes in fyg_fft.hip do
- * the rest.
+ * weight over u = f^2, evaluated in double (paganin.c:212-230).  They live in
+ * the calling stream's table slots (uploaded asynchronously on a miss), or in
+ * the process-wide cache when too large for a slot; the GPU passes in
+ * fyg_fft.hip do the rest.
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -48,7 +49,7 @@ typedef struct {
 } table_key;
 
 int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, int kind,
-                   double cap, double fz2_scale, fyh_tables *t) {
+                   double cap, double fz2_scale, fyg_stream *s, fyh_tables *t) {
     int pz = fy_next_fft_size(nz), py = fy_next_fft_size(ny), px = fy_next_fft_size(nx);
     int hx = px / 2 + 1;
     if (fz2_scale <= 0) fz2_scale = 1.0;
@@ -84,7 +85,11 @@ int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, in
     key.kind = kind; key.pz = pz; key.py = py; key.px = px;
     memcpy(key.phys, p, sizeof(double) * 8);
     key.param = param; key.cap = cap; key.fz2_scale = fz2_scale;
-    const unsigned char *dev = (const unsigned char *)fyg_table_find(&key, sizeof key);
+    /* per-stream slots first (no device wait on a miss), the process-wide cache
+     * for tables too large for a slot */
+    int slot = bytes <= FYG_ARENA_SLOT;
+    const unsigned char *dev = (const unsigned char *)(slot ? fyg_stream_table(s, &key, sizeof key, NULL, 0)
+                                                            : fyg_table_find(&key, sizeof key));
     if (!dev) {
         unsigned char *blk = (unsigned char *)malloc(bytes);
         if (!blk) { free(fz2); return 1; }
@@ -97,7 +102,8 @@ int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, in
         }
         lut[4097] = lut[4098] = lut[4099] = 0.0f;
         memcpy(blk + lut_bytes, fz2, sizeof(double) * (size_t)(pz + py + hx));
-        dev = (const unsigned char *)fyg_table(&key, sizeof key, blk, bytes);
+        dev = (const unsigned char *)(slot ? fyg_stream_table(s, &key, sizeof key, blk, bytes)
+                                           : fyg_table(&key, sizeof key, blk, bytes));
         free(blk);
     }
     free(fz2);
@@ -113,7 +119,7 @@ int fyh_deconv_dev(const float *d_in, float *d_out, int nz, int ny, int nx, cons
                    double param, int kind, double cap, double fz2_scale, fyg_stream *s) {
     if (nz <= 0 || ny <= 0 || nx <= 0) return 0;
     fyh_tables t;
-    if (fyh_tables_get(nz, ny, nx, p, param, kind, cap, fz2_scale, &t)) return 1;
+    if (fyh_tables_get(nz, ny, nx, p, param, kind, cap, fz2_scale, s, &t)) return 1;
     if (!fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px)) return 1;
     void *spec = fyg_ws(s, WS_SPEC, fyg_deconv_spec_bytes(t.pz, t.py, t.px));
     if (!spec) return 1;

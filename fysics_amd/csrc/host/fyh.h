@@ -24,7 +24,7 @@ typedef struct {
 
 /* kind 0: Wiener weight of the recon transfer (cap 8); kind 1: matched PSF*unsharp */
 int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, int kind,
-                   double cap, double fz2_scale, fyh_tables *t);
+                   double cap, double fz2_scale, fyg_stream *s, fyh_tables *t);
 double fyh_matched_fz2_scale(const fy_physics *p);
 /* the physics fy_process_buffer deconvolves with (pipeline.c:89-104) */
 void fyh_scaled_physics(const fy_pipeline_cfg *c, fy_physics *p);

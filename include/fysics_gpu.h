@@ -80,8 +80,27 @@ typedef struct {
     const fy_cube_params *params;
 } fy_cube;
 
-/* process n cubes on one stream */
+/* process n cubes on one stream.  Cubes of one region shape (and stage set)
+ * share launches: one per stage per 8 cubes, in order of first appearance.  A
+ * cube's output does not depend on what it is submitted with.  No reference
+ * counterpart: the batched form of process_tile (pipeline.c:252-297). */
 int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream);
+/* device bytes one cube holds while it runs (workspace admission) */
+size_t fy_cube_workspace_bytes(const fy_cube *cube);
+
+/* ---- served form: a request loop over one stream (no reference counterpart).
+ * fy_serve_submit queues a request (its buffers and params must stay valid
+ * until it is retired).  fy_serve_poll retires finished waves, then, while
+ * fewer than max_waves waves are in flight, launches the pending requests that
+ * fit the HBM budget next to the running ones (FIFO, at most max_batch) as one
+ * wave: continuous batching.  Returns the number of tickets written to done
+ * (retired in submission order), or -1 on error. */
+typedef struct fy_server fy_server;
+fy_server *fy_serve_create(size_t hbm_budget, int max_batch, int max_waves, void *stream);
+int  fy_serve_submit(fy_server *sv, const fy_cube *cube, long ticket);   /* 1: exceeds the budget */
+int  fy_serve_poll(fy_server *sv, long *done, int max_done);
+int  fy_serve_outstanding(const fy_server *sv);                          /* queued + in flight */
+void fy_serve_destroy(fy_server *sv);
 /* the cube parameters fy_process_buffer derives from a pipeline cfg
  * (delta/beta back-off, matched vs plain weight, per-tile eps) */
 int fy_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long ty, long tx,
