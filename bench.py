@@ -118,6 +118,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the 96-cube batched measurement")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-request measurement")
+    ap.add_argument("--served-requests", type=int, default=2000)
+    ap.add_argument("--served-rate", type=float, default=400.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference point that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters into HBM bytes)")
@@ -229,6 +233,12 @@ def main():
         assert rc == 0
     host_rate = mvox / statistics.median(host_t[1:])
 
+    batched = None if args.no_batched else batched_bench(F)
+    served = None if args.no_served else served_bench(F, args.served_requests, args.served_rate)
+    # the same request mix offered far above the one-cube-at-a-time capacity:
+    # where continuous batching pays
+    stress = None if args.no_served else served_bench(F, 4000, 12000.0, seed=8)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(vol, phys, eps, ring, args.cpu_threads)
@@ -246,10 +256,161 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
         "host_api_mvox_s": round(host_rate, 1),
+        "batched": batched,
+        "served": served,
+        "served_overload": stress,
         "cpu_baseline": cpu,
         "workload_gen_s": round(gen_s, 1),
     }
     print(json.dumps(line))
+
+
+def _generate(fn, shapes, seeds):
+    """Synthetic cubes in worker processes (spawned: they import numpy only)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    workers = max(1, min(8, (os.cpu_count() or 2) - 1))
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(fn, shapes, seeds))
+
+
+def _cube_params(F, phys, z_ratio, eps):
+    P = F.CubeParams()
+    P.phys = F.physics(**phys)
+    P.deconv, P.deconv_param, P.deconv_cap, P.fz2_scale = 1, 0.015, 8.0, z_ratio ** 2
+    P.window_lo, P.window_hi = -0.1, 1.2
+    P.guided_eps, P.guided_radius, P.guided_s = eps, 2, 2
+    return P
+
+
+def batched_bench(F, reps=5, seed=100):
+    """The batched workload: 96 cubes (32x64^3, 32x96^3, 16x128^3, 12x160^3 -> 192^3
+    FFT, 4x192x256x256; 169.7 M voxels), each with its own phantom and physics,
+    u8 in HBM -> deconv + guided -> u8 in HBM.  Timed as one fy_gpu_process_cubes
+    call over all 96 (grouped launches) and, beside it, one call per cube."""
+    import torch
+    from fysics_amd import phantom
+    lib = F.lib()
+    t0 = time.time()
+    keep, cubes = [], []
+    jobs = [shape for count, shape in phantom.BATCH_CLASSES for _ in range(count)]
+    made = _generate(phantom.batch_cube, jobs, [seed + k for k in range(len(jobs))])
+    for shape, (u8, phys, zr) in zip(jobs, made):
+        P = _cube_params(F, phys, zr, 2e-3)
+        d_in = torch.from_numpy(u8).cuda()
+        d_out = torch.empty_like(d_in)
+        keep += [P, d_in, d_out]
+        z = (0, 0, 0)
+        cubes.append(F.Cube(d_in.data_ptr(), *z, *shape, *z, *shape, d_out.data_ptr(), None, C.pointer(P)))
+    gen_s = time.time() - t0
+    nvox = sum(c.hz * c.hy * c.hx for c in cubes)
+    arr = (F.Cube * len(cubes))(*cubes)
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+
+    def grouped():
+        if lib.fy_gpu_process_cubes(arr, len(cubes), sh):
+            raise RuntimeError(lib.fy_gpu_last_error().decode())
+
+    def one_by_one():
+        for c in cubes:
+            if lib.fy_gpu_process_cubes(C.byref(c), 1, sh):
+                raise RuntimeError(lib.fy_gpu_last_error().decode())
+
+    out = {"workload": "96 cubes: 32x64^3, 32x96^3, 16x128^3, 12x160^3 (FFT 192^3), 4x192x256x256; "
+                       "per-cube phantom, delta/beta U[100,1000], pixel {2.4,4.3,7.9} um, sigma_z/sigma_xy "
+                       "U[1,1.3]; u8 -> dewindow -> Wiener deconv -> guided r=2 s=2 -> dithered u8",
+           "cubes": len(cubes), "voxels": int(nvox), "workload_gen_s": round(gen_s, 1)}
+    for name, fn in (("grouped", grouped), ("one_call_per_cube", one_by_one)):
+        fn()
+        stream.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        stream.synchronize()
+        dt = (time.perf_counter() - t1) / reps
+        out[name] = {"ms_per_batch": round(1000 * dt, 3), "mvox_s": round(nvox / dt / 1e6, 1)}
+    return out
+
+
+def served_bench(F, n_req=2000, rate=400.0, seed=7, pool=4, eps=2e-3):
+    """The served workload: open-loop Poisson arrivals at `rate` req/s, sizes 40%
+    64^3, 30% 128^3, 20% 192^3, 10% 256^3, per-request physics (delta/beta, E, D,
+    pixel, unsharp) so every request brings its own transfer table; inputs from a
+    pool of `pool` resident phantoms per size.  Latency = completion - scheduled
+    arrival.  'served' is the request loop with continuous batching (waves of up
+    to 64 requests, 2 in flight, 200 GB budget); 'plain' is one cube per launch
+    sequence as each request arrives."""
+    import torch
+    from fysics_amd import phantom
+    lib = F.lib()
+    rng = np.random.default_rng(seed)
+    sizes = [n for _, n in phantom.SERVE_SIZES]
+    probs = [p for p, _ in phantom.SERVE_SIZES]
+    jobs = [(n, n, n) for n in sizes for _ in range(pool)]
+    made = _generate(phantom.batch_cube, jobs, [seed * 100 + k for k in range(len(jobs))])
+    inputs = {n: [] for n in sizes}
+    for shape, (u8, _, _) in zip(jobs, made):
+        inputs[shape[0]].append(torch.from_numpy(u8).cuda())
+    slots = 256
+    outs = {n: torch.empty((slots, n, n, n), dtype=torch.uint8, device="cuda") for n in sizes}
+    kinds = rng.choice(len(sizes), size=n_req, p=probs)
+    arrive = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    params = []
+    for i in range(n_req):
+        P = _cube_params(F, phantom.serve_physics(rng), float(rng.uniform(1.0, 1.3)), eps)
+        params.append(P)
+    cubes = []
+    for i in range(n_req):
+        n = sizes[kinds[i]]
+        d_in = inputs[n][i % pool]
+        z = (0, 0, 0)
+        cubes.append(F.Cube(d_in.data_ptr(), *z, n, n, n, *z, n, n, n, outs[n][i % slots].data_ptr(), None,
+                            C.pointer(params[i])))
+    nvox = float(sum(sizes[k] ** 3 for k in kinds))
+
+    def run(max_batch, max_waves):
+        stream = torch.cuda.Stream()
+        sv = lib.fy_serve_create(200 << 30, max_batch, max_waves, C.c_void_p(stream.cuda_stream))
+        if not sv:
+            raise RuntimeError("fy_serve_create")
+        # warm the per-stream workspaces with one request of each size
+        for n in sizes:
+            j = int(np.flatnonzero(kinds == sizes.index(n))[0])
+            lib.fy_serve_submit(sv, C.byref(cubes[j]), -1)
+        buf = (C.c_long * 4096)()
+        while lib.fy_serve_outstanding(sv):
+            lib.fy_serve_poll(sv, buf, 4096)
+        done_t = np.zeros(n_req)
+        t0 = time.perf_counter()
+        i = nd = 0
+        while nd < n_req:
+            now = time.perf_counter() - t0
+            while i < n_req and arrive[i] <= now:
+                if i >= slots and done_t[i - slots] == 0:
+                    break                   # the output slot is still in flight
+                if lib.fy_serve_submit(sv, C.byref(cubes[i]), i):
+                    raise RuntimeError("fy_serve_submit")
+                i += 1
+            m = lib.fy_serve_poll(sv, buf, 4096)
+            if m < 0:
+                raise RuntimeError(lib.fy_gpu_last_error().decode())
+            t = time.perf_counter() - t0
+            for j in range(m):
+                done_t[buf[j]] = t
+            nd += m
+        lib.fy_serve_destroy(sv)
+        lat = np.sort(done_t - arrive) * 1000.0
+        span = done_t.max() - arrive[0]
+        return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "mean_ms": round(float(lat.mean()), 3), "mvox_s": round(nvox / span / 1e6, 1)}
+
+    res = {"workload": f"{n_req} requests, Poisson {rate:g} req/s, sizes 40% 64^3 / 30% 128^3 / 20% 192^3 / "
+                       "10% 256^3, per-request physics", "requests": n_req, "rate_req_s": rate,
+           "offered_mvox_s": round(nvox / (arrive[-1] - arrive[0]) / 1e6, 1)}
+    res["served"] = run(64, 2)
+    res["plain"] = run(1, 1 << 16)
+    return res
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):

@@ -120,3 +120,36 @@ def config2_volume(shape=(256, 256, 256), seed=2, z_ratio=1.17, rings=20):
     if rings:
         u8, _ = add_rings(u8, rng, rings)
     return u8
+
+
+# batched workload: 96 cubes of five shapes (169.7 M voxels), one seed and one
+# physics draw per cube
+BATCH_CLASSES = [(32, (64, 64, 64)), (32, (96, 96, 96)), (16, (128, 128, 128)), (12, (160, 160, 160)),
+                 (4, (192, 256, 256))]
+
+
+def batch_cube(shape, seed):
+    """One cube of the batched workload: its own sheets, delta/beta U[100,1000],
+    pixel in {2.4, 4.3, 7.9} um, sigma_z/sigma_xy U[1.0,1.3]; u8 through the
+    window.  Returns (u8, physics dict, z_ratio)."""
+    rng = np.random.default_rng(seed)
+    phys = dict(CONFIG1_PHYS)
+    phys["delta_beta"] = float(rng.uniform(100.0, 1000.0))
+    phys["pixel_um"] = float(rng.choice([2.4, 4.3, 7.9]))
+    zr = float(rng.uniform(1.0, 1.3))
+    vol = sheets(shape, rng, n_sheets=max(4, shape[0] // 6))
+    u8 = noisy_u8(forward(vol, z_ratio=zr, **phys), rng, WINDOW, 0.02, 4.0)
+    return u8, phys, zr
+
+
+# served workload: request sizes and their shares
+SERVE_SIZES = [(0.4, 64), (0.3, 128), (0.2, 192), (0.1, 256)]
+
+
+def serve_physics(rng):
+    """Per-request physics of the served workload: delta/beta U[100,1000],
+    E U[80,120] keV, D U[0.5,2] m, pixel U[2.4,7.9] um, unsharp coeff U[0.3,0.7],
+    sigma U[0.6,1.2] px."""
+    return dict(delta_beta=float(rng.uniform(100.0, 1000.0)), energy_kev=float(rng.uniform(80.0, 120.0)),
+                distance_mm=float(rng.uniform(500.0, 2000.0)), pixel_um=float(rng.uniform(2.4, 7.9)),
+                unsharp_coeff=float(rng.uniform(0.3, 0.7)), unsharp_sigma=float(rng.uniform(0.6, 1.2)))
