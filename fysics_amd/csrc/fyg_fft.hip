@@ -23,6 +23,7 @@
 //   k_irfft_x   Hermitian repack of rows (y, y+1), complex inverse, 1/(pz*py*px),
 //               crop, fused rewindow / dec-range / clamp     (paganin.c:263-289)
 #include <cstdio>
+#include <type_traits>
 #include <vector>
 
 #include "fyg_internal.hpp"
@@ -274,7 +275,10 @@ __device__ __forceinline__ void run_from(float sg, const float2 *tw, float2 *buf
             __syncthreads();
             run_from<L, NL, CM, REV, S + 1, END>(sg, tw, buf, first, last);
         } else if constexpr (FINAL) {
-            stage<L, NL, T, CM, R, NS, false>(sg, tw, LdsRd<L, NL>{buf}, last);
+            // a final stage that stores back into the same LDS lines (a partial
+            // plan, END < n) needs the barrier between its read and write phases
+            constexpr bool IN_PLACE = std::is_same<Last, LdsWr<L, NL>>::value;
+            stage<L, NL, T, CM, R, NS, IN_PLACE>(sg, tw, LdsRd<L, NL>{buf}, last);
         } else {
             stage<L, NL, T, CM, R, NS, true>(sg, tw, LdsRd<L, NL>{buf}, LdsWr<L, NL>{buf});
             __syncthreads();

@@ -76,6 +76,25 @@ def test_deconvolve_vs_oracle(orc, shape):
     assert rel_l2(got, want32) <= REL_L2
 
 
+# every FFT length from 96 up on each axis (plans of three or more stages, and
+# column passes with fewer than 16 lines per workgroup)
+LONG_AXES = [(96, 6, 6), (384, 4, 4), (512, 4, 4), (768, 4, 4), (1024, 3, 3), (2048, 2, 2), (4096, 2, 2)]
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("base", LONG_AXES)
+def test_deconvolve_long_axes_vs_oracle(orc, base, axis):
+    shape = tuple(np.roll(base, axis))
+    rng = np.random.default_rng(sum(shape) + axis)
+    x = rng.random(shape).astype(np.float32)
+    p = F.physics()
+    got = F.deconvolve(x, p, 0.015)
+    want64 = np.empty(shape, np.float64)
+    pc = clibs.physics()
+    assert orc.orc_deconv_radial_f64(clibs.fp(x), clibs.dp(want64), *shape, C.byref(pc), 0.015, 0, 8.0, 1.0) == 0
+    assert rel_l2(got, want64) <= REL_L2
+
+
 @pytest.mark.parametrize("shape", [(20, 24, 28), (48, 48, 48), (7, 9, 11)])
 def test_deconvolve_matched_aniso_vs_oracle(orc, shape):
     rng = np.random.default_rng(3)
