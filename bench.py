@@ -120,6 +120,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the 96-cube batched measurement")
     ap.add_argument("--no-served", action="store_true", help="skip the served-request measurement")
+    ap.add_argument("--no-precision", action="store_true", help="skip the 512^3 fp32 / fp64 timing")
     ap.add_argument("--served-requests", type=int, default=2000)
     ap.add_argument("--served-rate", type=float, default=400.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -238,6 +239,7 @@ def main():
     # the same request mix offered far above the one-cube-at-a-time capacity:
     # where continuous batching pays
     stress = None if args.no_served else served_bench(F, 4000, 12000.0, seed=8)
+    precision = None if args.no_precision else precision_bench(F)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -259,6 +261,7 @@ def main():
         "batched": batched,
         "served": served,
         "served_overload": stress,
+        "precision_512": precision,
         "cpu_baseline": cpu,
         "workload_gen_s": round(gen_s, 1),
     }
@@ -411,6 +414,42 @@ def served_bench(F, n_req=2000, rate=400.0, seed=7, pool=4, eps=2e-3):
     res["served"] = run(64, 2)
     res["plain"] = run(1, 1 << 16)
     return res
+
+
+def precision_bench(F, n=512, reps=5, seed=3):
+    """The 512^3 precision configuration timed two ways: the fp32 chain (the
+    product path) and its all-FP64 variant (deconvolution spectrum, weights and
+    the fast guided filter in double), both u8 in HBM -> u8 in HBM.  The accuracy
+    of both against the fp64 oracle is the subject of tests/test_gpu_precision.py."""
+    import torch
+    lib = F.lib()
+    rng = np.random.default_rng(seed)
+    d_in = torch.from_numpy(rng.integers(0, 256, size=(n, n, n), dtype=np.uint8)).cuda()
+    d_out = torch.empty_like(d_in)
+    phys = F.physics(delta_beta=300.0 * 0.35, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
+                     unsharp_sigma=0.8, unsharp_coeff=0.5)
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+    out = {"workload": f"{n}^3 u8 cube: dewindow -> Wiener deconv (sigma ratio 1.17) -> guided r=2 s=2 -> u8"}
+    for name, fp64 in (("fp32", 0), ("fp64", 1)):
+        P = F.CubeParams()
+        P.phys = phys
+        P.deconv, P.deconv_param, P.deconv_cap, P.fz2_scale = 1, 0.015, 8.0, 1.17 ** 2
+        P.window_lo, P.window_hi = -0.1, 1.2
+        P.guided_eps, P.guided_radius, P.guided_s = 2e-3, 2, 2
+        P.fp64 = fp64
+        z = (0, 0, 0)
+        cube = F.Cube(d_in.data_ptr(), *z, n, n, n, *z, n, n, n, d_out.data_ptr(), None, C.pointer(P))
+        if lib.fy_gpu_process_cubes(C.byref(cube), 1, sh):
+            raise RuntimeError(lib.fy_gpu_last_error().decode())
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            lib.fy_gpu_process_cubes(C.byref(cube), 1, sh)
+        stream.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        out[name] = {"ms_per_cube": round(1000 * dt, 3), "mvox_s": round(n ** 3 / dt / 1e6, 1)}
+    return out
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):

@@ -91,7 +91,7 @@ class CubeParams(C.Structure):
         ("dec_hi", C.c_double), ("guided_eps", C.c_double), ("guided_radius", C.c_int),
         ("guided_s", C.c_int), ("no_dither", C.c_int), ("recenter", C.c_int), ("norm_lo", C.c_int),
         ("norm_hi", C.c_int), ("dering", C.POINTER(Dering)), ("zdrift_factor", c_float_p),
-        ("vol_z", C.c_long)]
+        ("vol_z", C.c_long), ("fp64", C.c_int)]
 
 
 class Cube(C.Structure):
@@ -159,6 +159,8 @@ SIGNATURES = {
                                             _VP]),
     "fy_gpu_deconv_radial": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_int, _P, C.c_double,
                                        C.c_int, C.c_double, C.c_double, _VP]),
+    "fy_gpu_deconv_radial_f64": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_int, _P, C.c_double,
+                                           C.c_int, C.c_double, C.c_double, _VP]),
     "fy_gpu_guided": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                 _VP]),
     "fy_gpu_process_buffer": (C.c_int, [C.POINTER(PipelineCfg), _VP] + [_L] * 14 +

@@ -17,7 +17,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfysics_amd.so")
 ARCH = os.environ.get("FYSICS_OFFLOAD_ARCH", "gfx950")
 
-HIP_SRC = ["fyg_runtime.hip", "fyg_fft.hip", "fyg_guided.hip", "fyg_elem.hip", "fyg_noise.hip"]
+HIP_SRC = ["fyg_runtime.hip", "fyg_fft.hip", "fyg_fft64.hip", "fyg_guided.hip", "fyg_elem.hip",
+           "fyg_noise.hip"]
 C_SRC = ["host/fy_core.c", "host/fy_deconv.c", "host/fy_guided.c", "host/fy_dering.c",
          "host/fy_chain.c", "host/fy_noise.c", "host/fy_serve.c"]
 
@@ -38,6 +39,7 @@ C_FLAGS = ["-std=gnu11", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parame
 
 def _deps(src):
     hdrs = [os.path.join(CSRC, "fyg.h"), os.path.join(CSRC, "fyg_internal.hpp"),
+            os.path.join(CSRC, "fyg_fft_engine.hpp"),
             os.path.join(CSRC, "host", "fyh.h"), os.path.join(ROOT, "include", "fysics.h"),
             os.path.join(ROOT, "include", "fysics_gpu.h")]
     return [src] + [h for h in hdrs if os.path.exists(h)]

@@ -72,6 +72,8 @@ void fyg_event_destroy(void *ev);
 
 /* fp32 twiddles cos/sin(2 pi m / L), m < L, evaluated in double (float2[L]) */
 const void *fyg_twiddles(int L);
+/* the same in fp64 (double2[L]) */
+const void *fyg_twiddles64(int L);
 
 /* per-kernel timing with HIP events recorded on the launch stream */
 void fyg_prof_enable(int on);
@@ -114,21 +116,24 @@ typedef struct {
 typedef struct {
     int nz, ny, nx;                  /* logical (cropped) volume */
     int pz, py, px, hx, hxp;         /* padded FFT sizes, half-spectrum width and row pitch */
-    const float *lut;                /* 4097 fp32 weights over u = f^2 */
+    const void *lut;                 /* 4097 weights over u = f^2: fp32, or fp64 when f64 */
     double us;                       /* LUT index scale: t = u*us */
     const double *fz2, *fy2, *fx2;   /* squared frequencies (fz2 pre-scaled by sigma_z ratio) */
     const void *in; int in_u8;       /* fp32 or u8 input (nz*ny*nx) */
     fyg_prologue pro;                /* u8 input only */
-    float *out;                      /* fp32 nz*ny*nx, may alias a fp32 in */
+    void *out;                       /* fp32 nz*ny*nx (double when f64), may alias a fp32 in */
     fyg_epilogue epi;
-    void *spec;                      /* complex workspace pz*py*hxp float2 */
+    void *spec;                      /* complex workspace pz*py*hxp float2 (double2 when f64) */
     int *nonzero_flag;               /* optional: receives nonzero_gen if any input u8 is nonzero */
     int nonzero_gen;
+    int f64;                         /* all-FP64 variant: spectrum, weights and out in double */
 } fyg_deconv_args;
 int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s);
 /* n deconvolutions of one shape (same nz..hxp, in_u8 and pro.dering) */
 int fyg_deconv_run_batch(const fyg_deconv_args *a, int n, fyg_stream *s);
 size_t fyg_deconv_spec_bytes(int pz, int py, int px);
+size_t fyg_deconv_spec_bytes64(int pz, int py, int px);
+int fyg_deconv_run_batch_f64(const fyg_deconv_args *a, int n, fyg_stream *s);
 int fyg_deconv_hxp(int px);
 
 /* complex FFT of split re/im device arrays (fy_fft1d/fy_fft3d) */
@@ -155,8 +160,11 @@ typedef struct {
     const float *in;
     fyg_emit em;
     void *ws;                        /* device workspace, fyg_guided_ws_bytes() */
+    int f64;                         /* all-FP64 variant (fast path, s >= 2): guide in64, ws in */
+    const double *in64;              /* doubles of fyg_guided_ws_bytes64(); emitted in fp32 */
 } fyg_guided_args;
 size_t fyg_guided_ws_bytes(int nz, int ny, int nx, int s);
+size_t fyg_guided_ws_bytes64(int nz, int ny, int nx, int s);
 int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s);
 /* n guided filters; cubes of one shape on the fast path share launches */
 int fyg_guided_run_batch(const fyg_guided_args *a, int n, fyg_stream *s);

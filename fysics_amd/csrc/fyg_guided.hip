@@ -38,10 +38,11 @@ struct Grid {
     int fny, fnx;       // full-resolution source dims (row pitch)
 };
 
-__device__ __forceinline__ float count_recip(int i, int n, int rho) {
+template <class P = float>
+__device__ __forceinline__ P count_recip(int i, int n, int rho) {
     int lo = i - rho < 0 ? 0 : i - rho;
     int hi = i + rho > n - 1 ? n - 1 : i + rho;
-    return 1.0f / (float)(hi - lo + 1);
+    return (P)1 / (P)(hi - lo + 1);
 }
 
 // Generic tiled box over NF fields.  load(gz,gy,gx,v[NF]) fills in-grid values;
@@ -307,29 +308,30 @@ struct March {
     static constexpr int SP = 48, AP = 16;
 };
 
+template <class P>
 struct CoefArgs {
     Grid g;                 // decimated grid; source rows are s*y, planes s*z, columns s*x
-    const float *in;
+    const P *in;
     double eps;
-    float *ma, *mb;
+    P *ma, *mb;
 };
 
 // grid: (x blocks, y blocks, cube * nzc + z chunk)
-template <int RHO>
-__global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs> bt, int nzc) {
+template <int RHO, class P>
+__global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
     using M = March<RHO>;
-    const CoefArgs &p = bt.a[blockIdx.z / nzc];
+    const CoefArgs<P> &p = bt.a[blockIdx.z / nzc];
     const int zchunk = blockIdx.z % nzc;
     static_assert(M::AW * M::AW == NT, "one thread per a, b column");
     static_assert(M::SW <= M::SP, "sample row fits its pitch");
-    __shared__ float sv[M::SW][M::SP];
-    __shared__ float sa[M::AW][M::AP], sb[M::AW][M::AP];
+    __shared__ P sv[M::SW][M::SP];
+    __shared__ P sa[M::AW][M::AP], sb[M::AW][M::AP];
     const Grid &g = p.g;
     const int bx = blockIdx.x * M::OW, by = blockIdx.y * M::OW, zs = zchunk * M::ZC;
     const int ze = min(zs + M::ZC, g.nz);
     const size_t sz = (size_t)g.s * g.fny * g.fnx, sy = (size_t)g.s * g.fnx;
-    const float c0 = p.in[(size_t)zs * sz + (size_t)by * sy + (size_t)bx * g.s];
-    const float epsf = (float)p.eps;
+    const P c0 = p.in[(size_t)zs * sz + (size_t)by * sy + (size_t)bx * g.s];
+    const P epsf = (P)p.eps;
 
     // sample slots: in-plane source offset (0 outside the grid) and LDS cell
     size_t soff[M::KS];
@@ -348,32 +350,32 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs> bt, int nzc) {
     const int ax = threadIdx.x % M::AW, ay = threadIdx.x / M::AW;
     const int gx = bx - RHO + ax, gy = by - RHO + ay;
     const bool colin = gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny;
-    const float rcxy = colin ? count_recip(gx, g.nx, RHO) * count_recip(gy, g.ny, RHO) : 0.0f;
+    const P rcxy = colin ? count_recip<P>(gx, g.nx, RHO) * count_recip<P>(gy, g.ny, RHO) : 0.0f;
     const bool inner = ax >= RHO && ax < M::AW - RHO && ay >= RHO && ay < M::AW - RHO && colin;
 
-    float q1[M::Q], q2[M::Q], qa[M::Q], qb[M::Q];
+    P q1[M::Q], q2[M::Q], qa[M::Q], qb[M::Q];
 #pragma unroll
     for (int j = 0; j < M::Q; j++) q1[j] = q2[j] = qa[j] = qb[j] = 0.0f;
 
-    auto fetch = [&](int zp, float *dst) {
+    auto fetch = [&](int zp, P *dst) {
         size_t zo = (size_t)(zp >= 0 && zp < g.nz ? zp : zs) * sz;
 #pragma unroll
         for (int k = 0; k < M::KS; k++) dst[k] = p.in[zo + soff[k]];
     };
-    auto step = [&](int zp, const float *cur) {
+    auto step = [&](int zp, const P *cur) {
         const bool zin = zp >= 0 && zp < g.nz;
 #pragma unroll
         for (int k = 0; k < M::KS; k++)
             if (scell[k] >= 0) (&sv[0][0])[scell[k]] = (zin && sok[k]) ? cur[k] - c0 : 0.0f;
         __syncthreads();
         // x-then-y window sums of v and v^2 at plane zp
-        float s1 = 0.0f, s2 = 0.0f;
+        P s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
         for (int dy = 0; dy < M::Q; dy++) {
-            float r1 = 0.0f, r2 = 0.0f;
+            P r1 = 0.0f, r2 = 0.0f;
 #pragma unroll
             for (int dx = 0; dx < M::Q; dx++) {
-                float v = sv[ay + dy][ax + dx];
+                P v = sv[ay + dy][ax + dx];
                 r1 += v;
                 r2 += v * v;
             }
@@ -386,14 +388,14 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs> bt, int nzc) {
         q2[M::Q - 1] = s2;
         if (zp >= zs) {                             // a, b at plane za = zp - rho
             const int za = zp - RHO;
-            float av = 0.0f, bv = 0.0f;
+            P av = 0.0f, bv = 0.0f;
             if (colin && za >= 0 && za < g.nz) {
-                float t1 = 0.0f, t2 = 0.0f;
+                P t1 = 0.0f, t2 = 0.0f;
 #pragma unroll
                 for (int j = 0; j < M::Q; j++) { t1 += q1[j]; t2 += q2[j]; }
-                float rc = rcxy * count_recip(za, g.nz, RHO);
-                float m = t1 * rc, q = t2 * rc;
-                float var = q - m * m;
+                P rc = rcxy * count_recip<P>(za, g.nz, RHO);
+                P m = t1 * rc, q = t2 * rc;
+                P var = q - m * m;
                 if (var < 0.0f) var = 0.0f;
                 av = var / (var + epsf);
                 bv = (m + c0) * (1.0f - av);
@@ -403,11 +405,11 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs> bt, int nzc) {
         }
         __syncthreads();
         if (zp >= zs) {                             // window sums of a, b; output plane zp - 2 rho
-            float u1 = 0.0f, u2 = 0.0f;
+            P u1 = 0.0f, u2 = 0.0f;
             if (inner) {
 #pragma unroll
                 for (int dy = -RHO; dy <= RHO; dy++) {
-                    float r1 = 0.0f, r2 = 0.0f;
+                    P r1 = 0.0f, r2 = 0.0f;
 #pragma unroll
                     for (int dx = -RHO; dx <= RHO; dx++) {
                         r1 += sa[ay + dy][ax + dx];
@@ -423,10 +425,10 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs> bt, int nzc) {
             qb[M::Q - 1] = u2;
             const int zo = zp - 2 * RHO;
             if (inner && zo >= zs && zo < ze) {
-                float t1 = 0.0f, t2 = 0.0f;
+                P t1 = 0.0f, t2 = 0.0f;
 #pragma unroll
                 for (int j = 0; j < M::Q; j++) { t1 += qa[j]; t2 += qb[j]; }
-                float rc = rcxy * count_recip(zo, g.nz, RHO);
+                P rc = rcxy * count_recip<P>(zo, g.nz, RHO);
                 size_t o = ((size_t)zo * g.ny + gy) * g.nx + gx;
                 p.ma[o] = t1 * rc;
                 p.mb[o] = t2 * rc;
@@ -437,7 +439,7 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs> bt, int nzc) {
     // buffer ring rotates by full unrolling, so no register copy waits on a load
     constexpr int PD = M::PD;
     const int zfirst = zs - 2 * RHO, zlast = ze - 1 + 2 * RHO;
-    float buf[PD][M::KS];
+    P buf[PD][M::KS];
 #pragma unroll
     for (int d = 0; d < PD - 1; d++) fetch(zfirst + d, buf[d]);
     for (int z0 = zfirst; z0 <= zlast; z0 += PD) {
@@ -466,9 +468,10 @@ __global__ __launch_bounds__(NT) void k_box1(Box1Args p) {
     box_tile<RHO, 1>(g, sm, load, out);
 }
 
+template <class P>
 struct UpArgs {
     int nz, ny, nx, lz, ly, lx, s;
-    const float *ma, *mb, *in;
+    const P *ma, *mb, *in;
     int z0, y0, x0, dz, dy, dx;      // region of output voxels to produce
     int vin, vout;                   // 16-byte input rows / aligned 4-voxel output groups
     fyg_emit em;
@@ -495,10 +498,10 @@ __device__ __forceinline__ int up_col(int c) { return (c & 1) * UPH + (c >> 1); 
 // then lerps along x; a thread blends only the three low-res columns its four
 // consecutive voxels need.  Thread (q, g) owns x quad q of rows g, g+4, g+8, ...
 // P2: s is a power of two, so c/s is exactly c*(1/s) and needs no division.
-template <bool P2>
+template <bool P2, class P = float>
 struct UpCoord {
-    float fs, inv;
-    __device__ float u(int c) const { return P2 ? (float)c * inv : (float)c / fs; }
+    P fs, inv;
+    __device__ P u(int c) const { return P2 ? (P)c * inv : (P)c / fs; }
     __device__ int i0(int c, int l) const {
         int i = (int)u(c);
         return i > l - 2 ? l - 2 : i;
@@ -506,11 +509,11 @@ struct UpCoord {
 };
 
 // grid: (x chunks, y blocks, cube * nzb + z block)
-template <bool P2>
-__global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs> bt, int nzb) {
-    const UpArgs &p = bt.a[blockIdx.z / nzb];
-    __shared__ float pa[UPZ][UPY][UPK], pb[UPZ][UPY][UPK];
-    const UpCoord<P2> U{(float)p.s, 1.0f / (float)p.s};
+template <bool P2, class P>
+__global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
+    const UpArgs<P> &p = bt.a[blockIdx.z / nzb];
+    __shared__ P pa[UPZ][UPY][UPK], pb[UPZ][UPY][UPK];
+    const UpCoord<P2, P> U{(P)p.s, (P)1 / (P)p.s};
     const int Z0 = p.z0 + (int)(blockIdx.z % nzb) * URZ, Y0 = p.y0 + blockIdx.y * URY;
     const int zl = min(Z0 + URZ, p.z0 + p.dz) - 1, yl = min(Y0 + URY, p.y0 + p.dy) - 1;
     const int xc0 = p.x0 + blockIdx.x * 64 * UPX;
@@ -523,15 +526,20 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs> bt, int nzb) {
     const bool xin = xb <= xc1, full = xb + UPX - 1 <= xc1;
 
     // input voxels of the thread's rows: issued first, used last
-    float iv[RPT][UPX];
+    P iv[RPT][UPX];
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
         int r = g + 4 * j, z = Z0 + r / URY, y = Y0 + r % URY;
         bool ok = xin && z <= zl && y <= yl;
-        const float *irow = p.in + ((size_t)(ok ? z : Z0) * p.ny + (ok ? y : Y0)) * p.nx;
+        const P *irow = p.in + ((size_t)(ok ? z : Z0) * p.ny + (ok ? y : Y0)) * p.nx;
         if (ok && full && p.vin) {
-            float4 t = *(const float4 *)(irow + xb);
-            iv[j][0] = t.x; iv[j][1] = t.y; iv[j][2] = t.z; iv[j][3] = t.w;
+            if constexpr (sizeof(P) == sizeof(float)) {
+                float4 t = *(const float4 *)(irow + xb);
+                iv[j][0] = t.x; iv[j][1] = t.y; iv[j][2] = t.z; iv[j][3] = t.w;
+            } else {
+                double2 t0 = *(const double2 *)(irow + xb), t1 = *(const double2 *)(irow + xb + 2);
+                iv[j][0] = t0.x; iv[j][1] = t0.y; iv[j][2] = t1.x; iv[j][3] = t1.y;
+            }
         } else {
 #pragma unroll
             for (int e = 0; e < UPX; e++) iv[j][e] = (ok && xb + e <= xc1) ? irow[xb + e] : 0.0f;
@@ -541,7 +549,7 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs> bt, int nzb) {
     {
         constexpr int RW = (UPZ * UPY + 3) / 4, CW = (UPK + 63) / 64;
         const int nrow = nzp * nyp;
-        float va[RW][CW], vb[RW][CW];
+        P va[RW][CW], vb[RW][CW];
 #pragma unroll
         for (int i = 0; i < RW; i++) {
             int rr = g + 4 * i;
@@ -580,12 +588,12 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs> bt, int nzb) {
     if (!xin) return;
     // x-dependent weights, shared by the four rows
     const int k0 = U.i0(xb, p.lx) - kb;
-    float fxs[UPX];
+    P fxs[UPX];
     bool dsel[UPX];
 #pragma unroll
     for (int e = 0; e < UPX; e++) {
         int x = xb + e, x0 = U.i0(x, p.lx);
-        float fx = U.u(x) - x0;
+        P fx = U.u(x) - x0;
         fxs[e] = fx > 1.0f ? 1.0f : fx;
         dsel[e] = x0 - kb - k0 != 0;            // 0 or 1
     }
@@ -596,28 +604,31 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs> bt, int nzb) {
         int r = g + 4 * j, z = Z0 + r / URY, y = Y0 + r % URY;
         if (z > zl || y > yl) continue;
         int z0 = U.i0(z, p.lz), y0 = U.i0(y, p.ly);
-        float zf = U.u(z) - z0; if (zf > 1.0f) zf = 1.0f;
-        float yf = U.u(y) - y0; if (yf > 1.0f) yf = 1.0f;
-        const float w00 = (1 - zf) * (1 - yf), w01 = (1 - zf) * yf, w10 = zf * (1 - yf), w11 = zf * yf;
+        P zf = U.u(z) - z0; if (zf > 1.0f) zf = 1.0f;
+        P yf = U.u(y) - y0; if (yf > 1.0f) yf = 1.0f;
+        const P w00 = (1 - zf) * (1 - yf), w01 = (1 - zf) * yf, w10 = zf * (1 - yf), w11 = zf * yf;
         const int pz = z0 - zlo, py = y0 - ylo;
-        auto blend = [&](const float (*P)[UPY][UPK], int c) {
-            return w00 * P[pz][py][c] + w01 * P[pz][py + 1][c] + w10 * P[pz + 1][py][c] + w11 * P[pz + 1][py + 1][c];
+        auto blend = [&](const P (*Q)[UPY][UPK], int c) {
+            return w00 * Q[pz][py][c] + w01 * Q[pz][py + 1][c] + w10 * Q[pz + 1][py][c] + w11 * Q[pz + 1][py + 1][c];
         };
-        const float A0 = blend(pa, c0), A1 = blend(pa, c1), A2 = blend(pa, c2);
-        const float B0 = blend(pb, c0), B1 = blend(pb, c1), B2 = blend(pb, c2);
-        float ov[UPX];
+        const P A0 = blend(pa, c0), A1 = blend(pa, c1), A2 = blend(pa, c2);
+        const P B0 = blend(pb, c0), B1 = blend(pb, c1), B2 = blend(pb, c2);
+        P ov[UPX];
 #pragma unroll
         for (int e = 0; e < UPX; e++) {
-            float fx = fxs[e], gx = 1 - fx;
-            float a0 = dsel[e] ? A1 : A0, a1 = dsel[e] ? A2 : A1;
-            float b0 = dsel[e] ? B1 : B0, b1 = dsel[e] ? B2 : B1;
-            float av = gx * a0 + fx * a1, bv = gx * b0 + fx * b1;
+            P fx = fxs[e], gx = 1 - fx;
+            P a0 = dsel[e] ? A1 : A0, a1 = dsel[e] ? A2 : A1;
+            P b0 = dsel[e] ? B1 : B0, b1 = dsel[e] ? B2 : B1;
+            P av = gx * a0 + fx * a1, bv = gx * b0 + fx * b1;
             ov[e] = av * iv[j][e] + bv;
         }
+        float of[UPX];                          // emitted (and quantised) in fp32
+#pragma unroll
+        for (int e = 0; e < UPX; e++) of[e] = (float)ov[e];
         if (full) {
-            emit4(p.em, p.ny, p.nx, z, y, xb, ov, p.vout != 0, keep);
+            emit4(p.em, p.ny, p.nx, z, y, xb, of, p.vout != 0, keep);
         } else {
-            for (int e = 0; e < UPX && xb + e <= xc1; e++) emit(p.em, p.ny, p.nx, z, y, xb + e, ov[e]);
+            for (int e = 0; e < UPX && xb + e <= xc1; e++) emit(p.em, p.ny, p.nx, z, y, xb + e, of[e]);
         }
     }
 }
@@ -771,6 +782,11 @@ void box_global(fyg_stream *s, const Grid &g, int rho, int nf, const float *in0,
 
 }  // namespace
 
+extern "C" size_t fyg_guided_ws_bytes64(int nz, int ny, int nx, int s) {
+    size_t lz = (nz + s - 1) / s, ly = (ny + s - 1) / s, lx = (nx + s - 1) / s;
+    return sizeof(double) * 4 * lz * ly * lx;
+}
+
 extern "C" size_t fyg_guided_ws_bytes(int nz, int ny, int nx, int s) {
     size_t n = (size_t)nz * ny * nx;
     if (s > 1) {
@@ -782,38 +798,51 @@ extern "C" size_t fyg_guided_ws_bytes(int nz, int ny, int nx, int s) {
 }
 
 // upsample + emit arguments of one fast-path filter (region = inner tile when quantising)
-static UpArgs up_args(const fyg_guided_args *a, const float *MA, const float *MB) {
+template <class P>
+static const P *guide(const fyg_guided_args *a) {
+    if constexpr (sizeof(P) == sizeof(float)) return a->in;
+    else return a->in64;
+}
+
+template <class P>
+static UpArgs<P> up_args(const fyg_guided_args *a, const P *MA, const P *MB) {
     const int sd = a->s, nz = a->nz, ny = a->ny, nx = a->nx;
     const int lz = (nz + sd - 1) / sd, ly = (ny + sd - 1) / sd, lx = (nx + sd - 1) / sd;
-    UpArgs u{nz, ny, nx, lz, ly, lx, sd, MA, MB, a->in, 0, 0, 0, nz, ny, nx, 0, 0, a->em};
+    const P *in = guide<P>(a);
+    UpArgs<P> u{nz, ny, nx, lz, ly, lx, sd, MA, MB, in, 0, 0, 0, nz, ny, nx, 0, 0, a->em};
     if (a->em.quant) {
         u.z0 = (int)a->em.iz0; u.y0 = (int)a->em.iy0; u.x0 = (int)a->em.ix0;
         u.dz = (int)a->em.tz; u.dy = (int)a->em.ty; u.dx = (int)a->em.tx;
     }
-    u.vin = (nx % 4 == 0) && (u.x0 % 4 == 0) && ((uintptr_t)a->in % 16 == 0);
+    u.vin = (nx % 4 == 0) && (u.x0 % 4 == 0) && ((uintptr_t)in % 16 == 0);
     u.vout = a->em.quant ? (a->em.tx % 4 == 0 && (uintptr_t)a->em.q % 4 == 0)
                          : (u.vin && (uintptr_t)a->em.out % 16 == 0);
     return u;
 }
 
 // upsample launches for n filters whose output regions have one shape
-static void launch_upsample(fyg_stream *s, const UpArgs *u, int n, int sd) {
+template <class P>
+static void launch_upsample(fyg_stream *s, const UpArgs<P> *u, int n, int sd) {
     if (n < 1 || (size_t)u[0].dz * u[0].dy * u[0].dx == 0) return;
     const int nzb = (u[0].dz + URZ - 1) / URZ;
     for (int i = 0; i < n; i += FYG_BATCH) {
         int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
-        Batch<UpArgs> b{};
+        Batch<UpArgs<P>> b{};
         for (int j = 0; j < m; j++) b.a[j] = u[i + j];
         dim3 grid((unsigned)((u[0].dx + 64 * UPX - 1) / (64 * UPX)), (unsigned)((u[0].dy + URY - 1) / URY),
                   (unsigned)(nzb * m));
-        if ((sd & (sd - 1)) == 0) fyg_launch(s, "guided_upsample", k_upsample<true>, grid, dim3(NT), 0, b, nzb);
-        else fyg_launch(s, "guided_upsample", k_upsample<false>, grid, dim3(NT), 0, b, nzb);
+        if ((sd & (sd - 1)) == 0)
+            fyg_launch(s, "guided_upsample", k_upsample<true, P>, grid, dim3(NT), 0, b, nzb);
+        else
+            fyg_launch(s, "guided_upsample", k_upsample<false, P>, grid, dim3(NT), 0, b, nzb);
     }
 }
 
+template <class P>
 static int fast_batch(const fyg_guided_args *a, int n, fyg_stream *s);
 
 extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
+    if (a->f64) return fyg_guided_run_batch(a, 1, s);
     int r = a->r < 1 ? 2 : a->r;
     int sd = a->s;
     int nz = a->nz, ny = a->ny, nx = a->nx;
@@ -856,7 +885,7 @@ extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
     size_t nl = (size_t)lz * ly * lx;
     int rl = r / sd;
     if (rl < 1) rl = 1;
-    if (rl <= 2) return fast_batch(a, 1, s);
+    if (rl <= 2) return fast_batch<float>(a, 1, s);
     float *A = ws, *B = ws + nl, *MA = ws + 2 * nl, *MB = ws + 3 * nl;
     Grid gl{lz, ly, lx, sd, ny, nx};
     {
@@ -871,13 +900,14 @@ extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
         box_global(s, gl, rl, 2, A, B, O0, O1, T0, T1);
         if (fyg_d2d(MA, O0, sizeof(float) * nl, s) || fyg_d2d(MB, O1, sizeof(float) * nl, s)) return 1;
     }
-    UpArgs u = up_args(a, MA, MB);
+    UpArgs<float> u = up_args<float>(a, MA, MB);
     launch_upsample(s, &u, 1, sd);
     return fyg_last_error();
 }
 
 // fast path with rl = r/s <= 2 for n filters of one shape: per stage one launch
-// per FYG_BATCH cubes (coefficients, then upsample + emit)
+// per FYG_BATCH cubes (coefficients, then upsample + emit), in precision P
+template <class P>
 static int fast_batch(const fyg_guided_args *a, int n, fyg_stream *s) {
     const fyg_guided_args &h = a[0];
     const int sd = h.s, r = h.r < 1 ? 2 : h.r;
@@ -886,28 +916,28 @@ static int fast_batch(const fyg_guided_args *a, int n, fyg_stream *s) {
     int rl = r / sd;
     if (rl < 1) rl = 1;
     const Grid gl{lz, ly, lx, sd, h.ny, h.nx};
-    std::vector<CoefArgs> c(n);
-    std::vector<UpArgs> u(n);
+    std::vector<CoefArgs<P>> c(n);
+    std::vector<UpArgs<P>> u(n);
     for (int i = 0; i < n; i++) {
-        float *ws = (float *)a[i].ws;
-        float *MA = ws + 2 * nl, *MB = ws + 3 * nl;
-        c[i] = CoefArgs{gl, a[i].in, a[i].eps, MA, MB};
-        u[i] = up_args(&a[i], MA, MB);
+        P *ws = (P *)a[i].ws;
+        P *MA = ws + 2 * nl, *MB = ws + 3 * nl;
+        c[i] = CoefArgs<P>{gl, guide<P>(&a[i]), a[i].eps, MA, MB};
+        u[i] = up_args<P>(&a[i], MA, MB);
     }
     for (int i = 0; i < n; i += FYG_BATCH) {
         int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
-        Batch<CoefArgs> b{};
+        Batch<CoefArgs<P>> b{};
         for (int j = 0; j < m; j++) b.a[j] = c[i + j];
         if (rl == 1) {
             dim3 g = march_grid<1>(gl);
             int nzc = (int)g.z;
             g.z *= m;
-            fyg_launch(s, "guided_coef", k_coef<1>, g, dim3(NT), 0, b, nzc);
+            fyg_launch(s, "guided_coef", k_coef<1, P>, g, dim3(NT), 0, b, nzc);
         } else {
             dim3 g = march_grid<2>(gl);
             int nzc = (int)g.z;
             g.z *= m;
-            fyg_launch(s, "guided_coef", k_coef<2>, g, dim3(NT), 0, b, nzc);
+            fyg_launch(s, "guided_coef", k_coef<2, P>, g, dim3(NT), 0, b, nzc);
         }
     }
     launch_upsample(s, u.data(), n, sd);
@@ -926,7 +956,11 @@ extern "C" int fyg_guided_run_batch(const fyg_guided_args *a, int n, fyg_stream 
                 (b.em.quant != 0) == (h.em.quant != 0) &&
                 (!h.em.quant || (b.em.tz == h.em.tz && b.em.ty == h.em.ty && b.em.tx == h.em.tx));
     }
-    if (batch) return fast_batch(a, n, s);
+    if (h.f64) {                     // all-FP64 variant: fast path only
+        for (int i = 0; i < n && batch; i++) batch = a[i].f64 && a[i].in64;
+        return batch ? fast_batch<double>(a, n, s) : 1;
+    }
+    if (batch) return fast_batch<float>(a, n, s);
     for (int i = 0; i < n; i++)
         if (fyg_guided_run(&a[i], s)) return 1;
     return 0;

@@ -305,6 +305,20 @@ extern "C" const void *fyg_twiddles(int L) {
     return fyg_table(key, (size_t)kl, tw.data(), sizeof(float) * tw.size());
 }
 
+extern "C" const void *fyg_twiddles64(int L) {
+    char key[16];
+    int kl = snprintf(key, sizeof key, "Tw64%d", L);
+    const void *d = fyg_table_find(key, (size_t)kl);
+    if (d) return d;
+    std::vector<double> tw(2 * (size_t)L);
+    for (int m = 0; m < L; m++) {
+        double a = 2.0 * M_PI * (double)m / (double)L;
+        tw[2 * m] = cos(a);
+        tw[2 * m + 1] = sin(a);
+    }
+    return fyg_table(key, (size_t)kl, tw.data(), sizeof(double) * tw.size());
+}
+
 // ------------------------------------------------------------------ profiling
 namespace fyg_prof {
 std::atomic<int> enabled{0};

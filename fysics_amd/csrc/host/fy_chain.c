@@ -103,7 +103,8 @@ typedef struct {
     const fy_cube *c;
     int skip;                   /* empty region: nothing to do */
     size_t hn;                  /* voxels of the halo'd region */
-    int deconv, guided;
+    int deconv, guided, f64;
+    size_t tile_bytes;          /* the deconvolved tile: fp32, or fp64 for the all-FP64 variant */
     fyg_prologue pro;           /* u8 prologue (both paths) */
     fyg_deconv_args d;          /* deconv = 1 */
     fyg_guided_args g;          /* guided = 1 */
@@ -124,6 +125,10 @@ static int plan_cube(const fy_cube *c, cube_plan *pl, fyg_stream *s) {
         c->y0 + c->ty > ry0 + hy || c->x0 + c->tx > rx0 + hx)
         return 1;                                               /* inner tile must lie in the region */
     pl->hn = (size_t)hz * hy * hx;
+    pl->f64 = P->fp64 != 0;
+    /* the all-FP64 variant covers deconvolution + fast guided filtering */
+    if (pl->f64 && (!P->deconv || P->guided_eps <= 0 || P->guided_s == 1)) return 1;
+    pl->tile_bytes = (pl->f64 ? sizeof(double) : sizeof(float)) * pl->hn;
 
     /* prologue (pipeline.c:492-494 and the dewindow of :266-268) */
     fyg_prologue *pro = &pl->pro;
@@ -152,7 +157,7 @@ static int plan_cube(const fy_cube *c, cube_plan *pl, fyg_stream *s) {
         if (param <= 0) param = matched ? 0.05 : fy_auto_reg(&P->phys, 0.015);
         fyh_tables t;
         if (fyh_tables_get((int)hz, (int)hy, (int)hx, &P->phys, param, matched, P->deconv_cap,
-                           P->fz2_scale > 0 ? P->fz2_scale : 1.0, s, &t))
+                           P->fz2_scale > 0 ? P->fz2_scale : 1.0, pl->f64, s, &t))
             return 1;
         if (!fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px)) return 1;
         fyg_deconv_args *a = &pl->d;
@@ -167,7 +172,8 @@ static int plan_cube(const fy_cube *c, cube_plan *pl, fyg_stream *s) {
             a->epi.dec = 1; a->epi.dlo = (float)P->dec_lo; a->epi.dinv = 1.0f / (float)(P->dec_hi - P->dec_lo);
         }
         a->epi.clamp01 = 1;
-        pl->spec_bytes = fyg_deconv_spec_bytes(t.pz, t.py, t.px);
+        a->f64 = pl->f64;
+        pl->spec_bytes = pl->f64 ? fyg_deconv_spec_bytes64(t.pz, t.py, t.px) : fyg_deconv_spec_bytes(t.pz, t.py, t.px);
         if (pro->dering) pl->img_bytes = fyg_ring_image_bytes(pro, (int)hz, (int)hy, (int)hx);
     }
 
@@ -191,7 +197,12 @@ static int plan_cube(const fy_cube *c, cube_plan *pl, fyg_stream *s) {
         g->r = P->guided_radius < 1 ? 2 : P->guided_radius;
         g->s = P->guided_s == 0 ? 2 : P->guided_s;
         g->eps = P->guided_eps;
-        pl->gws_bytes = fyg_guided_ws_bytes(g->nz, g->ny, g->nx, g->s);
+        g->f64 = pl->f64;
+        pl->gws_bytes = pl->f64 ? fyg_guided_ws_bytes64(g->nz, g->ny, g->nx, g->s)
+                                : fyg_guided_ws_bytes(g->nz, g->ny, g->nx, g->s);
+        if (pl->f64 && (g->s < 2 || g->r / g->s > 2 || (hz + g->s - 1) / g->s < 2 || (hy + g->s - 1) / g->s < 2 ||
+                        (hx + g->s - 1) / g->s < 2))
+            return 1;
     }
     return 0;
 }
@@ -201,7 +212,8 @@ static int same_group(const cube_plan *a, const cube_plan *b) {
     const fy_cube *x = a->c, *y = b->c;
     if (x->hz != y->hz || x->hy != y->hy || x->hx != y->hx || x->tz != y->tz || x->ty != y->ty || x->tx != y->tx)
         return 0;
-    if (a->deconv != b->deconv || a->guided != b->guided || a->pro.dering != b->pro.dering) return 0;
+    if (a->deconv != b->deconv || a->guided != b->guided || a->pro.dering != b->pro.dering || a->f64 != b->f64)
+        return 0;
     if ((x->d_float != NULL) != (y->d_float != NULL)) return 0;
     if (a->guided && (a->g.s != b->g.s || a->g.r != b->g.r)) return 0;
     return 1;
@@ -212,7 +224,7 @@ static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 /* the chain of n planned cubes of one group; flags[i] is cube i's all-air flag */
 static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) {
     const cube_plan *h = &pl[0];
-    size_t tile = al256(sizeof(float) * h->hn), spec = al256(h->spec_bytes), img = 0, gws = al256(h->gws_bytes);
+    size_t tile = al256(h->tile_bytes), spec = al256(h->spec_bytes), img = 0, gws = al256(h->gws_bytes);
     for (int i = 0; i < n; i++)                                 /* ring images may span more slabs */
         if (al256(pl[i].img_bytes) > img) img = al256(pl[i].img_bytes);
     char *T = (char *)fyg_ws(s, WS_TILE, tile * n);
@@ -226,7 +238,7 @@ static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) 
         if (!da) return 1;
         for (int i = 0; i < n; i++) {
             da[i] = pl[i].d;
-            da[i].out = (float *)(T + tile * i);
+            da[i].out = T + tile * i;
             da[i].spec = S + spec * i;
             if (pl[i].pro.dering) da[i].pro.cimg = (float *)(I + img * i);
             da[i].nonzero_flag = flags[i];
@@ -251,7 +263,8 @@ static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) 
         if (!ga) return 1;
         for (int i = 0; i < n; i++) {
             ga[i] = pl[i].g;
-            ga[i].in = (const float *)(T + tile * i);
+            if (pl[i].f64) ga[i].in64 = (const double *)(T + tile * i);
+            else ga[i].in = (const float *)(T + tile * i);
             ga[i].ws = G + gws * i;
             ga[i].em = pl[i].em;
         }
@@ -264,7 +277,8 @@ static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) 
         const float *src = (const float *)(T + tile * i);
         if (h->guided) {
             fyg_guided_args g = pl[i].g;
-            g.in = src;
+            if (pl[i].f64) g.in64 = (const double *)(T + tile * i);
+            else g.in = src;
             g.ws = G + gws * i;
             g.em = pl[i].em;
             g.em.quant = 0;
@@ -311,7 +325,7 @@ static int run_chunk(const fy_cube *cubes, int n, int *d_flags, int gen, fyg_str
                 m++;
             }
         }
-        size_t per = al256(sizeof(float) * pl[i].hn) + al256(pl[i].spec_bytes) + al256(pl[i].gws_bytes) +
+        size_t per = al256(pl[i].tile_bytes) + al256(pl[i].spec_bytes) + al256(pl[i].gws_bytes) +
                      2 * al256(pl[i].img_bytes);
         int part = (int)(FYH_GROUP_BYTES / (per ? per : 1));
         if (part < 1) part = 1;
@@ -336,10 +350,11 @@ size_t fy_cube_workspace_bytes(const fy_cube *c) {
     if (!c || !c->params || c->hz <= 0 || c->hy <= 0 || c->hx <= 0) return 0;
     const fy_cube_params *P = c->params;
     size_t hn = (size_t)c->hz * c->hy * c->hx;
-    size_t b = al256(sizeof(float) * hn);
+    int f64 = P->fp64 != 0;
+    size_t b = al256((f64 ? sizeof(double) : sizeof(float)) * hn);
     if (P->deconv) {
         int pz = fy_next_fft_size((int)c->hz), py = fy_next_fft_size((int)c->hy), px = fy_next_fft_size((int)c->hx);
-        b += al256(fyg_deconv_spec_bytes(pz, py, px));
+        b += al256(f64 ? fyg_deconv_spec_bytes64(pz, py, px) : fyg_deconv_spec_bytes(pz, py, px));
         if (P->dering && P->dering->have_rings) {
             fyg_prologue pro;
             memset(&pro, 0, sizeof pro);
@@ -347,8 +362,11 @@ size_t fy_cube_workspace_bytes(const fy_cube *c) {
             b += al256(fyg_ring_image_bytes(&pro, (int)c->hz, (int)c->hy, (int)c->hx));
         }
     }
-    if (P->guided_eps > 0)
-        b += al256(fyg_guided_ws_bytes((int)c->hz, (int)c->hy, (int)c->hx, P->guided_s == 0 ? 2 : P->guided_s));
+    if (P->guided_eps > 0) {
+        int sd = P->guided_s == 0 ? 2 : P->guided_s;
+        b += al256(f64 ? fyg_guided_ws_bytes64((int)c->hz, (int)c->hy, (int)c->hx, sd)
+                       : fyg_guided_ws_bytes((int)c->hz, (int)c->hy, (int)c->hx, sd));
+    }
     return b;
 }
 

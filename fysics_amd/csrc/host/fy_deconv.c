@@ -49,13 +49,14 @@ typedef struct {
 } table_key;
 
 int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, int kind,
-                   double cap, double fz2_scale, fyg_stream *s, fyh_tables *t) {
+                   double cap, double fz2_scale, int f64, fyg_stream *s, fyh_tables *t) {
     int pz = fy_next_fft_size(nz), py = fy_next_fft_size(ny), px = fy_next_fft_size(nx);
     int hx = px / 2 + 1;
     if (fz2_scale <= 0) fz2_scale = 1.0;
     t->pz = pz; t->py = py; t->px = px; t->hx = hx; t->hxp = fyg_deconv_hxp(px);
-    /* layout of the device block: lut[4097] (padded to 4100 floats), fz2, fy2, fx2 */
-    size_t lut_bytes = sizeof(float) * 4100;
+    /* layout of the device block: lut[4097] (padded to 4100 entries; fp32, or the
+     * same weights in fp64 for the all-FP64 variant), fz2, fy2, fx2 */
+    size_t lut_bytes = (f64 ? sizeof(double) : sizeof(float)) * 4100;
     size_t bytes = lut_bytes + sizeof(double) * (size_t)(pz + py + hx);
     double *fz2 = (double *)malloc(sizeof(double) * (size_t)(pz + py + hx));
     if (!fz2) return 1;
@@ -81,7 +82,7 @@ int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, in
 
     table_key key;
     memset(&key, 0, sizeof key);
-    memcpy(key.tag, "LUT", 4);
+    memcpy(key.tag, f64 ? "L64" : "LUT", 4);
     key.kind = kind; key.pz = pz; key.py = py; key.px = px;
     memcpy(key.phys, p, sizeof(double) * 8);
     key.param = param; key.cap = cap; key.fz2_scale = fz2_scale;
@@ -94,13 +95,18 @@ int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, in
         unsigned char *blk = (unsigned char *)malloc(bytes);
         if (!blk) { free(fz2); return 1; }
         float *lut = (float *)blk;
+        double *lut64 = (double *)blk;
         for (int i = 0; i <= FYH_LUT_N; i++) {
             double fr = sqrt((double)i / us);
             double w = kind == 0 ? wiener_weight(fr, p, param) : matched_weight(fr, p, param);
             if (cap > 0 && w > cap) w = cap;
-            lut[i] = (float)w;
+            if (f64) lut64[i] = w;
+            else lut[i] = (float)w;
         }
-        lut[4097] = lut[4098] = lut[4099] = 0.0f;
+        for (int i = 4097; i < 4100; i++) {
+            if (f64) lut64[i] = 0.0;
+            else lut[i] = 0.0f;
+        }
         memcpy(blk + lut_bytes, fz2, sizeof(double) * (size_t)(pz + py + hx));
         dev = (const unsigned char *)(slot ? fyg_stream_table(s, &key, sizeof key, blk, bytes)
                                            : fyg_table(&key, sizeof key, blk, bytes));
@@ -108,7 +114,7 @@ int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, in
     }
     free(fz2);
     if (!dev) return 1;
-    t->lut = (const float *)dev;
+    t->lut = dev;
     t->fz2 = (const double *)(dev + lut_bytes);
     t->fy2 = t->fz2 + pz;
     t->fx2 = t->fy2 + py;
@@ -119,7 +125,7 @@ int fyh_deconv_dev(const float *d_in, float *d_out, int nz, int ny, int nx, cons
                    double param, int kind, double cap, double fz2_scale, fyg_stream *s) {
     if (nz <= 0 || ny <= 0 || nx <= 0) return 0;
     fyh_tables t;
-    if (fyh_tables_get(nz, ny, nx, p, param, kind, cap, fz2_scale, s, &t)) return 1;
+    if (fyh_tables_get(nz, ny, nx, p, param, kind, cap, fz2_scale, 0, s, &t)) return 1;
     if (!fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px)) return 1;
     void *spec = fyg_ws(s, WS_SPEC, fyg_deconv_spec_bytes(t.pz, t.py, t.px));
     if (!spec) return 1;
@@ -180,4 +186,28 @@ int fy_gpu_deconv_radial(const float *d_in, float *d_out, int nz, int ny, int nx
     fyg_stream *s = fyg_stream_wrap(stream);
     if (!s) return 1;
     return fyh_deconv_dev(d_in, d_out, nz, ny, nx, p, param, kind, cap, fz2_scale, s);
+}
+
+/* the all-FP64 variant of deconv_radial: fp32 input, fp64 spectrum, weights and
+ * output (the speed reference of the precision study) */
+int fy_gpu_deconv_radial_f64(const float *d_in, double *d_out, int nz, int ny, int nx, const fy_physics *p,
+                             double param, int kind, double cap, double fz2_scale, void *stream) {
+    if (nz <= 0 || ny <= 0 || nx <= 0) return 0;
+    fyg_stream *s = fyg_stream_wrap(stream);
+    if (!s) return 1;
+    fyh_tables t;
+    if (fyh_tables_get(nz, ny, nx, p, param, kind, cap, fz2_scale, 1, s, &t)) return 1;
+    if (!fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px)) return 1;
+    void *spec = fyg_ws(s, WS_SPEC, fyg_deconv_spec_bytes64(t.pz, t.py, t.px));
+    if (!spec) return 1;
+    fyg_deconv_args a;
+    memset(&a, 0, sizeof a);
+    a.nz = nz; a.ny = ny; a.nx = nx;
+    a.pz = t.pz; a.py = t.py; a.px = t.px; a.hx = t.hx; a.hxp = t.hxp;
+    a.lut = t.lut; a.us = t.us; a.fz2 = t.fz2; a.fy2 = t.fy2; a.fx2 = t.fx2;
+    a.in = d_in; a.in_u8 = 0;
+    a.out = d_out;
+    a.spec = spec;
+    a.f64 = 1;
+    return fyg_deconv_run(&a, s);
 }

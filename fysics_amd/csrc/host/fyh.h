@@ -17,14 +17,14 @@ enum { WS_IN = 0, WS_SPEC = 1, WS_TILE = 2, WS_GUIDED = 3, WS_U8 = 4, WS_FLAG = 
 /* device-resident transfer tables of one (weight, physics, padded shape) */
 typedef struct {
     int pz, py, px, hx, hxp;
-    const float *lut;
+    const void *lut;                /* fp32 (or fp64 for the all-FP64 variant) */
     double us;
     const double *fz2, *fy2, *fx2;
 } fyh_tables;
 
 /* kind 0: Wiener weight of the recon transfer (cap 8); kind 1: matched PSF*unsharp */
 int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, int kind,
-                   double cap, double fz2_scale, fyg_stream *s, fyh_tables *t);
+                   double cap, double fz2_scale, int f64, fyg_stream *s, fyh_tables *t);
 double fyh_matched_fz2_scale(const fy_physics *p);
 /* the physics fy_process_buffer deconvolves with (pipeline.c:89-104) */
 void fyh_scaled_physics(const fy_pipeline_cfg *c, fy_physics *p);

@@ -31,6 +31,13 @@ int fy_gpu_deconv_radial(const float *d_in, float *d_out, int nz, int ny, int nx
                          const fy_physics *p, double param, int kind, double cap,
                          double fz2_scale, void *stream);
 
+/* the all-FP64 variant of fy_gpu_deconv_radial: fp32 input, spectrum, weights
+ * and output in double (the speed reference of the precision study; no
+ * reference counterpart) */
+int fy_gpu_deconv_radial_f64(const float *d_in, double *d_out, int nz, int ny, int nx,
+                             const fy_physics *p, double param, int kind, double cap,
+                             double fz2_scale, void *stream);
+
 /* guided filter on device buffers; s <= 1 exact (guided.c:113), else fast (guided.c:152) */
 int fy_gpu_guided(const float *d_in, float *d_out, int nz, int ny, int nx,
                   int radius, double eps, int s, void *stream);
@@ -69,6 +76,7 @@ typedef struct {
     int    recenter; int norm_lo, norm_hi;   /* pipeline.c:413-421 */
     const fy_dering *dering;   /* finalized ring state, or NULL */
     const float *zdrift_factor; long vol_z;  /* per-slice factors, or NULL */
+    int    fp64;               /* all-FP64 variant of deconv + fast guided (needs guided_s >= 2) */
 } fy_cube_params;
 
 typedef struct {

@@ -84,7 +84,7 @@ class CubeParams(C.Structure):
         ("dec_hi", C.c_double), ("guided_eps", C.c_double), ("guided_radius", C.c_int),
         ("guided_s", C.c_int), ("no_dither", C.c_int), ("recenter", C.c_int), ("norm_lo", C.c_int),
         ("norm_hi", C.c_int), ("dering", C.POINTER(Dering)), ("zdrift_factor", c_float_p),
-        ("vol_z", C.c_long)]
+        ("vol_z", C.c_long), ("fp64", C.c_int)]
 
 
 def fp(a):

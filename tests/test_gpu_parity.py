@@ -95,6 +95,26 @@ def test_deconvolve_long_axes_vs_oracle(orc, base, axis):
     assert rel_l2(got, want64) <= REL_L2
 
 
+@pytest.mark.parametrize("shape", [(22, 27, 30), (5, 7, 9), (64, 64, 64), (96, 40, 24), (512, 4, 4), (4, 4, 768)])
+def test_deconvolve_f64_vs_oracle(orc, shape):
+    """The all-FP64 variant against the fp64 oracle (same fp64 LUT weights):
+    relative L2 <= 1e-12."""
+    import torch
+    rng = np.random.default_rng(sum(shape) + 64)
+    x = rng.random(shape).astype(np.float32)
+    p = F.physics()
+    d_in = torch.from_numpy(x).cuda()
+    d_out = torch.empty(shape, dtype=torch.float64, device="cuda")
+    assert F.lib().fy_gpu_deconv_radial_f64(d_in.data_ptr(), d_out.data_ptr(), *shape, C.byref(p), 0.015, 0,
+                                            8.0, 1.17 ** 2, None) == 0
+    torch.cuda.synchronize()
+    want64 = np.empty(shape, np.float64)
+    pc = clibs.physics()
+    assert orc.orc_deconv_radial_f64(clibs.fp(x), clibs.dp(want64), *shape, C.byref(pc), 0.015, 0, 8.0,
+                                     1.17 ** 2) == 0
+    assert rel_l2(d_out.cpu().numpy(), want64) <= 1e-12
+
+
 @pytest.mark.parametrize("shape", [(20, 24, 28), (48, 48, 48), (7, 9, 11)])
 def test_deconvolve_matched_aniso_vs_oracle(orc, shape):
     rng = np.random.default_rng(3)
