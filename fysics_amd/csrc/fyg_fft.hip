@@ -20,13 +20,13 @@ __global__ __launch_bounds__(256) void k_cfft_rows(Rows a) {
     load_twiddles<L>(tw_s, a.tw);
     __syncthreads();
     const long g0 = (long)blockIdx.x * NL;
-    auto first = [&](int l, int x) -> float2 {
+    auto first = [&](int l, int p0, int off) -> float2 {
         long g = g0 + l;
-        return g < a.nlines ? a.data[(size_t)g * L + x] : make_float2(0.f, 0.f);
+        return g < a.nlines ? a.data[(size_t)g * L + p0 + off] : make_float2(0.f, 0.f);
     };
-    auto last = [&](int l, int x, float2 v) {
+    auto last = [&](int l, int p0, int off, float2 v) {
         long g = g0 + l;
-        if (g < a.nlines) a.data[(size_t)g * L + x] = v;
+        if (g < a.nlines) a.data[(size_t)g * L + p0 + off] = v;
     };
     if (a.sg < 0)
         run_from<L, NL, false, false, 0>(-1.0f, tw_s, buf, first, last);

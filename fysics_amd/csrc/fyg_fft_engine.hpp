@@ -264,8 +264,12 @@ __device__ __forceinline__ void item_of(int i, int &l, int &j) {
     }
 }
 
-// one Stockham stage; LDIN/STOUT read/write (line, pos).  SYNC: barrier between
-// the read and the write phase (needed when both touch the same LDS buffer).
+// one Stockham stage.  The loader ld(l, p0, off) returns element p0 + off of line
+// l and the storer st(l, p0, off, v) writes it: p0 is the item's base position
+// and off = r*L/R (loads) or r*NS (stores) a compile-time constant, so address
+// work that depends only on (l, p0) is done once per item.  SYNC: barrier
+// between the read and the write phase (needed when both touch the same LDS
+// buffer).
 template <int L, int NL, int T, bool CM, int R, int NS, bool SYNC, class V, class LdF, class StF>
 __device__ __forceinline__ void stage(scal_t<V> sg, const V *__restrict__ tw, LdF ld, StF st) {
     constexpr int NB = L / R;
@@ -280,7 +284,7 @@ __device__ __forceinline__ void stage(scal_t<V> sg, const V *__restrict__ tw, Ld
             int l, j;
             item_of<NL, NB, CM>(i, l, j);
 #pragma unroll
-            for (int r = 0; r < R; r++) v[b][r] = ld(l, j + r * NB);
+            for (int r = 0; r < R; r++) v[b][r] = ld(l, j, r * NB);
         }
     }
     if (SYNC) __syncthreads();
@@ -301,7 +305,7 @@ __device__ __forceinline__ void stage(scal_t<V> sg, const V *__restrict__ tw, Ld
             dft<R>(v[b], sg);
             const int ob = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int r = 0; r < R; r++) st(l, ob + r * NS, v[b][r]);
+            for (int r = 0; r < R; r++) st(l, ob, r * NS, v[b][r]);
         }
     }
 }
@@ -309,12 +313,12 @@ __device__ __forceinline__ void stage(scal_t<V> sg, const V *__restrict__ tw, Ld
 template <int L, int NL, class V>
 struct LdsRd {
     const V *buf;
-    __device__ V operator()(int l, int pos) const { return buf[l * Shape<L>::LD + lds_pos(pos)]; }
+    __device__ V operator()(int l, int p0, int off) const { return buf[l * Shape<L>::LD + lds_pos(p0 + off)]; }
 };
 template <int L, int NL, class V>
 struct LdsWr {
     V *buf;
-    __device__ void operator()(int l, int pos, V v) const { buf[l * Shape<L>::LD + lds_pos(pos)] = v; }
+    __device__ void operator()(int l, int p0, int off, V v) const { buf[l * Shape<L>::LD + lds_pos(p0 + off)] = v; }
 };
 
 // stages [S, END) of a plan (END < 0: all); stage 0 reads `first`, the final
@@ -471,9 +475,9 @@ __global__ __launch_bounds__(256) void k_rfft_x(Batch<XFwd<P>> bt) {
     }
     __syncthreads();
     int any = 0;
-    auto first = [&](int l, int x) -> V {
+    auto first = [&](int l, int p0, int off) -> V {
         const XLine &m = meta[l];
-        int rx = reflect_idx(x, a.nx);
+        int rx = reflect_idx(p0 + off, a.nx);
         float va, vb;
         if (U8) {
             const unsigned char *u = (const unsigned char *)a.in;
@@ -542,11 +546,16 @@ __global__ __launch_bounds__(256) void k_cfft_cols(Batch<Cols<P>> bt) {
     const long c0 = (long)blockIdx.x * NL;
     V *base = a.data + (size_t)blockIdx.y * a.bstride + c0;
     const long nc = a.ncols - c0;
-    auto first = [&](int l, int pos) -> V {
-        return l < nc ? base[(size_t)pos * a.estride + l] : Cx<P>::make(0, 0);
+    const size_t es = (size_t)a.estride;
+    // Columns past ncols (the last group of a row) load column nc-1 instead --
+    // in bounds, no branch around the load -- and are never stored; the lines
+    // of a workgroup are independent transforms.
+    auto first = [&](int l, int p0, int off) -> V {
+        int lc = l < nc ? l : (int)nc - 1;
+        return base[(size_t)p0 * es + lc + (size_t)off * es];
     };
-    auto last = [&](int l, int pos, V v) {
-        if (l < nc) base[(size_t)pos * a.estride + l] = v;
+    auto last = [&](int l, int p0, int off, V v) {
+        if (l < nc) base[(size_t)p0 * es + l + (size_t)off * es] = v;
     };
     if (a.sg < 0)
         run_from<L, NL, true, false, 0>((P)-1, tw_s, buf, first, last);
@@ -595,12 +604,15 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
     }
     __syncthreads();
     V *base = a.data + c0;
+    const size_t es = (size_t)a.estride;
     auto valid = [&](int l) { return s_ok[l] != 0; };
-    auto first = [&](int l, int pos) -> V {
-        return valid(l) ? base[(size_t)pos * a.estride + l] : Cx<P>::make(0, 0);
+    // Every column of a group lies inside its row's pitch, so loads need no
+    // guard: pitch-padding columns (k >= hx) are transformed and not stored.
+    auto first = [&](int l, int p0, int off) -> V {
+        return base[(size_t)p0 * es + l + (size_t)off * es];
     };
-    auto last = [&](int l, int pos, V v) {
-        if (valid(l)) base[(size_t)pos * a.estride + l] = v;
+    auto last = [&](int l, int p0, int off, V v) {
+        if (valid(l)) base[(size_t)p0 * es + l + (size_t)off * es] = v;
     };
     // LUT weight of deconv_radial (paganin.c:231-242) at z = pos:
     // t = ((fz2[z] + fy2[y]) + fx2[k]) * us, linear interpolation between knots
@@ -633,9 +645,8 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
             item_of<NL, NB, true>(i, l, j);
 #pragma unroll
             for (int r = 0; r < RL; r++) {
-                int pos = j + r * NB;
-                if constexpr (n > 1) v[b][r] = buf[l * Shape<L>::LD + lds_pos(pos)];
-                else v[b][r] = first(l, pos);
+                if constexpr (n > 1) v[b][r] = buf[l * Shape<L>::LD + lds_pos(j + r * NB)];
+                else v[b][r] = first(l, j, r * NB);
             }
             if constexpr (NSL > 1) {
                 constexpr int TWS = L / (NSL * RL);
@@ -678,7 +689,7 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
                 int l, j;
                 item_of<NL, NB, true>(i, l, j);
 #pragma unroll
-                for (int r = 0; r < RL; r++) last(l, j * RL + r, v[b][r]);
+                for (int r = 0; r < RL; r++) last(l, j * RL, r, v[b][r]);
             }
         }
     }
@@ -729,8 +740,9 @@ __global__ __launch_bounds__(256) void k_irfft_x(Batch<XInv<P>> bt) {
     }
     __syncthreads();
     // L(x) = A(x) + i B(x) for x <= px/2, Hermitian extension above (paganin.c:271-280)
-    auto first = [&](int l, int x) -> V {
+    auto first = [&](int l, int p0, int off) -> V {
         const XInvLine &m = meta[l];
+        const int x = p0 + off;
         int k = x < a.hx ? x : L - x;
         // Loads are unconditional so the unrolled gather keeps them all in flight:
         // a line past the end recomputes line 0, and a row y0+1 >= ny still packs
@@ -740,8 +752,9 @@ __global__ __launch_bounds__(256) void k_irfft_x(Batch<XInv<P>> bt) {
         V B = a.spec[m.sb + k];
         return x < a.hx ? Cx<P>::make(A.x - B.y, A.y + B.x) : Cx<P>::make(A.x + B.y, B.x - A.y);
     };
-    auto last = [&](int l, int x, V v) {
+    auto last = [&](int l, int p0, int off, V v) {
         const XInvLine &m = meta[l];
+        const int x = p0 + off;
         if (x >= a.nx || !m.valid) return;
         a.out[m.oo + x] = epilogue<P>(a.epi, v.x * a.invn);
         if (m.pair) a.out[m.oo + a.nx + x] = epilogue<P>(a.epi, v.y * a.invn);
