@@ -45,6 +45,25 @@ __device__ __forceinline__ P count_recip(int i, int n, int rho) {
     return (P)1 / (P)(hi - lo + 1);
 }
 
+// the same reciprocals from a table (window lengths up to 7): for a
+// wave-uniform position this is one scalar load instead of a division
+__constant__ float kRecipF[8] = {0.0f, 1.0f, 1.0f / 2, 1.0f / 3, 1.0f / 4, 1.0f / 5, 1.0f / 6, 1.0f / 7};
+__constant__ double kRecipD[8] = {0.0, 1.0, 1.0 / 2, 1.0 / 3, 1.0 / 4, 1.0 / 5, 1.0 / 6, 1.0 / 7};
+template <class P>
+__device__ __forceinline__ P count_recip_tab(int i, int n, int rho) {
+    int lo = i - rho < 0 ? 0 : i - rho;
+    int hi = i + rho > n - 1 ? n - 1 : i + rho;
+    if constexpr (sizeof(P) == sizeof(float)) return kRecipF[hi - lo + 1];
+    else return kRecipD[hi - lo + 1];
+}
+
+// a / (a + e) for the coefficient a = var/(var+eps): the hardware reciprocal
+// (1 ulp) in fp32, a correctly rounded division in fp64
+__device__ __forceinline__ float ratio(float v, float e) { return v * __builtin_amdgcn_rcpf(v + e); }
+__device__ __forceinline__ double ratio(double v, double e) { return v / (v + e); }
+
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+
 // Generic tiled box over NF fields.  load(gz,gy,gx,v[NF]) fills in-grid values;
 // cells outside the grid contribute 0.  out(gz,gy,gx,sum[NF],recip) is called for
 // every in-grid output voxel of the tile.
@@ -294,15 +313,19 @@ struct March {
     static constexpr int SW = AW + 2 * RHO;         // sample columns per side
     static constexpr int NS = SW * SW;
     static constexpr int KS = (NS + NT - 1) / NT;   // sample slots per thread
-#ifndef FYG_COEF_ZC
-#define FYG_COEF_ZC 16
+#ifdef FYG_COEF_ZC
+    static constexpr int ZC = FYG_COEF_ZC;
+#else
+    static constexpr int ZC = RHO == 1 ? 8 : 12;    // output planes per workgroup
 #endif
-    static constexpr int ZC = FYG_COEF_ZC;          // output planes per workgroup
     static constexpr int Q = 2 * RHO + 1;           // window length
     static constexpr int STEPS = ZC + 4 * RHO;      // sample planes of a full chunk
     // sample planes in flight + 1; divides STEPS so the buffer ring rotates by
     // full unrolling (a short last chunk runs idle steps)
     static constexpr int PD = STEPS % 5 == 0 ? 5 : STEPS % 6 == 0 ? 6 : STEPS % 4 == 0 ? 4 : 1;
+    // steps per unrolled loop body: the sample ring (PD) and the window queues
+    // (Q) both rotate by renaming, so no value moves between registers
+    static constexpr int U = PD / gcd_c(PD, Q) * Q;
     // LDS row pitches: the two rows of 16 columns a half-wave reads land in
     // disjoint banks (pitch 48 = 16 mod 32 for samples, 16 for a and b)
     static constexpr int SP = 48, AP = 16;
@@ -316,18 +339,30 @@ struct CoefArgs {
     P *ma, *mb;
 };
 
-// grid: (x blocks, y blocks, cube * nzc + z chunk)
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup h runs on XCD
+// h % 8) and each XCD has its own L2.  xcd_tile renumbers them so that XCD k
+// runs the k-th contiguous eighth of the logical tiles: neighbouring tiles,
+// which read each other's halo rows, then meet in one L2.
+__device__ __forceinline__ int xcd_tile(int h, int T) {
+    const int c = T / 8, r = T % 8, x = h % 8, i = h / 8;
+    return (x < r ? x * (c + 1) : r * (c + 1) + (x - r) * c) + i;
+}
+
+// grid: (x blocks, y blocks, cube * nzc + z chunk), renumbered by xcd_tile
 template <int RHO, class P>
 __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
     using M = March<RHO>;
-    const CoefArgs<P> &p = bt.a[blockIdx.z / nzc];
-    const int zchunk = blockIdx.z % nzc;
+    const int T = gridDim.x * gridDim.y * gridDim.z;
+    const int L = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), T);
+    const int tx = L % gridDim.x, ty = (L / gridDim.x) % gridDim.y, tz = L / (gridDim.x * gridDim.y);
+    const CoefArgs<P> &p = bt.a[tz / nzc];
+    const int zchunk = tz % nzc;
     static_assert(M::AW * M::AW == NT, "one thread per a, b column");
     static_assert(M::SW <= M::SP, "sample row fits its pitch");
     __shared__ P sv[M::SW][M::SP];
     __shared__ P sa[M::AW][M::AP], sb[M::AW][M::AP];
     const Grid &g = p.g;
-    const int bx = blockIdx.x * M::OW, by = blockIdx.y * M::OW, zs = zchunk * M::ZC;
+    const int bx = tx * M::OW, by = ty * M::OW, zs = zchunk * M::ZC;
     const int ze = min(zs + M::ZC, g.nz);
     const size_t sz = (size_t)g.s * g.fny * g.fnx, sy = (size_t)g.s * g.fnx;
     const P c0 = p.in[(size_t)zs * sz + (size_t)by * sy + (size_t)bx * g.s];
@@ -362,7 +397,10 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
 #pragma unroll
         for (int k = 0; k < M::KS; k++) dst[k] = p.in[zo + soff[k]];
     };
-    auto step = [&](int zp, const P *cur) {
+    // u: the step's position in the unrolled loop body; the window queues are
+    // rings whose newest entry is slot u % Q (steps before zs push zeros)
+    auto step = [&](int zp, const P *cur, int u) {
+        const int slot = u % M::Q;
         const bool zin = zp >= 0 && zp < g.nz;
 #pragma unroll
         for (int k = 0; k < M::KS; k++)
@@ -382,10 +420,8 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
             s1 += r1;
             s2 += r2;
         }
-#pragma unroll
-        for (int j = 0; j < M::Q - 1; j++) { q1[j] = q1[j + 1]; q2[j] = q2[j + 1]; }
-        q1[M::Q - 1] = s1;
-        q2[M::Q - 1] = s2;
+        q1[slot] = s1;
+        q2[slot] = s2;
         if (zp >= zs) {                             // a, b at plane za = zp - rho
             const int za = zp - RHO;
             P av = 0.0f, bv = 0.0f;
@@ -393,20 +429,20 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
                 P t1 = 0.0f, t2 = 0.0f;
 #pragma unroll
                 for (int j = 0; j < M::Q; j++) { t1 += q1[j]; t2 += q2[j]; }
-                P rc = rcxy * count_recip<P>(za, g.nz, RHO);
+                P rc = rcxy * count_recip_tab<P>(za, g.nz, RHO);
                 P m = t1 * rc, q = t2 * rc;
                 P var = q - m * m;
                 if (var < 0.0f) var = 0.0f;
-                av = var / (var + epsf);
+                av = ratio(var, epsf);
                 bv = (m + c0) * (1.0f - av);
             }
             sa[ay][ax] = av;
             sb[ay][ax] = bv;
         }
         __syncthreads();
-        if (zp >= zs) {                             // window sums of a, b; output plane zp - 2 rho
+        {                                           // window sums of a, b; output plane zp - 2 rho
             P u1 = 0.0f, u2 = 0.0f;
-            if (inner) {
+            if (inner && zp >= zs) {
 #pragma unroll
                 for (int dy = -RHO; dy <= RHO; dy++) {
                     P r1 = 0.0f, r2 = 0.0f;
@@ -419,16 +455,14 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
                     u2 += r2;
                 }
             }
-#pragma unroll
-            for (int j = 0; j < M::Q - 1; j++) { qa[j] = qa[j + 1]; qb[j] = qb[j + 1]; }
-            qa[M::Q - 1] = u1;
-            qb[M::Q - 1] = u2;
+            qa[slot] = u1;
+            qb[slot] = u2;
             const int zo = zp - 2 * RHO;
             if (inner && zo >= zs && zo < ze) {
                 P t1 = 0.0f, t2 = 0.0f;
 #pragma unroll
                 for (int j = 0; j < M::Q; j++) { t1 += qa[j]; t2 += qb[j]; }
-                P rc = rcxy * count_recip<P>(zo, g.nz, RHO);
+                P rc = rcxy * count_recip_tab<P>(zo, g.nz, RHO);
                 size_t o = ((size_t)zo * g.ny + gy) * g.nx + gx;
                 p.ma[o] = t1 * rc;
                 p.mb[o] = t2 * rc;
@@ -437,16 +471,16 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
     };
     // sample planes PD-1 ahead are in flight while a plane is processed; the
     // buffer ring rotates by full unrolling, so no register copy waits on a load
-    constexpr int PD = M::PD;
+    constexpr int PD = M::PD, U = M::U;
     const int zfirst = zs - 2 * RHO, zlast = ze - 1 + 2 * RHO;
     P buf[PD][M::KS];
 #pragma unroll
     for (int d = 0; d < PD - 1; d++) fetch(zfirst + d, buf[d]);
-    for (int z0 = zfirst; z0 <= zlast; z0 += PD) {
+    for (int z0 = zfirst; z0 <= zlast; z0 += U) {
 #pragma unroll
-        for (int u = 0; u < PD; u++) {
+        for (int u = 0; u < U; u++) {
             fetch(z0 + u + PD - 1, buf[(u + PD - 1) % PD]);
-            step(z0 + u, buf[u]);
+            step(z0 + u, buf[u % PD], u);
         }
     }
 }
@@ -521,7 +555,9 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
     const int kb = U.i0(xc0, p.lx), nk = U.i0(xc1, p.lx) - kb + 2;
     const int zlo = U.i0(Z0, p.lz), nzp = U.i0(zl, p.lz) - zlo + 2;
     const int ylo = U.i0(Y0, p.ly), nyp = U.i0(yl, p.ly) - ylo + 2;
-    const int q = threadIdx.x & 63, g = threadIdx.x >> 6;
+    // the wave index is uniform across a wave: row and patch-row arithmetic below
+    // runs on the scalar unit, only column offsets are per lane
+    const int q = threadIdx.x & 63, g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int xb = xc0 + q * UPX;
     const bool xin = xb <= xc1, full = xb + UPX - 1 <= xc1;
 
@@ -530,8 +566,9 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
         int r = g + 4 * j, z = Z0 + r / URY, y = Y0 + r % URY;
-        bool ok = xin && z <= zl && y <= yl;
-        const P *irow = p.in + ((size_t)(ok ? z : Z0) * p.ny + (ok ? y : Y0)) * p.nx;
+        const bool rok = z <= zl && y <= yl;
+        bool ok = xin && rok;
+        const P *irow = p.in + ((size_t)(rok ? z : Z0) * p.ny + (rok ? y : Y0)) * p.nx;
         if (ok && full && p.vin) {
             if constexpr (sizeof(P) == sizeof(float)) {
                 float4 t = *(const float4 *)(irow + xb);
@@ -552,19 +589,20 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
         P va[RW][CW], vb[RW][CW];
 #pragma unroll
         for (int i = 0; i < RW; i++) {
-            int rr = g + 4 * i;
+            const int rr = g + 4 * i;
             int pz = 0;
 #pragma unroll
             for (int k = 1; k < UPZ; k++) pz += rr >= k * nyp;
-            int py = rr - pz * nyp;
-            size_t ro = ((size_t)(zlo + pz) * p.ly + (ylo + py)) * p.lx + kb;
+            const int py = rr - pz * nyp;
+            const bool rok = rr < nrow;
+            const size_t ro = rok ? ((size_t)(zlo + pz) * p.ly + (ylo + py)) * p.lx + kb : (size_t)zlo * p.ly * p.lx;
+            const P *ra = p.ma + ro, *rb = p.mb + ro;
 #pragma unroll
             for (int c = 0; c < CW; c++) {
                 int col = q + 64 * c;
-                bool ok = rr < nrow && col < nk;
-                size_t o = ok ? ro + col : (size_t)zlo * p.ly * p.lx;
-                va[i][c] = p.ma[o];
-                vb[i][c] = p.mb[o];
+                int cc = (rok && col < nk) ? col : 0;
+                va[i][c] = ra[cc];
+                vb[i][c] = rb[cc];
             }
         }
 #pragma unroll
