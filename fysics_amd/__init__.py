@@ -41,6 +41,11 @@ class NoiseModel(C.Structure):
                 ("ref_intensity", C.c_double), ("n_bins_used", C.c_int)]
 
 
+class HistState(C.Structure):
+    """fy_hist_state (fysics.h:54)."""
+    _fields_ = [("hist", C.c_long * 256), ("total", C.c_long)]
+
+
 class Dering(C.Structure):
     """fy_dering (fysics.h:406-415)."""
     _fields_ = [("Z", C.c_long), ("Y", C.c_long), ("X", C.c_long),
@@ -142,6 +147,18 @@ SIGNATURES = {
                                     C.c_double, C.POINTER(NoiseModel)]),
     "fy_noise_aniso": (C.c_int, [c_float_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p]),
     "fy_zdrift_apply": (None, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int, c_float_p]),
+    "fy_zdrift_accumulate": (None, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int, c_double_p,
+                                    C.POINTER(C.c_long), C.c_float]),
+    "fy_zdrift_finalize": (None, [c_double_p, C.POINTER(C.c_long), C.c_int, c_float_p]),
+    "fy_correct_zdrift": (C.c_int, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_float]),
+    "fy_hist_init": (None, [C.POINTER(HistState)]),
+    "fy_hist_accumulate_u8": (None, [C.POINTER(HistState), c_u8_p, C.c_size_t]),
+    "fy_hist_merge": (None, [C.POINTER(HistState), C.POINTER(HistState)]),
+    "fy_hist_percentile_u8": (C.c_int, [C.POINTER(HistState), C.c_double]),
+    "fy_hist_percentile_u8_inner": (C.c_int, [C.POINTER(HistState), C.c_double]),
+    "fy_norm_finalize": (None, [C.POINTER(HistState), C.c_double, C.c_double, c_u8_p, c_u8_p]),
+    "fy_norm_apply_u8": (None, [c_u8_p, c_float_p, C.c_size_t, C.c_ubyte, C.c_ubyte]),
+    "fy_flat_noise": (C.c_double, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "fy_dering_init": (C.c_int, [C.POINTER(Dering), _L, _L, _L, C.c_double, C.c_double, C.c_int,
                                  C.c_int]),
     "fy_dering_free": (None, [C.POINTER(Dering)]),
@@ -174,6 +191,12 @@ SIGNATURES = {
     "fy_serve_destroy": (None, [_VP]),
     "fy_cube_params_from_cfg": (C.c_int, [C.POINTER(PipelineCfg), _L, _L, _L, _L, _L, _L,
                                           C.POINTER(CubeParams)]),
+    "fy_calib_create": (_VP, [_L, _VP]),
+    "fy_calib_accumulate_u8": (C.c_int, [_VP, _VP, _L, _L, _L, _L, C.c_int]),
+    "fy_calib_read": (C.c_int, [_VP, C.POINTER(HistState), c_double_p, C.POINTER(C.c_long)]),
+    "fy_calib_reset": (C.c_int, [_VP]),
+    "fy_calib_destroy": (None, [_VP]),
+    "fy_gpu_flat_noise": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, c_double_p, _VP]),
     "fy_gpu_sync": (C.c_int, [_VP]),
     "fy_gpu_profile": (None, [C.c_int]),
     "fy_gpu_profile_read": (C.c_int, [C.c_char_p, C.c_int, c_double_p, C.POINTER(C.c_long),

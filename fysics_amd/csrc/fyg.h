@@ -186,6 +186,24 @@ int fyg_dering_accum_run(const unsigned char *buf, long dz, long dy, long dx, in
                          const int *map, int nbins, unsigned long long *sum,
                          unsigned int *cnt, fyg_stream *s);
 
+/* ------------------------------------------------------------------ pass-1 calibration */
+/* hist[b] += count of byte value b in d[0..n) (any alignment) */
+int fyg_hist_u8_run(const unsigned char *d, size_t n, unsigned long long *hist, fyg_stream *s);
+/* acc[0..255] += tile[0..255], acc[256] += n; skip_zero: not when tile[0] == n */
+int fyg_hist_merge_run(const unsigned long long *tile, unsigned long long n, unsigned long long *acc,
+                       int skip_zero, fyg_stream *s);
+/* per slice z < dz: sums[z] += sum of u8/255 > 0.10 over (y, x) multiples of ss, cnts[z] += count */
+int fyg_zdrift_u8_run(const unsigned char *d, long dz, long dy, long dx, int ss, double *sums,
+                      long long *cnts, fyg_stream *s);
+/* per slice z < nz: sums[z] += sum of values > thresh (double), cnts[z] += count */
+int fyg_zdrift_f32_run(const float *d, int nz, int ny, int nx, float thresh, double *sums,
+                       long long *cnts, fyg_stream *s);
+/* std of each blk^3 block with mean > 0.1, -1 for the others (blocks x fastest) */
+int fyg_flat_blocks_run(const float *v, int nz, int ny, int nx, int blk, double *std_out, fyg_stream *s);
+/* out = clamp((in - lo) / (hi - lo), 0, 1) in fp32 (1/(hi-lo) -> 1 when hi <= lo) */
+int fyg_norm_u8_run(const unsigned char *in, float *out, size_t n, unsigned char lo, unsigned char hi,
+                    fyg_stream *s);
+
 /* ------------------------------------------------------------------ noise model */
 /* local (2r+1)^3 box mean m and variance v (double) with sliding sums per line */
 int fyg_noise_local(const float *d_in, int nz, int ny, int nx, int r, double wc, double *m, double *v,

@@ -9,10 +9,10 @@
  * and served forms live beside these in fysics_gpu.h.
  *
  * Only the hot-path subset is provided (FFT, transfers, Wiener deconvolution,
- * guided filter, u8/physical conversions, ring removal, noise fit and the
- * per-tile chain).  Registration, zarr I/O, z-drift estimation, MUSICA,
- * downscale and the whole-volume calibrate/run drivers are not part of this
- * library.
+ * guided filter, u8/physical conversions, ring removal, noise fit, the pass-1
+ * calibration statistics -- histogram, z-drift sums, flat-block noise floor --
+ * and the per-tile chain).  Registration, zarr I/O, MUSICA, downscale and the
+ * whole-volume calibrate/run drivers are not part of this library.
  *
  * Layout: row-major, x fastest, idx = (z*ny + y)*nx + x (fysics.h:120).
  */
@@ -56,6 +56,25 @@ void  fy_u8_to_phys(const unsigned char *in, float *out, size_t n,
 void  fy_phys_to_u8(const float *in, unsigned char *out, size_t n,
                     double f32_min, double f32_max);
 float fy_phys_to_u8_level(double phys, double f32_min, double f32_max);
+
+/* ---- pass-1 calibration statistics (fysics.h:44-95; stream.c:65-200) ----
+ * Global 256-bin histogram of u8 values: accumulated per chunk on the GPU,
+ * merged and read on the host.  Percentiles: the first bin whose running count
+ * reaches (long)(pct/100*total); the _inner form counts bins 1..254 only. */
+typedef struct { long hist[256]; long total; } fy_hist_state;       /* fysics.h:54 */
+void fy_hist_init(fy_hist_state *s);
+void fy_hist_accumulate_u8(fy_hist_state *s, const unsigned char *chunk, size_t n);
+void fy_hist_merge(fy_hist_state *dst, const fy_hist_state *src);
+int  fy_hist_percentile_u8(const fy_hist_state *s, double pct);
+int  fy_hist_percentile_u8_inner(const fy_hist_state *s, double pct);
+void fy_norm_finalize(const fy_hist_state *s, double lo_pct, double hi_pct,
+                      unsigned char *lo_out, unsigned char *hi_out);
+/* out = clamp((u8 - lo) / (hi - lo), 0, 1) (1/(hi-lo) -> 1 when hi <= lo) */
+void fy_norm_apply_u8(const unsigned char *in, float *out, size_t n,
+                      unsigned char lo, unsigned char hi);
+/* noise floor: 10th percentile of the stds of the non-overlapping blk^3 blocks
+ * whose mean exceeds 0.1 (blk < 4 -> 8); 0 when there is no such block */
+double fy_flat_noise(const float *v, int nz, int ny, int nx, int blk);
 
 /* ---- transfer functions, f in cycles/voxel (fysics.h:98-117; paganin.c:17-91) ---- */
 double fy_paganin_transfer(double f_cyc_per_voxel, const fy_physics *p);
@@ -107,8 +126,17 @@ int fy_estimate_noise(const float *in, int nz, int ny, int nx,
 int fy_noise_aniso(const float *v, int nz, int ny, int nx,
                    double *rho_z, double *rho_xy);
 
-/* ---- z-drift apply (fysics.h:227; zdrift.c:64-71), used inside the tile chain ---- */
+/* ---- z-drift (fysics.h:216-228; zdrift.c) ----
+ * accumulate: per slice z0+z, the sum (double) and count of values above
+ * papyrus_thresh; finalize: factor[z] = global mean / (mean over the slices within
+ * nz/20+1 that have papyrus); apply: chunk *= factor[z0+z] (used inside the tile
+ * chain); correct: the three over a whole volume.  Accumulate and apply run on the
+ * GPU, finalize (one scalar per slice) on the host. */
+void fy_zdrift_accumulate(const float *chunk, int nz_slab, int ny, int nx,
+                          int z0, double *sums, long *counts, float papyrus_thresh);
+void fy_zdrift_finalize(const double *sums, const long *counts, int nz, float *factor);
 void fy_zdrift_apply(float *chunk, int nz_slab, int ny, int nx, int z0, const float *factor);
+int  fy_correct_zdrift(float *vol, int nz, int ny, int nx, float papyrus_thresh);
 
 /* ---- ring removal (fysics.h:406-433; dering.c) ---- */
 typedef struct {

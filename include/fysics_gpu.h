@@ -114,6 +114,31 @@ void fy_serve_destroy(fy_server *sv);
 int fy_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long ty, long tx,
                             long vol_y, long vol_x, fy_cube_params *out);
 
+/* ---- pass-1 calibration sweep on device tiles (pipeline.c:576-701).
+ * A sweep holds device accumulators for the volume's u8 histogram and the
+ * per-slice z-drift sums of Z slices.  fy_calib_accumulate_u8 queues the
+ * statistics of one contiguous u8 ptile (dz*dy*dx, device memory, first plane
+ * at global z0) on the sweep's stream:
+ *   FY_CALIB_HIST    the tile's histogram is merged unless the tile is all zero
+ *                    (pipeline.c:677-679, fy_hist_accumulate_u8 stream.c:76-86)
+ *   FY_CALIB_ZDRIFT  per slice, the sum and count of v = u8/255 > 0.10 over the
+ *                    4x4 (y, x) subsample (pipeline.c:686-701)
+ * Sums are exact (integers, and multiples of 2^-27 below 2^26 per slice), so
+ * they do not depend on the order of tiles or of the voxels within a tile.
+ * fy_calib_read waits for the stream and adds the totals to hist / zsums /
+ * zcnts (any may be NULL); fy_calib_reset clears the accumulators.
+ * No reference counterpart: the reference accumulates on host threads. */
+enum { FY_CALIB_HIST = 1, FY_CALIB_ZDRIFT = 2 };
+typedef struct fy_calib fy_calib;
+fy_calib *fy_calib_create(long Z, void *stream);
+int  fy_calib_accumulate_u8(fy_calib *c, const unsigned char *d_tile, long z0,
+                            long dz, long dy, long dx, int flags);
+int  fy_calib_read(fy_calib *c, fy_hist_state *hist, double *zsums, long *zcnts);
+int  fy_calib_reset(fy_calib *c);
+void fy_calib_destroy(fy_calib *c);
+/* fy_flat_noise of a device volume (stream.c:174-200) */
+int  fy_gpu_flat_noise(const float *d_v, int nz, int ny, int nx, int blk, double *out, void *stream);
+
 int fy_gpu_sync(void *stream);
 
 /* per-kernel timing: HIP events recorded around every launch on its own stream */

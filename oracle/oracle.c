@@ -547,6 +547,125 @@ int orc_noise_aniso(const float *v, int nz, int ny, int nx, double *rho_z, doubl
     return 0;
 }
 
+/* ------------------------------------------------------------------ pass-1 calibration */
+void orc_hist_accumulate_u8(fy_hist_state *s, const unsigned char *chunk, size_t n) {
+    for (size_t i = 0; i < n; i++) s->hist[chunk[i]]++;           /* stream.c:76-86 */
+    s->total += (long)n;
+}
+int orc_hist_percentile_u8(const fy_hist_state *s, double pct) {  /* stream.c:96-101 */
+    long target = (long)(pct / 100.0 * (double)s->total), acc = 0;
+    for (int i = 0; i < 256; i++) {
+        acc += s->hist[i];
+        if (acc >= target) return i;
+    }
+    return 255;
+}
+int orc_hist_percentile_u8_inner(const fy_hist_state *s, double pct) {  /* stream.c:105-112 */
+    long total = 0, acc = 0;
+    for (int i = 1; i <= 254; i++) total += s->hist[i];
+    if (total <= 0) return 0;
+    long target = (long)(pct / 100.0 * (double)total);
+    for (int i = 1; i <= 254; i++) {
+        acc += s->hist[i];
+        if (acc >= target) return i;
+    }
+    return 254;
+}
+void orc_norm_apply_u8(const unsigned char *in, float *out, size_t n,
+                       unsigned char lo, unsigned char hi) {      /* stream.c:122-129 */
+    float flo = lo, inv = hi > lo ? 1.0f / (float)(hi - lo) : 1.0f;
+    for (size_t i = 0; i < n; i++) {
+        float v = ((float)in[i] - flo) * inv;
+        out[i] = v < 0 ? 0 : (v > 1 ? 1 : v);
+    }
+}
+void orc_zdrift_accumulate(const float *chunk, int nz, int ny, int nx, int z0,
+                           double *sums, long *counts, float thresh) {  /* zdrift.c:23-35 */
+    size_t plane = (size_t)ny * nx;
+    for (int z = 0; z < nz; z++) {
+        double sm = 0;
+        long c = 0;
+        for (size_t i = 0; i < plane; i++) {
+            float v = chunk[(size_t)z * plane + i];
+            if (v > thresh) { sm += v; c++; }
+        }
+        sums[z0 + z] += sm;
+        counts[z0 + z] += c;
+    }
+}
+void orc_zdrift_finalize(const double *sums, const long *counts, int nz, float *factor) {
+    double *mean = (double *)malloc(sizeof(double) * (nz > 0 ? nz : 1));  /* zdrift.c:41-62 */
+    double gsum = 0;
+    long gcount = 0;
+    for (int z = 0; z < nz; z++) {
+        mean[z] = counts[z] ? sums[z] / counts[z] : 0.0;
+        gsum += sums[z];
+        gcount += counts[z];
+    }
+    double gmean = gcount ? gsum / gcount : 1.0;
+    int win = nz / 20 + 1;
+    for (int z = 0; z < nz; z++) {
+        double sm = 0;
+        int c = 0;
+        for (int d = -win; d <= win; d++) {
+            int zz = z + d;
+            if (zz >= 0 && zz < nz && mean[zz] > 0) { sm += mean[zz]; c++; }
+        }
+        double m = c ? sm / c : gmean;
+        factor[z] = m > 1e-6 ? (float)(gmean / m) : 1.0f;
+    }
+    free(mean);
+}
+/* the pass-1 z-drift sums of one u8 ptile: 4x4 (y, x) subsample, v = u8/255 in
+ * fp32, papyrus where v > 0.10f (pipeline.c:686-701) */
+void orc_zdrift_u8_sub(const unsigned char *pb, long dz, long dy, long dx, long z0,
+                       double *sums, long *counts) {
+    const int ss = 4;
+    for (long z = 0; z < dz; z++) {
+        const unsigned char *sl = pb + (size_t)z * dy * dx;
+        double sm = 0;
+        long c = 0;
+        for (long y = 0; y < dy; y += ss)
+            for (long x = 0; x < dx; x += ss) {
+                float v = sl[(size_t)y * dx + x] * (1.0f / 255.0f);
+                if (v > 0.10f) { sm += v; c++; }
+            }
+        sums[z0 + z] += sm;
+        counts[z0 + z] += c;
+    }
+}
+double orc_flat_noise(const float *v, int nz, int ny, int nx, int blk) {  /* stream.c:174-200 */
+    if (blk < 4) blk = 8;
+    long nb = (long)(nz / blk) * (ny / blk) * (nx / blk);
+    if (nb <= 0) return 0;
+    double *stds = (double *)malloc(sizeof(double) * nb);
+    long m = 0;
+    for (int z = 0; z + blk <= nz; z += blk)
+        for (int y = 0; y + blk <= ny; y += blk)
+            for (int x = 0; x + blk <= nx; x += blk) {
+                double sm = 0, s2 = 0;
+                int c = 0;
+                for (int dz = 0; dz < blk; dz++)
+                    for (int dy = 0; dy < blk; dy++)
+                        for (int dx = 0; dx < blk; dx++) {
+                            float val = v[((size_t)(z + dz) * ny + (y + dy)) * nx + (x + dx)];
+                            float sq = val * val;                 /* fp32 square */
+                            sm += val;
+                            s2 += sq;
+                            c++;
+                        }
+                double mean = sm / c, var = s2 / c - mean * mean;
+                if (mean > 0.1) stds[m++] = sqrt(var > 0 ? var : 0);
+            }
+    double nf = 0;
+    if (m > 0) {
+        qsort(stds, m, sizeof(double), orc_cmpd);
+        nf = stds[m / 10];
+    }
+    free(stds);
+    return nf;
+}
+
 /* ------------------------------------------------------------------ chain pieces */
 void orc_zdrift_apply(float *chunk, int nz, int ny, int nx, int z0, const float *factor) {
     size_t plane = (size_t)ny * nx;                              /* zdrift.c:64-71 */

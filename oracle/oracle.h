@@ -95,6 +95,19 @@ int    orc_estimate_noise(const float *in, int nz, int ny, int nx, int win,
 double orc_guided_eps_for_noise(double noise_ref);
 int    orc_noise_aniso(const float *v, int nz, int ny, int nx, double *rho_z, double *rho_xy);
 
+/* stream.c histogram, normalisation, flat-block noise; zdrift.c accumulate and
+ * finalize; the pass-1 u8 z-drift subsample sums of pipeline.c */
+void   orc_hist_accumulate_u8(fy_hist_state *s, const unsigned char *chunk, size_t n);
+int    orc_hist_percentile_u8(const fy_hist_state *s, double pct);
+int    orc_hist_percentile_u8_inner(const fy_hist_state *s, double pct);
+void   orc_norm_apply_u8(const unsigned char *in, float *out, size_t n, unsigned char lo, unsigned char hi);
+void   orc_zdrift_accumulate(const float *chunk, int nz, int ny, int nx, int z0,
+                             double *sums, long *counts, float thresh);
+void   orc_zdrift_finalize(const double *sums, const long *counts, int nz, float *factor);
+void   orc_zdrift_u8_sub(const unsigned char *pb, long dz, long dy, long dx, long z0,
+                         double *sums, long *counts);
+double orc_flat_noise(const float *v, int nz, int ny, int nx, int blk);
+
 /* zdrift.c apply; pipeline.c per-tile eps and the tile chain */
 void   orc_zdrift_apply(float *chunk, int nz, int ny, int nx, int z0, const float *factor);
 double orc_eps_for_tile(const fy_pipeline_cfg *cfg, double ecy, double ecx,

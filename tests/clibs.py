@@ -34,6 +34,11 @@ class NoiseModel(C.Structure):
                 ("ref_intensity", C.c_double), ("n_bins_used", C.c_int)]
 
 
+class HistState(C.Structure):
+    """fy_hist_state (fysics.h:54)."""
+    _fields_ = [("hist", C.c_long * 256), ("total", C.c_long)]
+
+
 class Dering(C.Structure):
     """fy_dering (fysics.h:406-415)."""
     _fields_ = [("Z", C.c_long), ("Y", C.c_long), ("X", C.c_long),
@@ -99,6 +104,10 @@ def up(a):
     return a.ctypes.data_as(c_u8_p)
 
 
+def lp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_long))
+
+
 # signature table shared by the fy_ (reference) and orc_ (oracle) names
 _SIGS = {
     "next_fft_size": (C.c_int, [C.c_int]),
@@ -135,6 +144,14 @@ _SIGS = {
     "guided_eps_for_noise": (C.c_double, [C.c_double]),
     "noise_aniso": (C.c_int, [c_float_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p]),
     "zdrift_apply": (None, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int, c_float_p]),
+    "hist_accumulate_u8": (None, [C.POINTER(HistState), c_u8_p, C.c_size_t]),
+    "hist_percentile_u8": (C.c_int, [C.POINTER(HistState), C.c_double]),
+    "hist_percentile_u8_inner": (C.c_int, [C.POINTER(HistState), C.c_double]),
+    "norm_apply_u8": (None, [c_u8_p, c_float_p, C.c_size_t, C.c_ubyte, C.c_ubyte]),
+    "zdrift_accumulate": (None, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int, c_double_p,
+                                 C.POINTER(C.c_long), C.c_float]),
+    "zdrift_finalize": (None, [c_double_p, C.POINTER(C.c_long), C.c_int, c_float_p]),
+    "flat_noise": (C.c_double, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "process_buffer": (C.c_int, [C.POINTER(PipelineCfg), c_u8_p] + [C.c_long] * 14 +
                        [c_u8_p, c_int_p]),
 }
@@ -170,6 +187,8 @@ _ORC_ONLY = {
                                       C.c_long, C.c_long, C.c_long, C.c_long]),
     "orc_chain_cube": (C.c_int, [C.POINTER(CubeParams), c_u8_p] + [C.c_long] * 12 +
                        [c_u8_p, c_int_p, c_float_p]),
+    "orc_zdrift_u8_sub": (None, [c_u8_p, C.c_long, C.c_long, C.c_long, C.c_long, c_double_p,
+                                 C.POINTER(C.c_long)]),
     "orc_process_buffer_f": (C.c_int, [C.POINTER(PipelineCfg), c_u8_p] + [C.c_long] * 14 +
                              [c_u8_p, c_int_p, c_float_p]),
 }
@@ -188,6 +207,8 @@ _REF_ONLY = {
     "ref_eps_for_tile": (C.c_double, [C.POINTER(PipelineCfg), C.c_double, C.c_double,
                                       C.c_long, C.c_long, C.c_long, C.c_long]),
     "fy_dering_tile_reset": (None, [C.POINTER(Dering)]),
+    "fy_hist_init": (None, [C.POINTER(HistState)]),
+    "fy_correct_zdrift": (C.c_int, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_float]),
 }
 
 

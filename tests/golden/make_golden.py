@@ -150,6 +150,54 @@ def main():
     out["aniso_vol"] = bz
     out["aniso_rho"] = np.array([rz.value, rxy.value])
 
+    # ---- pass-1 calibration statistics (stream.c:70-200, zdrift.c:23-62, pipeline.c:686-701)
+    crng = np.random.default_rng(11)
+    hu8 = np.clip(crng.normal(120, 50, 6001), 0, 255).astype(np.uint8)
+    hu8[:700] = 0
+    hu8[700:900] = 255
+    hs = clibs.HistState()
+    ref.fy_hist_init(C.byref(hs))
+    ref.fy_hist_accumulate_u8(C.byref(hs), clibs.up(hu8), 3001)
+    ref.fy_hist_accumulate_u8(C.byref(hs), clibs.up(hu8[3001:]), hu8.size - 3001)
+    pcts = np.array([0.0, 0.5, 1.0, 10.0, 50.0, 90.0, 99.5, 100.0])
+    out["hist_u8"] = hu8
+    out["hist_counts"] = np.array(hs.hist[:], np.int64)
+    out["hist_total"] = np.array([hs.total], np.int64)
+    out["hist_pcts"] = pcts
+    out["hist_pct"] = np.array([ref.fy_hist_percentile_u8(C.byref(hs), q) for q in pcts])
+    out["hist_pct_inner"] = np.array([ref.fy_hist_percentile_u8_inner(C.byref(hs), q) for q in pcts])
+    nin = crng.integers(0, 256, 777).astype(np.uint8)
+    norms = []
+    for lo, hi in ((10, 200), (0, 255), (90, 90), (200, 10)):
+        o = np.empty(nin.size, np.float32)
+        ref.fy_norm_apply_u8(clibs.up(nin), clibs.fp(o), nin.size, lo, hi)
+        norms.append(o)
+    out["norm_in"], out["norm_out"] = nin, np.stack(norms)
+    zch = crng.random((8, 20, 24)).astype(np.float32) * np.linspace(0.5, 1.0, 8, dtype=np.float32)[:, None, None]
+    zsum = np.linspace(0.0, 5.0, 12)
+    zcnt = np.arange(12, dtype=np.int64) * 3
+    ref.fy_zdrift_accumulate(clibs.fp(zch), 8, 20, 24, 3, clibs.dp(zsum), clibs.lp(zcnt), 0.3)
+    zfac = np.empty(12, np.float32)
+    ref.fy_zdrift_finalize(clibs.dp(zsum), clibs.lp(zcnt), 12, clibs.fp(zfac))
+    out["zdrift_chunk"], out["zdrift_sums"], out["zdrift_counts"], out["zdrift_factor"] = zch, zsum, zcnt, zfac
+    # the pass-1 u8 sums of pipeline.c:686-701 are fy_zdrift_accumulate of the 4x4
+    # subsample of v = u8 * (1.0f/255) at threshold 0.10f
+    zu8 = crng.integers(0, 256, (6, 19, 23)).astype(np.uint8)
+    zu8[2] = 0
+    sub = np.ascontiguousarray(zu8[:, ::4, ::4]).astype(np.float32) * np.float32(1.0 / 255.0)
+    usum = np.zeros(9)
+    ucnt = np.zeros(9, np.int64)
+    ref.fy_zdrift_accumulate(clibs.fp(sub), 6, sub.shape[1], sub.shape[2], 2, clibs.dp(usum), clibs.lp(ucnt),
+                             np.float32(0.10))
+    out["zdrift_u8"], out["zdrift_u8_sums"], out["zdrift_u8_counts"] = zu8, usum, ucnt
+    fu = np.clip(crng.normal(40, 25, (40, 40, 48)), 0, 255).astype(np.uint8)
+    fu[:, :20] = np.clip(crng.normal(150, 6, (40, 20, 48)), 0, 255).astype(np.uint8)
+    fv = fu.astype(np.float32) * np.float32(1.0 / 255.0)
+    fg = (0.3 + 0.1 * crng.standard_normal((33, 34, 35))).astype(np.float32)
+    out["flat_u8vol"], out["flat_gvol"] = fv, fg
+    out["flat_noise"] = np.array([ref.fy_flat_noise(clibs.fp(fv), 40, 40, 48, b) for b in (8, 5, 3, 16, 41)] +
+                                 [ref.fy_flat_noise(clibs.fp(fg), 33, 34, 35, b) for b in (8, 4)])
+
     # ---- per-tile chain (pipeline.c:467-510)
     sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
     from fysics_amd import phantom

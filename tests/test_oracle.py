@@ -440,3 +440,132 @@ def test_deconv_guided_vs_live_reference(orc, ref, shape):
     orc.orc_guided_fast_f32(clibs.fp(x), clibs.fp(a), *shape, 2, 1e-3, 3)
     ref.fy_guided_denoise_fast_ws(clibs.fp(x), clibs.fp(b), *shape, 2, 1e-3, 3, clibs.fp(ws))
     assert rel_l2(a, b) < 1e-6
+
+
+# ------------------------------------------------------------------ pass-1 calibration
+def _hist_of(orc, data, chunks):
+    hs = clibs.HistState()
+    for a, b in chunks:
+        part = np.ascontiguousarray(data[a:b])
+        orc.orc_hist_accumulate_u8(C.byref(hs), clibs.up(part), part.size)
+    return hs
+
+
+def test_hist_golden_and_bincount(orc):
+    u8 = GOLD["hist_u8"]
+    hs = _hist_of(orc, u8, [(0, 1000), (1000, 1000), (1000, u8.size)])
+    counts = np.array(hs.hist[:], np.int64)
+    assert np.array_equal(counts, GOLD["hist_counts"])
+    assert np.array_equal(counts, np.bincount(u8, minlength=256))
+    assert hs.total == GOLD["hist_total"][0] == u8.size
+    for q, want, want_in in zip(GOLD["hist_pcts"], GOLD["hist_pct"], GOLD["hist_pct_inner"]):
+        assert orc.orc_hist_percentile_u8(C.byref(hs), q) == want
+        assert orc.orc_hist_percentile_u8_inner(C.byref(hs), q) == want_in
+        # first bin whose running count reaches (long)(q/100 * total)
+        cs = np.cumsum(counts)
+        assert want == min(int(np.searchsorted(cs, int(q / 100.0 * u8.size))), 255)
+        inner = counts[1:255]
+        tgt = int(q / 100.0 * inner.sum())
+        assert want_in == min(1 + int(np.searchsorted(np.cumsum(inner), tgt)), 254)
+
+
+def test_hist_empty_and_inner_empty(orc):
+    hs = clibs.HistState()
+    assert orc.orc_hist_percentile_u8(C.byref(hs), 50.0) == 0
+    assert orc.orc_hist_percentile_u8_inner(C.byref(hs), 50.0) == 0
+    z = np.zeros(10, np.uint8)
+    orc.orc_hist_accumulate_u8(C.byref(hs), clibs.up(z), 10)
+    assert orc.orc_hist_percentile_u8_inner(C.byref(hs), 50.0) == 0      # bins 1..254 empty
+
+
+def test_norm_apply_golden(orc):
+    nin = GOLD["norm_in"]
+    for k, (lo, hi) in enumerate(((10, 200), (0, 255), (90, 90), (200, 10))):
+        o = np.empty(nin.size, np.float32)
+        orc.orc_norm_apply_u8(clibs.up(nin), clibs.fp(o), nin.size, lo, hi)
+        assert np.array_equal(o, GOLD["norm_out"][k])
+        inv = np.float32(1.0) / np.float32(hi - lo) if hi > lo else np.float32(1.0)
+        want = np.clip((nin.astype(np.float32) - np.float32(lo)) * inv, 0, 1)
+        assert np.array_equal(o, want)
+
+
+def test_zdrift_accumulate_finalize_golden(orc):
+    zch = GOLD["zdrift_chunk"]
+    zsum = np.linspace(0.0, 5.0, 12)
+    zcnt = np.arange(12, dtype=np.int64) * 3
+    orc.orc_zdrift_accumulate(clibs.fp(zch), 8, 20, 24, 3, clibs.dp(zsum), clibs.lp(zcnt), 0.3)
+    assert np.array_equal(zcnt, GOLD["zdrift_counts"])
+    assert np.allclose(zsum, GOLD["zdrift_sums"], rtol=1e-13, atol=0)
+    # independent: per-slice masked sums
+    m = zch > np.float32(0.3)
+    base = np.linspace(0.0, 5.0, 12)
+    want = base.copy()
+    want[3:11] += (zch.astype(np.float64) * m).sum(axis=(1, 2))
+    assert np.allclose(zsum, want, rtol=1e-12, atol=0)
+    fac = np.empty(12, np.float32)
+    orc.orc_zdrift_finalize(clibs.dp(GOLD["zdrift_sums"]), clibs.lp(np.ascontiguousarray(GOLD["zdrift_counts"])),
+                            12, clibs.fp(fac))
+    assert np.array_equal(fac, GOLD["zdrift_factor"])
+
+
+def test_zdrift_finalize_closed_form(orc):
+    # constant papyrus mean with one empty slice: every factor is 1; a linear
+    # ramp is flattened to the global mean by the windowed means
+    nz = 30
+    counts = np.full(nz, 10, np.int64)
+    counts[7] = 0
+    sums = 10 * 0.5 * np.ones(nz)
+    sums[7] = 0
+    fac = np.empty(nz, np.float32)
+    orc.orc_zdrift_finalize(clibs.dp(sums), clibs.lp(counts), nz, clibs.fp(fac))
+    assert np.array_equal(fac, np.ones(nz, np.float32))
+    mean = np.linspace(0.4, 0.6, nz)
+    sums = mean * 10
+    counts = np.full(nz, 10, np.int64)
+    orc.orc_zdrift_finalize(clibs.dp(sums), clibs.lp(counts), nz, clibs.fp(fac))
+    win = nz // 20 + 1
+    g = sums.sum() / counts.sum()
+    sm = np.array([mean[max(0, z - win):z + win + 1].mean() for z in range(nz)])
+    assert np.allclose(fac, (g / sm).astype(np.float32), rtol=1e-6)
+
+
+def test_zdrift_u8_sub_golden_and_exact(orc):
+    zu8 = GOLD["zdrift_u8"]
+    s = np.zeros(9)
+    c = np.zeros(9, np.int64)
+    orc.orc_zdrift_u8_sub(clibs.up(zu8), 6, 19, 23, 2, clibs.dp(s), clibs.lp(c))
+    assert np.array_equal(c, GOLD["zdrift_u8_counts"])
+    assert np.array_equal(s, GOLD["zdrift_u8_sums"])          # sums of multiples of 2^-27: exact
+    v = zu8[:, ::4, ::4].astype(np.float32) * np.float32(1.0 / 255.0)
+    m = v > np.float32(0.10)
+    assert np.array_equal(c[2:8], m.sum(axis=(1, 2)))
+    assert np.array_equal(s[2:8], (v.astype(np.float64) * m).sum(axis=(1, 2)))
+
+
+def _flat_noise_np(v, blk):
+    if blk < 4:
+        blk = 8
+    nz, ny, nx = (d // blk for d in v.shape)
+    if nz * ny * nx == 0:
+        return 0.0
+    b = v[:nz * blk, :ny * blk, :nx * blk].reshape(nz, blk, ny, blk, nx, blk)
+    s1 = b.astype(np.float64).sum(axis=(1, 3, 5))
+    s2 = (b * b).astype(np.float64).sum(axis=(1, 3, 5))       # fp32 squares
+    c = blk ** 3
+    mean = s1 / c
+    var = s2 / c - mean * mean
+    sd = np.sqrt(np.maximum(var, 0))[mean > 0.1]
+    return float(np.sort(sd)[sd.size // 10]) if sd.size else 0.0
+
+
+def test_flat_noise_golden_and_blocks(orc):
+    fv, fg = GOLD["flat_u8vol"], GOLD["flat_gvol"]
+    got = [orc.orc_flat_noise(clibs.fp(fv), 40, 40, 48, b) for b in (8, 5, 3, 16, 41)] + \
+          [orc.orc_flat_noise(clibs.fp(fg), 33, 34, 35, b) for b in (8, 4)]
+    # the reference build (-Ofast) divides by the block count through a
+    # reciprocal: it agrees to ~1e-14
+    assert np.allclose(got, GOLD["flat_noise"], rtol=1e-12, atol=0)
+    for b, g in zip((8, 5, 3, 16, 41), got[:5]):
+        assert g == _flat_noise_np(fv, b)
+    for b, g in zip((8, 4), got[5:]):
+        assert abs(g - _flat_noise_np(fg, b)) <= 1e-12 * max(g, 1e-30)
