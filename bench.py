@@ -121,6 +121,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 96-cube batched measurement")
     ap.add_argument("--no-served", action="store_true", help="skip the served-request measurement")
     ap.add_argument("--no-precision", action="store_true", help="skip the 512^3 fp32 / fp64 timing")
+    ap.add_argument("--no-calibration", action="store_true", help="skip the pass-1 statistics timing")
     ap.add_argument("--served-requests", type=int, default=2000)
     ap.add_argument("--served-rate", type=float, default=400.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -240,6 +241,7 @@ def main():
     # where continuous batching pays
     stress = None if args.no_served else served_bench(F, 4000, 12000.0, seed=8)
     precision = None if args.no_precision else precision_bench(F)
+    calib = None if args.no_calibration else calibration_bench(F, vol)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -262,6 +264,7 @@ def main():
         "served": served,
         "served_overload": stress,
         "precision_512": precision,
+        "calibration": calib,
         "cpu_baseline": cpu,
         "workload_gen_s": round(gen_s, 1),
     }
@@ -450,6 +453,62 @@ def precision_bench(F, n=512, reps=5, seed=3):
         dt = (time.perf_counter() - t0) / reps
         out[name] = {"ms_per_cube": round(1000 * dt, 3), "mvox_s": round(n ** 3 / dt / 1e6, 1)}
     return out
+
+
+def calibration_bench(F, vol, reps=50):
+    """Pass-1 calibration statistics of the workload cube, all on the device:
+    the u8 histogram plus the per-slice z-drift sums of the 4x4 subsample
+    (fy_calib_accumulate_u8, one 256^3 ptile per call), and the flat-block noise
+    floor of the central 128^3 sub-block as f = u8/255 (fy_gpu_flat_noise,
+    including its read-back and the host selection)."""
+    import torch
+    lib = F.lib()
+    n = vol.shape[0]
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+    d = torch.from_numpy(vol).cuda()
+    torch.cuda.synchronize()
+    cal = lib.fy_calib_create(n, sh)
+    if not cal:
+        raise RuntimeError(lib.fy_gpu_last_error().decode())
+    flags = 3                                   # FY_CALIB_HIST | FY_CALIB_ZDRIFT
+    for _ in range(3):
+        lib.fy_calib_accumulate_u8(cal, C.c_void_p(d.data_ptr()), 0, n, n, n, flags)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        if lib.fy_calib_accumulate_u8(cal, C.c_void_p(d.data_ptr()), 0, n, n, n, flags):
+            raise RuntimeError(lib.fy_gpu_last_error().decode())
+    stream.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    lib.fy_gpu_profile_reset()
+    lib.fy_gpu_profile(1)
+    for _ in range(20):
+        lib.fy_calib_accumulate_u8(cal, C.c_void_p(d.data_ptr()), 0, n, n, n, flags)
+    stream.synchronize()
+    lib.fy_gpu_profile(0)
+    prof = F.profile_read()
+    lib.fy_calib_destroy(cal)
+    kern = {k: round(1000.0 * ms / max(c, 1), 2) for k, (ms, c) in prof.items()}
+    nbytes = vol.size                           # the tile is read once: N bytes
+    hist_us = kern.get("hist_u8")
+    h = n // 2 - 64
+    sub = (d[h:h + 128, h:h + 128, h:h + 128].float() * np.float32(1.0 / 255.0)).contiguous()
+    torch.cuda.synchronize()
+    nf = C.c_double()
+    lib.fy_gpu_flat_noise(C.c_void_p(sub.data_ptr()), 128, 128, 128, 8, C.byref(nf), sh)
+    t1 = time.perf_counter()
+    for _ in range(10):
+        lib.fy_gpu_flat_noise(C.c_void_p(sub.data_ptr()), 128, 128, 128, 8, C.byref(nf), sh)
+    fdt = (time.perf_counter() - t1) / 10
+    return {"workload": f"pass-1 statistics of the {n}^3 u8 cube: histogram + 4x4 z-drift sums per call",
+            "ms_per_tile": round(1000 * dt, 4), "gvox_s": round(vol.size / dt / 1e9, 1),
+            "kernels_us": kern,
+            "hist_roofline": {"bound": "hbm", "algorithmic_bytes": nbytes,
+                              "achieved": round(nbytes / (hist_us * 1e-6) / 1e9, 1) if hist_us else None,
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(nbytes / (hist_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if hist_us else None},
+            "flat_noise_128": {"ms_per_call": round(1000 * fdt, 3), "noise_floor": nf.value}}
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):

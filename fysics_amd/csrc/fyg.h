@@ -187,14 +187,19 @@ int fyg_dering_accum_run(const unsigned char *buf, long dz, long dy, long dx, in
                          unsigned int *cnt, fyg_stream *s);
 
 /* ------------------------------------------------------------------ pass-1 calibration */
-/* hist[b] += count of byte value b in d[0..n) (any alignment) */
-int fyg_hist_u8_run(const unsigned char *d, size_t n, unsigned long long *hist, fyg_stream *s);
-/* acc[0..255] += tile[0..255], acc[256] += n; skip_zero: not when tile[0] == n */
-int fyg_hist_merge_run(const unsigned long long *tile, unsigned long long n, unsigned long long *acc,
+/* hist[b] += count of byte value b in d[0..n) (any alignment); part is scratch
+ * of fyg_hist_part_bytes(n) bytes */
+size_t fyg_hist_part_bytes(size_t n);
+int fyg_hist_u8_run(const unsigned char *d, size_t n, unsigned int *part, unsigned long long *hist,
+                    fyg_stream *s);
+/* acc[0..255] += tile[0..255], acc[256] += n; skip_zero: not when tile[0] == n.
+ * Clears tile. */
+int fyg_hist_merge_run(unsigned long long *tile, unsigned long long n, unsigned long long *acc,
                        int skip_zero, fyg_stream *s);
-/* per slice z < dz: sums[z] += sum of u8/255 > 0.10 over (y, x) multiples of ss, cnts[z] += count */
-int fyg_zdrift_u8_run(const unsigned char *d, long dz, long dy, long dx, int ss, double *sums,
-                      long long *cnts, fyg_stream *s);
+/* per slice z < dz: zs[z] += sum of u8/255 > 0.10 over (y, x) multiples of ss in
+ * units of 2^-27, zc[z] += count (integer atomics) */
+int fyg_zdrift_u8_run(const unsigned char *d, long dz, long dy, long dx, int ss, unsigned long long *zs,
+                      unsigned long long *zc, fyg_stream *s);
 /* per slice z < nz: sums[z] += sum of values > thresh (double), cnts[z] += count */
 int fyg_zdrift_f32_run(const float *d, int nz, int ny, int nx, float thresh, double *sums,
                        long long *cnts, fyg_stream *s);

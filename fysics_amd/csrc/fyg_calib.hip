@@ -26,9 +26,11 @@ __device__ __forceinline__ void hist_word(unsigned int *h, unsigned int w, int c
 }
 
 // grid-stride over the 16-byte-aligned body (uint4 loads); the unaligned head
-// and the tail (< 32 bytes together) are taken by the first threads of the grid
+// and the tail (< 32 bytes together) are taken by the first threads of the grid.
+// Each workgroup writes its 256 counts to part[blockIdx.x]: no global atomics,
+// which would serialise on the few bins that hold most voxels.
 __global__ __launch_bounds__(NT) void k_hist_u8(const unsigned char *d, size_t n, size_t head,
-                                                 unsigned long long *hist) {
+                                                 unsigned int *part) {
     __shared__ unsigned int h[256 * HC];
     for (int i = threadIdx.x; i < 256 * HC; i += NT) h[i] = 0;
     __syncthreads();
@@ -36,7 +38,20 @@ __global__ __launch_bounds__(NT) void k_hist_u8(const unsigned char *d, size_t n
     const size_t nv = (n - head) / 16;
     const uint4 *v = (const uint4 *)(d + head);
     const size_t t0 = (size_t)blockIdx.x * NT + threadIdx.x, stride = (size_t)gridDim.x * NT;
-    for (size_t i = t0; i < nv; i += stride) {
+    size_t i = t0;
+    for (; i + 3 * stride < nv; i += 4 * stride) {       // four loads in flight
+        uint4 w[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) w[k] = v[i + k * stride];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            hist_word(h, w[k].x, c);
+            hist_word(h, w[k].y, c);
+            hist_word(h, w[k].z, c);
+            hist_word(h, w[k].w, c);
+        }
+    }
+    for (; i < nv; i += stride) {
         uint4 w = v[i];
         hist_word(h, w.x, c);
         hist_word(h, w.y, c);
@@ -55,17 +70,32 @@ __global__ __launch_bounds__(NT) void k_hist_u8(const unsigned char *d, size_t n
     unsigned int t = 0;
 #pragma unroll 8
     for (int k = 0; k < HC; k++) t += h[b * HC + ((k + b) & (HC - 1))];
-    if (t) atomicAdd(&hist[b], (unsigned long long)t);
+    part[(size_t)blockIdx.x * 256 + b] = t;
 }
 
-// acc[0..255] += tile[0..255], acc[256] += n; with skip_zero an all-zero tile
-// (tile[0] == n, including n == 0) is not merged (pipeline.c:677-679)
-__global__ void k_hist_merge(const unsigned long long *tile, unsigned long long n, unsigned long long *acc,
-                             int skip_zero) {
-    if (skip_zero && tile[0] == n) return;
+// hist[b] += sum over the rows of part: workgroup r takes rows r, r+R, ...
+__global__ __launch_bounds__(NT) void k_hist_reduce(const unsigned int *part, int rows,
+                                                     unsigned long long *hist) {
     const int b = threadIdx.x;
-    acc[b] += tile[b];
-    if (b == 0) acc[256] += n;
+    unsigned long long t = 0;
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) t += part[(size_t)r * 256 + b];
+    if (t) atomicAdd(&hist[b], t);
+}
+
+// acc[0..255] += tile[0..255], acc[256] += n, unless skip_zero and the tile is
+// all zero (tile[0] == n, including n == 0: pipeline.c:677-679); tile is
+// cleared for the next tile either way
+__global__ void k_hist_merge(unsigned long long *tile, unsigned long long n, unsigned long long *acc,
+                             int skip_zero) {
+    const int b = threadIdx.x;
+    const unsigned long long t = tile[b];
+    const bool merge = !(skip_zero && tile[0] == n);
+    __syncthreads();
+    tile[b] = 0;
+    if (merge) {
+        acc[b] += t;
+        if (b == 0) acc[256] += n;
+    }
 }
 
 template <class T>
@@ -79,16 +109,19 @@ __device__ __forceinline__ T block_sum(T v, T *red) {
     return t;
 }
 
-// one workgroup per slice: sums of v = u8*(1/255) > 0.10 over (y, x) = multiples
-// of ss, in units of 2^-27, and their count; added to sums[z], cnts[z]
+// grid (row blocks, slices): sums of v = u8*(1/255) > 0.10 over (y, x) =
+// multiples of ss, in units of 2^-27, and their count, added to zs[z], zc[z].
+// Integer atomics: exact in any order.
+constexpr int ZR = 8;                    // subsampled rows per workgroup
 __global__ __launch_bounds__(NT) void k_zdrift_u8(const unsigned char *d, long dy, long dx, int ss,
-                                                   double *sums, long long *cnts) {
+                                                   unsigned long long *zs, unsigned long long *zc) {
     __shared__ unsigned long long red[NT / 64];
-    const unsigned char *pl = d + (size_t)blockIdx.x * dy * dx;
-    const int sy = (int)((dy + ss - 1) / ss), sx = (int)((dx + ss - 1) / ss), np = sy * sx;
+    const unsigned char *pl = d + (size_t)blockIdx.y * dy * dx;
+    const int sy = (int)((dy + ss - 1) / ss), sx = (int)((dx + ss - 1) / ss);
+    const int y0 = blockIdx.x * ZR, ny = min(ZR, sy - y0), np = ny * sx;
     unsigned long long s = 0, c = 0;
     for (int i = threadIdx.x; i < np; i += NT) {
-        const int yi = i / sx, xi = i - yi * sx;
+        const int yi = y0 + i / sx, xi = i % sx;
         const float v = (float)pl[(size_t)yi * ss * dx + (size_t)xi * ss] * (1.0f / 255.0f);
         const bool ok = v > 0.10f;
         s += ok ? (unsigned long long)(v * kZUnit) : 0ull;
@@ -97,9 +130,9 @@ __global__ __launch_bounds__(NT) void k_zdrift_u8(const unsigned char *d, long d
     const unsigned long long S = block_sum(s, red);
     __syncthreads();
     const unsigned long long Cn = block_sum(c, red);
-    if (threadIdx.x == 0) {
-        sums[blockIdx.x] += (double)S * (1.0 / (double)kZUnit);
-        cnts[blockIdx.x] += (long long)Cn;
+    if (threadIdx.x == 0 && Cn) {
+        atomicAdd(&zs[blockIdx.y], S);
+        atomicAdd(&zc[blockIdx.y], Cn);
     }
 }
 
@@ -169,30 +202,41 @@ unsigned grid_for(size_t work, unsigned cap) {
 
 }  // namespace
 
-extern "C" int fyg_hist_u8_run(const unsigned char *d, size_t n, unsigned long long *hist, fyg_stream *s) {
+extern "C" size_t fyg_hist_part_bytes(size_t n) {
+    (void)n;
+    return sizeof(unsigned int) * 256 * 4096;
+}
+
+extern "C" int fyg_hist_u8_run(const unsigned char *d, size_t n, unsigned int *part, unsigned long long *hist,
+                               fyg_stream *s) {
     if (!n) return 0;
     size_t head = (16 - ((uintptr_t)d & 15)) & 15;
     if (head > n) head = n;
     const size_t nv = (n - head) / 16, extra = n - nv * 16;
-    // about 4 uint4 per thread (four 32 KB workgroups fit a CU), at most 4096
-    // workgroups; enough threads for the head and tail bytes
-    size_t want = nv / 4 > extra ? nv / 4 : extra;
-    fyg_launch(s, "hist_u8", k_hist_u8, dim3(grid_for(want, 4096)), dim3(NT), 0, d, n, head, hist);
+    // about 8 uint4 per thread (four 32 KB workgroups fit a CU), at most 4096
+    // workgroups (the rows of part); enough threads for the head and tail bytes
+    size_t want = nv / 8 > extra ? nv / 8 : extra;
+    const unsigned g = grid_for(want, 4096);
+    fyg_launch(s, "hist_u8", k_hist_u8, dim3(g), dim3(NT), 0, d, n, head, part);
+    const unsigned r = g < 16 ? g : 16;
+    fyg_launch(s, "hist_reduce", k_hist_reduce, dim3(r), dim3(NT), 0, (const unsigned int *)part, (int)g, hist);
     return fyg_last_error();
 }
 
-extern "C" int fyg_hist_merge_run(const unsigned long long *tile, unsigned long long n, unsigned long long *acc,
+extern "C" int fyg_hist_merge_run(unsigned long long *tile, unsigned long long n, unsigned long long *acc,
                                   int skip_zero, fyg_stream *s) {
     fyg_launch(s, "hist_merge", k_hist_merge, dim3(1), dim3(256), 0, tile, n, acc, skip_zero);
     return fyg_last_error();
 }
 
-extern "C" int fyg_zdrift_u8_run(const unsigned char *d, long dz, long dy, long dx, int ss, double *sums,
-                                 long long *cnts, fyg_stream *s) {
+extern "C" int fyg_zdrift_u8_run(const unsigned char *d, long dz, long dy, long dx, int ss, unsigned long long *zs,
+                                 unsigned long long *zc, fyg_stream *s) {
     if (dz <= 0 || dy <= 0 || dx <= 0) return 0;
     if (ss < 1) ss = 1;
-    if ((dy + ss - 1) / ss * ((dx + ss - 1) / ss) > 0x7fffffffL || dz > 0x7fffffffL) return 1;
-    fyg_launch(s, "zdrift_u8", k_zdrift_u8, dim3((unsigned)dz), dim3(NT), 0, d, dy, dx, ss, sums, cnts);
+    const long sy = (dy + ss - 1) / ss, sx = (dx + ss - 1) / ss;
+    if (sx * ZR > 0x7fffffffL || dz > 65535) return 1;
+    fyg_launch(s, "zdrift_u8", k_zdrift_u8, dim3((unsigned)((sy + ZR - 1) / ZR), (unsigned)dz), dim3(NT), 0, d, dy,
+               dx, ss, zs, zc);
     return fyg_last_error();
 }
 

@@ -24,11 +24,13 @@ void fy_hist_init(fy_hist_state *s) { memset(s, 0, sizeof(*s)); }      /* stream
 void fy_hist_accumulate_u8(fy_hist_state *st, const unsigned char *chunk, size_t n) {
     if (!n) return;                                                     /* stream.c:76-86 */
     fyg_stream *s = fyg_thread_stream();
-    unsigned char *d = s ? (unsigned char *)fyg_ws(s, WS_U8, n) : NULL;
-    unsigned long long *dh = s ? (unsigned long long *)fyg_ws(s, WS_MISC, 256 * sizeof(unsigned long long)) : NULL;
     unsigned long long hh[256];
-    if (!d || !dh) return;
-    if (fyg_h2d(d, chunk, n, s) || fyg_zero(dh, sizeof(hh), s) || fyg_hist_u8_run(d, n, dh, s) ||
+    unsigned char *d = s ? (unsigned char *)fyg_ws(s, WS_U8, n) : NULL;
+    char *m = s ? (char *)fyg_ws(s, WS_MISC, al256(sizeof(hh)) + fyg_hist_part_bytes(n)) : NULL;
+    if (!d || !m) return;
+    unsigned long long *dh = (unsigned long long *)m;
+    unsigned int *part = (unsigned int *)(m + al256(sizeof(hh)));
+    if (fyg_h2d(d, chunk, n, s) || fyg_zero(dh, sizeof(hh), s) || fyg_hist_u8_run(d, n, part, dh, s) ||
         fyg_d2h(hh, dh, sizeof(hh), s) || fyg_sync(s))
         return;
     for (int b = 0; b < 256; b++) st->hist[b] += (long)hh[b];
@@ -212,9 +214,10 @@ struct fy_calib {
     void *mem;
     size_t bytes;
     unsigned long long *acc;    /* 256 bins + total */
-    unsigned long long *tile;   /* one tile's bins */
-    double *zs;                 /* Z slice sums */
-    long long *zc;              /* Z slice counts */
+    unsigned long long *tile;   /* one tile's bins (cleared by the merge) */
+    unsigned long long *zs;     /* Z slice sums, units of 2^-27 */
+    unsigned long long *zc;     /* Z slice counts */
+    unsigned int *part;         /* per-workgroup bins of the histogram kernel */
 };
 
 fy_calib *fy_calib_create(long Z, void *stream) {
@@ -223,15 +226,17 @@ fy_calib *fy_calib_create(long Z, void *stream) {
     if (!c) return NULL;
     c->Z = Z;
     c->s = fyg_stream_wrap(stream);
+    size_t zb = al256(sizeof(unsigned long long) * (size_t)(Z ? Z : 1));
     size_t o1 = al256(257 * sizeof(unsigned long long)), o2 = o1 + al256(256 * sizeof(unsigned long long));
-    size_t o3 = o2 + al256(sizeof(double) * (size_t)(Z ? Z : 1));
-    c->bytes = o3 + al256(sizeof(long long) * (size_t)(Z ? Z : 1));
+    size_t o3 = o2 + zb, o4 = o3 + zb;
+    c->bytes = o4 + fyg_hist_part_bytes(0);
     if (!c->s || fyg_malloc(&c->mem, c->bytes)) { free(c); return NULL; }
     char *m = (char *)c->mem;
     c->acc = (unsigned long long *)m;
     c->tile = (unsigned long long *)(m + o1);
-    c->zs = (double *)(m + o2);
-    c->zc = (long long *)(m + o3);
+    c->zs = (unsigned long long *)(m + o2);
+    c->zc = (unsigned long long *)(m + o3);
+    c->part = (unsigned int *)(m + o4);
     if (fyg_zero(c->mem, c->bytes, c->s)) { fy_calib_destroy(c); return NULL; }
     return c;
 }
@@ -246,8 +251,7 @@ int fy_calib_accumulate_u8(fy_calib *c, const unsigned char *d_tile, long z0, lo
     size_t n = (size_t)dz * dy * dx;
     if (n && !d_tile) return 1;
     if (flags & FY_CALIB_HIST) {
-        if (fyg_zero(c->tile, 256 * sizeof(unsigned long long), c->s) ||
-            (n && fyg_hist_u8_run(d_tile, n, c->tile, c->s)) ||
+        if ((n && fyg_hist_u8_run(d_tile, n, c->part, c->tile, c->s)) ||
             fyg_hist_merge_run(c->tile, (unsigned long long)n, c->acc, 1, c->s))
             return 1;
     }
@@ -258,22 +262,24 @@ int fy_calib_accumulate_u8(fy_calib *c, const unsigned char *d_tile, long z0, lo
     return 0;
 }
 
+/* slice sums leave the device as integers in units of 2^-27: exact in double
+ * while a slice's papyrus total stays below 2^26 */
 int fy_calib_read(fy_calib *c, fy_hist_state *hist, double *zsums, long *zcnts) {
     if (!c) return 1;
     unsigned long long acc[257];
     size_t nz = (size_t)c->Z;
-    double *hs = (double *)malloc(sizeof(double) * (nz ? nz : 1));
-    long long *hc = (long long *)malloc(sizeof(long long) * (nz ? nz : 1));
+    unsigned long long *hs = (unsigned long long *)malloc(sizeof(unsigned long long) * (nz ? nz : 1));
+    unsigned long long *hc = (unsigned long long *)malloc(sizeof(unsigned long long) * (nz ? nz : 1));
     int rc = !hs || !hc || fyg_d2h(acc, c->acc, sizeof(acc), c->s) ||
-             (nz && fyg_d2h(hs, c->zs, sizeof(double) * nz, c->s)) ||
-             (nz && fyg_d2h(hc, c->zc, sizeof(long long) * nz, c->s)) || fyg_sync(c->s);
+             (nz && fyg_d2h(hs, c->zs, sizeof(unsigned long long) * nz, c->s)) ||
+             (nz && fyg_d2h(hc, c->zc, sizeof(unsigned long long) * nz, c->s)) || fyg_sync(c->s);
     if (!rc) {
         if (hist) {
             for (int b = 0; b < 256; b++) hist->hist[b] += (long)acc[b];
             hist->total += (long)acc[256];
         }
         for (size_t z = 0; z < nz; z++) {
-            if (zsums) zsums[z] += hs[z];
+            if (zsums) zsums[z] += (double)hs[z] * (1.0 / 134217728.0);
             if (zcnts) zcnts[z] += (long)hc[z];
         }
     }
