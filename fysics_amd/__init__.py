@@ -96,7 +96,10 @@ class CubeParams(C.Structure):
         ("dec_hi", C.c_double), ("guided_eps", C.c_double), ("guided_radius", C.c_int),
         ("guided_s", C.c_int), ("no_dither", C.c_int), ("recenter", C.c_int), ("norm_lo", C.c_int),
         ("norm_hi", C.c_int), ("dering", C.POINTER(Dering)), ("zdrift_factor", c_float_p),
-        ("vol_z", C.c_long), ("fp64", C.c_int)]
+        ("vol_z", C.c_long), ("fp64", C.c_int),
+        ("air_zero", C.c_int), ("air_cut_u8", C.c_int), ("air_cut_band", C.c_int),
+        ("air_thresh", C.c_double), ("air_cut_aggr", C.c_double), ("air_min_component", C.c_int),
+        ("scratch_passes", C.c_int)]
 
 
 class Cube(C.Structure):
@@ -159,6 +162,8 @@ SIGNATURES = {
     "fy_norm_finalize": (None, [C.POINTER(HistState), C.c_double, C.c_double, c_u8_p, c_u8_p]),
     "fy_norm_apply_u8": (None, [c_u8_p, c_float_p, C.c_size_t, C.c_ubyte, C.c_ubyte]),
     "fy_flat_noise": (C.c_double, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "fy_valley_depth": (C.c_double, [C.POINTER(C.c_long), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]),
     "fy_dering_init": (C.c_int, [C.POINTER(Dering), _L, _L, _L, C.c_double, C.c_double, C.c_int,
                                  C.c_int]),
     "fy_dering_free": (None, [C.POINTER(Dering)]),

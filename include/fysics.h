@@ -72,6 +72,10 @@ void fy_norm_finalize(const fy_hist_state *s, double lo_pct, double hi_pct,
 /* out = clamp((u8 - lo) / (hi - lo), 0, 1) (1/(hi-lo) -> 1 when hi <= lo) */
 void fy_norm_apply_u8(const unsigned char *in, float *out, size_t n,
                       unsigned char lo, unsigned char hi);
+/* bimodal valley depth of a 256-bin histogram (rails 0/254/255 excluded):
+ * 1 - h_valley/min(h_dark, h_light) of the 5-bin smoothed histogram, writing the
+ * dark/light peak and valley bins (each may be NULL); -1 if not bimodal */
+double fy_valley_depth(const long hist[256], int *dark_out, int *light_out, int *valley_out);
 /* noise floor: 10th percentile of the stds of the non-overlapping blk^3 blocks
  * whose mean exceeds 0.1 (blk < 4 -> 8); 0 when there is no such block */
 double fy_flat_noise(const float *v, int nz, int ny, int nx, int blk);

@@ -77,6 +77,17 @@ typedef struct {
     const fy_dering *dering;   /* finalized ring state, or NULL */
     const float *zdrift_factor; long vol_z;  /* per-slice factors, or NULL */
     int    fp64;               /* all-FP64 variant of deconv + fast guided (needs guided_s >= 2) */
+    /* (d) air-zero (pipeline.c:302-399): the pre-deconvolution tile is box-smoothed
+     * on a 2x decimated grid, a cut is chosen from the smooth's histogram valley
+     * (fy_valley_depth) and voxels whose upsampled smooth falls below it -- a
+     * harder cut where the tile is locally uniform -- are zeroed; then 6-connected
+     * material islands that do not touch the region's faces and are smaller than
+     * air_min_component voxels are zeroed (pipeline.c:227-250) */
+    int    air_zero;
+    int    air_cut_u8, air_cut_band;   /* global cut and band; air_cut_u8 < 0: dark-peak rule */
+    double air_thresh, air_cut_aggr;   /* physics floor (<= 0 -> u8 39), valley aggressiveness */
+    int    air_min_component;          /* <= 0: no despeckle */
+    int    scratch_passes;             /* scratch box radius (<= 0 -> 5) */
 } fy_cube_params;
 
 typedef struct {

@@ -666,6 +666,186 @@ double orc_flat_noise(const float *v, int nz, int ny, int nx, int blk) {  /* str
     return nf;
 }
 
+/* ------------------------------------------------------------------ air-zero */
+double orc_valley_depth(const long hist[256], int *dark_out, int *light_out, int *valley_out) {
+    double hf[256];                                             /* stream.c:138-170 */
+    for (int i = 0; i < 256; i++) {
+        double sm = 0;
+        int cnt = 0;
+        for (int k = -2; k <= 2; k++) {
+            int j = i + k;
+            if (j >= 0 && j < 256) { sm += hist[j]; cnt++; }
+        }
+        hf[i] = sm / cnt;
+    }
+    hf[0] = hf[254] = hf[255] = 0;
+    double mx = 0;
+    for (int i = 0; i < 256; i++) if (hf[i] > mx) mx = hf[i];
+    if (mx <= 0) return -1;
+    int peaks[256], np = 0;
+    for (int u = 3; u < 253; u++)
+        if (hf[u] >= hf[u - 1] && hf[u] >= hf[u + 1] && hf[u] > 0.05 * mx) {
+            if (np && u - peaks[np - 1] <= 10) {
+                if (hf[u] > hf[peaks[np - 1]]) peaks[np - 1] = u;
+            } else {
+                peaks[np++] = u;
+            }
+        }
+    if (np < 2) return -1;
+    int p1 = peaks[0], p2 = peaks[1];
+    for (int i = 0; i < np; i++) {
+        if (hf[peaks[i]] > hf[p1]) { p2 = p1; p1 = peaks[i]; }
+        else if (hf[peaks[i]] > hf[p2] && peaks[i] != p1) p2 = peaks[i];
+    }
+    int a = p1 < p2 ? p1 : p2, b = p1 < p2 ? p2 : p1;
+    int v = a;
+    double vmin = hf[a];
+    for (int u = a; u <= b; u++) if (hf[u] < vmin) { vmin = hf[u]; v = u; }
+    if (dark_out) *dark_out = a;
+    if (light_out) *light_out = b;
+    if (valley_out) *valley_out = v;
+    double mn = hf[a] < hf[b] ? hf[a] : hf[b];
+    return 1.0 - vmin / (mn + 1e-12);
+}
+
+/* 6-connected components of f > 0: zero those that touch no face of the region
+ * and hold fewer than min_size voxels (pipeline.c:227-250) */
+void orc_air_cc_despeckle(float *f, int nz, int ny, int nx, int min_size) {
+    size_t n = (size_t)nz * ny * nx;
+    int *lab = (int *)malloc(sizeof(int) * (n ? n : 1)), *q = (int *)malloc(sizeof(int) * (n ? n : 1));
+    if (!lab || !q) { free(lab); free(q); return; }
+    for (size_t i = 0; i < n; i++) lab[i] = f[i] > 0.0f ? -1 : 0;
+    int plane = ny * nx;
+    for (size_t s0 = 0; s0 < n; s0++) {
+        if (lab[s0] != -1) continue;
+        int head = 0, tail = 0, edge = 0;
+        q[tail++] = (int)s0;
+        lab[s0] = 1;
+        while (head < tail) {
+            int p = q[head++];
+            int z = p / plane, r = p - z * plane, y = r / nx, x = r - y * nx;
+            if (z == 0 || z == nz - 1 || y == 0 || y == ny - 1 || x == 0 || x == nx - 1) edge = 1;
+            int nb[6] = {z > 0 ? p - plane : -1, z < nz - 1 ? p + plane : -1, y > 0 ? p - nx : -1,
+                         y < ny - 1 ? p + nx : -1, x > 0 ? p - 1 : -1, x < nx - 1 ? p + 1 : -1};
+            for (int k = 0; k < 6; k++)
+                if (nb[k] >= 0 && lab[nb[k]] == -1) { lab[nb[k]] = 1; q[tail++] = nb[k]; }
+        }
+        if (!edge && tail < min_size)
+            for (int k = 0; k < tail; k++) f[q[k]] = 0.0f;
+    }
+    free(lab);
+    free(q);
+}
+
+/* process_tile stage (d) on f (after the guided filter) with orig = the
+ * corrected tile before the deconvolution (pipeline.c:302-399) */
+void orc_air_zero(float *f, const float *orig, int nz, int ny, int nx, const fy_cube_params *P) {
+    size_t n = (size_t)nz * ny * nx;
+    int sr = P->scratch_passes > 0 ? P->scratch_passes : 5;
+    long hist[256];
+    memset(hist, 0, sizeof(hist));
+    int lz = (nz + 1) / 2, ly = (ny + 1) / 2, lx = (nx + 1) / 2;
+    size_t nl = (size_t)lz * ly * lx;
+    int lowres = lz >= 2 && ly >= 2 && lx >= 2;
+    float *sm = (float *)malloc(sizeof(float) * (lowres ? nl : n));
+    float *low = lowres ? (float *)malloc(sizeof(float) * nl) : NULL;
+    if (!sm || (lowres && !low)) { free(sm); free(low); return; }
+    if (lowres) {
+        for (int z = 0; z < lz; z++)
+            for (int y = 0; y < ly; y++)
+                for (int x = 0; x < lx; x++)
+                    low[((size_t)z * ly + y) * lx + x] = orig[((size_t)(2 * z) * ny + 2 * y) * nx + 2 * x];
+        int rl = (sr + 1) / 2;
+        if (rl < 1) rl = 1;
+        orc_box_mean_f32(low, sm, lz, ly, lx, rl);
+    } else {
+        orc_box_mean_f32(orig, sm, nz, ny, nx, sr);
+    }
+    size_t nh = lowres ? nl : n;
+    for (size_t i = 0; i < nh; i++) {
+        int b = (int)(sm[i] * 255.0f + 0.5f);
+        if (b < 0) b = 0;
+        if (b > 255) b = 255;
+        hist[b]++;
+    }
+    int dark = 0, light = 0, valley = 0;
+    double d = orc_valley_depth(hist, &dark, &light, &valley);
+    int cut;
+    if (P->air_cut_u8 >= 0) {
+        int gg = P->air_cut_u8, band = P->air_cut_band;
+        if (d >= 0) {
+            int pf = P->air_thresh > 0 ? (int)(P->air_thresh * 255 + 0.5) : 39;
+            double a = P->air_cut_aggr;
+            if (a < 0) a = 0;
+            if (a > 2) a = 2;
+            int local;
+            if (valley <= pf) local = pf;
+            else if (a <= 1.0) local = (int)(pf + a * (valley - pf) + 0.5);
+            else local = (int)(valley + (a - 1.0) * (light > valley ? light - valley : 0) + 0.5);
+            cut = local;
+            if (cut < gg - band) cut = gg - band;
+            if (cut > gg + band) cut = gg + band;
+        } else {
+            cut = gg;
+        }
+    } else if (d >= 0) {
+        int mid = (dark + valley) / 2;
+        cut = dark + 8 < mid ? dark + 8 : mid;
+    } else {
+        cut = (int)((P->air_thresh > 0 ? P->air_thresh : 0.05) * 255);
+    }
+    int pf2 = P->air_thresh > 0 ? (int)(P->air_thresh * 255 + 0.5) : 39;
+    int spread = valley > pf2 ? (valley - pf2) / 2 : 8;
+    int cu = cut + spread;
+    if (valley > cut && cu > valley) cu = valley;
+    int ct = cut - spread;
+    if (ct < pf2) ct = pf2;
+    float cutf_uni = cu / 255.0f, cutf_tex = ct / 255.0f;
+    const float AIR_TEX = 0.03f;
+    if (lowres) {
+        float *row = (float *)malloc(sizeof(float) * lx);
+        for (int z = 0; z < nz; z++) {
+            int z0 = z >> 1;
+            if (z0 > lz - 2) z0 = lz - 2;
+            float zf = 0.5f * z - z0;
+            if (zf > 1.0f) zf = 1.0f;
+            for (int y = 0; y < ny; y++) {
+                int y0 = y >> 1;
+                if (y0 > ly - 2) y0 = ly - 2;
+                float yf = 0.5f * y - y0;
+                if (yf > 1.0f) yf = 1.0f;
+                size_t r00 = ((size_t)z0 * ly + y0) * lx, r01 = r00 + lx;
+                size_t r10 = r00 + (size_t)ly * lx, r11 = r10 + lx;
+                float w00 = (1 - zf) * (1 - yf), w01 = (1 - zf) * yf, w10 = zf * (1 - yf), w11 = zf * yf;
+                for (int k = 0; k < lx; k++)
+                    row[k] = w00 * sm[r00 + k] + w01 * sm[r01 + k] + w10 * sm[r10 + k] + w11 * sm[r11 + k];
+                float *frow = f + ((size_t)z * ny + y) * nx;
+                const float *orow = orig + ((size_t)z * ny + y) * nx;
+                for (int x = 0; x < nx; x++) {
+                    int x0 = x >> 1;
+                    if (x0 > lx - 2) x0 = lx - 2;
+                    float fx = 0.5f * x - x0;
+                    if (fx > 1.0f) fx = 1.0f;
+                    float v = row[x0] + (row[x0 + 1] - row[x0]) * fx;
+                    float tex = orow[x] - v;
+                    if (tex < 0) tex = -tex;
+                    if (v < (tex < AIR_TEX ? cutf_uni : cutf_tex)) frow[x] = 0.0f;
+                }
+            }
+        }
+        free(row);
+    } else {
+        for (size_t i = 0; i < n; i++) {
+            float tex = orig[i] - sm[i];
+            if (tex < 0) tex = -tex;
+            if (sm[i] < (tex < AIR_TEX ? cutf_uni : cutf_tex)) f[i] = 0.0f;
+        }
+    }
+    free(sm);
+    free(low);
+    if (P->air_min_component > 0) orc_air_cc_despeckle(f, nz, ny, nx, P->air_min_component);
+}
+
 /* ------------------------------------------------------------------ chain pieces */
 void orc_zdrift_apply(float *chunk, int nz, int ny, int nx, int z0, const float *factor) {
     size_t plane = (size_t)ny * nx;                              /* zdrift.c:64-71 */
@@ -715,6 +895,12 @@ int orc_chain_cube(const fy_cube_params *P, const unsigned char *u8buf,
     if (P->dering && P->dering->have_rings) orc_dering_apply(P->dering, f, rz0, ry0, rx0, hz, hy, hx, 1.0 / 255.0);
     if (P->zdrift_factor) orc_zdrift_apply(f, (int)hz, (int)hy, (int)hx, (int)rz0, P->zdrift_factor);
     int nz = (int)hz, ny = (int)hy, nx = (int)hx;
+    float *orig = NULL;                                        /* air-blend baseline */
+    if (P->air_zero) {
+        orig = malloc(sizeof(float) * hn);
+        if (!orig) { free(f); free(b1); return 1; }
+        memcpy(orig, f, sizeof(float) * hn);
+    }
     if (P->deconv) {                                           /* (b) */
         int win = P->window_hi > P->window_lo;
         float wlo = (float)P->window_lo, wspan = (float)(P->window_hi - P->window_lo);
@@ -741,6 +927,10 @@ int orc_chain_cube(const fy_cube_params *P, const unsigned char *u8buf,
         int r = P->guided_radius < 1 ? 2 : P->guided_radius;
         if (orc_guided_fast_f32(f, b1, nz, ny, nx, r, P->guided_eps, gs) == 0) memcpy(f, b1, sizeof(float) * hn);
     }
+    if (orig) {                                                /* (d) */
+        orc_air_zero(f, orig, nz, ny, nx, P);
+        free(orig);
+    }
     if (ftile) memcpy(ftile, f, sizeof(float) * hn);
     if (P->recenter && P->norm_hi > P->norm_lo) {              /* (d.5) */
         float mid = 0.5f * (float)(P->norm_lo + P->norm_hi) / 255.0f;
@@ -766,7 +956,7 @@ int orc_chain_cube(const fy_cube_params *P, const unsigned char *u8buf,
 int orc_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long ty, long tx,
                              long vol_y, long vol_x, fy_cube_params *o) {
     memset(o, 0, sizeof *o);
-    if (cfg->do_air_zero || cfg->do_musica) return 1;
+    if (cfg->do_musica) return 1;
     orc_scaled_db(cfg, &o->phys);
     if (cfg->do_deconv) {
         int matched = cfg->use_matched_deconv && cfg->psf_sigma_vox > 0;
@@ -788,6 +978,12 @@ int orc_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long 
     o->no_dither = cfg->no_dither;
     if (cfg->have_norm && cfg->norm_hi > cfg->norm_lo) { o->recenter = 1; o->norm_lo = cfg->norm_lo; o->norm_hi = cfg->norm_hi; }
     if (cfg->have_dering && cfg->dering) o->dering = cfg->dering;
+    if (cfg->do_air_zero) {                                    /* pipeline.c:302-399 */
+        o->air_zero = 1;
+        o->air_cut_u8 = cfg->air_cut_u8; o->air_cut_band = cfg->air_cut_band;
+        o->air_thresh = cfg->air_thresh; o->air_cut_aggr = cfg->air_cut_aggr;
+        o->air_min_component = cfg->air_min_component; o->scratch_passes = cfg->scratch_passes;
+    }
     if (cfg->have_zdrift && cfg->zdrift_factor) { o->zdrift_factor = cfg->zdrift_factor; o->vol_z = cfg->vol_z; }
     return 0;
 }

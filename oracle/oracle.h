@@ -108,6 +108,11 @@ void   orc_zdrift_u8_sub(const unsigned char *pb, long dz, long dy, long dx, lon
                          double *sums, long *counts);
 double orc_flat_noise(const float *v, int nz, int ny, int nx, int blk);
 
+/* stream.c valley depth; pipeline.c air-zero stage (d) and CC despeckle */
+double orc_valley_depth(const long hist[256], int *dark_out, int *light_out, int *valley_out);
+void   orc_air_cc_despeckle(float *f, int nz, int ny, int nx, int min_size);
+void   orc_air_zero(float *f, const float *orig, int nz, int ny, int nx, const fy_cube_params *P);
+
 /* zdrift.c apply; pipeline.c per-tile eps and the tile chain */
 void   orc_zdrift_apply(float *chunk, int nz, int ny, int nx, int z0, const float *factor);
 double orc_eps_for_tile(const fy_pipeline_cfg *cfg, double ecy, double ecx,

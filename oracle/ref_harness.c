@@ -6,6 +6,9 @@
  *                         so the anisotropic Paganin+unsharp case is reachable
  *   ref_quant_u8       -- pipeline.c:60
  *   ref_eps_for_tile   -- pipeline.c:447
+ *   ref_air_cc_despeckle -- pipeline.c:227
+ *   ref_process_tile   -- pipeline.c:252 (stage d, the air-zero, in isolation
+ *                         when the cfg turns the other stages off)
  * TEST INFRASTRUCTURE ONLY.
  */
 #include "paganin.c"
@@ -79,4 +82,23 @@ double ref_chain_many(int n, const unsigned char **u8, unsigned char **out, long
                                 hx - 2 * halo, out[i], p, deconv, param, cap, fz2_scale, wlo, whi, eps, s, ring, 0);
     double t = omp_get_wtime() - t0;
     return fails ? -t : t;
+}
+
+void ref_air_cc_despeckle(float *f, int nz, int ny, int nx, int min_size) {
+    size_t n = (size_t)nz * ny * nx;
+    int *labels = malloc(sizeof(int) * (n ? n : 1)), *queue = malloc(sizeof(int) * (n ? n : 1));
+    if (labels && queue) air_cc_despeckle(f, nz, ny, nx, min_size, labels, queue);
+    free(labels);
+    free(queue);
+}
+
+int ref_process_tile(float *f, const float *orig, const unsigned char *u8orig, int nz, int ny, int nx,
+                     const fy_pipeline_cfg *cfg) {
+    size_t n = (size_t)nz * ny * nx;
+    float *b1 = malloc(sizeof(float) * n), *b2 = malloc(sizeof(float) * n), *ws = malloc(sizeof(float) * 4 * n);
+    int rc = !b1 || !b2 || !ws;
+    if (!rc) process_tile(f, orig, u8orig, nz, ny, nx, cfg, cfg->have_dec_range, cfg->dec_lo, cfg->dec_hi,
+                          b1, b2, ws, 0.0);
+    free(b1); free(b2); free(ws);
+    return rc;
 }

@@ -89,7 +89,10 @@ class CubeParams(C.Structure):
         ("dec_hi", C.c_double), ("guided_eps", C.c_double), ("guided_radius", C.c_int),
         ("guided_s", C.c_int), ("no_dither", C.c_int), ("recenter", C.c_int), ("norm_lo", C.c_int),
         ("norm_hi", C.c_int), ("dering", C.POINTER(Dering)), ("zdrift_factor", c_float_p),
-        ("vol_z", C.c_long), ("fp64", C.c_int)]
+        ("vol_z", C.c_long), ("fp64", C.c_int),
+        ("air_zero", C.c_int), ("air_cut_u8", C.c_int), ("air_cut_band", C.c_int),
+        ("air_thresh", C.c_double), ("air_cut_aggr", C.c_double), ("air_min_component", C.c_int),
+        ("scratch_passes", C.c_int)]
 
 
 def fp(a):
@@ -152,6 +155,7 @@ _SIGS = {
                                  C.POINTER(C.c_long), C.c_float]),
     "zdrift_finalize": (None, [c_double_p, C.POINTER(C.c_long), C.c_int, c_float_p]),
     "flat_noise": (C.c_double, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "valley_depth": (C.c_double, [C.POINTER(C.c_long), c_int_p, c_int_p, c_int_p]),
     "process_buffer": (C.c_int, [C.POINTER(PipelineCfg), c_u8_p] + [C.c_long] * 14 +
                        [c_u8_p, c_int_p]),
 }
@@ -189,6 +193,8 @@ _ORC_ONLY = {
                        [c_u8_p, c_int_p, c_float_p]),
     "orc_zdrift_u8_sub": (None, [c_u8_p, C.c_long, C.c_long, C.c_long, C.c_long, c_double_p,
                                  C.POINTER(C.c_long)]),
+    "orc_air_cc_despeckle": (None, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_air_zero": (None, [c_float_p, c_float_p, C.c_int, C.c_int, C.c_int, C.POINTER(CubeParams)]),
     "orc_process_buffer_f": (C.c_int, [C.POINTER(PipelineCfg), c_u8_p] + [C.c_long] * 14 +
                              [c_u8_p, c_int_p, c_float_p]),
 }
@@ -208,6 +214,9 @@ _REF_ONLY = {
                                       C.c_long, C.c_long, C.c_long, C.c_long]),
     "fy_dering_tile_reset": (None, [C.POINTER(Dering)]),
     "fy_hist_init": (None, [C.POINTER(HistState)]),
+    "ref_air_cc_despeckle": (None, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "ref_process_tile": (C.c_int, [c_float_p, c_float_p, c_u8_p, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(PipelineCfg)]),
     "fy_correct_zdrift": (C.c_int, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_float]),
 }
 

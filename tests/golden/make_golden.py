@@ -198,6 +198,60 @@ def main():
     out["flat_noise"] = np.array([ref.fy_flat_noise(clibs.fp(fv), 40, 40, 48, b) for b in (8, 5, 3, 16, 41)] +
                                  [ref.fy_flat_noise(clibs.fp(fg), 33, 34, 35, b) for b in (8, 4)])
 
+    # ---- air-zero (pipeline.c:227-250,302-399; stream.c:138-170)
+    arng = np.random.default_rng(21)
+    bins = np.arange(256)
+    hists = [np.zeros(256, np.int64),                                     # empty
+             (1000 * np.exp(-0.5 * ((bins - 120) / 9.0) ** 2)).astype(np.int64),   # unimodal
+             (900 * np.exp(-0.5 * ((bins - 40) / 6.0) ** 2) + 500 * np.exp(-0.5 * ((bins - 150) / 15.0) ** 2)
+              + 3).astype(np.int64),                                      # bimodal
+             (700 * np.exp(-0.5 * ((bins - 60) / 3.0) ** 2) + 650 * np.exp(-0.5 * ((bins - 68) / 3.0) ** 2)
+              + 400 * np.exp(-0.5 * ((bins - 200) / 8.0) ** 2)).astype(np.int64),  # close peaks merge
+             arng.integers(0, 50, 256).astype(np.int64)]                 # ragged
+    hists[2][0] = 10 ** 6
+    hists[2][255] = 10 ** 5                                               # rails are excluded
+    vd = []
+    for h in hists:
+        dk, lt, vl = C.c_int(-7), C.c_int(-7), C.c_int(-7)
+        dv = ref.fy_valley_depth(h.ctypes.data_as(C.POINTER(C.c_long)), C.byref(dk), C.byref(lt), C.byref(vl))
+        vd.append([dv, dk.value, lt.value, vl.value])
+    out["valley_hists"], out["valley_out"] = np.stack(hists), np.array(vd)
+    blob = (arng.random((18, 20, 22)) < 0.18).astype(np.float32) * arng.random((18, 20, 22)).astype(np.float32)
+    out["despeckle_in"] = blob
+    for ms in (3, 8, 40):
+        o = blob.copy()
+        ref.ref_air_cc_despeckle(clibs.fp(o), 18, 20, 22, ms)
+        out[f"despeckle_out_{ms}"] = o
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from fysics_amd import phantom
+    av = phantom.sheets((30, 34, 38), arng, n_sheets=5)
+    au8 = phantom.noisy_u8(phantom.forward(av, **phantom.CONFIG1_PHYS), arng)
+    aorig = au8.astype(np.float32) * np.float32(1.0 / 255.0)
+    af = np.clip(aorig + np.float32(0.02) * arng.standard_normal(aorig.shape).astype(np.float32), 0, 1)
+    out["air_u8"], out["air_orig"], out["air_f"] = au8, aorig, af
+    air_cases = [dict(air_cut_u8=-1), dict(air_cut_u8=60, air_cut_band=12, air_cut_aggr=0.5),
+                 dict(air_cut_u8=60, air_cut_band=30, air_cut_aggr=1.5, air_thresh=0.12),
+                 dict(air_cut_u8=-1, air_min_component=40, scratch_passes=3),
+                 dict(air_cut_u8=45, air_cut_band=5, air_cut_aggr=1.0, air_min_component=300)]
+    for k, case in enumerate(air_cases):
+        cfg = clibs.PipelineCfg()
+        cfg.do_air_zero = 1
+        for kk, vv in case.items():
+            setattr(cfg, kk, vv)
+        o = af.copy()
+        assert ref.ref_process_tile(clibs.fp(o), clibs.fp(aorig), clibs.up(au8), 30, 34, 38, C.byref(cfg)) == 0
+        out[f"air_out_{k}"] = o
+        out[f"air_case_{k}"] = np.array([case.get("air_cut_u8", 0), case.get("air_cut_band", 0),
+                                         case.get("air_cut_aggr", 0.0), case.get("air_thresh", 0.0),
+                                         case.get("air_min_component", 0), case.get("scratch_passes", 0)])
+    # a tile too thin for the decimated scratch (nz = 2: full-resolution branch)
+    cfg = clibs.PipelineCfg()
+    cfg.do_air_zero, cfg.air_cut_u8 = 1, -1
+    o2 = np.ascontiguousarray(af[:2]).copy()
+    assert ref.ref_process_tile(clibs.fp(o2), clibs.fp(np.ascontiguousarray(aorig[:2])), clibs.up(au8), 2, 34, 38,
+                                C.byref(cfg)) == 0
+    out["air_out_thin"] = o2
+
     # ---- per-tile chain (pipeline.c:467-510)
     sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
     from fysics_amd import phantom

@@ -569,3 +569,77 @@ def test_flat_noise_golden_and_blocks(orc):
         assert g == _flat_noise_np(fv, b)
     for b, g in zip((8, 4), got[5:]):
         assert abs(g - _flat_noise_np(fg, b)) <= 1e-12 * max(g, 1e-30)
+
+
+# ------------------------------------------------------------------ air-zero
+def test_valley_depth_golden(orc):
+    for h, want in zip(GOLD["valley_hists"], GOLD["valley_out"]):
+        h = np.ascontiguousarray(h, np.int64)
+        dk, lt, vl = C.c_int(-7), C.c_int(-7), C.c_int(-7)
+        d = orc.orc_valley_depth(h.ctypes.data_as(C.POINTER(C.c_long)), C.byref(dk), C.byref(lt), C.byref(vl))
+        assert [d, dk.value, lt.value, vl.value] == list(want)
+
+
+def test_valley_depth_closed_form(orc):
+    # two separated boxes of counts: peaks at their centres' smoothed maxima, the
+    # valley in the empty gap, depth 1 - 0/min
+    h = np.zeros(256, np.int64)
+    h[50:55] = 100
+    h[150:155] = 80
+    dk, lt, vl = C.c_int(), C.c_int(), C.c_int()
+    d = orc.orc_valley_depth(h.ctypes.data_as(C.POINTER(C.c_long)), C.byref(dk), C.byref(lt), C.byref(vl))
+    assert (dk.value, lt.value) == (52, 152)
+    assert 55 <= vl.value <= 149 and d == 1.0
+
+
+def _despeckle_np(f, min_size):
+    from scipy import ndimage
+    lab, n = ndimage.label(f > 0)                       # 6-connectivity (the default structure)
+    out = f.copy()
+    nz, ny, nx = f.shape
+    for k, sl in enumerate(ndimage.find_objects(lab), start=1):
+        m = lab[sl] == k
+        touches = sl[0].start == 0 or sl[1].start == 0 or sl[2].start == 0 or sl[0].stop == nz or \
+            sl[1].stop == ny or sl[2].stop == nx
+        if not touches and m.sum() < min_size:
+            out[sl][m] = 0
+    return out
+
+
+@pytest.mark.parametrize("ms", [3, 8, 40])
+def test_despeckle_golden_and_scipy(orc, ms):
+    f = GOLD["despeckle_in"].copy()
+    orc.orc_air_cc_despeckle(clibs.fp(f), 18, 20, 22, ms)
+    assert np.array_equal(f, GOLD[f"despeckle_out_{ms}"])
+    assert np.array_equal(f, _despeckle_np(GOLD["despeckle_in"], ms))
+
+
+def air_params(case):
+    P = clibs.CubeParams()
+    P.air_zero = 1
+    P.air_cut_u8, P.air_cut_band = int(case[0]), int(case[1])
+    P.air_cut_aggr, P.air_thresh = float(case[2]), float(case[3])
+    P.air_min_component, P.scratch_passes = int(case[4]), int(case[5])
+    return P
+
+
+@pytest.mark.parametrize("k", range(5))
+def test_air_zero_golden(orc, k):
+    """The reference build contracts the upsample blend into FMAs, so a voxel whose
+    smoothed value sits on the cut may fall either way: at most 0.1% of the zero
+    mask differs, every other voxel is identical."""
+    f = GOLD["air_f"].copy()
+    P = air_params(GOLD[f"air_case_{k}"])
+    orc.orc_air_zero(clibs.fp(f), clibs.fp(GOLD["air_orig"]), 30, 34, 38, C.byref(P))
+    want = GOLD[f"air_out_{k}"]
+    diff = f != want
+    assert diff.mean() <= 1e-3
+    assert np.all((f[diff] == 0) | (want[diff] == 0))
+
+
+def test_air_zero_thin_tile_golden(orc):
+    f = np.ascontiguousarray(GOLD["air_f"][:2]).copy()
+    P = clibs.CubeParams()
+    P.air_zero, P.air_cut_u8 = 1, -1
+    orc.orc_air_zero(clibs.fp(f), clibs.fp(np.ascontiguousarray(GOLD["air_orig"][:2])), 2, 34, 38, C.byref(P))
+    assert np.array_equal(f, GOLD["air_out_thin"])
