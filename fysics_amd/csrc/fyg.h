@@ -28,7 +28,7 @@ void *fyg_stream_handle(fyg_stream *s);          /* the hipStream_t */
 int   fyg_sync(fyg_stream *s);
 
 /* per-stream grow-only device workspace: slot in [0, FYG_WS_SLOTS) */
-#define FYG_WS_SLOTS 8
+#define FYG_WS_SLOTS 9
 void *fyg_ws(fyg_stream *s, int slot, size_t bytes);
 /* per-stream grow-only pinned host staging buffer */
 void *fyg_pinned(fyg_stream *s, int slot, size_t bytes);
@@ -208,6 +208,25 @@ int fyg_flat_blocks_run(const float *v, int nz, int ny, int nx, int blk, double 
 /* out = clamp((in - lo) / (hi - lo), 0, 1) in fp32 (1/(hi-lo) -> 1 when hi <= lo) */
 int fyg_norm_u8_run(const unsigned char *in, float *out, size_t n, unsigned char lo, unsigned char hi,
                     fyg_stream *s);
+
+/* hist[b] += sum of part[r*256 + b] over rows r < rows */
+int fyg_hist_reduce_run(const unsigned int *part, int rows, unsigned long long *hist, fyg_stream *s);
+
+/* ------------------------------------------------------------------ air-zero */
+typedef struct {
+    int scratch_passes;             /* <= 0 -> 5 */
+    int cut_u8, band;               /* cut_u8 < 0: dark-peak rule */
+    int min_component;              /* <= 0: no despeckle */
+    double thresh, aggr;
+} fyg_air_args;
+/* valley depth of a 256-bin histogram (stream.c:138-170) */
+double fyg_valley_depth(const long *hist, int *dark, int *light, int *valley);
+size_t fyg_air_ws_bytes(int nz, int ny, int nx);
+/* stage (d) on f in place; orig = the corrected tile before the deconvolution */
+int fyg_air_zero_run(float *f, const float *orig, int nz, int ny, int nx, const fyg_air_args *a, void *ws,
+                     fyg_stream *s);
+/* the union-find's broken-invariant flag of the last run (waits for the stream) */
+int fyg_air_err_read(void *ws, int nz, int ny, int nx, int *out, fyg_stream *s);
 
 /* ------------------------------------------------------------------ noise model */
 /* local (2r+1)^3 box mean m and variance v (double) with sliding sums per line */

@@ -223,6 +223,13 @@ extern "C" int fyg_hist_u8_run(const unsigned char *d, size_t n, unsigned int *p
     return fyg_last_error();
 }
 
+extern "C" int fyg_hist_reduce_run(const unsigned int *part, int rows, unsigned long long *hist, fyg_stream *s) {
+    if (rows <= 0) return 0;
+    const unsigned r = rows < 16 ? rows : 16;
+    fyg_launch(s, "hist_reduce", k_hist_reduce, dim3(r), dim3(NT), 0, part, rows, hist);
+    return fyg_last_error();
+}
+
 extern "C" int fyg_hist_merge_run(unsigned long long *tile, unsigned long long n, unsigned long long *acc,
                                   int skip_zero, fyg_stream *s) {
     fyg_launch(s, "hist_merge", k_hist_merge, dim3(1), dim3(256), 0, tile, n, acc, skip_zero);

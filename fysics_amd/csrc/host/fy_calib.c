@@ -82,6 +82,12 @@ void fy_norm_apply_u8(const unsigned char *in, float *out, size_t n, unsigned ch
     fyg_sync(s);
 }
 
+/* bimodal valley depth (stream.c:138-170): the same routine the device air-zero
+ * cut runs, compiled for the host */
+double fy_valley_depth(const long hist[256], int *dark_out, int *light_out, int *valley_out) {
+    return fyg_valley_depth(hist, dark_out, light_out, valley_out);
+}
+
 /* ------------------------------------------------------------------ flat-block noise */
 static int cmp_double(const void *a, const void *b) {
     double x = *(const double *)a, y = *(const double *)b;

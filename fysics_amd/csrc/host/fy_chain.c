@@ -61,7 +61,7 @@ static double eps_for_tile(const fy_pipeline_cfg *cfg, double ecy, double ecx, l
 int fy_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long ty, long tx, long vol_y,
                             long vol_x, fy_cube_params *o) {
     memset(o, 0, sizeof *o);
-    if (cfg->do_air_zero || cfg->do_musica) return 1;          /* stages not provided */
+    if (cfg->do_musica) return 1;                              /* stage not provided */
     fyh_scaled_physics(cfg, &o->phys);
     if (cfg->do_deconv) {
         int matched = cfg->use_matched_deconv && cfg->psf_sigma_vox > 0;
@@ -94,6 +94,15 @@ int fy_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long t
         o->zdrift_factor = cfg->zdrift_factor;
         o->vol_z = cfg->vol_z;
     }
+    if (cfg->do_air_zero) {                                     /* pipeline.c:302-399 */
+        o->air_zero = 1;
+        o->air_cut_u8 = cfg->air_cut_u8;
+        o->air_cut_band = cfg->air_cut_band;
+        o->air_thresh = cfg->air_thresh;
+        o->air_cut_aggr = cfg->air_cut_aggr;
+        o->air_min_component = cfg->air_min_component;
+        o->scratch_passes = cfg->scratch_passes;
+    }
     return 0;
 }
 
@@ -109,8 +118,19 @@ typedef struct {
     fyg_deconv_args d;          /* deconv = 1 */
     fyg_guided_args g;          /* guided = 1 */
     fyg_emit em;
-    size_t spec_bytes, img_bytes, gws_bytes;
+    int air;                    /* stage (d): air-zero + despeckle */
+    fyg_air_args air_args;
+    size_t spec_bytes, img_bytes, gws_bytes, air_bytes;
 } cube_plan;
+
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+/* stage (d) workspace: the pre-deconvolution tile, the float tile it zeroes
+ * (when the caller gives none) and the kernels' scratch */
+static size_t air_bytes(long hz, long hy, long hx) {
+    size_t hn = (size_t)hz * hy * hx;
+    return 2 * al256(sizeof(float) * hn) + al256(fyg_air_ws_bytes((int)hz, (int)hy, (int)hx));
+}
 
 static int plan_cube(const fy_cube *c, cube_plan *pl, fyg_stream *s) {
     memset(pl, 0, sizeof *pl);
@@ -204,6 +224,18 @@ static int plan_cube(const fy_cube *c, cube_plan *pl, fyg_stream *s) {
                         (hx + g->s - 1) / g->s < 2))
             return 1;
     }
+    if (P->air_zero) {                                          /* (d) */
+        if (pl->f64 && !pl->guided) return 1;
+        pl->air = 1;
+        fyg_air_args *aa = &pl->air_args;
+        aa->scratch_passes = P->scratch_passes;
+        aa->cut_u8 = P->air_cut_u8;
+        aa->band = P->air_cut_band;
+        aa->min_component = P->air_min_component;
+        aa->thresh = P->air_thresh;
+        aa->aggr = P->air_cut_aggr;
+        pl->air_bytes = air_bytes(hz, hy, hx);
+    }
     return 0;
 }
 
@@ -212,14 +244,13 @@ static int same_group(const cube_plan *a, const cube_plan *b) {
     const fy_cube *x = a->c, *y = b->c;
     if (x->hz != y->hz || x->hy != y->hy || x->hx != y->hx || x->tz != y->tz || x->ty != y->ty || x->tx != y->tx)
         return 0;
-    if (a->deconv != b->deconv || a->guided != b->guided || a->pro.dering != b->pro.dering || a->f64 != b->f64)
+    if (a->deconv != b->deconv || a->guided != b->guided || a->pro.dering != b->pro.dering || a->f64 != b->f64 ||
+        a->air != b->air)
         return 0;
     if ((x->d_float != NULL) != (y->d_float != NULL)) return 0;
     if (a->guided && (a->g.s != b->g.s || a->g.r != b->g.r)) return 0;
     return 1;
 }
-
-static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 /* the chain of n planned cubes of one group; flags[i] is cube i's all-air flag */
 static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) {
@@ -231,7 +262,9 @@ static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) 
     char *S = spec ? (char *)fyg_ws(s, WS_SPEC, spec * n) : NULL;
     char *I = img ? (char *)fyg_ws(s, WS_MISC, img * n) : NULL;
     char *G = gws ? (char *)fyg_ws(s, WS_GUIDED, gws * n) : NULL;
-    if (!T || (spec && !S) || (img && !I) || (gws && !G)) return 1;
+    size_t air = al256(h->air_bytes);
+    char *A = air ? (char *)fyg_ws(s, WS_AIR, air * n) : NULL;
+    if (!T || (spec && !S) || (img && !I) || (gws && !G) || (air && !A)) return 1;
 
     if (h->deconv) {
         fyg_deconv_args *da = (fyg_deconv_args *)malloc(sizeof *da * n);
@@ -258,7 +291,7 @@ static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) 
         pl[i].em.nonzero_flag = flags[i];
         pl[i].em.nonzero_gen = gen;
     }
-    if (h->guided && !h->c->d_float) {                          /* filter + emit in one pass */
+    if (h->guided && !h->c->d_float && !h->air) {               /* filter + emit in one pass */
         fyg_guided_args *ga = (fyg_guided_args *)malloc(sizeof *ga * n);
         if (!ga) return 1;
         for (int i = 0; i < n; i++) {
@@ -272,9 +305,12 @@ static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) 
         free(ga);
         return rc;
     }
-    for (int i = 0; i < n; i++) {                               /* fp32 copy requested, or no filter */
+    for (int i = 0; i < n; i++) {               /* fp32 copy requested, air-zero, or no filter */
         const fy_cube *c = pl[i].c;
         const float *src = (const float *)(T + tile * i);
+        size_t fb = al256(sizeof(float) * pl[i].hn);
+        float *orig = h->air ? (float *)(A + air * i) : NULL;
+        float *F = c->d_float ? c->d_float : (h->air ? (float *)(A + air * i + fb) : NULL);
         if (h->guided) {
             fyg_guided_args g = pl[i].g;
             if (pl[i].f64) g.in64 = (const double *)(T + tile * i);
@@ -282,11 +318,18 @@ static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) 
             g.ws = G + gws * i;
             g.em = pl[i].em;
             g.em.quant = 0;
-            g.em.out = c->d_float;
+            g.em.out = F;
             if (fyg_guided_run(&g, s)) return 1;
-            src = c->d_float;
-        } else if (c->d_float) {
-            if (fyg_d2d(c->d_float, src, sizeof(float) * pl[i].hn, s)) return 1;
+            src = F;
+        } else if (F) {
+            if (fyg_d2d(F, src, sizeof(float) * pl[i].hn, s)) return 1;
+            src = F;
+        }
+        if (h->air) {                                           /* (d), pipeline.c:302-399 */
+            if (fyg_prologue_run(c->d_in, orig, (int)c->hz, (int)c->hy, (int)c->hx, &pl[i].pro, flags[i], gen, s) ||
+                fyg_air_zero_run(F, orig, (int)c->hz, (int)c->hy, (int)c->hx, &pl[i].air_args, A + air * i + 2 * fb,
+                                 s))
+                return 1;
         }
         if (fyg_emit_run(src, (int)c->hz, (int)c->hy, (int)c->hx, &pl[i].em, s)) return 1;
     }
@@ -326,7 +369,7 @@ static int run_chunk(const fy_cube *cubes, int n, int *d_flags, int gen, fyg_str
             }
         }
         size_t per = al256(pl[i].tile_bytes) + al256(pl[i].spec_bytes) + al256(pl[i].gws_bytes) +
-                     2 * al256(pl[i].img_bytes);
+                     2 * al256(pl[i].img_bytes) + al256(pl[i].air_bytes);
         int part = (int)(FYH_GROUP_BYTES / (per ? per : 1));
         if (part < 1) part = 1;
         for (int k = 0; k < m && !rc; k += part)
@@ -367,6 +410,7 @@ size_t fy_cube_workspace_bytes(const fy_cube *c) {
         b += al256(f64 ? fyg_guided_ws_bytes64((int)c->hz, (int)c->hy, (int)c->hx, sd)
                        : fyg_guided_ws_bytes((int)c->hz, (int)c->hy, (int)c->hx, sd));
     }
+    if (P->air_zero) b += al256(air_bytes(c->hz, c->hy, c->hx));
     return b;
 }
 

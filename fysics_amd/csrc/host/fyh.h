@@ -12,7 +12,8 @@
 #define FYH_MAX_GAIN 8.0        /* Wiener gain cap (paganin.c:50) */
 
 /* workspace slots of a stream */
-enum { WS_IN = 0, WS_SPEC = 1, WS_TILE = 2, WS_GUIDED = 3, WS_U8 = 4, WS_FLAG = 5, WS_OUT = 6, WS_MISC = 7 };
+enum { WS_IN = 0, WS_SPEC = 1, WS_TILE = 2, WS_GUIDED = 3, WS_U8 = 4, WS_FLAG = 5, WS_OUT = 6, WS_MISC = 7,
+       WS_AIR = 8 };
 
 /* device-resident transfer tables of one (weight, physics, padded shape) */
 typedef struct {
