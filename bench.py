@@ -122,6 +122,7 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served-request measurement")
     ap.add_argument("--no-precision", action="store_true", help="skip the 512^3 fp32 / fp64 timing")
     ap.add_argument("--no-calibration", action="store_true", help="skip the pass-1 statistics timing")
+    ap.add_argument("--no-air-zero", action="store_true", help="skip the chain-with-air-zero timing")
     ap.add_argument("--served-requests", type=int, default=2000)
     ap.add_argument("--served-rate", type=float, default=400.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -242,6 +243,7 @@ def main():
     stress = None if args.no_served else served_bench(F, 4000, 12000.0, seed=8)
     precision = None if args.no_precision else precision_bench(F)
     calib = None if args.no_calibration else calibration_bench(F, vol)
+    airz = None if args.no_air_zero else air_zero_bench(F, vol, eps)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -265,6 +267,7 @@ def main():
         "served_overload": stress,
         "precision_512": precision,
         "calibration": calib,
+        "air_zero": airz,
         "cpu_baseline": cpu,
         "workload_gen_s": round(gen_s, 1),
     }
@@ -509,6 +512,51 @@ def calibration_bench(F, vol, reps=50):
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(nbytes / (hist_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if hist_us else None},
             "flat_noise_128": {"ms_per_call": round(1000 * fdt, 3), "noise_floor": nf.value}}
+
+
+def air_zero_bench(F, vol, eps, reps=10):
+    """The workload cube through the chain with stage (d) added: air-zero value
+    cut (valley rule) and despeckle of islands under 500 voxels, u8 in HBM -> u8
+    in HBM, with its per-kernel times."""
+    import torch
+    lib = F.lib()
+    n = vol.shape[0]
+    P = F.CubeParams()
+    P.phys = F.physics(delta_beta=300.0 * 0.35, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
+                       unsharp_sigma=0.8, unsharp_coeff=0.5)
+    P.deconv, P.deconv_param, P.deconv_cap, P.fz2_scale = 1, 0.015, 8.0, 1.17 ** 2
+    P.window_lo, P.window_hi = -0.1, 1.2
+    P.guided_eps, P.guided_radius, P.guided_s = eps, 2, 2
+    P.air_zero, P.air_cut_u8, P.air_min_component = 1, -1, 500
+    d_in = torch.from_numpy(vol).cuda()
+    d_out = torch.empty_like(d_in)
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+    z = (0, 0, 0)
+    cube = F.Cube(d_in.data_ptr(), *z, n, n, n, *z, n, n, n, d_out.data_ptr(), None, C.pointer(P))
+    torch.cuda.synchronize()
+    if lib.fy_gpu_process_cubes(C.byref(cube), 1, sh):
+        raise RuntimeError(lib.fy_gpu_last_error().decode())
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        lib.fy_gpu_process_cubes(C.byref(cube), 1, sh)
+    stream.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    lib.fy_gpu_profile_reset()
+    lib.fy_gpu_profile(1)
+    for _ in range(5):
+        lib.fy_gpu_process_cubes(C.byref(cube), 1, sh)
+    stream.synchronize()
+    lib.fy_gpu_profile(0)
+    prof = F.profile_read()
+    stage = {k: round(1000.0 * ms / max(c, 1), 2) for k, (ms, c) in prof.items()
+             if k.startswith("air_") or k.startswith("cc_") or k in ("prologue", "emit", "guided_coef",
+                                                                     "guided_upsample")}
+    zeroed = float((d_out == 0).float().mean().item())
+    return {"workload": f"{n}^3 u8 cube, the chain plus air-zero (valley rule) and despeckle (< 500 voxels)",
+            "ms_per_cube": round(1000 * dt, 3), "mvox_s": round(n ** 3 / dt / 1e6, 1),
+            "zeroed_fraction": round(zeroed, 4), "kernels_us": stage}
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):
