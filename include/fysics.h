@@ -11,8 +11,9 @@
  * Only the hot-path subset is provided (FFT, transfers, Wiener deconvolution,
  * guided filter, u8/physical conversions, ring removal, noise fit, the pass-1
  * calibration statistics -- histogram, z-drift sums, flat-block noise floor --
- * and the per-tile chain).  Registration, zarr I/O, MUSICA, downscale and the
- * whole-volume calibrate/run drivers are not part of this library.
+ * and the per-tile chain with its air-zero stage).  Registration, zarr I/O,
+ * MUSICA, downscale and the whole-volume calibrate/run drivers are not part of
+ * this library.
  *
  * Layout: row-major, x fastest, idx = (z*ny + y)*nx + x (fysics.h:120).
  */
@@ -205,9 +206,9 @@ typedef struct {
 
 /* Process one inner tile from an already-read halo'd u8 buffer (fysics.h:507-511;
  * pipeline.c:467-510): u8 -> f01 -> dering -> z-drift -> (deconv chain) -> guided
- * (per-tile eps) -> recenter -> dithered quantisation of the inner tile.
- * The air-zero (do_air_zero) and MUSICA (do_musica) stages are not provided:
- * a cfg that enables either returns 1 without touching out. */
+ * (per-tile eps) -> air-zero + despeckle (do_air_zero) -> recenter -> dithered
+ * quantisation of the inner tile.  The MUSICA stage (do_musica) is not
+ * provided: a cfg that enables it returns 1 without touching out. */
 int fy_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
                       long rz0, long ry0, long rx0, long hz, long hy, long hx,
                       long z0, long y0, long x0, long tz, long ty, long tx,
