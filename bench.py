@@ -61,14 +61,14 @@ def kernel_bytes(name, n, p, s=2):
 # moved, write 1.0), from the run summarised in
 # profiles/chain_256cube_kernel_trace.md.  Used for roofline.traffic only.
 MEASURED_TRAFFIC_256 = {
-    "ring_image": 296045,
-    "rfft_x_u8": 86636153,
-    "cfft_y_fwd": 144839817,
-    "zfused": 145100598,
-    "cfft_y_inv": 144839817,
-    "irfft_x": 142886317,
-    "guided_coef": 92672173,
-    "guided_upsample": 105242624,
+    "ring_image": 300561,
+    "rfft_x_u8": 86681663,
+    "cfft_y_fwd": 144805470,
+    "zfused": 145066779,
+    "cfft_y_inv": 144805470,
+    "irfft_x": 142812617,
+    "guided_coef": 42136861,
+    "guided_upsample": 105213257,
 }
 
 
