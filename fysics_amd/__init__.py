@@ -162,6 +162,13 @@ SIGNATURES = {
     "fy_norm_finalize": (None, [C.POINTER(HistState), C.c_double, C.c_double, c_u8_p, c_u8_p]),
     "fy_norm_apply_u8": (None, [c_u8_p, c_float_p, C.c_size_t, C.c_ubyte, C.c_ubyte]),
     "fy_flat_noise": (C.c_double, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "fy_phase_correlate": (C.c_int, [c_float_p, c_float_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p,
+                                     C.c_int]),
+    "fy_gpu_phase_correlate": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p, C.c_int,
+                                         _VP]),
+    "fy_downscale2x": (None, [c_u8_p, C.c_int, C.c_int, C.c_int, c_u8_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                              C.POINTER(C.c_int), C.c_int, C.c_float]),
+    "fy_gpu_downscale2x": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _VP, C.c_int, C.c_float, _VP]),
     "fy_valley_depth": (C.c_double, [C.POINTER(C.c_long), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]),
     "fy_dering_init": (C.c_int, [C.POINTER(Dering), _L, _L, _L, C.c_double, C.c_double, C.c_int,

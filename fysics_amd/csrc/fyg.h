@@ -140,6 +140,21 @@ int fyg_deconv_hxp(int px);
 int fyg_fft3d_run(float *d_re, float *d_im, int nz, int ny, int nx, int sign,
                   void *work /* 8*nz*ny*nx bytes */, fyg_stream *s);
 int fyg_scale_run(float *d, size_t n, float k, fyg_stream *s);
+/* in-place complex 3-D FFT of interleaved (re, im) fp32 device data */
+int fyg_cfft3d_run(void *data, int nz, int ny, int nx, int sign, fyg_stream *s);
+
+/* ------------------------------------------------------------------ phase correlation */
+typedef struct {
+    float best;                     /* surface maximum (first in index order) */
+    float maxmag;                   /* largest cross-power magnitude */
+    long long index;                /* its padded index */
+    float c[7];                     /* surface at the peak, then -z, +z, -y, +y, -x, +x (circular) */
+} fyg_pc_result;
+size_t fyg_pc_ws_bytes(int pz, int py, int px);
+/* phasecorr.c:44-145 on device volumes: w* are the per-axis windows (device) */
+int fyg_phase_correlate_run(const float *fixed, const float *moving, int nz, int ny, int nx, int pz, int py,
+                            int px, const float *wz, const float *wy, const float *wx, void *ws,
+                            fyg_pc_result *out, fyg_stream *s);
 
 /* ------------------------------------------------------------------ guided / box */
 typedef struct {
@@ -178,6 +193,9 @@ int fyg_box_mean_run(const float *in, float *out, int nz, int ny, int nx, int r,
 
 /* ------------------------------------------------------------------ elementwise */
 int fyg_u8_to_f32_run(const unsigned char *in, float *out, size_t n, float lo, float scale, fyg_stream *s);
+/* 2x LOD downscale, method 0 box / 1 contrast-maintaining box (downscale.c) */
+int fyg_downscale2x_run(const unsigned char *in, int nx, int ny, int nz, unsigned char *out, int method,
+                        float alpha, fyg_stream *s);
 int fyg_f32_to_u8_run(const float *in, unsigned char *out, size_t n, float lo, float k, fyg_stream *s);
 int fyg_zdrift_run(float *f, int nz, int ny, int nx, const float *factor_at_z0, fyg_stream *s);
 int fyg_dering_apply_run(float *f, long dz, long dy, long dx, const fyg_prologue *pro, fyg_stream *s);

@@ -132,7 +132,102 @@ __global__ void k_dering_accum_global(const unsigned char *buf, long dz, long dy
     }
 }
 
+// ---- 2x LOD downscale (downscale.c:15-45): one output voxel from its 2x2x2
+// parent cell; an all-zero cell stays zero
+__device__ __forceinline__ unsigned int ds_cell(const unsigned int c[8], int method, float alpha) {
+    #pragma clang fp contract(off)
+    unsigned int acc = c[0] | c[1] | c[2] | c[3] | c[4] | c[5] | c[6] | c[7];
+    if (!acc) return 0;
+    const float mean = (float)(int)(c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7]) * 0.125f;
+    float v = mean;
+    if (method != 0) {
+        float best = mean, bestdev = -1.0f;
+        for (int i = 0; i < 8; i++) {
+            const float d = (float)c[i] - mean, ad = d < 0 ? -d : d;
+            if (ad > bestdev) {
+                bestdev = ad;
+                best = (float)c[i];
+            }
+        }
+        v = mean + alpha * (best - mean);
+    }
+    return (unsigned int)(unsigned char)(v < 0 ? 0.0f : v > 255 ? 255.0f : (v + 0.5f));
+}
+
+// grid (x groups, output rows): a thread makes 4 consecutive outputs of one row
+// from two planes x two rows x 8 bytes (one 8-byte load each where the rows are
+// 8-byte aligned and the cells lie inside the volume)
+__global__ void k_downscale2x(const unsigned char *in, int nx, int ny, int nz, unsigned char *out, int ox, int oy,
+                              int method, float alpha, int vec) {
+    const int row = blockIdx.y, z = row / oy, y = row - z * oy;
+    const int x0 = (blockIdx.x * NT + threadIdx.x) * 4;
+    if (x0 >= ox) return;
+    const int za = 2 * z, zb = 2 * z + 1 < nz ? 2 * z + 1 : nz - 1;
+    const int ya = 2 * y, yb = 2 * y + 1 < ny ? 2 * y + 1 : ny - 1;
+    const unsigned char *r[4] = {in + ((size_t)za * ny + ya) * nx, in + ((size_t)za * ny + yb) * nx,
+                                 in + ((size_t)zb * ny + ya) * nx, in + ((size_t)zb * ny + yb) * nx};
+    unsigned int q[4];
+    if (vec && 2 * x0 + 8 <= nx) {
+        unsigned int w[4][2];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint2 t = *(const uint2 *)(r[k] + 2 * x0);
+            w[k][0] = t.x;
+            w[k][1] = t.y;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int sh = (j & 1) * 16, h = j >> 1;
+            // cell order: (dz, dy, dx) with x fastest
+            unsigned int c[8] = {(w[0][h] >> sh) & 255, (w[0][h] >> (sh + 8)) & 255,
+                                 (w[1][h] >> sh) & 255, (w[1][h] >> (sh + 8)) & 255,
+                                 (w[2][h] >> sh) & 255, (w[2][h] >> (sh + 8)) & 255,
+                                 (w[3][h] >> sh) & 255, (w[3][h] >> (sh + 8)) & 255};
+            q[j] = ds_cell(c, method, alpha);
+        }
+    } else {
+        for (int j = 0; j < 4; j++) {
+            const int x = x0 + j;
+            if (x >= ox) { q[j] = 0; continue; }
+            const int xa = 2 * x, xb = 2 * x + 1 < nx ? 2 * x + 1 : nx - 1;
+            unsigned int c[8] = {r[0][xa], r[0][xb], r[1][xa], r[1][xb], r[2][xa], r[2][xb], r[3][xa], r[3][xb]};
+            q[j] = ds_cell(c, method, alpha);
+        }
+    }
+    unsigned char *o = out + ((size_t)z * oy + y) * ox + x0;
+    if (vec && x0 + 4 <= ox) {
+        *(unsigned int *)o = q[0] | (q[1] << 8) | (q[2] << 16) | (q[3] << 24);
+    } else {
+        for (int j = 0; j < 4 && x0 + j < ox; j++) o[j] = (unsigned char)q[j];
+    }
+}
+
 }  // namespace
+
+extern "C" int fyg_downscale2x_run(const unsigned char *in, int nx, int ny, int nz, unsigned char *out, int method,
+                                   float alpha, fyg_stream *s) {
+    if (nx < 1 || ny < 1 || nz < 1) return 1;
+    const int ox = (nx + 1) / 2, oy = (ny + 1) / 2, oz = (nz + 1) / 2;
+    if ((long)oz * oy > 65535L * 32768L) return 1;
+    // 8-byte row loads need 8-aligned rows; 4-byte stores 4-aligned output rows
+    const int vec = nx % 8 == 0 && ((uintptr_t)in & 7) == 0 && ox % 4 == 0 && ((uintptr_t)out & 3) == 0;
+    const unsigned gx = (unsigned)(((ox + 3) / 4 + NT - 1) / NT);
+    if ((long)oz * oy > 65535) {
+        // rows beyond the grid's y limit: one launch per slab of output planes
+        const int per = 65535 / oy;
+        for (int z0 = 0; z0 < oz; z0 += per) {
+            const int nzs = oz - z0 < per ? oz - z0 : per;
+            const int nin = 2 * nzs < nz - 2 * z0 ? 2 * nzs : nz - 2 * z0;
+            fyg_launch(s, "downscale2x", k_downscale2x, dim3(gx, (unsigned)(nzs * oy)), dim3(NT), 0,
+                       in + (size_t)2 * z0 * ny * nx, nx, ny, nin, out + (size_t)z0 * oy * ox, ox, oy, method, alpha,
+                       vec);
+        }
+    } else {
+        fyg_launch(s, "downscale2x", k_downscale2x, dim3(gx, (unsigned)(oz * oy)), dim3(NT), 0, in, nx, ny, nz, out,
+                   ox, oy, method, alpha, vec);
+    }
+    return fyg_last_error();
+}
 
 extern "C" int fyg_u8_to_f32_run(const unsigned char *in, float *out, size_t n, float lo, float scale,
                                  fyg_stream *s) {

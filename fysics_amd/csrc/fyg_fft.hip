@@ -87,14 +87,11 @@ extern "C" size_t fyg_ring_image_bytes(const fyg_prologue *p, int nz, int ny, in
     return sizeof(float) * (size_t)(s1 - s0 + 1) * ny * nx;
 }
 
-extern "C" int fyg_fft3d_run(float *d_re, float *d_im, int nz, int ny, int nx, int sign, void *work,
-                             fyg_stream *s) {
+// in-place complex 3-D FFT of interleaved data, axis order X, Y, Z (fft.c:178-213)
+extern "C" int fyg_cfft3d_run(void *data, int nz, int ny, int nx, int sign, fyg_stream *s) {
     if (!fyg_fft_supported(nx) || !fyg_fft_supported(ny) || !fyg_fft_supported(nz)) return 1;
-    size_t n = (size_t)nz * ny * nx;
-    float2 *c = (float2 *)work;
+    float2 *c = (float2 *)data;
     float sg = sign < 0 ? -1.0f : 1.0f;
-    fyg_launch(s, "interleave", k_interleave, dim3(grid1d(n)), dim3(256), 0, (const float *)d_re,
-               (const float *)d_im, c, n);
     Rows rx{c, (long)nz * ny, sg, (const float2 *)fyg_twiddles(nx)};
     dispatch(nx, [&](auto cl) {
         constexpr int L = decltype(cl)::value;
@@ -109,6 +106,17 @@ extern "C" int fyg_fft3d_run(float *d_re, float *d_im, int nz, int ny, int nx, i
         Cols<float> cz{c, (long)ny * nx, (long)ny * nx, 0, sg, (const float2 *)fyg_twiddles(nz)};
         dispatch(nz, [&](auto cl) { Launch<decltype(cl)::value, float>::cols(s, &cz, 1, 1, "cfft_cols"); });
     }
+    return fyg_last_error();
+}
+
+extern "C" int fyg_fft3d_run(float *d_re, float *d_im, int nz, int ny, int nx, int sign, void *work,
+                             fyg_stream *s) {
+    if (!fyg_fft_supported(nx) || !fyg_fft_supported(ny) || !fyg_fft_supported(nz)) return 1;
+    size_t n = (size_t)nz * ny * nx;
+    float2 *c = (float2 *)work;
+    fyg_launch(s, "interleave", k_interleave, dim3(grid1d(n)), dim3(256), 0, (const float *)d_re,
+               (const float *)d_im, c, n);
+    if (fyg_cfft3d_run(c, nz, ny, nx, sign, s)) return 1;
     fyg_launch(s, "deinterleave", k_deinterleave, dim3(grid1d(n)), dim3(256), 0, (const float2 *)c, d_re,
                d_im, n);
     return fyg_last_error();

@@ -195,6 +195,103 @@ int fy_kernel_halo(const fy_physics *p) {
 }
 
 /* ---------------------------------------------------------------- device hooks */
+/* ---------------------------------------------------------------- phase correlation (phasecorr.c) */
+/* Hann weights 0.5 - 0.5 cos(2 pi i / (n-1)) in fp32 with the C library's cosf,
+ * as phasecorr.c:36-40 evaluates them; all ones without the window */
+static void pc_window(float *w, int n, int window) {
+    if (!window || n == 1) {
+        for (int i = 0; i < n; i++) w[i] = 1.0f;
+        return;
+    }
+    for (int i = 0; i < n; i++) w[i] = 0.5f - 0.5f * cosf(2.0f * (float)M_PI * i / (float)(n - 1));
+}
+
+/* Foroosh's estimator on one axis (phasecorr.c:150-176) */
+static double pc_foroosh(double c0, double cm, double cp) {
+    double c1 = cp >= cm ? cp : cm, sg = cp >= cm ? 1.0 : -1.0, d = 0.0;
+    if (fabs(c1 - c0) > 1e-20 && fabs(c1 + c0) > 1e-20) {
+        double d1 = c1 / (c1 - c0), d2 = c1 / (c1 + c0);
+        d = sg * (fabs(d1) <= fabs(d2) ? d1 : d2);
+    }
+    return d > 0.5 ? 0.5 : (d < -0.5 ? -0.5 : d);
+}
+
+static int pc_dev(const float *df, const float *dm, int nz, int ny, int nx, double *shift, double *peak, int window,
+                  fyg_stream *s) {
+    int pz = fy_next_pow2(nz), py = fy_next_pow2(ny), px = fy_next_pow2(nx);
+    size_t wb = sizeof(float) * (size_t)(nz + ny + nx);
+    float *hw = (float *)malloc(wb);
+    char *ws = (char *)fyg_ws(s, WS_SPEC, fyg_pc_ws_bytes(pz, py, px) + ((wb + 255) & ~(size_t)255));
+    if (!hw || !ws) { free(hw); return 1; }
+    pc_window(hw, nz, window);
+    pc_window(hw + nz, ny, window);
+    pc_window(hw + nz + ny, nx, window);
+    float *dw = (float *)(ws + fyg_pc_ws_bytes(pz, py, px));
+    fyg_pc_result r;
+    int rc = fyg_h2d(dw, hw, wb, s) ||
+             fyg_phase_correlate_run(df, dm, nz, ny, nx, pz, py, px, dw, dw + nz, dw + nz + ny, ws, &r, s);
+    free(hw);
+    if (rc) return 1;
+    if (r.maxmag < 1e-20) {                                     /* a flat input: no phase */
+        shift[0] = shift[1] = shift[2] = 0.0;
+        if (peak) *peak = 0.0;
+        return 1;
+    }
+    long long pk = r.index;
+    int kz = (int)(pk / ((long long)py * px)), ky = (int)((pk / px) % py), kx = (int)(pk % px);
+    double c0 = r.c[0];
+    shift[0] = (kz <= pz / 2 ? kz : kz - pz) + pc_foroosh(c0, r.c[1], r.c[2]);
+    shift[1] = (ky <= py / 2 ? ky : ky - py) + pc_foroosh(c0, r.c[3], r.c[4]);
+    shift[2] = (kx <= px / 2 ? kx : kx - px) + pc_foroosh(c0, r.c[5], r.c[6]);
+    if (peak) *peak = (double)r.best / (double)((size_t)pz * py * px);
+    return 0;
+}
+
+int fy_phase_correlate(const float *fixed, const float *moving, int nz, int ny, int nx, double *shift, double *peak,
+                       int window) {                            /* phasecorr.c:44-186 */
+    if (!fixed || !moving || !shift || nz < 2 || ny < 2 || nx < 2) return 1;
+    fyg_stream *s = fyg_thread_stream();
+    size_t n = (size_t)nz * ny * nx;
+    float *d = s ? (float *)fyg_ws(s, WS_IN, 2 * n * sizeof(float)) : NULL;
+    if (!d || fyg_h2d(d, fixed, n * sizeof(float), s) || fyg_h2d(d + n, moving, n * sizeof(float), s)) return 1;
+    return pc_dev(d, d + n, nz, ny, nx, shift, peak, window, s);
+}
+
+int fy_gpu_phase_correlate(const float *d_fixed, const float *d_moving, int nz, int ny, int nx, double *shift,
+                           double *peak, int window, void *stream) {
+    if (!d_fixed || !d_moving || !shift || nz < 2 || ny < 2 || nx < 2) return 1;
+    fyg_stream *s = fyg_stream_wrap(stream);
+    return !s || pc_dev(d_fixed, d_moving, nz, ny, nx, shift, peak, window, s);
+}
+
+/* ---------------------------------------------------------------- 2x LOD (downscale.c) */
+void fy_downscale2x(const unsigned char *in, int nx, int ny, int nz, unsigned char *out, int *oxp, int *oyp,
+                    int *ozp, fy_ds_method method, float alpha) {
+    int ox = (nx + 1) / 2, oy = (ny + 1) / 2, oz = (nz + 1) / 2;
+    if (ox < 1) ox = 1;
+    if (oy < 1) oy = 1;
+    if (oz < 1) oz = 1;
+    if (oxp) *oxp = ox;
+    if (oyp) *oyp = oy;
+    if (ozp) *ozp = oz;
+    if (nx < 1 || ny < 1 || nz < 1) return;
+    fyg_stream *s = fyg_thread_stream();
+    size_t n = (size_t)nx * ny * nz, no = (size_t)ox * oy * oz;
+    unsigned char *d = s ? (unsigned char *)fyg_ws(s, WS_U8, n) : NULL;
+    unsigned char *o = s ? (unsigned char *)fyg_ws(s, WS_OUT, no) : NULL;
+    if (!d || !o) return;
+    if (fyg_h2d(d, in, n, s) || fyg_downscale2x_run(d, nx, ny, nz, o, (int)method, alpha, s) ||
+        fyg_d2h(out, o, no, s))
+        return;
+    fyg_sync(s);
+}
+
+int fy_gpu_downscale2x(const unsigned char *d_in, int nx, int ny, int nz, unsigned char *d_out, fy_ds_method method,
+                       float alpha, void *stream) {
+    fyg_stream *s = fyg_stream_wrap(stream);
+    return !s || fyg_downscale2x_run(d_in, nx, ny, nz, d_out, (int)method, alpha, s);
+}
+
 int fy_gpu_sync(void *stream) {
     fyg_stream *s = fyg_stream_wrap(stream);
     return s ? fyg_sync(s) : 1;

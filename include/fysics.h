@@ -143,6 +143,28 @@ void fy_zdrift_finalize(const double *sums, const long *counts, int nz, float *f
 void fy_zdrift_apply(float *chunk, int nz_slab, int ny, int nx, int z0, const float *factor);
 int  fy_correct_zdrift(float *vol, int nz, int ny, int nx, float papyrus_thresh);
 
+/* ---- 2x LOD downscale (fysics.h:388-396; downscale.c) ----
+ * Each output voxel is a function of its own 2x2x2 parent cell (edge cells
+ * repeat the last plane/row/column); an all-zero cell stays zero.  BOX: the
+ * cell mean; CBOX: the mean moved by alpha toward the cell's farthest-from-mean
+ * voxel.  Rounded to u8 by +0.5 and truncation.  ox/oy/oz (may be NULL)
+ * receive (n+1)/2 per axis.  Note the argument order nx, ny, nz. */
+typedef enum { FY_DS_BOX = 0, FY_DS_CBOX } fy_ds_method;
+void fy_downscale2x(const unsigned char *in, int nx, int ny, int nz,
+                    unsigned char *out, int *ox, int *oy, int *oz,
+                    fy_ds_method method, float alpha);
+
+/* ---- sub-voxel translation by phase correlation (fysics.h:296-313; phasecorr.c:44-186) ----
+ * shift (dz,dy,dx) such that moving(p) ~ fixed(p - shift): mean removal and an
+ * optional separable Hann window, zero padding to the next power of two per
+ * axis, the unit-magnitude cross-power spectrum conj(F)*M/|conj(F)*M|, its
+ * inverse FFT, the first maximum of the surface unwrapped to a signed shift and
+ * refined per axis by Foroosh's side-lobe estimator (clamped to +-0.5).
+ * peak (may be NULL) = surface maximum / padded voxel count.  1 on bad
+ * arguments (any axis < 2) or a flat input. */
+int fy_phase_correlate(const float *fixed, const float *moving, int nz, int ny, int nx,
+                       double *shift, double *peak, int window);
+
 /* ---- ring removal (fysics.h:406-433; dering.c) ---- */
 typedef struct {
     long Z, Y, X;

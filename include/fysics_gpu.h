@@ -147,6 +147,13 @@ int  fy_calib_accumulate_u8(fy_calib *c, const unsigned char *d_tile, long z0,
 int  fy_calib_read(fy_calib *c, fy_hist_state *hist, double *zsums, long *zcnts);
 int  fy_calib_reset(fy_calib *c);
 void fy_calib_destroy(fy_calib *c);
+/* fy_downscale2x on device buffers (downscale.c:27-45); d_out holds
+ * ((nz+1)/2)*((ny+1)/2)*((nx+1)/2) bytes */
+int  fy_gpu_downscale2x(const unsigned char *d_in, int nx, int ny, int nz, unsigned char *d_out,
+                        fy_ds_method method, float alpha, void *stream);
+/* fy_phase_correlate of two device volumes (phasecorr.c:44-186); waits for the stream */
+int  fy_gpu_phase_correlate(const float *d_fixed, const float *d_moving, int nz, int ny, int nx,
+                            double *shift, double *peak, int window, void *stream);
 /* fy_flat_noise of a device volume (stream.c:174-200) */
 int  fy_gpu_flat_noise(const float *d_v, int nz, int ny, int nx, int blk, double *out, void *stream);
 

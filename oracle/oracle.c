@@ -846,6 +846,134 @@ void orc_air_zero(float *f, const float *orig, int nz, int ny, int nx, const fy_
     if (P->air_min_component > 0) orc_air_cc_despeckle(f, nz, ny, nx, P->air_min_component);
 }
 
+/* ------------------------------------------------------------------ LOD downscale */
+static unsigned char orc_ds_u8(float v) { return (unsigned char)(v < 0 ? 0 : v > 255 ? 255 : (v + 0.5f)); }
+void orc_downscale2x(const unsigned char *in, int nx, int ny, int nz, unsigned char *out,
+                     int *oxp, int *oyp, int *ozp, int method, float alpha) {   /* downscale.c:15-45 */
+    int ox = (nx + 1) / 2, oy = (ny + 1) / 2, oz = (nz + 1) / 2;
+    if (ox < 1) ox = 1;
+    if (oy < 1) oy = 1;
+    if (oz < 1) oz = 1;
+    if (oxp) *oxp = ox;
+    if (oyp) *oyp = oy;
+    if (ozp) *ozp = oz;
+    for (int z = 0; z < oz; z++)
+        for (int y = 0; y < oy; y++)
+            for (int x = 0; x < ox; x++) {
+                unsigned char c[8];
+                int k = 0;
+                unsigned acc = 0;
+                for (int dz = 0; dz < 2; dz++)
+                    for (int dy = 0; dy < 2; dy++)
+                        for (int dx = 0; dx < 2; dx++) {
+                            int z2 = 2 * z + dz, y2 = 2 * y + dy, x2 = 2 * x + dx;
+                            if (z2 >= nz) z2 = nz - 1;
+                            if (y2 >= ny) y2 = ny - 1;
+                            if (x2 >= nx) x2 = nx - 1;
+                            c[k] = in[((size_t)z2 * ny + y2) * nx + x2];
+                            acc |= c[k++];
+                        }
+                unsigned char r = 0;
+                if (acc) {
+                    float mean = (c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7]) * 0.125f;
+                    if (method == 0) {
+                        r = orc_ds_u8(mean);
+                    } else {
+                        float best = mean, bestdev = -1.0f;
+                        for (int i = 0; i < 8; i++) {
+                            float d = (float)c[i] - mean, ad = d < 0 ? -d : d;
+                            if (ad > bestdev) { bestdev = ad; best = c[i]; }
+                        }
+                        r = orc_ds_u8(mean + alpha * (best - mean));
+                    }
+                }
+                out[((size_t)z * oy + y) * ox + x] = r;
+            }
+}
+
+/* ------------------------------------------------------------------ phase correlation */
+static void orc_hann(float *w, int n) {                        /* phasecorr.c:36-40 */
+    if (n == 1) { w[0] = 1.0f; return; }
+    for (int i = 0; i < n; i++) w[i] = 0.5f - 0.5f * cosf(2.0f * (float)M_PI * i / (float)(n - 1));
+}
+
+/* Foroosh's estimator along one axis from the peak c0 and its circular
+ * neighbours cm, cp (phasecorr.c:150-176) */
+static double orc_foroosh(double c0, double cm, double cp) {
+    double c1 = cp >= cm ? cp : cm, sg = cp >= cm ? 1.0 : -1.0, d = 0.0;
+    if (fabs(c1 - c0) > 1e-20 && fabs(c1 + c0) > 1e-20) {
+        double d1 = c1 / (c1 - c0), d2 = c1 / (c1 + c0);
+        d = sg * (fabs(d1) <= fabs(d2) ? d1 : d2);
+    }
+    if (d > 0.5) d = 0.5;
+    if (d < -0.5) d = -0.5;
+    return d;
+}
+
+int orc_phase_correlate(const float *fixed, const float *moving, int nz, int ny, int nx,
+                        double *shift, double *peak, int window) {   /* phasecorr.c:44-186 */
+    if (!fixed || !moving || !shift || nz < 2 || ny < 2 || nx < 2) return 1;
+    int pz = orc_next_pow2(nz), py = orc_next_pow2(ny), px = orc_next_pow2(nx);
+    size_t pn = (size_t)pz * py * px, n = (size_t)nz * ny * nx;
+    float *fr = calloc(pn, sizeof(float)), *fi = calloc(pn, sizeof(float));
+    float *mr = calloc(pn, sizeof(float)), *mi = calloc(pn, sizeof(float));
+    float *wz = malloc(sizeof(float) * nz), *wy = malloc(sizeof(float) * ny), *wx = malloc(sizeof(float) * nx);
+    if (window) { orc_hann(wz, nz); orc_hann(wy, ny); orc_hann(wx, nx); }
+    else {
+        for (int i = 0; i < nz; i++) wz[i] = 1;
+        for (int i = 0; i < ny; i++) wy[i] = 1;
+        for (int i = 0; i < nx; i++) wx[i] = 1;
+    }
+    double sf = 0, sm = 0;
+    for (size_t i = 0; i < n; i++) { sf += fixed[i]; sm += moving[i]; }
+    float mf = (float)(sf / n), mm = (float)(sm / n);
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++) {
+            float wzy = wz[z] * wy[y];
+            for (int x = 0; x < nx; x++) {
+                float w = wzy * wx[x];
+                size_t i = ((size_t)z * ny + y) * nx + x, o = ((size_t)z * py + y) * px + x;
+                fr[o] = (fixed[i] - mf) * w;
+                mr[o] = (moving[i] - mm) * w;
+            }
+        }
+    free(wz); free(wy); free(wx);
+    orc_fft3d_f32(fr, fi, pz, py, px, -1);
+    orc_fft3d_f32(mr, mi, pz, py, px, -1);
+    double maxmag = 0.0;
+    for (size_t i = 0; i < pn; i++) {
+        float cr = fr[i] * mr[i] + fi[i] * mi[i], ci = fr[i] * mi[i] - fi[i] * mr[i];
+        float mag = sqrtf(cr * cr + ci * ci);
+        if (mag > maxmag) maxmag = mag;
+        if (mag > 1e-12f) { fr[i] = cr / mag; fi[i] = ci / mag; }
+        else { fr[i] = 0.0f; fi[i] = 0.0f; }
+    }
+    free(mr); free(mi);
+    if (maxmag < 1e-20) {
+        free(fr); free(fi);
+        shift[0] = shift[1] = shift[2] = 0.0;
+        if (peak) *peak = 0.0;
+        return 1;
+    }
+    orc_fft3d_f32(fr, fi, pz, py, px, 1);
+    size_t pk = 0;
+    float best = fr[0];
+    for (size_t i = 1; i < pn; i++) if (fr[i] > best) { best = fr[i]; pk = i; }
+    int kz = (int)(pk / ((size_t)py * px)), ky = (int)((pk / px) % py), kx = (int)(pk % px);
+#define ORC_S(Z, Y, X) fr[((size_t)(Z) * py + (Y)) * px + (X)]
+    double c0 = ORC_S(kz, ky, kx);
+    double dz = orc_foroosh(c0, ORC_S((kz - 1 + pz) % pz, ky, kx), ORC_S((kz + 1) % pz, ky, kx));
+    double dy = orc_foroosh(c0, ORC_S(kz, (ky - 1 + py) % py, kx), ORC_S(kz, (ky + 1) % py, kx));
+    double dx = orc_foroosh(c0, ORC_S(kz, ky, (kx - 1 + px) % px), ORC_S(kz, ky, (kx + 1) % px));
+#undef ORC_S
+    shift[0] = (kz <= pz / 2 ? kz : kz - pz) + dz;
+    shift[1] = (ky <= py / 2 ? ky : ky - py) + dy;
+    shift[2] = (kx <= px / 2 ? kx : kx - px) + dx;
+    if (peak) *peak = (double)best / (double)pn;
+    free(fr); free(fi);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ chain pieces */
 void orc_zdrift_apply(float *chunk, int nz, int ny, int nx, int z0, const float *factor) {
     size_t plane = (size_t)ny * nx;                              /* zdrift.c:64-71 */

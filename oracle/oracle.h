@@ -113,6 +113,14 @@ double orc_valley_depth(const long hist[256], int *dark_out, int *light_out, int
 void   orc_air_cc_despeckle(float *f, int nz, int ny, int nx, int min_size);
 void   orc_air_zero(float *f, const float *orig, int nz, int ny, int nx, const fy_cube_params *P);
 
+/* downscale.c 2x LOD (method 0 box, 1 contrast-maintaining box) */
+void   orc_downscale2x(const unsigned char *in, int nx, int ny, int nz, unsigned char *out,
+                       int *ox, int *oy, int *oz, int method, float alpha);
+
+/* phasecorr.c phase correlation (with the restated fft.c FFT) */
+int    orc_phase_correlate(const float *fixed, const float *moving, int nz, int ny, int nx,
+                           double *shift, double *peak, int window);
+
 /* zdrift.c apply; pipeline.c per-tile eps and the tile chain */
 void   orc_zdrift_apply(float *chunk, int nz, int ny, int nx, int z0, const float *factor);
 double orc_eps_for_tile(const fy_pipeline_cfg *cfg, double ecy, double ecx,

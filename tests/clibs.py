@@ -156,6 +156,10 @@ _SIGS = {
     "zdrift_finalize": (None, [c_double_p, C.POINTER(C.c_long), C.c_int, c_float_p]),
     "flat_noise": (C.c_double, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "valley_depth": (C.c_double, [C.POINTER(C.c_long), c_int_p, c_int_p, c_int_p]),
+    "phase_correlate": (C.c_int, [c_float_p, c_float_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p,
+                                  C.c_int]),
+    "downscale2x": (None, [c_u8_p, C.c_int, C.c_int, C.c_int, c_u8_p, c_int_p, c_int_p, c_int_p, C.c_int,
+                           C.c_float]),
     "process_buffer": (C.c_int, [C.POINTER(PipelineCfg), c_u8_p] + [C.c_long] * 14 +
                        [c_u8_p, c_int_p]),
 }

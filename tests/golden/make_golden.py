@@ -252,6 +252,51 @@ def main():
                                 C.byref(cfg)) == 0
     out["air_out_thin"] = o2
 
+    # ---- 2x LOD downscale (downscale.c:27-45)
+    drng = np.random.default_rng(31)
+    dv = drng.integers(0, 256, (9, 14, 17)).astype(np.uint8)
+    dv[:4, :6, :6] = 0                                                     # all-zero cells
+    dv[4:, 6:, 6:] = np.where(drng.random((5, 8, 11)) < 0.15, 250, 30)     # thin bright structure
+    out["ds_in"] = dv
+    for mth, al in ((0, 0.0), (1, 0.5), (1, 0.8)):
+        o = np.zeros((5, 7, 9), np.uint8)
+        ox, oy, oz = C.c_int(), C.c_int(), C.c_int()
+        ref.fy_downscale2x(clibs.up(dv), 17, 14, 9, clibs.up(o), C.byref(ox), C.byref(oy), C.byref(oz), mth,
+                           C.c_float(al))
+        assert (oz.value, oy.value, ox.value) == o.shape
+        out[f"ds_out_{mth}_{int(al * 10)}"] = o
+
+    # ---- phase correlation (phasecorr.c:44-186)
+    prng = np.random.default_rng(41)
+
+    def texture(shape):
+        t = prng.standard_normal(shape)
+        for ax in range(3):                                  # smooth a little: a textured patch
+            t = 0.5 * t + 0.25 * (np.roll(t, 1, ax) + np.roll(t, -1, ax))
+        return t
+
+    def fshift(v, s):
+        k = np.meshgrid(*[np.fft.fftfreq(n) for n in v.shape], indexing="ij")
+        ph = np.exp(-2j * np.pi * sum(ki * si for ki, si in zip(k, s)))
+        return np.real(np.fft.ifftn(np.fft.fftn(v) * ph))
+
+    pc_cases = []
+    a = texture((20, 24, 28)).astype(np.float32)
+    pc_cases.append((a, np.roll(a, (2, -3, 1), (0, 1, 2)).astype(np.float32), 0))
+    pc_cases.append((a, np.roll(a, (2, -3, 1), (0, 1, 2)).astype(np.float32), 1))
+    b = texture((32, 32, 32))
+    pc_cases.append((b.astype(np.float32), fshift(b, (0.3, -1.6, 2.25)).astype(np.float32), 0))
+    pc_cases.append((b.astype(np.float32), (2.0 * fshift(b, (-4.4, 0.7, 1.5)) + 5.0).astype(np.float32), 1))
+    c = texture((17, 9, 33)).astype(np.float32)
+    pc_cases.append((c, np.roll(c, (-5, 2, 7), (0, 1, 2)).astype(np.float32), 1))
+    pc_cases.append((np.full((8, 8, 8), 3.0, np.float32), texture((8, 8, 8)).astype(np.float32), 0))  # flat
+    for i, (fx_, mv, win) in enumerate(pc_cases):
+        sh = np.zeros(3)
+        pk = C.c_double(-1.0)
+        rc = ref.fy_phase_correlate(clibs.fp(fx_), clibs.fp(mv), *fx_.shape, clibs.dp(sh), C.byref(pk), win)
+        out[f"pc_fixed_{i}"], out[f"pc_moving_{i}"] = fx_, mv
+        out[f"pc_out_{i}"] = np.array([rc, win, sh[0], sh[1], sh[2], pk.value])
+
     # ---- per-tile chain (pipeline.c:467-510)
     sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
     from fysics_amd import phantom

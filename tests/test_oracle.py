@@ -643,3 +643,79 @@ def test_air_zero_thin_tile_golden(orc):
     P.air_zero, P.air_cut_u8 = 1, -1
     orc.orc_air_zero(clibs.fp(f), clibs.fp(np.ascontiguousarray(GOLD["air_orig"][:2])), 2, 34, 38, C.byref(P))
     assert np.array_equal(f, GOLD["air_out_thin"])
+
+
+# ------------------------------------------------------------------ LOD downscale
+def _ds_np(v, method, alpha):
+    nz, ny, nx = v.shape
+    oz, oy, ox = (nz + 1) // 2, (ny + 1) // 2, (nx + 1) // 2
+    pad = np.pad(v, ((0, 2 * oz - nz), (0, 2 * oy - ny), (0, 2 * ox - nx)), mode="edge")
+    c = pad.reshape(oz, 2, oy, 2, ox, 2).transpose(0, 2, 4, 1, 3, 5).reshape(oz, oy, ox, 8).astype(np.int32)
+    mean = c.sum(-1).astype(np.float32) * np.float32(0.125)
+    if method == 0:
+        v = mean
+    else:
+        dev = np.abs(c.astype(np.float32) - mean[..., None])
+        best = np.take_along_axis(c, dev.argmax(-1)[..., None], -1)[..., 0].astype(np.float32)
+        v = mean + np.float32(alpha) * (best - mean)
+    q = np.where(v < 0, 0, np.where(v > 255, 255, v + np.float32(0.5))).astype(np.uint8)
+    return np.where(c.max(-1) == 0, 0, q).astype(np.uint8)
+
+
+@pytest.mark.parametrize("mth,al", [(0, 0.0), (1, 0.5), (1, 0.8)])
+def test_downscale_golden_and_numpy(orc, mth, al):
+    dv = GOLD["ds_in"]
+    o = np.zeros((5, 7, 9), np.uint8)
+    ox, oy, oz = C.c_int(), C.c_int(), C.c_int()
+    orc.orc_downscale2x(clibs.up(dv), 17, 14, 9, clibs.up(o), C.byref(ox), C.byref(oy), C.byref(oz), mth,
+                        C.c_float(al))
+    assert (oz.value, oy.value, ox.value) == (5, 7, 9)
+    assert np.array_equal(o, GOLD[f"ds_out_{mth}_{int(al * 10)}"])
+    assert np.array_equal(o, _ds_np(dv, mth, al))
+
+
+# ------------------------------------------------------------------ phase correlation
+def _pc(lib, prefix, fx_, mv, win):
+    sh = np.zeros(3)
+    pk = C.c_double(-1.0)
+    rc = getattr(lib, prefix + "phase_correlate")(clibs.fp(fx_), clibs.fp(mv), *fx_.shape, clibs.dp(sh),
+                                                  C.byref(pk), win)
+    return rc, sh, pk.value
+
+
+# Unwindowed, zero-padded patches are ill-conditioned: the unit-magnitude
+# normalisation divides by cross-power magnitudes that are tiny where the padded
+# patch has little energy, so fp32 rounding (the -Ofast reference's or this
+# restatement's) moves the estimate by up to ~0.04 voxel around the fp64 value;
+# windowed cases agree to ~1e-4.
+PC_ATOL = {0: 5e-2, 1: 1e-4, 2: 5e-2, 3: 1e-4, 4: 1e-4, 5: 0.0}
+
+
+@pytest.mark.parametrize("i", range(6))
+def test_phase_correlate_golden(orc, i):
+    want = GOLD[f"pc_out_{i}"]
+    rc, sh, pk = _pc(orc, "orc_", GOLD[f"pc_fixed_{i}"], GOLD[f"pc_moving_{i}"], int(want[1]))
+    assert rc == want[0]
+    if rc == 0:
+        assert np.allclose(sh, want[2:5], rtol=0, atol=PC_ATOL[i])
+        assert abs(pk - want[5]) <= max(PC_ATOL[i], 1e-5) * want[5] * 10
+
+
+def test_phase_correlate_fourier_shift(orc):
+    """A band-limited sub-voxel shift of a periodic volume is recovered to the
+    estimator's accuracy, in the right direction on every axis."""
+    b = GOLD["pc_fixed_2"]
+    rc, sh, pk = _pc(orc, "orc_", b, GOLD["pc_moving_2"], 0)
+    assert rc == 0 and np.allclose(sh, (0.3, -1.6, 2.25), atol=0.05)
+
+
+def test_phase_correlate_circular_shift_exact(orc):
+    """A circular integer shift of a power-of-two volume makes the normalised
+    cross-power spectrum an exact phase ramp: the surface is a delta of height
+    pn at the shift (peak 1, no sub-voxel part)."""
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((16, 32, 8)).astype(np.float32)
+    for s in [(3, -5, 2), (0, 0, 0), (-8, 16, 4)]:
+        rc, sh, pk = _pc(orc, "orc_", a, np.roll(a, s, (0, 1, 2)).astype(np.float32), 0)
+        want = [((si + n // 2) % n) - n // 2 if ((si % n) > n // 2) else si % n for si, n in zip(s, a.shape)]
+        assert rc == 0 and np.allclose(sh, want, atol=1e-4) and abs(pk - 1.0) < 1e-4
