@@ -123,6 +123,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the 512^3 fp32 / fp64 timing")
     ap.add_argument("--no-calibration", action="store_true", help="skip the pass-1 statistics timing")
     ap.add_argument("--no-air-zero", action="store_true", help="skip the chain-with-air-zero timing")
+    ap.add_argument("--no-lod", action="store_true", help="skip the downscale / phase-correlation timing")
     ap.add_argument("--served-requests", type=int, default=2000)
     ap.add_argument("--served-rate", type=float, default=400.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -244,6 +245,7 @@ def main():
     precision = None if args.no_precision else precision_bench(F)
     calib = None if args.no_calibration else calibration_bench(F, vol)
     airz = None if args.no_air_zero else air_zero_bench(F, vol, eps)
+    lod = None if args.no_lod else lod_phasecorr_bench(F, vol)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -268,6 +270,7 @@ def main():
         "precision_512": precision,
         "calibration": calib,
         "air_zero": airz,
+        "lod_phasecorr": lod,
         "cpu_baseline": cpu,
         "workload_gen_s": round(gen_s, 1),
     }
@@ -557,6 +560,49 @@ def air_zero_bench(F, vol, eps, reps=10):
     return {"workload": f"{n}^3 u8 cube, the chain plus air-zero (valley rule) and despeckle (< 500 voxels)",
             "ms_per_cube": round(1000 * dt, 3), "mvox_s": round(n ** 3 / dt / 1e6, 1),
             "zeroed_fraction": round(zeroed, 4), "kernels_us": stage}
+
+
+def lod_phasecorr_bench(F, vol, reps=20):
+    """The adjacent consumers: the 2x LOD downscale of the workload cube
+    (contrast-maintaining box, device to device; algorithmic bytes N + N/8) and
+    phase correlation of its central 128^3 block against a copy shifted by
+    (3, -2, 5) voxels (windowed; two forward and one inverse 128^3 FFT)."""
+    import torch
+    lib = F.lib()
+    n = vol.shape[0]
+    d = torch.from_numpy(vol).cuda()
+    o = torch.empty((n + 1) // 2, (n + 1) // 2, (n + 1) // 2, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        lib.fy_gpu_downscale2x(C.c_void_p(d.data_ptr()), n, n, n, C.c_void_p(o.data_ptr()), 1, C.c_float(0.5), sh)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        lib.fy_gpu_downscale2x(C.c_void_p(d.data_ptr()), n, n, n, C.c_void_p(o.data_ptr()), 1, C.c_float(0.5), sh)
+    stream.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    nbytes = vol.size + o.numel()
+    h = n // 2 - 64
+    a = (d[h:h + 128, h:h + 128, h:h + 128].float() / 255.0).contiguous()
+    b = torch.roll(a, shifts=(3, -2, 5), dims=(0, 1, 2)).contiguous()
+    torch.cuda.synchronize()
+    s3 = np.zeros(3)
+    pk = C.c_double()
+    lib.fy_gpu_phase_correlate(C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), 128, 128, 128,
+                               s3.ctypes.data_as(F.c_double_p), C.byref(pk), 1, sh)
+    t1 = time.perf_counter()
+    for _ in range(5):
+        lib.fy_gpu_phase_correlate(C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), 128, 128, 128,
+                                   s3.ctypes.data_as(F.c_double_p), C.byref(pk), 1, sh)
+    pdt = (time.perf_counter() - t1) / 5
+    return {"downscale2x": {"workload": f"{n}^3 u8 -> {(n + 1) // 2}^3, contrast-maintaining box",
+                            "ms": round(1000 * dt, 4), "GBps": round(nbytes / dt / 1e9, 1),
+                            "frac_of_hbm_peak": round(nbytes / dt / 1e9 / HBM_PEAK_GBS, 4)},
+            "phase_correlate": {"workload": "128^3 patch vs itself rolled by (3, -2, 5), Hann window",
+                                "ms_per_call": round(1000 * pdt, 3), "shift": [round(v, 4) for v in s3],
+                                "peak": round(pk.value, 4)}}
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):
