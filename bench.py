@@ -299,7 +299,9 @@ def batched_bench(F, reps=5, seed=100):
     """The batched workload: 96 cubes (32x64^3, 32x96^3, 16x128^3, 12x160^3 -> 192^3
     FFT, 4x192x256x256; 169.7 M voxels), each with its own phantom and physics,
     u8 in HBM -> deconv + guided -> u8 in HBM.  Timed as one fy_gpu_process_cubes
-    call over all 96 (grouped launches) and, beside it, one call per cube."""
+    call over all 96 (grouped launches), as the replay of those launches captured
+    in a HIP graph (fy_gpu_graph_cubes / fy_graph_launch), and, beside them, one
+    call per cube."""
     import torch
     from fysics_amd import phantom
     lib = F.lib()
@@ -333,7 +335,15 @@ def batched_bench(F, reps=5, seed=100):
                        "per-cube phantom, delta/beta U[100,1000], pixel {2.4,4.3,7.9} um, sigma_z/sigma_xy "
                        "U[1,1.3]; u8 -> dewindow -> Wiener deconv -> guided r=2 s=2 -> dithered u8",
            "cubes": len(cubes), "voxels": int(nvox), "workload_gen_s": round(gen_s, 1)}
-    for name, fn in (("grouped", grouped), ("one_call_per_cube", one_by_one)):
+    graph = lib.fy_gpu_graph_cubes(arr, len(cubes), sh)
+    if not graph:
+        raise RuntimeError(lib.fy_gpu_last_error().decode())
+
+    def captured():
+        if lib.fy_graph_launch(C.c_void_p(graph)):
+            raise RuntimeError(lib.fy_gpu_last_error().decode())
+
+    for name, fn in (("grouped", grouped), ("graph", captured), ("one_call_per_cube", one_by_one)):
         fn()
         stream.synchronize()
         t1 = time.perf_counter()
@@ -342,6 +352,7 @@ def batched_bench(F, reps=5, seed=100):
         stream.synchronize()
         dt = (time.perf_counter() - t1) / reps
         out[name] = {"ms_per_batch": round(1000 * dt, 3), "mvox_s": round(nvox / dt / 1e6, 1)}
+    lib.fy_graph_destroy(C.c_void_p(graph))
     return out
 
 

@@ -195,6 +195,9 @@ SIGNATURES = {
     "fy_gpu_process_buffer": (C.c_int, [C.POINTER(PipelineCfg), _VP] + [_L] * 14 +
                               [_VP, c_int_p, _VP]),
     "fy_gpu_process_cubes": (C.c_int, [C.POINTER(Cube), C.c_int, _VP]),
+    "fy_gpu_graph_cubes": (_VP, [C.POINTER(Cube), C.c_int, _VP]),
+    "fy_graph_launch": (C.c_int, [_VP]),
+    "fy_graph_destroy": (None, [_VP]),
     "fy_cube_workspace_bytes": (C.c_size_t, [C.POINTER(Cube)]),
     "fy_serve_create": (_VP, [C.c_size_t, C.c_int, C.c_int, _VP]),
     "fy_serve_submit": (C.c_int, [_VP, C.POINTER(Cube), _L]),
@@ -209,6 +212,8 @@ SIGNATURES = {
     "fy_calib_reset": (C.c_int, [_VP]),
     "fy_calib_destroy": (None, [_VP]),
     "fy_gpu_flat_noise": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, c_double_p, _VP]),
+    "fy_gpu_tuning": (C.c_int, [C.c_int]),
+    "fy_gpu_tuning_report": (C.c_int, [C.c_char_p, C.c_int]),
     "fy_gpu_sync": (C.c_int, [_VP]),
     "fy_gpu_profile": (None, [C.c_int]),
     "fy_gpu_profile_read": (C.c_int, [C.c_char_p, C.c_int, c_double_p, C.POINTER(C.c_long),
@@ -338,3 +343,11 @@ def profile_read():
     n = lib().fy_gpu_profile_read(names, 8192, ms, cnt, 128)
     keys = names.raw.split(b"\0")
     return {keys[i].decode(): (ms[i], cnt[i]) for i in range(n)}
+
+
+def tuning_report():
+    """The fast guided filter's tuned tile shapes, one line per shape."""
+    n = lib().fy_gpu_tuning_report(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    lib().fy_gpu_tuning_report(buf, n + 1)
+    return buf.value.decode()

@@ -25,6 +25,17 @@ fyg_stream *fyg_thread_stream(void);
 /* wrap an external hipStream_t (NULL -> the thread stream) */
 fyg_stream *fyg_stream_wrap(void *hip_stream);
 void *fyg_stream_handle(fyg_stream *s);          /* the hipStream_t */
+/* a private stream object on hip_stream (NULL: the thread stream's), with its
+ * own workspaces and flags and tables from the process-wide cache, for work
+ * that is recorded into a graph */
+fyg_stream *fyg_stream_private(void *hip_stream);
+void  fyg_stream_private_free(fyg_stream *s);
+int   fyg_stream_persistent(fyg_stream *s);
+/* graph capture of the launches queued on s between begin and end */
+int   fyg_capture_begin(fyg_stream *s);
+int   fyg_capture_end(fyg_stream *s, void **exec);
+int   fyg_graph_launch(void *exec, fyg_stream *s);
+void  fyg_graph_free(void *exec);
 int   fyg_sync(fyg_stream *s);
 
 /* per-stream grow-only device workspace: slot in [0, FYG_WS_SLOTS) */
@@ -40,12 +51,14 @@ int  fyg_d2h(void *dst, const void *src, size_t bytes, fyg_stream *s);
 int  fyg_d2d(void *dst, const void *src, size_t bytes, fyg_stream *s);
 int  fyg_zero(void *dst, size_t bytes, fyg_stream *s);    /* fill kernel, graph-safe */
 int  fyg_last_error(void);                                /* sticky launch error check */
-/* per-stream all-air flag: returns a fresh generation number g >= 1 and the
- * device flag; kernels that see a nonzero input voxel store g, consumers test
- * (*flag == g), so no clearing launch is needed between calls */
-int  fyg_flag(fyg_stream *s, int **d_flag);
+/* per-stream all-air flags: enqueues the step of a device generation word g
+ * (>= 1, a one-thread kernel, so a captured graph steps it on every replay) and
+ * returns the flags and g's address; kernels that see a nonzero input voxel
+ * store g, consumers test (*flag == g), so flags are never cleared between
+ * calls.  Returns 0 on error. */
+int  fyg_flag(fyg_stream *s, int **d_flag, const int **d_gen);
 /* the same with n flags (one per cube of a batch), all tested against one g */
-int  fyg_flags(fyg_stream *s, int n, int **d_flags);
+int  fyg_flags(fyg_stream *s, int n, int **d_flags, const int **d_gen);
 const char *fyg_error_string(void);
 
 /* process-wide cache of small read-only device tables (twiddles, transfer
@@ -124,8 +137,8 @@ typedef struct {
     void *out;                       /* fp32 nz*ny*nx (double when f64), may alias a fp32 in */
     fyg_epilogue epi;
     void *spec;                      /* complex workspace pz*py*hxp float2 (double2 when f64) */
-    int *nonzero_flag;               /* optional: receives nonzero_gen if any input u8 is nonzero */
-    int nonzero_gen;
+    int *nonzero_flag;               /* optional: receives *nonzero_gen if any input u8 is nonzero */
+    const int *nonzero_gen;          /* device generation word of the call */
     int f64;                         /* all-FP64 variant: spectrum, weights and out in double */
 } fyg_deconv_args;
 int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s);
@@ -165,8 +178,8 @@ typedef struct {
     long gz0, gy0, gx0;              /* global origin of the buffer (dither coordinates) */
     int no_dither;
     int recenter; float rmid, rk;    /* pipeline.c:413-421 */
-    const int *nonzero_flag;         /* optional: unless it holds nonzero_gen the tile is all air -> zeros */
-    int nonzero_gen;
+    const int *nonzero_flag;         /* optional: unless it holds *nonzero_gen the tile is all air -> zeros */
+    const int *nonzero_gen;
 } fyg_emit;
 
 typedef struct {
@@ -181,11 +194,16 @@ typedef struct {
 size_t fyg_guided_ws_bytes(int nz, int ny, int nx, int s);
 size_t fyg_guided_ws_bytes64(int nz, int ny, int nx, int s);
 int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s);
+/* tile-shape tuner of the fast guided filter: mode 0 first candidates, 1 tune
+ * on first use (default), 2 + k force candidate k; returns the previous mode */
+int fyg_tune_mode(int mode);
+/* one line per tuned shape; returns the report's length */
+int fyg_tune_report(char *buf, int len);
 /* n guided filters; cubes of one shape on the fast path share launches */
 int fyg_guided_run_batch(const fyg_guided_args *a, int n, fyg_stream *s);
 /* prologue-only path (deconv disabled): u8 -> f32 through the tile prologue */
 int fyg_prologue_run(const unsigned char *in, float *out, int nz, int ny, int nx,
-                     const fyg_prologue *pro, int *nonzero_flag, int gen, fyg_stream *s);
+                     const fyg_prologue *pro, int *nonzero_flag, const int *gen, fyg_stream *s);
 /* emit without guided filtering: recenter + quantise the inner tile */
 int fyg_emit_run(const float *f, int nz, int ny, int nx, const fyg_emit *em, fyg_stream *s);
 int fyg_box_mean_run(const float *in, float *out, int nz, int ny, int nx, int r,

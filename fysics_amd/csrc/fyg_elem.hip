@@ -66,7 +66,7 @@ __global__ void k_dering_apply(float *f, long dz, long dy, long dx, fyg_prologue
 }
 
 __global__ void k_prologue(const unsigned char *in, float *out, int nz, int ny, int nx, fyg_prologue P,
-                           int *flag, int gen) {
+                           int *flag, const int *gen) {
     #pragma clang fp contract(off)
     size_t n = (size_t)nz * ny * nx;
     int any = 0;
@@ -82,7 +82,7 @@ __global__ void k_prologue(const unsigned char *in, float *out, int nz, int ny, 
         if (P.window) f = P.wlo + f * P.wspan;
         out[i] = f;
     }
-    if (flag && __syncthreads_or(any) && threadIdx.x == 0) *flag = gen;
+    if (flag && __syncthreads_or(any) && threadIdx.x == 0) *flag = *gen;
 }
 
 // Sector/radius histogram of nonzero u8 samples on the ss-subsampled (y,x) grid.
@@ -258,7 +258,7 @@ extern "C" int fyg_dering_apply_run(float *f, long dz, long dy, long dx, const f
 }
 
 extern "C" int fyg_prologue_run(const unsigned char *in, float *out, int nz, int ny, int nx,
-                                const fyg_prologue *pro, int *flag, int gen, fyg_stream *s) {
+                                const fyg_prologue *pro, int *flag, const int *gen, fyg_stream *s) {
     size_t n = (size_t)nz * ny * nx;
     if (!n) return 0;
     fyg_launch(s, "prologue", k_prologue, dim3(grid1d(n)), dim3(NT), 0, in, out, nz, ny, nx, *pro, flag, gen);

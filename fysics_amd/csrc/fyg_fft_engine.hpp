@@ -427,7 +427,7 @@ struct XFwd {
     const V *tw;
     fyg_prologue pro;
     int *flag;
-    int gen;
+    const int *gen;
 };
 
 // per-line constants of a row-pair line, computed once per workgroup
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(256) void k_rfft_x(Batch<XFwd<P>> bt) {
     run_from<L, NL, false, false, 0>((P)-1, tw_s, buf, first, LdsWr<L, NL, V>{buf});
     __syncthreads();
     if (U8 && a.flag) {
-        if (__syncthreads_or(any) && threadIdx.x == 0) *a.flag = a.gen;
+        if (__syncthreads_or(any) && threadIdx.x == 0) *a.flag = *a.gen;
     }
     // untangle A(k) = (X(k) + conj X(-k))/2, B(k) = (X(k) - conj X(-k))/(2i)
     const int dl = Shape<L>::T / a.hxp, dk = Shape<L>::T - dl * a.hxp;

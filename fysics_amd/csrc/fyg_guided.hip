@@ -15,7 +15,11 @@
 // Exact path (guided.c:113-137):   k_moments(I) -> a,b ; k_box2(a,b) -> out = ma*I + mb
 // Fast path  (guided.c:152-231):   k_moments(I decimated by s) -> a,b (low-res);
 //                                  k_box2 -> ma, mb (low-res); k_upsample -> out
+#include <atomic>
 #include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
 
 #include <vector>
 
@@ -190,7 +194,7 @@ __device__ __forceinline__ unsigned char emit_q(const fyg_emit &E, bool keep, lo
 }
 
 __device__ __forceinline__ bool emit_keep(const fyg_emit &E) {
-    return !E.nonzero_flag || *E.nonzero_flag == E.nonzero_gen;
+    return !E.nonzero_flag || *E.nonzero_flag == *E.nonzero_gen;
 }
 
 // write one filtered voxel (local coords z,y,x of an nz*ny*nx buffer)
@@ -306,18 +310,14 @@ __global__ __launch_bounds__(NT) void k_box2(Box2Args p) {
 // The halo costs 1.3x in samples instead of the 3.4x of a 3-D tile and a and b
 // never reach HBM.  Cells outside the grid contribute 0 and windows shrink at
 // the faces (guided.c:22-98).
-template <int RHO>
+template <int RHO, int ZC_>
 struct March {
     static constexpr int AW = 16;                   // a, b columns per side (NT = AW*AW)
     static constexpr int OW = AW - 2 * RHO;         // output columns per side
     static constexpr int SW = AW + 2 * RHO;         // sample columns per side
     static constexpr int NS = SW * SW;
     static constexpr int KS = (NS + NT - 1) / NT;   // sample slots per thread
-#ifdef FYG_COEF_ZC
-    static constexpr int ZC = FYG_COEF_ZC;
-#else
-    static constexpr int ZC = RHO == 1 ? 8 : 12;    // output planes per workgroup
-#endif
+    static constexpr int ZC = ZC_;                  // output planes per workgroup
     static constexpr int Q = 2 * RHO + 1;           // window length
     static constexpr int STEPS = ZC + 4 * RHO;      // sample planes of a full chunk
     // sample planes in flight + 1; divides STEPS so the buffer ring rotates by
@@ -349,9 +349,9 @@ __device__ __forceinline__ int xcd_tile(int h, int T) {
 }
 
 // grid: (x blocks, y blocks, cube * nzc + z chunk), renumbered by xcd_tile
-template <int RHO, class P>
+template <int RHO, class P, int ZC>
 __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
-    using M = March<RHO>;
+    using M = March<RHO, ZC>;
     const int T = gridDim.x * gridDim.y * gridDim.z;
     const int L = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), T);
     const int tx = L % gridDim.x, ty = (L / gridDim.x) % gridDim.y, tz = L / (gridDim.x * gridDim.y);
@@ -365,7 +365,9 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
     const int bx = tx * M::OW, by = ty * M::OW, zs = zchunk * M::ZC;
     const int ze = min(zs + M::ZC, g.nz);
     const size_t sz = (size_t)g.s * g.fny * g.fnx, sy = (size_t)g.s * g.fnx;
-    const P c0 = p.in[(size_t)zs * sz + (size_t)by * sy + (size_t)bx * g.s];
+    // samples are centred on one sample of the column block (plane 0, so the
+    // result does not depend on the z chunking)
+    const P c0 = p.in[(size_t)by * sy + (size_t)bx * g.s];
     const P epsf = (P)p.eps;
 
     // sample slots: in-plane source offset (0 outside the grid) and LDS cell
@@ -398,7 +400,9 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
         for (int k = 0; k < M::KS; k++) dst[k] = p.in[zo + soff[k]];
     };
     // u: the step's position in the unrolled loop body; the window queues are
-    // rings whose newest entry is slot u % Q (steps before zs push zeros)
+    // rings whose newest entry is slot u % Q (steps before zs push zeros).  The
+    // window sums run oldest to newest plane, so a column's arithmetic does not
+    // depend on where its z chunk starts (nor on the chunk length).
     auto step = [&](int zp, const P *cur, int u) {
         const int slot = u % M::Q;
         const bool zin = zp >= 0 && zp < g.nz;
@@ -428,7 +432,7 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
             if (colin && za >= 0 && za < g.nz) {
                 P t1 = 0.0f, t2 = 0.0f;
 #pragma unroll
-                for (int j = 0; j < M::Q; j++) { t1 += q1[j]; t2 += q2[j]; }
+                for (int j = 1; j <= M::Q; j++) { t1 += q1[(slot + j) % M::Q]; t2 += q2[(slot + j) % M::Q]; }
                 P rc = rcxy * count_recip_tab<P>(za, g.nz, RHO);
                 P m = t1 * rc, q = t2 * rc;
                 P var = q - m * m;
@@ -461,7 +465,7 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
             if (inner && zo >= zs && zo < ze) {
                 P t1 = 0.0f, t2 = 0.0f;
 #pragma unroll
-                for (int j = 0; j < M::Q; j++) { t1 += qa[j]; t2 += qb[j]; }
+                for (int j = 1; j <= M::Q; j++) { t1 += qa[(slot + j) % M::Q]; t2 += qb[(slot + j) % M::Q]; }
                 P rc = rcxy * count_recip_tab<P>(zo, g.nz, RHO);
                 size_t o = ((size_t)zo * g.ny + gy) * g.nx + gx;
                 p.ma[o] = t1 * rc;
@@ -513,12 +517,13 @@ struct UpArgs {
 
 constexpr int UPX = 4;               // consecutive x outputs per thread
 constexpr int UPK = 64 * UPX / 2 + 4;  // low-res columns one 256-wide chunk touches (s >= 2)
-#ifndef FYG_UP_RZ
-#define FYG_UP_RZ 2
-#endif
-constexpr int URZ = FYG_UP_RZ, URY = 8;  // output planes x rows of one workgroup
-constexpr int RPT = URZ * URY / 4;       // rows per thread (4 row groups)
-constexpr int UPZ = URZ / 2 + 2, UPY = 6;  // low-res planes x rows they touch (s >= 2)
+constexpr int URY = 8, UPY = 6;          // output rows of one workgroup; low-res rows they touch (s >= 2)
+// URZ output planes per workgroup: RPT rows per thread (4 row groups), and the
+// UPZ low-res planes they touch (s >= 2)
+template <int URZ>
+struct UpShape {
+    static constexpr int RPT = URZ * URY / 4, UPZ = URZ / 2 + 2;
+};
 // patch rows store even columns first, then odd ones: a wave's reads of
 // columns 2q+e (s = 2) then hit consecutive words instead of every other one
 constexpr int UPH = UPK / 2;
@@ -543,9 +548,10 @@ struct UpCoord {
 };
 
 // grid: (x chunks, y blocks, cube * nzb + z block)
-template <bool P2, class P>
+template <bool P2, class P, int URZ>
 __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
     const UpArgs<P> &p = bt.a[blockIdx.z / nzb];
+    constexpr int RPT = UpShape<URZ>::RPT, UPZ = UpShape<URZ>::UPZ;
     __shared__ P pa[UPZ][UPY][UPK], pb[UPZ][UPY][UPK];
     const UpCoord<P2, P> U{(P)p.s, (P)1 / (P)p.s};
     const int Z0 = p.z0 + (int)(blockIdx.z % nzb) * URZ, Y0 = p.y0 + blockIdx.y * URY;
@@ -692,9 +698,9 @@ __global__ __launch_bounds__(NT) void k_emit(EmitArgs p) {
     }
 }
 
-template <int RHO>
+template <int RHO, int ZC>
 dim3 march_grid(const Grid &g) {
-    using M = March<RHO>;
+    using M = March<RHO, ZC>;
     return dim3((unsigned)((g.nx + M::OW - 1) / M::OW), (unsigned)((g.ny + M::OW - 1) / M::OW),
                 (unsigned)((g.nz + M::ZC - 1) / M::ZC));
 }
@@ -858,21 +864,115 @@ static UpArgs<P> up_args(const fyg_guided_args *a, const P *MA, const P *MB) {
     return u;
 }
 
+// ---- tile-shape autotuning.  The candidates change only how the work is cut
+// into workgroups -- the coefficient march's z chunk, the upsample's planes per
+// workgroup -- never the arithmetic of an output voxel, so every choice gives
+// bit-identical results and the tuner may pick by speed alone.  The first time a
+// (precision, radius, decimation, grid, region) shape is filtered, each
+// candidate runs once on the real data, timed with events on the launch stream,
+// and the fastest is kept for the process.
+template <int RHO>
+struct CoefCand;
+template <>
+struct CoefCand<1> {
+    static constexpr int z[3] = {8, 14, 20};
+};
+template <>
+struct CoefCand<2> {
+    static constexpr int z[3] = {12, 7, 22};
+};
+constexpr int kNCoef = 3, kNUp = 2;
+constexpr int kUpRZ[kNUp] = {2, 4};
+
+struct TuneRec {
+    int coef = 0, up = 0;
+    bool coef_done = false, up_done = false;
+    float coef_us[kNCoef] = {}, up_us[kNUp] = {};
+};
+
+// [p, p + n) and [q, q + m) share a byte
+static bool overlaps(const void *p, size_t n, const void *q, size_t m) {
+    const uintptr_t a = (uintptr_t)p, b = (uintptr_t)q;
+    return n && m && a < b + m && b < a + n;
+}
+
+// the upsample candidates of a group may only be raced against each other when
+// no filter's output overlaps any filter's guide: each candidate rereads the
+// guides and rewrites the outputs
+template <class P>
+static bool up_rerunnable(const UpArgs<P> *u, int m) {
+    for (int j = 0; j < m; j++) {
+        const fyg_emit &e = u[j].em;
+        const void *o = e.quant ? (const void *)e.q : (const void *)e.out;
+        const size_t on = e.quant ? (size_t)e.tz * e.ty * e.tx : sizeof(float) * u[j].nz * u[j].ny * u[j].nx;
+        for (int k = 0; k < m; k++)
+            if (overlaps(o, on, u[k].in, sizeof(P) * u[k].nz * u[k].ny * u[k].nx)) return false;
+    }
+    return true;
+}
+static std::mutex g_tune_mu;
+static std::map<std::string, TuneRec> g_tune;
+static std::atomic<int> g_tune_mode{1};   // 0: first candidates, 1: autotune, 2 + k: candidate k
+
+template <class F>
+static float timed(fyg_stream *s, F launch) {
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 1e30f;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return 1e30f; }
+    (void)hipEventRecord(e0, s->s);
+    launch();
+    (void)hipEventRecord(e1, s->s);
+    float ms = 1e30f;
+    if (hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 1000.0f * ms;
+}
+
+template <int RHO, class P, int ZC>
+static void coef_go(fyg_stream *s, const Batch<CoefArgs<P>> &b, int m, const Grid &gl) {
+    dim3 g = march_grid<RHO, ZC>(gl);
+    int nzc = (int)g.z;
+    g.z *= m;
+    fyg_launch(s, "guided_coef", k_coef<RHO, P, ZC>, g, dim3(NT), 0, b, nzc);
+}
+
+template <int RHO, class P>
+static void launch_coef(fyg_stream *s, int v, const Batch<CoefArgs<P>> &b, int m, const Grid &gl) {
+    switch (v) {
+    case 1: coef_go<RHO, P, CoefCand<RHO>::z[1]>(s, b, m, gl); break;
+    case 2: coef_go<RHO, P, CoefCand<RHO>::z[2]>(s, b, m, gl); break;
+    default: coef_go<RHO, P, CoefCand<RHO>::z[0]>(s, b, m, gl); break;
+    }
+}
+
+template <class P, int URZ>
+static void up_go(fyg_stream *s, const Batch<UpArgs<P>> &b, int m, int sd) {
+    const UpArgs<P> &u = b.a[0];
+    const int nzb = (u.dz + URZ - 1) / URZ;
+    dim3 grid((unsigned)((u.dx + 64 * UPX - 1) / (64 * UPX)), (unsigned)((u.dy + URY - 1) / URY),
+              (unsigned)(nzb * m));
+    if ((sd & (sd - 1)) == 0)
+        fyg_launch(s, "guided_upsample", k_upsample<true, P, URZ>, grid, dim3(NT), 0, b, nzb);
+    else
+        fyg_launch(s, "guided_upsample", k_upsample<false, P, URZ>, grid, dim3(NT), 0, b, nzb);
+}
+
+template <class P>
+static void launch_up_group(fyg_stream *s, int v, const Batch<UpArgs<P>> &b, int m, int sd) {
+    if (v == 1) up_go<P, kUpRZ[1]>(s, b, m, sd);
+    else up_go<P, kUpRZ[0]>(s, b, m, sd);
+}
+
 // upsample launches for n filters whose output regions have one shape
 template <class P>
-static void launch_upsample(fyg_stream *s, const UpArgs<P> *u, int n, int sd) {
+static void launch_upsample(fyg_stream *s, const UpArgs<P> *u, int n, int sd, int v = 0, int from = 0) {
     if (n < 1 || (size_t)u[0].dz * u[0].dy * u[0].dx == 0) return;
-    const int nzb = (u[0].dz + URZ - 1) / URZ;
-    for (int i = 0; i < n; i += FYG_BATCH) {
+    for (int i = from; i < n; i += FYG_BATCH) {
         int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
         Batch<UpArgs<P>> b{};
         for (int j = 0; j < m; j++) b.a[j] = u[i + j];
-        dim3 grid((unsigned)((u[0].dx + 64 * UPX - 1) / (64 * UPX)), (unsigned)((u[0].dy + URY - 1) / URY),
-                  (unsigned)(nzb * m));
-        if ((sd & (sd - 1)) == 0)
-            fyg_launch(s, "guided_upsample", k_upsample<true, P>, grid, dim3(NT), 0, b, nzb);
-        else
-            fyg_launch(s, "guided_upsample", k_upsample<false, P>, grid, dim3(NT), 0, b, nzb);
+        launch_up_group(s, v, b, m, sd);
     }
 }
 
@@ -962,24 +1062,90 @@ static int fast_batch(const fyg_guided_args *a, int n, fyg_stream *s) {
         c[i] = CoefArgs<P>{gl, guide<P>(&a[i]), a[i].eps, MA, MB};
         u[i] = up_args<P>(&a[i], MA, MB);
     }
+    // tile-shape choice: cached, tuned on first use, or forced
+    char kb[160];
+    snprintf(kb, sizeof kb, "%s r%d s%d grid %dx%dx%d region %dx%dx%d %s", sizeof(P) == 8 ? "fp64" : "fp32", rl,
+             sd, lz, ly, lx, u[0].dz, u[0].dy, u[0].dx, u[0].em.quant ? "u8" : "fp32");
+    const std::string key(kb);
+    const int mode = g_tune_mode.load();
+    const int m0 = n < FYG_BATCH ? n : FYG_BATCH;
+    TuneRec rec;
+    bool tune_coef = false, tune_up = false;
+    if (mode == 1) {
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) rec = it->second;
+        }
+        tune_coef = !rec.coef_done && !s->capturing;
+        tune_up = !rec.up_done && !s->capturing && up_rerunnable(u.data(), m0);
+    } else if (mode >= 2) {
+        rec.coef = mode - 2 < kNCoef ? mode - 2 : kNCoef - 1;
+        rec.up = mode - 2 < kNUp ? mode - 2 : kNUp - 1;
+    }
     for (int i = 0; i < n; i += FYG_BATCH) {
         int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
         Batch<CoefArgs<P>> b{};
         for (int j = 0; j < m; j++) b.a[j] = c[i + j];
-        if (rl == 1) {
-            dim3 g = march_grid<1>(gl);
-            int nzc = (int)g.z;
-            g.z *= m;
-            fyg_launch(s, "guided_coef", k_coef<1, P>, g, dim3(NT), 0, b, nzc);
+        auto go = [&](int v) {
+            if (rl == 1) launch_coef<1, P>(s, v, b, m, gl);
+            else launch_coef<2, P>(s, v, b, m, gl);
+        };
+        if (tune_coef && i == 0) {             // every candidate writes the same coefficients
+            int best = 0;
+            for (int v = 0; v < kNCoef; v++) {
+                rec.coef_us[v] = timed(s, [&] { go(v); });
+                if (rec.coef_us[v] < rec.coef_us[best]) best = v;
+            }
+            rec.coef = best;
+            rec.coef_done = true;
         } else {
-            dim3 g = march_grid<2>(gl);
-            int nzc = (int)g.z;
-            g.z *= m;
-            fyg_launch(s, "guided_coef", k_coef<2, P>, g, dim3(NT), 0, b, nzc);
+            go(rec.coef);
         }
     }
-    launch_upsample(s, u.data(), n, sd);
+    int from = 0;
+    if (tune_up) {                              // the same for the upsample's first group
+        Batch<UpArgs<P>> b{};
+        for (int j = 0; j < m0; j++) b.a[j] = u[j];
+        int best = 0;
+        for (int v = 0; v < kNUp; v++) {
+            rec.up_us[v] = timed(s, [&] { launch_up_group(s, v, b, m0, sd); });
+            if (rec.up_us[v] < rec.up_us[best]) best = v;
+        }
+        rec.up = best;
+        rec.up_done = true;
+        from = FYG_BATCH;
+    }
+    if (tune_coef || tune_up) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = rec;
+    }
+    launch_upsample(s, u.data(), n, sd, rec.up, from);
     return fyg_last_error();
+}
+
+extern "C" int fyg_tune_mode(int mode) { return g_tune_mode.exchange(mode); }
+
+// one line per tuned shape: key, chosen candidates and their times (us)
+extern "C" int fyg_tune_report(char *buf, int len) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::string out;
+    char line[320];
+    for (auto &kv : g_tune) {
+        const TuneRec &r = kv.second;
+        if (r.up_done)
+            snprintf(line, sizeof line, "%s: coef zchunk #%d (%.1f %.1f %.1f us), upsample planes #%d (%.1f %.1f us)\n",
+                     kv.first.c_str(), r.coef, r.coef_us[0], r.coef_us[1], r.coef_us[2], r.up, r.up_us[0],
+                     r.up_us[1]);
+        else
+            snprintf(line, sizeof line, "%s: coef zchunk #%d (%.1f %.1f %.1f us), upsample planes #0 (untuned: "
+                     "outputs overlap guides)\n", kv.first.c_str(), r.coef, r.coef_us[0], r.coef_us[1], r.coef_us[2]);
+        out += line;
+    }
+    if (buf && len > 0) {
+        snprintf(buf, (size_t)len, "%s", out.c_str());
+    }
+    return (int)out.size();
 }
 
 extern "C" int fyg_guided_run_batch(const fyg_guided_args *a, int n, fyg_stream *s) {

@@ -13,9 +13,10 @@ struct fyg_stream {
     size_t wcap[FYG_WS_SLOTS] = {};
     void *pin[2] = {};
     size_t pcap[2] = {};
-    int *flag = nullptr;     // all-air flags: hold the generation of the last call that saw data
+    int *flag = nullptr;     // [0] the device generation word, then the all-air flags
     int nflag = 0;
-    int gen = 0;
+    int persist = 0;         // tables from the process-wide cache (stable addresses for graphs)
+    int capturing = 0;       // launches are being recorded into a graph: no timing events
     struct Arena *arena = nullptr;   // per-stream table slots (fyg_stream_table)
 };
 

@@ -61,6 +61,57 @@ extern "C" fyg_stream *fyg_stream_wrap(void *hip_stream) {
 
 extern "C" void *fyg_stream_handle(fyg_stream *s) { return s ? (void *)s->s : nullptr; }
 
+// a stream object of its own (workspaces, flags) on an existing hipStream_t, for
+// work recorded into a graph: nothing else can grow or free the buffers the
+// graph's kernels address.  Tables come from the process-wide cache.
+extern "C" fyg_stream *fyg_stream_private(void *hip_stream) {
+    fyg_stream *base = hip_stream ? nullptr : fyg_thread_stream();
+    if (!hip_stream && !base) return nullptr;
+    fyg_stream *st = new fyg_stream();
+    st->s = hip_stream ? (hipStream_t)hip_stream : base->s;
+    st->persist = 1;
+    return st;
+}
+
+extern "C" void fyg_stream_private_free(fyg_stream *st) {
+    if (!st) return;
+    (void)hipStreamSynchronize(st->s);
+    for (int i = 0; i < FYG_WS_SLOTS; i++)
+        if (st->ws[i]) (void)hipFree(st->ws[i]);
+    for (int i = 0; i < 2; i++)
+        if (st->pin[i]) (void)hipHostFree(st->pin[i]);
+    if (st->flag) (void)hipFree(st->flag);
+    delete st;
+}
+
+extern "C" int fyg_stream_persistent(fyg_stream *s) { return s && s->persist; }
+
+extern "C" int fyg_capture_begin(fyg_stream *s) {
+    if (note(hipStreamBeginCapture(s->s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return 1;
+    s->capturing = 1;
+    return 0;
+}
+
+extern "C" int fyg_capture_end(fyg_stream *s, void **exec) {
+    s->capturing = 0;
+    hipGraph_t g = nullptr;
+    if (note(hipStreamEndCapture(s->s, &g), "hipStreamEndCapture")) return 1;
+    hipGraphExec_t e = nullptr;
+    int rc = note(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    (void)hipGraphDestroy(g);
+    if (rc) return 1;
+    *exec = (void *)e;
+    return 0;
+}
+
+extern "C" int fyg_graph_launch(void *exec, fyg_stream *s) {
+    return note(hipGraphLaunch((hipGraphExec_t)exec, s->s), "hipGraphLaunch");
+}
+
+extern "C" void fyg_graph_free(void *exec) {
+    if (exec) (void)hipGraphExecDestroy((hipGraphExec_t)exec);
+}
+
 extern "C" int fyg_sync(fyg_stream *s) {
     if (!s) return 1;
     return note(hipStreamSynchronize(s->s), "hipStreamSynchronize");
@@ -137,7 +188,20 @@ extern "C" int fyg_zero(void *dst, size_t bytes, fyg_stream *s) {
     return fyg_last_error();
 }
 
-extern "C" int fyg_flags(fyg_stream *s, int n, int **d_flags) {
+// gen <- gen + 1; on reaching the wrap value the flags are cleared first, so
+// no stale generation can match a new one
+__global__ void k_gen_step(int *gen, int *flags, int nflag) {
+    __shared__ int g;
+    if (threadIdx.x == 0) g = *gen;
+    __syncthreads();
+    const bool wrap = g >= 0x7ffffff0;
+    if (wrap)
+        for (int i = threadIdx.x; i < nflag; i += blockDim.x) flags[i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) *gen = wrap ? 1 : g + 1;
+}
+
+extern "C" int fyg_flags(fyg_stream *s, int n, int **d_flags, const int **d_gen) {
     if (!s || n < 1) return 0;
     if (n > s->nflag) {
         // earlier generations may still be tested by queued work on this stream
@@ -148,19 +212,18 @@ extern "C" int fyg_flags(fyg_stream *s, int n, int **d_flags) {
             s->nflag = 0;
         }
         int cap = n < 16 ? 16 : n;
-        if (note(hipMalloc((void **)&s->flag, sizeof(int) * cap), "hipMalloc(flag)")) return 0;
-        if (fyg_zero(s->flag, sizeof(int) * cap, s)) return 0;   // zero never equals a generation
+        if (note(hipMalloc((void **)&s->flag, sizeof(int) * (cap + 1)), "hipMalloc(flag)")) return 0;
+        if (fyg_zero(s->flag, sizeof(int) * (cap + 1), s)) return 0;   // zero never equals a generation
         s->nflag = cap;
     }
-    if (s->gen >= 0x7ffffff0) {              // wrap: clear so no stale generation can match
-        if (fyg_zero(s->flag, sizeof(int) * s->nflag, s)) return 0;
-        s->gen = 0;
-    }
-    *d_flags = s->flag;
-    return ++s->gen;
+    fyg_launch(s, "gen_step", k_gen_step, dim3(1), dim3(256), 0, s->flag, s->flag + 1, s->nflag);
+    if (fyg_last_error()) return 0;
+    *d_flags = s->flag + 1;
+    *d_gen = s->flag;
+    return 1;
 }
 
-extern "C" int fyg_flag(fyg_stream *s, int **d_flag) { return fyg_flags(s, 1, d_flag); }
+extern "C" int fyg_flag(fyg_stream *s, int **d_flag, const int **d_gen) { return fyg_flags(s, 1, d_flag, d_gen); }
 
 extern "C" int fyg_last_error(void) {
     return note(hipGetLastError(), "kernel launch");
@@ -336,7 +399,7 @@ std::vector<std::pair<std::string, Acc>> acc;
 
 void begin(fyg_stream *s, const char *name, void **tok) {
     *tok = nullptr;
-    if (!enabled.load(std::memory_order_relaxed)) return;
+    if (!enabled.load(std::memory_order_relaxed) || s->capturing) return;
     Rec *r = new Rec{name, nullptr, nullptr};
     (void)hipEventCreate(&r->a);
     (void)hipEventCreate(&r->b);

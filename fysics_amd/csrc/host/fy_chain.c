@@ -253,7 +253,7 @@ static int same_group(const cube_plan *a, const cube_plan *b) {
 }
 
 /* the chain of n planned cubes of one group; flags[i] is cube i's all-air flag */
-static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) {
+static int run_group(cube_plan *pl, int n, int **flags, const int *gen, fyg_stream *s) {
     const cube_plan *h = &pl[0];
     size_t tile = al256(h->tile_bytes), spec = al256(h->spec_bytes), img = 0, gws = al256(h->gws_bytes);
     for (int i = 0; i < n; i++)                                 /* ring images may span more slabs */
@@ -347,7 +347,7 @@ static int run_group(cube_plan *pl, int n, int **flags, int gen, fyg_stream *s) 
  * stage set per group, in order of first appearance), each group in as few
  * launches per stage as FYG_BATCH and FYH_GROUP_BYTES allow.  Results do not
  * depend on the grouping: every cube has its own workspace slice and flag. */
-static int run_chunk(const fy_cube *cubes, int n, int *d_flags, int gen, fyg_stream *s) {
+static int run_chunk(const fy_cube *cubes, int n, int *d_flags, const int *gen, fyg_stream *s) {
     cube_plan *pl = (cube_plan *)malloc(sizeof *pl * (size_t)n);
     cube_plan *grp = (cube_plan *)malloc(sizeof *grp * (size_t)n);
     int *gi = (int *)malloc(sizeof(int) * (size_t)n);
@@ -379,13 +379,13 @@ static int run_chunk(const fy_cube *cubes, int n, int *d_flags, int gen, fyg_str
     return rc;
 }
 
-static int run_cubes(const fy_cube *cubes, int n, int *d_flags, int gen, fyg_stream *s) {
+static int run_cubes(const fy_cube *cubes, int n, int *d_flags, const int *gen, fyg_stream *s) {
     for (int i = 0; i < n; i += FYH_PLAN_CHUNK)
         if (run_chunk(cubes + i, n - i < FYH_PLAN_CHUNK ? n - i : FYH_PLAN_CHUNK, d_flags + i, gen, s)) return 1;
     return 0;
 }
 
-static int chain_cube(const fy_cube *c, int *d_flag, int gen, fyg_stream *s) {
+static int chain_cube(const fy_cube *c, int *d_flag, const int *gen, fyg_stream *s) {
     return run_cubes(c, 1, d_flag, gen, s);
 }
 
@@ -419,14 +419,57 @@ int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream) {
     fyg_stream *s = fyg_stream_wrap(stream);
     if (!s) return 1;
     int *dflags = NULL;
-    int gen = fyg_flags(s, n, &dflags);
-    if (!gen) return 1;
-    return run_cubes(cubes, n, dflags, gen, s);
+    const int *dgen = NULL;
+    if (!fyg_flags(s, n, &dflags, &dgen)) return 1;
+    return run_cubes(cubes, n, dflags, dgen, s);
+}
+
+/* ---- captured batched form: the launches of one fy_gpu_process_cubes call
+ * recorded once into a HIP graph and replayed.  The graph owns a private
+ * stream object, so its workspaces and flags are never regrown or shared, and
+ * its tables live in the process-wide cache. */
+struct fy_graph {
+    fyg_stream *s;
+    void *exec;
+};
+
+fy_graph *fy_gpu_graph_cubes(const fy_cube *cubes, int n, void *stream) {
+    if (n <= 0 || !cubes) return NULL;
+    fy_graph *g = (fy_graph *)calloc(1, sizeof *g);
+    if (!g) return NULL;
+    g->s = fyg_stream_private(stream);
+    int *dflags = NULL;
+    const int *dgen = NULL;
+    /* an uncaptured call first: sizes every workspace and makes every table resident */
+    int rc = !g->s || !fyg_flags(g->s, n, &dflags, &dgen) || run_cubes(cubes, n, dflags, dgen, g->s) ||
+             fyg_sync(g->s);
+    if (!rc) {
+        rc = fyg_capture_begin(g->s);
+        if (!rc) {
+            int r1 = !fyg_flags(g->s, n, &dflags, &dgen) || run_cubes(cubes, n, dflags, dgen, g->s);
+            int r2 = fyg_capture_end(g->s, &g->exec);
+            rc = r1 || r2;
+        }
+    }
+    if (rc) {
+        fy_graph_destroy(g);
+        return NULL;
+    }
+    return g;
+}
+
+int fy_graph_launch(fy_graph *g) { return !g || fyg_graph_launch(g->exec, g->s); }
+
+void fy_graph_destroy(fy_graph *g) {
+    if (!g) return;
+    fyg_graph_free(g->exec);
+    fyg_stream_private_free(g->s);
+    free(g);
 }
 
 int fyh_chain(const fy_pipeline_cfg *cfg, const unsigned char *d_u8, long rz0, long ry0, long rx0, long hz,
               long hy, long hx, long z0, long y0, long x0, long tz, long ty, long tx, long vol_y, long vol_x,
-              unsigned char *d_out, int *d_flag, int gen, fyg_stream *s) {
+              unsigned char *d_out, int *d_flag, const int *gen, fyg_stream *s) {
     fy_cube_params P;
     if (fy_cube_params_from_cfg(cfg, y0, x0, ty, tx, vol_y, vol_x, &P)) return 1;
     fy_cube c = {d_u8, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, d_out, NULL, &P};
@@ -446,13 +489,15 @@ int fy_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf, lo
     unsigned char *du = (unsigned char *)fyg_ws(s, WS_U8, hn);
     unsigned char *dq = (unsigned char *)fyg_ws(s, WS_OUT, tn);
     int *dflag = NULL;
-    int gen = fyg_flag(s, &dflag);
-    if (!du || !dq || !gen) return 1;
+    const int *dgen = NULL;
+    if (!du || !dq || !fyg_flag(s, &dflag, &dgen)) return 1;
     if (fyg_h2d(du, u8buf, hn, s)) return 1;
-    if (fyh_chain(cfg, du, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, dq, dflag, gen, s))
+    if (fyh_chain(cfg, du, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, dq, dflag, dgen, s))
         return 1;
-    int flag = 0;
-    if (fyg_d2h(out, dq, tn, s) || fyg_d2h(&flag, dflag, sizeof(int), s) || fyg_sync(s)) return 1;
+    int flag = 0, gen = 0;
+    if (fyg_d2h(out, dq, tn, s) || fyg_d2h(&flag, dflag, sizeof(int), s) || fyg_d2h(&gen, dgen, sizeof(int), s) ||
+        fyg_sync(s))
+        return 1;
     if (all_air) *all_air = flag != gen;
     return 0;
 }
@@ -463,13 +508,13 @@ int fy_gpu_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *d_u8,
     fyg_stream *s = fyg_stream_wrap(stream);
     if (!s) return 1;
     int *dflag = NULL;
-    int gen = fyg_flag(s, &dflag);
-    if (!gen) return 1;
-    if (fyh_chain(cfg, d_u8, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, d_out, dflag, gen, s))
+    const int *dgen = NULL;
+    if (!fyg_flag(s, &dflag, &dgen)) return 1;
+    if (fyh_chain(cfg, d_u8, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, d_out, dflag, dgen, s))
         return 1;
     if (all_air) {
-        int flag = 0;
-        if (fyg_d2h(&flag, dflag, sizeof(int), s) || fyg_sync(s)) return 1;
+        int flag = 0, gen = 0;
+        if (fyg_d2h(&flag, dflag, sizeof(int), s) || fyg_d2h(&gen, dgen, sizeof(int), s) || fyg_sync(s)) return 1;
         *all_air = flag != gen;
     }
     return 0;

@@ -292,6 +292,9 @@ int fy_gpu_downscale2x(const unsigned char *d_in, int nx, int ny, int nz, unsign
     return !s || fyg_downscale2x_run(d_in, nx, ny, nz, d_out, (int)method, alpha, s);
 }
 
+int fy_gpu_tuning(int mode) { return fyg_tune_mode(mode); }
+int fy_gpu_tuning_report(char *buf, int len) { return fyg_tune_report(buf, len); }
+
 int fy_gpu_sync(void *stream) {
     fyg_stream *s = fyg_stream_wrap(stream);
     return s ? fyg_sync(s) : 1;

@@ -88,7 +88,7 @@ int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, in
     key.param = param; key.cap = cap; key.fz2_scale = fz2_scale;
     /* per-stream slots first (no device wait on a miss), the process-wide cache
      * for tables too large for a slot */
-    int slot = bytes <= FYG_ARENA_SLOT;
+    int slot = bytes <= FYG_ARENA_SLOT && !fyg_stream_persistent(s);
     const unsigned char *dev = (const unsigned char *)(slot ? fyg_stream_table(s, &key, sizeof key, NULL, 0)
                                                             : fyg_table_find(&key, sizeof key));
     if (!dev) {

@@ -104,6 +104,15 @@ typedef struct {
  * cube's output does not depend on what it is submitted with.  No reference
  * counterpart: the batched form of process_tile (pipeline.c:252-297). */
 int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream);
+/* the batched form captured: fy_gpu_graph_cubes runs the cubes once, then
+ * records the same launches (one per stage per shape group and 8 cubes) into a
+ * HIP graph; fy_graph_launch replays them on `stream` with whatever the cubes'
+ * buffers hold at that time.  The cubes' buffers and parameters must stay valid
+ * until fy_graph_destroy.  No reference counterpart. */
+typedef struct fy_graph fy_graph;
+fy_graph *fy_gpu_graph_cubes(const fy_cube *cubes, int n, void *stream);
+int  fy_graph_launch(fy_graph *g);
+void fy_graph_destroy(fy_graph *g);
 /* device bytes one cube holds while it runs (workspace admission) */
 size_t fy_cube_workspace_bytes(const fy_cube *cube);
 
@@ -156,6 +165,15 @@ int  fy_gpu_phase_correlate(const float *d_fixed, const float *d_moving, int nz,
                             double *shift, double *peak, int window, void *stream);
 /* fy_flat_noise of a device volume (stream.c:174-200) */
 int  fy_gpu_flat_noise(const float *d_v, int nz, int ny, int nx, int blk, double *out, void *stream);
+
+/* the tile-shape autotuner of the fast guided filter (the coefficient march's z
+ * chunk, the upsample's planes per workgroup; every candidate gives
+ * bit-identical results).  mode 0: first candidates; 1: each new shape runs all
+ * candidates once, timed, and keeps the fastest (default); 2 + k: candidate k.
+ * Returns the previous mode.  The report lists each tuned shape with its
+ * candidates' times; returns its length.  No reference counterpart. */
+int fy_gpu_tuning(int mode);
+int fy_gpu_tuning_report(char *buf, int len);
 
 int fy_gpu_sync(void *stream);
 

@@ -310,7 +310,7 @@ def test_process_buffer_all_air():
 
 
 def test_process_buffer_rejects_unsupported_stages():
-    cfg = _chain_cfg(F, do_air_zero=1)
+    cfg = _chain_cfg(F, do_musica=1)
     u8 = np.full((12, 12, 12), 100, np.uint8)
     with pytest.raises(RuntimeError):
         F.process_buffer(cfg, u8, (0, 0, 0), (0, 0, 0), (12, 12, 12), (12, 12))
