@@ -111,9 +111,22 @@ class Cube(C.Structure):
                 ("params", C.POINTER(CubeParams))]
 
 
+class Zarr(C.Structure):
+    """fy_zarr (fysics.h:376-381): local zarr v2 raw-u8 volume handle."""
+    _fields_ = [("root", C.c_char * 1024), ("shape", C.c_long * 3), ("chunk", C.c_long * 3),
+                ("fill", C.c_ubyte)]
+
+
+class ExportStats(C.Structure):
+    """fy_export_stats (include/fysics_gpu.h)."""
+    _fields_ = [("tiles", C.c_long), ("written", C.c_long), ("air", C.c_long), ("io_fail", C.c_long),
+                ("wall_s", C.c_double), ("read_s", C.c_double), ("write_s", C.c_double)]
+
+
 _P = C.POINTER(Physics)
 _VP = C.c_void_p
 _L = C.c_long
+_ZP = C.POINTER(Zarr)
 
 # name -> (restype, argtypes); every function declared in include/fysics*.h
 SIGNATURES = {
@@ -212,6 +225,14 @@ SIGNATURES = {
     "fy_calib_reset": (C.c_int, [_VP]),
     "fy_calib_destroy": (None, [_VP]),
     "fy_gpu_flat_noise": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, c_double_p, _VP]),
+    "fy_zarr_open": (C.c_int, [_ZP, C.c_char_p]),
+    "fy_zarr_create": (C.c_int, [_ZP, C.c_char_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    "fy_zarr_read": (C.c_int, [_ZP, _L, _L, _L, _L, _L, _L, _VP]),
+    "fy_zarr_write_chunk": (C.c_int, [_ZP, _L, _L, _L, _VP, _L, _L, _L]),
+    "fy_process_chunk": (C.c_int, [_ZP, C.POINTER(PipelineCfg), _L, _L, _L, C.c_int, _VP, c_int_p, c_int_p,
+                                   c_int_p, c_int_p]),
+    "fy_gpu_export": (C.c_int, [_ZP, _ZP, C.POINTER(PipelineCfg), C.c_int, _L, _L, C.c_int, C.c_int,
+                                C.POINTER(ExportStats), _VP]),
     "fy_gpu_tuning": (C.c_int, [C.c_int]),
     "fy_gpu_tuning_report": (C.c_int, [C.c_char_p, C.c_int]),
     "fy_gpu_sync": (C.c_int, [_VP]),

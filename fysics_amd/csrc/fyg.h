@@ -31,6 +31,15 @@ void *fyg_stream_handle(fyg_stream *s);          /* the hipStream_t */
 fyg_stream *fyg_stream_private(void *hip_stream);
 void  fyg_stream_private_free(fyg_stream *s);
 int   fyg_stream_persistent(fyg_stream *s);
+/* an owned stream object on a new non-blocking hipStream_t; fyg_stream_free
+ * waits for it, frees its buffers and destroys the stream */
+fyg_stream *fyg_stream_new(void);
+void  fyg_stream_free(fyg_stream *s);
+/* work queued on s after this call waits for ev (recorded on any stream) */
+int   fyg_stream_wait(fyg_stream *s, void *ev);
+/* pinned host memory */
+int   fyg_host_alloc(void **p, size_t bytes);
+void  fyg_host_free(void *p);
 /* graph capture of the launches queued on s between begin and end */
 int   fyg_capture_begin(fyg_stream *s);
 int   fyg_capture_end(fyg_stream *s, void **exec);
@@ -60,6 +69,7 @@ int  fyg_flag(fyg_stream *s, int **d_flag, const int **d_gen);
 /* the same with n flags (one per cube of a batch), all tested against one g */
 int  fyg_flags(fyg_stream *s, int n, int **d_flags, const int **d_gen);
 const char *fyg_error_string(void);
+void fyg_set_error(const char *msg);            /* host-side failures (I/O, arguments) */
 
 /* process-wide cache of small read-only device tables (twiddles, transfer
  * LUTs).  Returns the device copy for key, uploading data on a miss.  Thread

@@ -32,6 +32,11 @@ static int note(hipError_t e, const char *what) {
 
 extern "C" const char *fyg_error_string(void) { return tls_err; }
 
+extern "C" void fyg_set_error(const char *msg) {
+    snprintf(tls_err, sizeof tls_err, "%s", msg ? msg : "");
+    g_err.store(1);
+}
+
 extern "C" fyg_stream *fyg_thread_stream(void) {
     if (tls_stream) return tls_stream;
     fyg_stream *st = new fyg_stream();
@@ -85,6 +90,39 @@ extern "C" void fyg_stream_private_free(fyg_stream *st) {
 }
 
 extern "C" int fyg_stream_persistent(fyg_stream *s) { return s && s->persist; }
+
+// an owned stream object on a new non-blocking hipStream_t (default priority)
+extern "C" fyg_stream *fyg_stream_new(void) {
+    fyg_stream *st = new fyg_stream();
+    if (note(hipStreamCreateWithFlags(&st->s, hipStreamNonBlocking), "hipStreamCreate")) {
+        delete st;
+        return nullptr;
+    }
+    st->owned = true;
+    return st;
+}
+
+extern "C" void fyg_stream_free(fyg_stream *st) {
+    if (!st) return;
+    const bool own = st->owned;
+    hipStream_t h = st->s;
+    fyg_stream_private_free(st);
+    if (own) (void)hipStreamDestroy(h);
+}
+
+extern "C" int fyg_stream_wait(fyg_stream *s, void *ev) {
+    if (!s || !ev) return 1;
+    return note(hipStreamWaitEvent(s->s, (hipEvent_t)ev, 0), "hipStreamWaitEvent");
+}
+
+extern "C" int fyg_host_alloc(void **p, size_t bytes) {
+    *p = nullptr;
+    return note(hipHostMalloc(p, bytes ? bytes : 256, hipHostMallocDefault), "hipHostMalloc");
+}
+
+extern "C" void fyg_host_free(void *p) {
+    if (p) (void)hipHostFree(p);
+}
 
 extern "C" int fyg_capture_begin(fyg_stream *s) {
     if (note(hipStreamBeginCapture(s->s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) return 1;

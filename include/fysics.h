@@ -237,6 +237,33 @@ int fy_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
                       long vol_y, long vol_x,
                       unsigned char *out, int *all_air);
 
+/* Local zarr v2 raw-u8 volumes (zarr_io.c; fysics.h:375-389, same struct
+ * layout): <root>/0/.zarray plus one uncompressed C-order file per chunk at
+ * <root>/0/cz/cy/cx; chunks on the far faces hold their true extent; a
+ * missing chunk (or the part past a short file's end) reads as fill_value.
+ * fy_zarr_open fails on S3 roots, compressed chunks and dtypes other than u1.
+ * fy_zarr_read fills voxels outside the volume with fill_value. */
+typedef struct {
+    char root[1024];
+    long shape[3];      /* z,y,x voxels */
+    long chunk[3];      /* chunk dims */
+    unsigned char fill; /* fill_value */
+} fy_zarr;
+
+int fy_zarr_open(fy_zarr *z, const char *root);
+int fy_zarr_create(fy_zarr *z, const char *root, const long shape[3], const long chunk[3]);
+int fy_zarr_read(const fy_zarr *z, long z0, long y0, long x0,
+                 long dz, long dy, long dx, unsigned char *out);
+int fy_zarr_write_chunk(const fy_zarr *z, long cz, long cy, long cx,
+                        const unsigned char *buf, long bz, long by, long bx);
+
+/* fy_process_buffer of the inner tile^3 at (z0,y0,x0), clamped to the volume,
+ * with its halo (cfg->halo, 8 if <= 0) read from zin (pipeline.c:518-538).
+ * Writes tz*ty*tx u8 to out and the clamped sizes to out_t* (any may be NULL). */
+int fy_process_chunk(const fy_zarr *zin, const fy_pipeline_cfg *cfg,
+                     long z0, long y0, long x0, int tile,
+                     unsigned char *out, int *out_tz, int *out_ty, int *out_tx, int *all_air);
+
 #ifdef __cplusplus
 }
 #endif

@@ -166,6 +166,31 @@ int  fy_gpu_phase_correlate(const float *d_fixed, const float *d_moving, int nz,
 /* fy_flat_noise of a device volume (stream.c:174-200) */
 int  fy_gpu_flat_noise(const float *d_v, int nz, int ny, int nx, int blk, double *out, void *stream);
 
+/* ---- pass-2 export through the GPU: the tile loop of fy_run_pipeline
+ * (pipeline.c:1000-1070) fed from zarr.  Tiles t in [t_begin, t_end) of the
+ * pass-2 grid (t = (iz*nty + iy)*ntx + ix; t_end <= 0: all) are read with
+ * their halo (cfg->halo, 8 if <= 0) from zin by `threads` host threads (<= 0:
+ * one per core, at most 16) straight into pinned staging, copied to the device,
+ * processed `batch` at a time (<= 0: 8) by fy_gpu_process_cubes on `stream`,
+ * copied back and written as chunk (iz, iy, ix) of zout, which must have zin's
+ * shape and tile^3 chunks.  Three batches are in flight, so reads, both PCIe
+ * directions, kernels and writes overlap.  All-air tiles are skipped (their
+ * chunks stay absent and read as 0), as in the reference; failed reads and
+ * writes are counted and make the call return 1.  Output chunks are
+ * bit-identical to fy_process_chunk's.  No reference counterpart (the
+ * reference runs this loop on host threads). */
+typedef struct {
+    long tiles;                     /* tiles in the range */
+    long written;                   /* chunks written */
+    long air;                       /* all-air tiles skipped */
+    long io_fail;                   /* failed reads or writes */
+    double wall_s;                  /* wall time of the call */
+    double read_s, write_s;         /* thread time spent reading / writing (summed) */
+} fy_export_stats;
+int fy_gpu_export(const fy_zarr *zin, const fy_zarr *zout, const fy_pipeline_cfg *cfg, int tile,
+                  long t_begin, long t_end, int threads, int batch, fy_export_stats *stats,
+                  void *stream);
+
 /* the tile-shape autotuner of the fast guided filter (the coefficient march's z
  * chunk, the upsample's planes per workgroup; every candidate gives
  * bit-identical results).  mode 0: first candidates; 1: each new shape runs all

@@ -203,7 +203,16 @@ _ORC_ONLY = {
                              [c_u8_p, c_int_p, c_float_p]),
 }
 
+class Zarr(C.Structure):
+    """fy_zarr (fysics.h:376-381)."""
+    _fields_ = [("root", C.c_char * 1024), ("shape", C.c_long * 3), ("chunk", C.c_long * 3),
+                ("fill", C.c_ubyte)]
+
+
 _REF_ONLY = {
+    "fy_zarr_open": (C.c_int, [C.POINTER(Zarr), C.c_char_p]),
+    "fy_zarr_read": (C.c_int, [C.POINTER(Zarr), C.c_long, C.c_long, C.c_long, C.c_long, C.c_long, C.c_long,
+                               C.c_void_p]),
     "fy_guided_denoise_ws": (C.c_int, [c_float_p, c_float_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_double, c_float_p]),
     "fy_guided_denoise_fast_ws": (C.c_int, [c_float_p, c_float_p, C.c_int, C.c_int, C.c_int,

@@ -59,7 +59,7 @@ def _c_sizeof(types, header_dir):
 
 
 STRUCTS = [("fy_physics", F.Physics), ("fy_noise_model", F.NoiseModel), ("fy_dering", F.Dering),
-           ("fy_pipeline_cfg", F.PipelineCfg)]
+           ("fy_pipeline_cfg", F.PipelineCfg), ("fy_zarr", F.Zarr)]
 
 
 def test_struct_layouts_match_ctypes():
