@@ -124,6 +124,7 @@ def main():
     ap.add_argument("--no-calibration", action="store_true", help="skip the pass-1 statistics timing")
     ap.add_argument("--no-air-zero", action="store_true", help="skip the chain-with-air-zero timing")
     ap.add_argument("--no-lod", action="store_true", help="skip the downscale / phase-correlation timing")
+    ap.add_argument("--no-export", action="store_true", help="skip the zarr -> GPU -> zarr export timing")
     ap.add_argument("--served-requests", type=int, default=2000)
     ap.add_argument("--served-rate", type=float, default=400.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -246,6 +247,7 @@ def main():
     calib = None if args.no_calibration else calibration_bench(F, vol)
     airz = None if args.no_air_zero else air_zero_bench(F, vol, eps)
     lod = None if args.no_lod else lod_phasecorr_bench(F, vol)
+    export = None if args.no_export else export_bench(F, vol, eps)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -271,6 +273,7 @@ def main():
         "calibration": calib,
         "air_zero": airz,
         "lod_phasecorr": lod,
+        "export": export,
         "cpu_baseline": cpu,
         "workload_gen_s": round(gen_s, 1),
     }
@@ -614,6 +617,82 @@ def lod_phasecorr_bench(F, vol, reps=20):
             "phase_correlate": {"workload": "128^3 patch vs itself rolled by (3, -2, 5), Hann window",
                                 "ms_per_call": round(1000 * pdt, 3), "shift": [round(v, 4) for v in s3],
                                 "peak": round(pk.value, 4)}}
+
+
+def export_bench(F, vol, eps, tile=128, threads=16, batch=8):
+    """Pass-2 export through the GPU (fy_gpu_export): a 512^3 u8 volume (the
+    workload cube tiled 2x2x2) stored as zarr v2 with 128^3 chunks in a
+    temporary directory, restored tile by tile (tile 128, halo 8, the chain of
+    the headline workload without rings) into an output zarr.  Inputs are read
+    from files the page cache holds and outputs written to files, so the rate
+    includes host I/O and both PCIe directions; it is set beside the GPU-only
+    rate of the same tiles with their regions already in HBM."""
+    import shutil
+    import tempfile
+    import torch
+    lib = F.lib()
+    big = np.tile(vol, (2, 2, 2))
+    Z = big.shape[0]
+    work = tempfile.mkdtemp(prefix="fy_export_")
+    try:
+        zi = F.Zarr()
+        src = os.path.join(work, "in.zarr").encode()
+        shp = (C.c_long * 3)(Z, Z, Z)
+        ch = (C.c_long * 3)(tile, tile, tile)
+        assert lib.fy_zarr_create(C.byref(zi), src, shp, ch) == 0
+        nt = Z // tile
+        for cz in range(nt):
+            for cy in range(nt):
+                for cx in range(nt):
+                    blk = np.ascontiguousarray(big[cz * tile:(cz + 1) * tile, cy * tile:(cy + 1) * tile,
+                                                   cx * tile:(cx + 1) * tile])
+                    assert lib.fy_zarr_write_chunk(C.byref(zi), cz, cy, cx, blk.ctypes.data, tile, tile, tile) == 0
+        cfg = F.pipeline_cfg(delta_beta=300.0, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
+                             unsharp_sigma=0.8, unsharp_coeff=0.5, window_lo=-0.1, window_hi=1.2,
+                             do_deconv=1, guided_eps=eps, halo=8)
+        stats = F.ExportStats()
+
+        def export(name, t1=0):
+            zo = F.Zarr()
+            assert lib.fy_zarr_create(C.byref(zo), os.path.join(work, name).encode(), shp, ch) == 0
+            t = time.perf_counter()
+            rc = lib.fy_gpu_export(C.byref(zi), C.byref(zo), C.byref(cfg), tile, 0, t1, threads, batch,
+                                   C.byref(stats), None)
+            if rc:
+                raise RuntimeError("fy_gpu_export: " + lib.fy_gpu_last_error().decode())
+            return time.perf_counter() - t
+
+        export("warm.zarr", t1=2 * batch)
+        walls = [export(f"out{i}.zarr") for i in range(3)]
+        wall = statistics.median(walls)
+        # the same tiles with their regions resident in HBM: one fy_gpu_process_cubes per batch
+        h = 8
+        reg = tile + 2 * h
+        d_in = torch.from_numpy(np.ascontiguousarray(big[:reg, :reg, :reg])).cuda()
+        d_out = torch.empty((batch, tile, tile, tile), dtype=torch.uint8, device="cuda")
+        prm = F.CubeParams()
+        assert lib.fy_cube_params_from_cfg(C.byref(cfg), h, h, tile, tile, Z, Z, C.byref(prm)) == 0
+        cubes = (F.Cube * batch)(*[F.Cube(d_in.data_ptr(), 0, 0, 0, reg, reg, reg, h, h, h, tile, tile, tile,
+                                          d_out[i].data_ptr(), None, C.pointer(prm)) for i in range(batch)])
+        torch.cuda.synchronize()
+        for _ in range(2):
+            assert lib.fy_gpu_process_cubes(cubes, batch, None) == 0
+        assert lib.fy_gpu_sync(None) == 0
+        t = time.perf_counter()
+        reps = 8
+        for _ in range(reps):
+            assert lib.fy_gpu_process_cubes(cubes, batch, None) == 0
+        assert lib.fy_gpu_sync(None) == 0
+        gpu_tile_s = (time.perf_counter() - t) / (reps * batch)
+        ntiles = nt ** 3
+        return {"workload": f"{Z}^3 u8 zarr (128^3 chunks) -> restored zarr, tile {tile} halo {h}, "
+                            f"{threads} I/O threads, batches of {batch}; inputs in the page cache",
+                "tiles": ntiles, "wall_s": round(wall, 4), "mvox_s": round(Z ** 3 / wall / 1e6, 1),
+                "read_thread_s": round(stats.read_s, 3), "write_thread_s": round(stats.write_s, 3),
+                "gpu_only_mvox_s": round(tile ** 3 / gpu_tile_s / 1e6, 1),
+                "gpu_only_wall_s": round(gpu_tile_s * ntiles, 4)}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):

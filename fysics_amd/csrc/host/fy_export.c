@@ -7,14 +7,17 @@
  * as a pipeline.  The reference gives each OpenMP thread one tile at a time
  * (read, chain, quantise, write).  Here host threads only do I/O -- they read
  * halo'd regions straight into pinned slots and write finished chunks -- while
- * the calling thread moves slots through three GPU stages on three streams:
+ * the calling thread moves slots through three GPU stages on three streams.
+ * The reads share one mapping of each input chunk (fyh_zcache): the up to 27
+ * chunks a halo'd tile touches are each mapped once per export, not once per
+ * neighbouring tile.
  *
  *   reads (threads) -> H2D (copy stream) -> fy_gpu_process_cubes (the caller's
  *   stream, shape-grouped launches) -> D2H (second copy stream) -> writes (threads)
  *
- * Three slots of `batch` tiles circulate, so the reads of slot b+1, the kernels
- * of slot b and the writes of slot b-1 overlap, and both PCIe directions run
- * beside the kernels.  Events order the stages; a slot is refilled only after
+ * Six slots of `batch` tiles circulate, so the reads of the next slots, the
+ * kernels of slot b and the writes of the previous ones overlap, and both PCIe
+ * directions run beside the kernels.  Events order the stages; a slot is refilled only after
  * its chunks are written, which is after every stage that touched its buffers.
  * Tiles whose region is all zero are skipped and their chunks stay absent
  * (read back as fill 0), as in the reference; failed reads and writes are
@@ -103,7 +106,7 @@ int fy_process_chunk(const fy_zarr *zin, const fy_pipeline_cfg *cfg, long z0, lo
 }
 
 /* ------------------------------------------------------------------ export */
-#define NSLOT 3
+#define NSLOT 6
 enum { S_FREE = 0, S_READING, S_SUBMITTED };
 
 typedef struct {
@@ -132,6 +135,7 @@ typedef struct {
     long Z, Y, X, nty, ntx;
     size_t cap, tcap;
     slot_t sl[NSLOT];
+    fyh_zcache *zc;             /* input chunk mappings shared by the reads */
     task_t *q;
     int qcap, qhead, qn;
     pthread_mutex_t mu;
@@ -157,7 +161,7 @@ static void do_read(exporter *e, slot_t *s, int k) {
     unsigned char *dst = s->hin + (size_t)k * e->cap;
     const size_t hn = (size_t)g->hz * g->hy * g->hx;
     double t = now_s();
-    int bad = fy_zarr_read(e->zin, g->rz0, g->ry0, g->rx0, g->hz, g->hy, g->hx, dst);
+    int bad = fyh_zarr_read_cached(e->zin, e->zc, g->rz0, g->ry0, g->rx0, g->hz, g->hy, g->hx, dst);
     int air = bad ? 1 : !any_nonzero(dst, hn);
     t = now_s() - t;
     pthread_mutex_lock(&e->mu);
@@ -176,8 +180,8 @@ static void do_read(exporter *e, slot_t *s, int k) {
 
 static void do_write(exporter *e, slot_t *s, int k) {
     const tile_geom *g = &s->g[k];
-    double t = now_s();
     int bad = fyg_event_wait(s->e_out) != 0;
+    double t = now_s();
     if (!bad)
         bad = fy_zarr_write_chunk(e->zout, g->iz, g->iy, g->ix, s->hout + (size_t)k * e->tcap, g->tz, g->ty,
                                   g->tx) != 0;
@@ -300,7 +304,8 @@ int fy_gpu_export(const fy_zarr *zin, const fy_zarr *zout, const fy_pipeline_cfg
     e->tcap = (size_t)tile * tile * tile;
     e->qcap = NSLOT * batch * 2 + threads + 4;
     e->q = (task_t *)calloc((size_t)e->qcap, sizeof(task_t));
-    int rc = e->q == NULL;
+    e->zc = fyh_zcache_new((size_t)1 << 31);
+    int rc = e->q == NULL || e->zc == NULL;
     for (int i = 0; i < NSLOT && !rc; i++) {
         slot_t *s = &e->sl[i];
         rc |= fyg_host_alloc((void **)&s->hin, e->cap * batch);
@@ -409,6 +414,7 @@ int fy_gpu_export(const fy_zarr *zin, const fy_zarr *zout, const fy_pipeline_cfg
     }
     free(th);
     free(e->q);
+    fyh_zcache_free(e->zc);
     if (e->io_fail) rc = 1;
     if (rc && e->err[0]) fyg_set_error(e->err);
     if (stats) {

@@ -16,6 +16,7 @@
 #define _GNU_SOURCE
 #include <errno.h>
 #include <fcntl.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -129,17 +130,18 @@ int fy_zarr_create(fy_zarr *z, const char *root, const long shape[3], const long
     return bad ? zerr("cannot write", path) : 0;
 }
 
-/* a chunk file's bytes: mapped, read into `scratch` when mapping fails, or
- * absent (p == NULL: all fill) */
+/* the bytes [off, off + len) of a chunk file's first `cap` bytes: mapped, read
+ * into `scratch` when mapping fails, or absent (p == NULL: all fill) */
 typedef struct {
-    const unsigned char *p;
-    size_t len;
+    const unsigned char *p;     /* byte `off` of the chunk */
+    size_t off, len;
     void *map;
     size_t maplen;
 } chunk_bytes;
 
-static int chunk_open(const fy_zarr *z, long cz, long cy, long cx, size_t cap, unsigned char **scratch,
-                      chunk_bytes *cb) {
+/* Only the pages holding bytes [lo, hi) of the chunk are mapped. */
+static int chunk_open(const fy_zarr *z, long cz, long cy, long cx, size_t cap, size_t lo, size_t hi,
+                      unsigned char **scratch, chunk_bytes *cb) {
     memset(cb, 0, sizeof *cb);
     char path[1100];
     snprintf(path, sizeof path, "%s/0/%ld/%ld/%ld", z->root, cz, cy, cx);
@@ -153,35 +155,41 @@ static int chunk_open(const fy_zarr *z, long cz, long cy, long cx, size_t cap, u
         close(fd);
         return zerr("cannot stat", path);
     }
-    if (st.st_size <= 0) {
+    size_t len = st.st_size > 0 ? (size_t)st.st_size : 0;
+    if (len > cap) len = cap;
+    if (hi > len) hi = len;
+    if (lo >= hi) {                                          /* nothing stored there: fill */
         close(fd);
         return 0;
     }
-    size_t len = (size_t)st.st_size;
-    void *m = mmap(NULL, len, PROT_READ, MAP_PRIVATE, fd, 0);
+    const size_t pg = (size_t)sysconf(_SC_PAGESIZE);
+    const size_t off = lo / pg * pg;
+    void *m = mmap(NULL, hi - off, PROT_READ, MAP_PRIVATE | (lo == 0 && hi == cap ? MAP_POPULATE : 0), fd, (off_t)off);
     if (m != MAP_FAILED) {
         close(fd);
         cb->map = m;
-        cb->maplen = len;
+        cb->maplen = hi - off;
         cb->p = (const unsigned char *)m;
-        cb->len = len < cap ? len : cap;
+        cb->off = off;
+        cb->len = hi - off;
         return 0;
     }
-    /* no mapping (e.g. a filesystem without mmap): read the chunk into a
+    /* no mapping (e.g. a filesystem without mmap): read the range into a
      * buffer of a whole chunk, which holds any chunk's extent */
     if (!*scratch && !(*scratch = (unsigned char *)malloc((size_t)z->chunk[0] * z->chunk[1] * z->chunk[2]))) {
         close(fd);
         return zerr("out of memory reading", path);
     }
-    size_t want = len < cap ? len : cap, got = 0;
-    while (got < want) {
-        ssize_t r = pread(fd, *scratch + got, want - got, (off_t)got);
+    size_t got = 0;
+    while (off + got < hi) {
+        ssize_t r = pread(fd, *scratch + got, hi - off - got, (off_t)(off + got));
         if (r < 0 && errno == EINTR) continue;
         if (r <= 0) break;
         got += (size_t)r;
     }
     close(fd);
     cb->p = *scratch;
+    cb->off = off;
     cb->len = got;
     return 0;
 }
@@ -193,7 +201,164 @@ static void chunk_close(chunk_bytes *cb) {
 static long lmin(long a, long b) { return a < b ? a : b; }
 static long lmax(long a, long b) { return a > b ? a : b; }
 
+/* ---- shared chunk mappings for a tile loop.  Neighbouring tiles read the
+ * same chunks (a tile-sized region with a halo touches up to 27), so the
+ * exporter maps each chunk file once, whole, and lets every tile that needs it
+ * copy from that mapping; unmapping (a TLB shootdown across the process's
+ * threads) also happens once per chunk.  Entries no read holds are unmapped,
+ * least recently used first, when the mapped bytes exceed the budget. */
+typedef struct {
+    long cz, cy, cx;
+    int state;                  /* 0 empty, 1 loading, 2 ready, 3 failed */
+    int refs;
+    long stamp;
+    chunk_bytes cb;             /* mapped, or (map == NULL) a malloc'd copy */
+} zc_entry;
+
+#define ZC_ENTRIES 256
+
+struct fyh_zcache {
+    pthread_mutex_t mu;
+    pthread_cond_t cv;
+    zc_entry e[ZC_ENTRIES];
+    long clock;
+    size_t bytes, budget;
+};
+
+static void zc_release(chunk_bytes *cb) {
+    if (cb->map) munmap(cb->map, cb->maplen);
+    else free((void *)cb->p);
+    memset(cb, 0, sizeof *cb);
+}
+
+fyh_zcache *fyh_zcache_new(size_t budget) {
+    fyh_zcache *c = (fyh_zcache *)calloc(1, sizeof *c);
+    if (!c) return NULL;
+    c->budget = budget;
+    pthread_mutex_init(&c->mu, NULL);
+    pthread_cond_init(&c->cv, NULL);
+    return c;
+}
+
+void fyh_zcache_free(fyh_zcache *c) {
+    if (!c) return;
+    for (int i = 0; i < ZC_ENTRIES; i++) zc_release(&c->e[i].cb);
+    pthread_mutex_destroy(&c->mu);
+    pthread_cond_destroy(&c->cv);
+    free(c);
+}
+
+/* the ready entry of chunk (cz,cy,cx), mapped whole and held until zc_put;
+ * NULL on error.  A reader holds one entry at a time, so with fewer reading
+ * threads than entries a free or unheld entry always turns up. */
+static zc_entry *zc_get(fyh_zcache *c, const fy_zarr *z, long cz, long cy, long cx, size_t cap) {
+    pthread_mutex_lock(&c->mu);
+    zc_entry *slot;
+    for (;;) {
+        zc_entry *hit = NULL, *empty = NULL, *lru = NULL;
+        for (int i = 0; i < ZC_ENTRIES; i++) {
+            zc_entry *e = &c->e[i];
+            if (!e->state) {
+                if (!empty) empty = e;
+            } else if (e->cz == cz && e->cy == cy && e->cx == cx) {
+                hit = e;
+            } else if (e->state >= 2 && e->refs == 0 && (!lru || e->stamp < lru->stamp)) {
+                lru = e;
+            }
+        }
+        if (hit && hit->state == 1) {               /* another thread is mapping it */
+            pthread_cond_wait(&c->cv, &c->mu);
+            continue;
+        }
+        if (hit) {
+            if (hit->state == 3) {                  /* failed before: fails again */
+                pthread_mutex_unlock(&c->mu);
+                return NULL;
+            }
+            hit->refs++;
+            hit->stamp = ++c->clock;
+            pthread_mutex_unlock(&c->mu);
+            return hit;
+        }
+        slot = (empty && (c->bytes <= c->budget || !lru)) ? empty : lru;
+        if (slot) break;
+        pthread_cond_wait(&c->cv, &c->mu);          /* every entry held or loading */
+    }
+    chunk_bytes old = slot->cb;
+    c->bytes -= slot->state == 2 ? (old.map ? old.maplen : old.len) : 0;
+    memset(slot, 0, sizeof *slot);
+    slot->cz = cz;
+    slot->cy = cy;
+    slot->cx = cx;
+    slot->state = 1;
+    slot->refs = 1;
+    slot->stamp = ++c->clock;
+    pthread_mutex_unlock(&c->mu);
+    zc_release(&old);
+    chunk_bytes cb;
+    unsigned char *scratch = NULL;
+    int bad = chunk_open(z, cz, cy, cx, cap, 0, cap, &scratch, &cb);
+    if (!bad && cb.p && !cb.map) {                  /* read, not mapped: keep a copy */
+        unsigned char *own = (unsigned char *)malloc(cb.len ? cb.len : 1);
+        if (own) memcpy(own, cb.p, cb.len);
+        else bad = zerr("out of memory in the chunk cache", z->root);
+        cb.p = own;
+    }
+    free(scratch);
+    pthread_mutex_lock(&c->mu);
+    slot->cb = bad ? (chunk_bytes){0} : cb;
+    if (bad && cb.p && !cb.map) free((void *)cb.p);
+    slot->state = bad ? 3 : 2;
+    if (bad) slot->refs = 0;
+    else c->bytes += cb.map ? cb.maplen : cb.len;
+    pthread_cond_broadcast(&c->cv);
+    pthread_mutex_unlock(&c->mu);
+    return bad ? NULL : slot;
+}
+
+static void zc_put(fyh_zcache *c, zc_entry *e) {
+    pthread_mutex_lock(&c->mu);
+    e->refs--;
+    pthread_cond_broadcast(&c->cv);
+    pthread_mutex_unlock(&c->mu);
+}
+
+/* nr rows of w bytes; a halo read is mostly slivers a few bytes wide, so
+ * narrow rows move as two overlapping words instead of a memcpy call each */
+static void copy_rows(unsigned char *dst, size_t dpitch, const unsigned char *src, size_t spitch, size_t w,
+                      long nr) {
+    if (w == spitch && w == dpitch) {
+        memcpy(dst, src, w * (size_t)nr);
+    } else if (w >= 8 && w <= 16) {
+        for (long j = 0; j < nr; j++, dst += dpitch, src += spitch) {
+            unsigned long long a, b;
+            memcpy(&a, src, 8);
+            memcpy(&b, src + w - 8, 8);
+            memcpy(dst, &a, 8);
+            memcpy(dst + w - 8, &b, 8);
+        }
+    } else if (w >= 4 && w < 8) {
+        for (long j = 0; j < nr; j++, dst += dpitch, src += spitch) {
+            unsigned a, b;
+            memcpy(&a, src, 4);
+            memcpy(&b, src + w - 4, 4);
+            memcpy(dst, &a, 4);
+            memcpy(dst + w - 4, &b, 4);
+        }
+    } else if (w < 4) {
+        for (long j = 0; j < nr; j++, dst += dpitch, src += spitch)
+            for (size_t i = 0; i < w; i++) dst[i] = src[i];
+    } else {
+        for (long j = 0; j < nr; j++, dst += dpitch, src += spitch) memcpy(dst, src, w);
+    }
+}
+
 int fy_zarr_read(const fy_zarr *z, long z0, long y0, long x0, long dz, long dy, long dx, unsigned char *out) {
+    return fyh_zarr_read_cached(z, NULL, z0, y0, x0, dz, dy, dx, out);
+}
+
+int fyh_zarr_read_cached(const fy_zarr *z, fyh_zcache *zc, long z0, long y0, long x0, long dz, long dy, long dx,
+                         unsigned char *out) {
     if (dz <= 0 || dy <= 0 || dx <= 0) return 0;
     const long C[3] = {z->chunk[0], z->chunk[1], z->chunk[2]};
     const long hi[3] = {z0 + dz, y0 + dy, x0 + dx};
@@ -219,22 +384,42 @@ int fy_zarr_read(const fy_zarr *z, long z0, long y0, long x0, long dz, long dy, 
                     b[i] = lmin(vhi[i], o[i] + e[i]) - o[i];
                 }
                 chunk_bytes cb;
-                if (chunk_open(z, cz, cy, cx, (size_t)e[0] * e[1] * e[2], &scratch, &cb)) {
+                zc_entry *held = NULL;
+                const size_t first = ((size_t)a[0] * e[1] + a[1]) * e[2] + a[2];
+                const size_t last = ((size_t)(b[0] - 1) * e[1] + (b[1] - 1)) * e[2] + b[2];
+                if (zc) {
+                    if (!(held = zc_get(zc, z, cz, cy, cx, (size_t)e[0] * e[1] * e[2]))) {
+                        rc = 1;
+                        break;
+                    }
+                    cb = held->cb;
+                } else if (chunk_open(z, cz, cy, cx, (size_t)e[0] * e[1] * e[2], first, last, &scratch, &cb)) {
                     rc = 1;
                     break;
                 }
-                const long w = b[2] - a[2];
-                for (long k = a[0]; k < b[0]; k++)
+                const long w = b[2] - a[2], nr = b[1] - a[1];
+                for (long k = a[0]; k < b[0]; k++) {
+                    /* fast path: the plane's rows are all stored */
+                    const size_t rel0 = ((size_t)k * e[1] + a[1]) * e[2] + a[2] - cb.off;
+                    if (cb.p && rel0 + (size_t)(nr - 1) * e[2] + w <= cb.len) {
+                        unsigned char *dst = out + ((size_t)(o[0] + k - z0) * dy + (size_t)(o[1] + a[1] - y0)) * dx +
+                                             (size_t)(o[2] + a[2] - x0);
+                        copy_rows(dst, (size_t)dx, cb.p + rel0, (size_t)e[2], (size_t)w, nr);
+                        continue;
+                    }
                     for (long j = a[1]; j < b[1]; j++) {
                         unsigned char *dst = out + ((size_t)(o[0] + k - z0) * dy + (size_t)(o[1] + j - y0)) * dx +
                                              (size_t)(o[2] + a[2] - x0);
                         const size_t src = ((size_t)k * e[1] + j) * e[2] + a[2];
-                        long have = cb.p && src < cb.len ? (long)(cb.len - src) : 0;
+                        const size_t rel = src - cb.off;           /* src >= first >= cb.off */
+                        long have = cb.p && rel < cb.len ? (long)(cb.len - rel) : 0;
                         if (have > w) have = w;
-                        if (have > 0) memcpy(dst, cb.p + src, (size_t)have);
+                        if (have > 0) memcpy(dst, cb.p + rel, (size_t)have);
                         if (have < w) memset(dst + have, z->fill, (size_t)(w - have));
                     }
-                chunk_close(&cb);
+                }
+                if (held) zc_put(zc, held);
+                else chunk_close(&cb);
             }
     free(scratch);
     return rc;

@@ -11,9 +11,10 @@
  * Only the hot-path subset is provided (FFT, transfers, Wiener deconvolution,
  * guided filter, u8/physical conversions, ring removal, noise fit, the pass-1
  * calibration statistics -- histogram, z-drift sums, flat-block noise floor --
- * the per-tile chain with its air-zero stage, the 2x LOD downscale and phase
- * correlation).  The rest of registration, zarr I/O, MUSICA and the
- * whole-volume calibrate/run drivers are not part of this library.
+ * the per-tile chain with its air-zero stage, local zarr tile I/O and
+ * fy_process_chunk, the 2x LOD downscale and phase correlation).  The rest of
+ * registration, S3 zarr, MUSICA and the whole-volume calibrate/run drivers are
+ * not part of this library.
  *
  * Layout: row-major, x fastest, idx = (z*ny + y)*nx + x (fysics.h:120).
  */

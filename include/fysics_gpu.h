@@ -173,7 +173,7 @@ int  fy_gpu_flat_noise(const float *d_v, int nz, int ny, int nx, int blk, double
  * one per core, at most 16) straight into pinned staging, copied to the device,
  * processed `batch` at a time (<= 0: 8) by fy_gpu_process_cubes on `stream`,
  * copied back and written as chunk (iz, iy, ix) of zout, which must have zin's
- * shape and tile^3 chunks.  Three batches are in flight, so reads, both PCIe
+ * shape and tile^3 chunks.  Six batches are in flight, so reads, both PCIe
  * directions, kernels and writes overlap.  All-air tiles are skipped (their
  * chunks stay absent and read as 0), as in the reference; failed reads and
  * writes are counted and make the call return 1.  Output chunks are

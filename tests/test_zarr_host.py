@@ -147,3 +147,43 @@ def test_header_is_readable_json(tmp_path):
     h = json.load(open(os.path.join(root, "0", ".zarray")))
     assert h["shape"] == [3, 4, 5] and h["chunks"] == [2, 2, 2] and h["dtype"] == "|u1"
     assert h["compressor"] is None and h["fill_value"] == 0 and h["dimension_separator"] == "/"
+
+
+def test_shared_chunk_mappings_under_threads(tmp_path):
+    """The exporter's reads go through one set of chunk mappings shared by its
+    I/O threads (mapped once, evicted least recently used past a byte budget).
+    Eight threads read overlapping regions with a budget of a few chunks, so
+    entries are evicted and remapped while others hold theirs; every region
+    must equal the uncached read."""
+    import threading
+    rng = np.random.default_rng(9)
+    vol = rng.integers(0, 256, SHAPE, dtype=np.uint8)
+    root = str(tmp_path / "c.zarr")
+    write_volume(root, vol, CHUNK, drop=[(1, 1, 1)], trunc=[((0, 0, 2), 100)])
+    lib = F.lib()
+    lib.fyh_zcache_new.restype = C.c_void_p
+    lib.fyh_zcache_new.argtypes = [C.c_size_t]
+    lib.fyh_zcache_free.argtypes = [C.c_void_p]
+    lib.fyh_zarr_read_cached.restype = C.c_int
+    lib.fyh_zarr_read_cached.argtypes = [C.POINTER(F.Zarr), C.c_void_p] + [C.c_long] * 6 + [C.c_void_p]
+    z = F.Zarr()
+    assert lib.fy_zarr_open(C.byref(z), root.encode()) == 0
+    cache = lib.fyh_zcache_new(3 * 16 * 20 * 12)
+    regs = list(regions(rng, SHAPE, 200))
+    errors = []
+
+    def work(part):
+        for org, shp in regs[part::8]:
+            a = np.empty(shp, np.uint8)
+            b = np.empty(shp, np.uint8)
+            if lib.fyh_zarr_read_cached(C.byref(z), cache, *org, *shp, a.ctypes.data) != 0 or \
+                    lib.fy_zarr_read(C.byref(z), *org, *shp, b.ctypes.data) != 0 or not np.array_equal(a, b):
+                errors.append((org, shp))
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(8)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    lib.fyh_zcache_free(cache)
+    assert not errors, errors[:3]
