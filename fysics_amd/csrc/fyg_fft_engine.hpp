@@ -615,23 +615,37 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
         if (valid(l)) base[(size_t)p0 * es + l + (size_t)off * es] = v;
     };
     // LUT weight of deconv_radial (paganin.c:231-242) at z = pos:
-    // t = ((fz2[z] + fy2[y]) + fx2[k]) * us, linear interpolation between knots
-    auto weight = [&](int l, int pos) -> P {
-        double fzy = a.fz2[pos] + s_fy[l];
-        double t = (fzy + s_fx[l]) * a.us;
-        int ti = (int)t;
-        if (ti >= 4096) ti = 4095;
-        P l0 = a.lut[ti], l1 = a.lut[ti + 1];
-        return l0 + (l1 - l0) * (P)(t - (double)ti);
-    };
+    // t = ((fz2[z] + fy2[y]) + fx2[k]) * us, linear interpolation between knots.
+    // The knots an item needs are loaded before the forward stages, so their
+    // latency hides behind the spectrum's loads instead of following them.
+    constexpr int NB = L / RL;
+    constexpr int TOT = NL * NB;
+    constexpr int BPT = (TOT + T - 1) / T;
+    P k0[BPT][RL], k1[BPT][RL], kf[BPT][RL];
+#pragma unroll
+    for (int b = 0; b < BPT; b++) {
+        int i = threadIdx.x + b * T;
+        if (TOT % T == 0 || i < TOT) {
+            int l, j;
+            item_of<NL, NB, true>(i, l, j);
+            const double fy = s_fy[l], fx = s_fx[l];
+#pragma unroll
+            for (int r = 0; r < RL; r++) {
+                double fzy = a.fz2[j + r * NB] + fy;
+                double t = (fzy + fx) * a.us;
+                int ti = (int)t;
+                if (ti >= 4096) ti = 4095;
+                k0[b][r] = a.lut[ti];
+                k1[b][r] = a.lut[ti + 1];
+                kf[b][r] = (P)(t - (double)ti);
+            }
+        }
+    }
     // Forward stages 0 .. n-2 (HBM -> LDS), then one fused step: the last forward
     // stage (radix RL, span NSL) leaves item j's outputs at positions j + r*L/RL --
     // exactly the inputs of the first inverse stage (radix RL, span 1) for the same
     // item -- so the weight multiply and that inverse stage run in registers and
     // the spectrum makes a single LDS round trip around them.
-    constexpr int NB = L / RL;
-    constexpr int TOT = NL * NB;
-    constexpr int BPT = (TOT + T - 1) / T;
     V v[BPT][RL];
     if constexpr (n > 1) {
         run_from<L, NL, true, false, 0, n - 1>((P)-1, tw_s, buf, first, LdsWr<L, NL, V>{buf});
@@ -660,7 +674,7 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
             dft<RL>(v[b], (P)-1);
 #pragma unroll
             for (int r = 0; r < RL; r++) {
-                P w = weight(l, j + r * NB);
+                P w = k0[b][r] + (k1[b][r] - k0[b][r]) * kf[b][r];
                 v[b][r].x *= w;
                 v[b][r].y *= w;
             }

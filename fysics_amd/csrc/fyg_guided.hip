@@ -869,8 +869,8 @@ static UpArgs<P> up_args(const fyg_guided_args *a, const P *MA, const P *MB) {
 // workgroup -- never the arithmetic of an output voxel, so every choice gives
 // bit-identical results and the tuner may pick by speed alone.  The first time a
 // (precision, radius, decimation, grid, region) shape is filtered, each
-// candidate runs once on the real data, timed with events on the launch stream,
-// and the fastest is kept for the process.
+// candidate runs on the real data -- once to warm up, then twice timed with
+// events on the launch stream -- and the fastest is kept for the process.
 template <int RHO>
 struct CoefCand;
 template <>
@@ -914,8 +914,18 @@ static std::mutex g_tune_mu;
 static std::map<std::string, TuneRec> g_tune;
 static std::atomic<int> g_tune_mode{1};   // 0: first candidates, 1: autotune, 2 + k: candidate k
 
+// a candidate's time: one untimed launch first (the first launch of a kernel
+// also loads its code object), then the best of two timed launches
+template <class F>
+static float timed1(fyg_stream *s, F launch);
 template <class F>
 static float timed(fyg_stream *s, F launch) {
+    launch();
+    const float a = timed1(s, launch), b = timed1(s, launch);
+    return a < b ? a : b;
+}
+template <class F>
+static float timed1(fyg_stream *s, F launch) {
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return 1e30f;
     if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return 1e30f; }
