@@ -61,14 +61,14 @@ def kernel_bytes(name, n, p, s=2):
 # moved, write 1.0), from the run summarised in
 # profiles/chain_256cube_kernel_trace.md.  Used for roofline.traffic only.
 MEASURED_TRAFFIC_256 = {
-    "ring_image": 300561,
-    "rfft_x_u8": 86681663,
-    "cfft_y_fwd": 144805470,
-    "zfused": 145066779,
-    "cfft_y_inv": 144805470,
-    "irfft_x": 142812617,
-    "guided_coef": 42136861,
-    "guided_upsample": 105213257,
+    "ring_image": 300186,
+    "rfft_x_u8": 86689202,
+    "cfft_y_fwd": 144820780,
+    "zfused": 145043174,
+    "cfft_y_inv": 144820780,
+    "irfft_x": 142826081,
+    "guided_coef": 42066697,
+    "guided_upsample": 105213486,
 }
 
 
@@ -112,8 +112,8 @@ def eps_for_cube(n):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -200,7 +200,7 @@ def main():
     # per-kernel HIP-event timing on the launch stream -> roofline of the dominant kernel
     lib.fy_gpu_profile_reset()
     lib.fy_gpu_profile(1)
-    for _ in range(min(args.steps, 20)):
+    for _ in range(min(args.steps, 50)):
         step()
     torch.cuda.synchronize()
     lib.fy_gpu_profile(0)
