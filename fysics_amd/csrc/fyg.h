@@ -106,7 +106,7 @@ void fyg_prof_reset(void);
 /* Cubes of one shape run as one launch per stage: up to FYG_BATCH cubes'
  * descriptors travel in the kernel arguments (no descriptor upload, so the
  * launches can be captured), and larger groups take several launches. */
-#define FYG_BATCH 8
+#define FYG_BATCH 16
 
 /* ------------------------------------------------------------------ FFT / deconv */
 int fyg_fft_supported(int L);                   /* 2^k or 3*2^k, L <= FYG_FFT_MAX */

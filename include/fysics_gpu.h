@@ -100,12 +100,12 @@ typedef struct {
 } fy_cube;
 
 /* process n cubes on one stream.  Cubes of one region shape (and stage set)
- * share launches: one per stage per 8 cubes, in order of first appearance.  A
+ * share launches: one per stage per 16 cubes, in order of first appearance.  A
  * cube's output does not depend on what it is submitted with.  No reference
  * counterpart: the batched form of process_tile (pipeline.c:252-297). */
 int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream);
 /* the batched form captured: fy_gpu_graph_cubes runs the cubes once, then
- * records the same launches (one per stage per shape group and 8 cubes) into a
+ * records the same launches (one per stage per shape group and 16 cubes) into a
  * HIP graph; fy_graph_launch replays them on `stream` with whatever the cubes'
  * buffers hold at that time.  The cubes' buffers and parameters must stay valid
  * until fy_graph_destroy.  No reference counterpart. */

@@ -99,12 +99,12 @@ def run(cases):
 
 
 def make_cases(rng, want_float=False):
-    """13 cubes of three shapes, interleaved: shape A (9 cubes, more than one
-    launch of FYG_BATCH), shape B (2), shape C with ring removal (2); one cube
-    of shape A is all air."""
+    """22 cubes of three shapes, interleaved: shape A (18 cubes, more than one
+    launch of FYG_BATCH = 16), shape B (2), shape C with ring removal (2); one
+    cube of shape A is all air."""
     ring, _ = ring_model((24, 48, 56), np.random.default_rng(7))
     assert ring.have_rings, "the ringed phantom must exercise the ring-removal group"
-    shapes = ["A", "B", "A", "C", "A", "A", "B", "A", "A", "C", "A", "A", "A"]
+    shapes = ["A", "B", "A", "C", "A", "A", "B", "A", "A", "C"] + ["A"] * 12
     dims = {"A": (20, 24, 28), "B": (24, 32, 40), "C": (16, 36, 40)}
     cases = []
     for i, k in enumerate(shapes):
