@@ -68,6 +68,10 @@ int  fyg_last_error(void);                                /* sticky launch error
 int  fyg_flag(fyg_stream *s, int **d_flag, const int **d_gen);
 /* the same with n flags (one per cube of a batch), all tested against one g */
 int  fyg_flags(fyg_stream *s, int n, int **d_flags, const int **d_gen);
+/* fyg_flags without the step: the caller steps g (nflag flags follow it) with
+ * fyg_gen_step or in the first kernel of its run, before anything reads g */
+int  fyg_flags_reserve(fyg_stream *s, int n, int **d_flags, int **d_gen, int *nflag);
+int  fyg_gen_step(int *d_gen, int nflag, fyg_stream *s);
 const char *fyg_error_string(void);
 void fyg_set_error(const char *msg);            /* host-side failures (I/O, arguments) */
 
@@ -150,6 +154,9 @@ typedef struct {
     int *nonzero_flag;               /* optional: receives *nonzero_gen if any input u8 is nonzero */
     const int *nonzero_gen;          /* device generation word of the call */
     int f64;                         /* all-FP64 variant: spectrum, weights and out in double */
+    int *step_gen;                   /* optional (a[0] of a batch): generation word to step before
+                                        anything reads it -- folded into the first ring-image launch */
+    int step_nflag;                  /* flags that follow *step_gen */
 } fyg_deconv_args;
 int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s);
 /* n deconvolutions of one shape (same nz..hxp, in_u8 and pro.dering) */

@@ -391,9 +391,11 @@ struct RingImg {
     int ny, nx, s0, nimg;
 };
 
-// grid: (x / 256, y, cube * mslab + slab)
-__global__ __launch_bounds__(256) void k_ring_image(Batch<RingImg> b, int mslab) {
+// grid: (x / 256, y, cube * mslab + slab).  step: the call's generation word
+// to step (workgroup 0, before any thread returns), or null
+__global__ __launch_bounds__(256) void k_ring_image(Batch<RingImg> b, int mslab, int *step, int nflag) {
     #pragma clang fp contract(off)
+    if (step && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) fyg_gen_step_block(step, nflag);
     const RingImg &r = b.a[blockIdx.z / mslab];
     const fyg_prologue &P = r.P;
     const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, si = blockIdx.z % mslab;
@@ -890,6 +892,7 @@ int deconv_batch(const fyg_deconv_args *a, int n, fyg_stream *s) {
         xi[i] = XInv<P>{spec, (P *)d->out, d->nz, d->ny, d->nx, d->py, d->hx, d->hxp,
                         (long)d->nz * ((d->ny + 1) / 2), (long)(d->ny + 1) / 2, invn, d->epi, twx};
     }
+    if (h.step_gen && !(h.in_u8 && h.pro.dering) && fyg_gen_step(h.step_gen, h.step_nflag, s)) return 1;
     if (h.in_u8 && h.pro.dering) {
         std::vector<RingImg> ri(n);
         int mslab = 1;
@@ -904,8 +907,9 @@ int deconv_batch(const fyg_deconv_args *a, int n, fyg_stream *s) {
             int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
             Batch<RingImg> b{};
             for (int j = 0; j < m; j++) b.a[j] = ri[i + j];
+            int *step = i == 0 ? h.step_gen : nullptr;   // the call's first launch steps the generation
             fyg_launch(s, "ring_image", k_ring_image, dim3((unsigned)((h.nx + 255) / 256), (unsigned)h.ny,
-                       (unsigned)(m * mslab)), dim3(256), 0, b, mslab);
+                       (unsigned)(m * mslab)), dim3(256), 0, b, mslab, step, h.step_nflag);
         }
     }
     const bool u8 = h.in_u8 != 0;

@@ -35,6 +35,21 @@ inline void fyg_launch(fyg_stream *s, const char *name, void (*k)(KArgs...), dim
     fyg_prof::end(s, tok);
 }
 
+// the step of the device generation word (flags follow it): gen <- gen + 1;
+// on reaching the wrap value the flags are cleared first, so no stale
+// generation can match a new one.  Run by one whole workgroup (it
+// synchronises), before any thread of that workgroup returns.
+__device__ __forceinline__ void fyg_gen_step_block(int *gen, int nflag) {
+    __shared__ int g;
+    if (threadIdx.x == 0) g = *gen;
+    __syncthreads();
+    const bool wrap = g >= 0x7ffffff0;
+    if (wrap)
+        for (int i = threadIdx.x; i < nflag; i += blockDim.x) gen[1 + i] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) *gen = wrap ? 1 : g + 1;
+}
+
 // per-cube descriptors of one batched launch, passed by value in the kernel
 // arguments; block index -> cube
 template <class T>

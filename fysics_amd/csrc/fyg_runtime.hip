@@ -226,20 +226,9 @@ extern "C" int fyg_zero(void *dst, size_t bytes, fyg_stream *s) {
     return fyg_last_error();
 }
 
-// gen <- gen + 1; on reaching the wrap value the flags are cleared first, so
-// no stale generation can match a new one
-__global__ void k_gen_step(int *gen, int *flags, int nflag) {
-    __shared__ int g;
-    if (threadIdx.x == 0) g = *gen;
-    __syncthreads();
-    const bool wrap = g >= 0x7ffffff0;
-    if (wrap)
-        for (int i = threadIdx.x; i < nflag; i += blockDim.x) flags[i] = 0;
-    __syncthreads();
-    if (threadIdx.x == 0) *gen = wrap ? 1 : g + 1;
-}
+__global__ void k_gen_step(int *gen, int nflag) { fyg_gen_step_block(gen, nflag); }
 
-extern "C" int fyg_flags(fyg_stream *s, int n, int **d_flags, const int **d_gen) {
+extern "C" int fyg_flags_reserve(fyg_stream *s, int n, int **d_flags, int **d_gen, int *nflag) {
     if (!s || n < 1) return 0;
     if (n > s->nflag) {
         // earlier generations may still be tested by queued work on this stream
@@ -254,10 +243,22 @@ extern "C" int fyg_flags(fyg_stream *s, int n, int **d_flags, const int **d_gen)
         if (fyg_zero(s->flag, sizeof(int) * (cap + 1), s)) return 0;   // zero never equals a generation
         s->nflag = cap;
     }
-    fyg_launch(s, "gen_step", k_gen_step, dim3(1), dim3(256), 0, s->flag, s->flag + 1, s->nflag);
-    if (fyg_last_error()) return 0;
     *d_flags = s->flag + 1;
     *d_gen = s->flag;
+    *nflag = s->nflag;
+    return 1;
+}
+
+extern "C" int fyg_gen_step(int *d_gen, int nflag, fyg_stream *s) {
+    fyg_launch(s, "gen_step", k_gen_step, dim3(1), dim3(256), 0, d_gen, nflag);
+    return fyg_last_error();
+}
+
+extern "C" int fyg_flags(fyg_stream *s, int n, int **d_flags, const int **d_gen) {
+    int *g = nullptr, nf = 0;
+    if (!fyg_flags_reserve(s, n, d_flags, &g, &nf)) return 0;
+    if (fyg_gen_step(g, nf, s)) return 0;
+    *d_gen = g;
     return 1;
 }
 

@@ -252,9 +252,23 @@ static int same_group(const cube_plan *a, const cube_plan *b) {
     return 1;
 }
 
-/* the chain of n planned cubes of one group; flags[i] is cube i's all-air flag */
-static int run_group(cube_plan *pl, int n, int **flags, const int *gen, fyg_stream *s) {
+/* the generation word of a run, still to be stepped before its first kernel
+ * reads it (NULL once stepped), and the number of flags that follow it */
+typedef struct {
+    int *gen;
+    int nflag;
+} gen_step;
+
+/* the chain of n planned cubes of one group; flags[i] is cube i's all-air flag.
+ * The run's first group steps the generation: inside its first ring-image
+ * launch when it has one, else with a launch of its own. */
+static int run_group(cube_plan *pl, int n, int **flags, const int *gen, gen_step *st, fyg_stream *s) {
     const cube_plan *h = &pl[0];
+    const int fold = st->gen && h->deconv && h->pro.dering;
+    if (st->gen && !fold) {
+        if (fyg_gen_step(st->gen, st->nflag, s)) return 1;
+        st->gen = NULL;
+    }
     size_t tile = al256(h->tile_bytes), spec = al256(h->spec_bytes), img = 0, gws = al256(h->gws_bytes);
     for (int i = 0; i < n; i++)                                 /* ring images may span more slabs */
         if (al256(pl[i].img_bytes) > img) img = al256(pl[i].img_bytes);
@@ -276,6 +290,11 @@ static int run_group(cube_plan *pl, int n, int **flags, const int *gen, fyg_stre
             if (pl[i].pro.dering) da[i].pro.cimg = (float *)(I + img * i);
             da[i].nonzero_flag = flags[i];
             da[i].nonzero_gen = gen;
+        }
+        if (fold) {
+            da[0].step_gen = st->gen;
+            da[0].step_nflag = st->nflag;
+            st->gen = NULL;
         }
         int rc = fyg_deconv_run_batch(da, n, s);
         free(da);
@@ -347,7 +366,7 @@ static int run_group(cube_plan *pl, int n, int **flags, const int *gen, fyg_stre
  * stage set per group, in order of first appearance), each group in as few
  * launches per stage as FYG_BATCH and FYH_GROUP_BYTES allow.  Results do not
  * depend on the grouping: every cube has its own workspace slice and flag. */
-static int run_chunk(const fy_cube *cubes, int n, int *d_flags, const int *gen, fyg_stream *s) {
+static int run_chunk(const fy_cube *cubes, int n, int *d_flags, const int *gen, gen_step *st, fyg_stream *s) {
     cube_plan *pl = (cube_plan *)malloc(sizeof *pl * (size_t)n);
     cube_plan *grp = (cube_plan *)malloc(sizeof *grp * (size_t)n);
     int *gi = (int *)malloc(sizeof(int) * (size_t)n);
@@ -373,20 +392,23 @@ static int run_chunk(const fy_cube *cubes, int n, int *d_flags, const int *gen, 
         int part = (int)(FYH_GROUP_BYTES / (per ? per : 1));
         if (part < 1) part = 1;
         for (int k = 0; k < m && !rc; k += part)
-            rc = run_group(grp + k, m - k < part ? m - k : part, gf + k, gen, s);
+            rc = run_group(grp + k, m - k < part ? m - k : part, gf + k, gen, st, s);
     }
     free(pl); free(grp); free(gi); free(gf);
     return rc;
 }
 
-static int run_cubes(const fy_cube *cubes, int n, int *d_flags, const int *gen, fyg_stream *s) {
+/* n cubes on the stream's flags (reserved, not yet stepped) */
+static int run_cubes(const fy_cube *cubes, int n, int *d_flags, int *gen, int nflag, fyg_stream *s) {
+    gen_step st = {gen, nflag};
     for (int i = 0; i < n; i += FYH_PLAN_CHUNK)
-        if (run_chunk(cubes + i, n - i < FYH_PLAN_CHUNK ? n - i : FYH_PLAN_CHUNK, d_flags + i, gen, s)) return 1;
-    return 0;
+        if (run_chunk(cubes + i, n - i < FYH_PLAN_CHUNK ? n - i : FYH_PLAN_CHUNK, d_flags + i, gen, &st, s))
+            return 1;
+    return st.gen ? fyg_gen_step(st.gen, st.nflag, s) : 0;    /* nothing ran: step anyway */
 }
 
-static int chain_cube(const fy_cube *c, int *d_flag, const int *gen, fyg_stream *s) {
-    return run_cubes(c, 1, d_flag, gen, s);
+static int chain_cube(const fy_cube *c, int *d_flag, int *gen, int nflag, fyg_stream *s) {
+    return run_cubes(c, 1, d_flag, gen, nflag, s);
 }
 
 size_t fy_cube_workspace_bytes(const fy_cube *c) {
@@ -418,10 +440,9 @@ int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream) {
     if (n <= 0) return 0;
     fyg_stream *s = fyg_stream_wrap(stream);
     if (!s) return 1;
-    int *dflags = NULL;
-    const int *dgen = NULL;
-    if (!fyg_flags(s, n, &dflags, &dgen)) return 1;
-    return run_cubes(cubes, n, dflags, dgen, s);
+    int *dflags = NULL, *dgen = NULL, nflag = 0;
+    if (!fyg_flags_reserve(s, n, &dflags, &dgen, &nflag)) return 1;
+    return run_cubes(cubes, n, dflags, dgen, nflag, s);
 }
 
 /* ---- captured batched form: the launches of one fy_gpu_process_cubes call
@@ -438,15 +459,15 @@ fy_graph *fy_gpu_graph_cubes(const fy_cube *cubes, int n, void *stream) {
     fy_graph *g = (fy_graph *)calloc(1, sizeof *g);
     if (!g) return NULL;
     g->s = fyg_stream_private(stream);
-    int *dflags = NULL;
-    const int *dgen = NULL;
+    int *dflags = NULL, *dgen = NULL, nflag = 0;
     /* an uncaptured call first: sizes every workspace and makes every table resident */
-    int rc = !g->s || !fyg_flags(g->s, n, &dflags, &dgen) || run_cubes(cubes, n, dflags, dgen, g->s) ||
-             fyg_sync(g->s);
+    int rc = !g->s || !fyg_flags_reserve(g->s, n, &dflags, &dgen, &nflag) ||
+             run_cubes(cubes, n, dflags, dgen, nflag, g->s) || fyg_sync(g->s);
     if (!rc) {
         rc = fyg_capture_begin(g->s);
         if (!rc) {
-            int r1 = !fyg_flags(g->s, n, &dflags, &dgen) || run_cubes(cubes, n, dflags, dgen, g->s);
+            int r1 = !fyg_flags_reserve(g->s, n, &dflags, &dgen, &nflag) ||
+                     run_cubes(cubes, n, dflags, dgen, nflag, g->s);
             int r2 = fyg_capture_end(g->s, &g->exec);
             rc = r1 || r2;
         }
@@ -469,11 +490,11 @@ void fy_graph_destroy(fy_graph *g) {
 
 int fyh_chain(const fy_pipeline_cfg *cfg, const unsigned char *d_u8, long rz0, long ry0, long rx0, long hz,
               long hy, long hx, long z0, long y0, long x0, long tz, long ty, long tx, long vol_y, long vol_x,
-              unsigned char *d_out, int *d_flag, const int *gen, fyg_stream *s) {
+              unsigned char *d_out, int *d_flag, int *gen, int nflag, fyg_stream *s) {
     fy_cube_params P;
     if (fy_cube_params_from_cfg(cfg, y0, x0, ty, tx, vol_y, vol_x, &P)) return 1;
     fy_cube c = {d_u8, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, d_out, NULL, &P};
-    return chain_cube(&c, d_flag, gen, s);
+    return chain_cube(&c, d_flag, gen, nflag, s);
 }
 
 int fy_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf, long rz0, long ry0, long rx0,
@@ -488,11 +509,11 @@ int fy_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf, lo
     if (!s) return 1;
     unsigned char *du = (unsigned char *)fyg_ws(s, WS_U8, hn);
     unsigned char *dq = (unsigned char *)fyg_ws(s, WS_OUT, tn);
-    int *dflag = NULL;
-    const int *dgen = NULL;
-    if (!du || !dq || !fyg_flag(s, &dflag, &dgen)) return 1;
+    int *dflag = NULL, *dgen = NULL, nflag = 0;
+    if (!du || !dq || !fyg_flags_reserve(s, 1, &dflag, &dgen, &nflag)) return 1;
     if (fyg_h2d(du, u8buf, hn, s)) return 1;
-    if (fyh_chain(cfg, du, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, dq, dflag, dgen, s))
+    if (fyh_chain(cfg, du, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, dq, dflag, dgen, nflag,
+                  s))
         return 1;
     int flag = 0, gen = 0;
     if (fyg_d2h(out, dq, tn, s) || fyg_d2h(&flag, dflag, sizeof(int), s) || fyg_d2h(&gen, dgen, sizeof(int), s) ||
@@ -507,10 +528,10 @@ int fy_gpu_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *d_u8,
                           long vol_y, long vol_x, unsigned char *d_out, int *all_air, void *stream) {
     fyg_stream *s = fyg_stream_wrap(stream);
     if (!s) return 1;
-    int *dflag = NULL;
-    const int *dgen = NULL;
-    if (!fyg_flag(s, &dflag, &dgen)) return 1;
-    if (fyh_chain(cfg, d_u8, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, d_out, dflag, dgen, s))
+    int *dflag = NULL, *dgen = NULL, nflag = 0;
+    if (!fyg_flags_reserve(s, 1, &dflag, &dgen, &nflag)) return 1;
+    if (fyh_chain(cfg, d_u8, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, vol_y, vol_x, d_out, dflag, dgen,
+                  nflag, s))
         return 1;
     if (all_air) {
         int flag = 0, gen = 0;

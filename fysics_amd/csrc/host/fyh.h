@@ -36,11 +36,13 @@ int fyh_ring_fast(double cy, double cx, long y0, long x0, long dy, long dx);
 /* device copy of a small host array, cached by content */
 const void *fyh_const_blob(char tag, const void *data, size_t bytes);
 
-/* run the whole tile chain of fy_process_buffer on device buffers */
+/* run the whole tile chain of fy_process_buffer on device buffers; gen and
+ * nflag come from fyg_flags_reserve and are stepped by the chain itself */
 int fyh_chain(const fy_pipeline_cfg *cfg, const unsigned char *d_u8,
               long rz0, long ry0, long rx0, long hz, long hy, long hx,
               long z0, long y0, long x0, long tz, long ty, long tx,
-              long vol_y, long vol_x, unsigned char *d_out, int *d_flag, const int *gen, fyg_stream *s);
+              long vol_y, long vol_x, unsigned char *d_out, int *d_flag, int *gen, int nflag,
+              fyg_stream *s);
 
 /* deconvolution on device buffers (fp32 in/out) */
 int fyh_deconv_dev(const float *d_in, float *d_out, int nz, int ny, int nx, const fy_physics *p,
