@@ -536,14 +536,22 @@ __device__ __forceinline__ int up_col(int c) { return (c & 1) * UPH + (c >> 1); 
 // the reference, each output row blends its four (z,y) low-res rows first and
 // then lerps along x; a thread blends only the three low-res columns its four
 // consecutive voxels need.  Thread (q, g) owns x quad q of rows g, g+4, g+8, ...
-// P2: s is a power of two, so c/s is exactly c*(1/s) and needs no division.
+// P2: s = 2^sh, so c/s is exactly c*(1/s) and needs no division; the knot
+// is then c >> sh and the fraction (c - (knot << sh)) / s, both exact, so the
+// row arithmetic (uniform across a wave) stays on the scalar unit.
 template <bool P2, class P = float>
 struct UpCoord {
     P fs, inv;
+    int sh;
     __device__ P u(int c) const { return P2 ? (P)c * inv : (P)c / fs; }
     __device__ int i0(int c, int l) const {
-        int i = (int)u(c);
+        int i = P2 ? c >> sh : (int)u(c);
         return i > l - 2 ? l - 2 : i;
+    }
+    // min(u(c) - i, 1) for the knot i = i0(c, l)
+    __device__ P frac(int c, int i) const {
+        P f = P2 ? (P)(c - (i << sh)) * inv : u(c) - (P)i;
+        return f > (P)1 ? (P)1 : f;
     }
 };
 
@@ -553,7 +561,7 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
     const UpArgs<P> &p = bt.a[blockIdx.z / nzb];
     constexpr int RPT = UpShape<URZ>::RPT, UPZ = UpShape<URZ>::UPZ;
     __shared__ P pa[UPZ][UPY][UPK], pb[UPZ][UPY][UPK];
-    const UpCoord<P2, P> U{(P)p.s, (P)1 / (P)p.s};
+    const UpCoord<P2, P> U{(P)p.s, (P)1 / (P)p.s, P2 ? __builtin_ctz((unsigned)p.s) : 0};
     const int Z0 = p.z0 + (int)(blockIdx.z % nzb) * URZ, Y0 = p.y0 + blockIdx.y * URY;
     const int zl = min(Z0 + URZ, p.z0 + p.dz) - 1, yl = min(Y0 + URY, p.y0 + p.dy) - 1;
     const int xc0 = p.x0 + blockIdx.x * 64 * UPX;
@@ -637,7 +645,7 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
 #pragma unroll
     for (int e = 0; e < UPX; e++) {
         int x = xb + e, x0 = U.i0(x, p.lx);
-        P fx = U.u(x) - x0;
+        P fx = U.frac(x, x0);
         fxs[e] = fx > 1.0f ? 1.0f : fx;
         dsel[e] = x0 - kb - k0 != 0;            // 0 or 1
     }
@@ -648,8 +656,8 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
         int r = g + 4 * j, z = Z0 + r / URY, y = Y0 + r % URY;
         if (z > zl || y > yl) continue;
         int z0 = U.i0(z, p.lz), y0 = U.i0(y, p.ly);
-        P zf = U.u(z) - z0; if (zf > 1.0f) zf = 1.0f;
-        P yf = U.u(y) - y0; if (yf > 1.0f) yf = 1.0f;
+        P zf = U.frac(z, z0);
+        P yf = U.frac(y, y0);
         const P w00 = (1 - zf) * (1 - yf), w01 = (1 - zf) * yf, w10 = zf * (1 - yf), w11 = zf * yf;
         const int pz = z0 - zlo, py = y0 - ylo;
         auto blend = [&](const P (*Q)[UPY][UPK], int c) {
