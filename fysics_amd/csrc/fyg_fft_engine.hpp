@@ -620,9 +620,8 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
     // t = ((fz2[z] + fy2[y]) + fx2[k]) * us, linear interpolation between knots.
     // The knots an item needs are loaded before the forward stages, so their
     // latency hides behind the spectrum's loads instead of following them.
-    // The reference forms t in double; the fp32 path forms it in fp32 (FP64 is
-    // kept to table generation), which moves t by ~1e-4 of a knot spacing and
-    // the weight by far less than the fp32 rounding of the product.
+    // t and its knot are formed in double, as the reference does: the knot
+    // index is part of the weight's semantics.
     constexpr int NB = L / RL;
     constexpr int TOT = NL * NB;
     constexpr int BPT = (TOT + T - 1) / T;
@@ -633,16 +632,16 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
         if (TOT % T == 0 || i < TOT) {
             int l, j;
             item_of<NL, NB, true>(i, l, j);
-            const P fy = (P)s_fy[l], fx = (P)s_fx[l], us = (P)a.us;
+            const double fy = s_fy[l], fx = s_fx[l];
 #pragma unroll
             for (int r = 0; r < RL; r++) {
-                P fzy = (P)a.fz2[j + r * NB] + fy;
-                P t = (fzy + fx) * us;
+                double fzy = a.fz2[j + r * NB] + fy;
+                double t = (fzy + fx) * a.us;
                 int ti = (int)t;
                 if (ti >= 4096) ti = 4095;
                 k0[b][r] = a.lut[ti];
                 k1[b][r] = a.lut[ti + 1];
-                kf[b][r] = t - (P)ti;
+                kf[b][r] = (P)(t - (double)ti);
             }
         }
     }
