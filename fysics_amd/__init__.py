@@ -235,6 +235,9 @@ SIGNATURES = {
                                 C.POINTER(ExportStats), _VP]),
     "fy_gpu_tuning": (C.c_int, [C.c_int]),
     "fy_gpu_tuning_report": (C.c_int, [C.c_char_p, C.c_int]),
+    "fy_gpu_tuning_save": (C.c_int, [C.c_char_p]),
+    "fy_gpu_tuning_load": (C.c_int, [C.c_char_p]),
+    "fy_gpu_tuning_clear": (None, []),
     "fy_gpu_sync": (C.c_int, [_VP]),
     "fy_gpu_profile": (None, [C.c_int]),
     "fy_gpu_profile_read": (C.c_int, [C.c_char_p, C.c_int, c_double_p, C.POINTER(C.c_long),

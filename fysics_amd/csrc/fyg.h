@@ -216,6 +216,10 @@ int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s);
 int fyg_tune_mode(int mode);
 /* one line per tuned shape; returns the report's length */
 int fyg_tune_report(char *buf, int len);
+/* the tuned shapes to / from a text file; clear forgets them all */
+int fyg_tune_save(const char *path);
+int fyg_tune_load(const char *path);
+void fyg_tune_clear(void);
 /* n guided filters; cubes of one shape on the fast path share launches */
 int fyg_guided_run_batch(const fyg_guided_args *a, int n, fyg_stream *s);
 /* prologue-only path (deconv disabled): u8 -> f32 through the tile prologue */

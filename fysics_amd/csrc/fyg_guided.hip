@@ -17,6 +17,8 @@
 //                                  k_box2 -> ma, mb (low-res); k_upsample -> out
 #include <atomic>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -895,6 +897,7 @@ constexpr int kUpRZ[kNUp] = {2, 4};
 struct TuneRec {
     int coef = 0, up = 0;
     bool coef_done = false, up_done = false;
+    bool loaded = false;                     // from a tuning file, not timed here
     float coef_us[kNCoef] = {}, up_us[kNUp] = {};
 };
 
@@ -1144,6 +1147,56 @@ static int fast_batch(const fyg_guided_args *a, int n, fyg_stream *s) {
 
 extern "C" int fyg_tune_mode(int mode) { return g_tune_mode.exchange(mode); }
 
+// Tuning file: one line per shape, "key<TAB>coef<TAB>up" (up -1: not tuned).
+// Saving writes every shape tuned or loaded so far; loading adds the file's
+// shapes, which are then used without timing (a later process starts tuned).
+extern "C" int fyg_tune_save(const char *path) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::string tmp = std::string(path) + ".tmp";
+    FILE *f = fopen(tmp.c_str(), "w");
+    if (!f) return 1;
+    int bad = fprintf(f, "# fysics_amd guided-filter tile shapes: key\tcoef candidate\tupsample candidate\n") < 0;
+    for (auto &kv : g_tune)
+        if (kv.second.coef_done)
+            bad |= fprintf(f, "%s\t%d\t%d\n", kv.first.c_str(), kv.second.coef,
+                           kv.second.up_done ? kv.second.up : -1) < 0;
+    bad |= fclose(f) != 0;
+    if (!bad) bad = rename(tmp.c_str(), path) != 0;
+    if (bad) remove(tmp.c_str());
+    return bad;
+}
+
+extern "C" int fyg_tune_load(const char *path) {
+    FILE *f = fopen(path, "r");
+    if (!f) return 1;
+    char line[512];
+    int bad = 0;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        char *t1 = strchr(line, '\t');
+        char *t2 = t1 ? strchr(t1 + 1, '\t') : nullptr;
+        if (!t1 || !t2) { bad = 1; continue; }
+        *t1 = 0;
+        const int coef = atoi(t1 + 1), up = atoi(t2 + 1);
+        if (coef < 0 || coef >= kNCoef || up < -1 || up >= kNUp) { bad = 1; continue; }
+        TuneRec r;
+        r.coef = coef;
+        r.coef_done = true;
+        r.up = up < 0 ? 0 : up;
+        r.up_done = up >= 0;
+        r.loaded = true;
+        g_tune[std::string(line)] = r;
+    }
+    fclose(f);
+    return bad;
+}
+
+extern "C" void fyg_tune_clear(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+}
+
 // one line per tuned shape: key, chosen candidates and their times (us)
 extern "C" int fyg_tune_report(char *buf, int len) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1151,7 +1204,10 @@ extern "C" int fyg_tune_report(char *buf, int len) {
     char line[320];
     for (auto &kv : g_tune) {
         const TuneRec &r = kv.second;
-        if (r.up_done)
+        if (r.loaded)
+            snprintf(line, sizeof line, "%s: coef zchunk #%d, upsample planes #%d%s (from a tuning file)\n",
+                     kv.first.c_str(), r.coef, r.up, r.up_done ? "" : " (untuned)");
+        else if (r.up_done)
             snprintf(line, sizeof line, "%s: coef zchunk #%d (%.1f %.1f %.1f us), upsample planes #%d (%.1f %.1f us)\n",
                      kv.first.c_str(), r.coef, r.coef_us[0], r.coef_us[1], r.coef_us[2], r.up, r.up_us[0],
                      r.up_us[1]);

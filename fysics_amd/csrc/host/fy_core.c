@@ -294,6 +294,9 @@ int fy_gpu_downscale2x(const unsigned char *d_in, int nx, int ny, int nz, unsign
 
 int fy_gpu_tuning(int mode) { return fyg_tune_mode(mode); }
 int fy_gpu_tuning_report(char *buf, int len) { return fyg_tune_report(buf, len); }
+int fy_gpu_tuning_save(const char *path) { return path ? fyg_tune_save(path) : 1; }
+int fy_gpu_tuning_load(const char *path) { return path ? fyg_tune_load(path) : 1; }
+void fy_gpu_tuning_clear(void) { fyg_tune_clear(); }
 
 int fy_gpu_sync(void *stream) {
     fyg_stream *s = fyg_stream_wrap(stream);

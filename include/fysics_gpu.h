@@ -199,6 +199,14 @@ int fy_gpu_export(const fy_zarr *zin, const fy_zarr *zout, const fy_pipeline_cfg
  * candidates' times; returns its length.  No reference counterpart. */
 int fy_gpu_tuning(int mode);
 int fy_gpu_tuning_report(char *buf, int len);
+/* persist the tuned shapes: save writes every shape tuned or loaded so far to
+ * a text file (one "key<TAB>coef<TAB>upsample" line each); load adds a file's
+ * shapes, which later calls then use without timing; clear forgets them all.
+ * Return 0 on success, 1 if the file cannot be written / read or has a bad
+ * line (the good lines are still loaded). */
+int fy_gpu_tuning_save(const char *path);
+int fy_gpu_tuning_load(const char *path);
+void fy_gpu_tuning_clear(void);
 
 int fy_gpu_sync(void *stream);
 

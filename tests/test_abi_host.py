@@ -134,3 +134,31 @@ def test_dering_tile_merge_host(fy):
     assert not np.ctypeslib.as_array(t.cnt, (nb,)).any()
     fy.fy_dering_free(C.byref(t))
     fy.fy_dering_free(C.byref(d))
+
+
+def test_tuning_file_round_trip(tmp_path):
+    """Tile shapes of the guided filter persist in a text file: loaded shapes
+    are reported as such (and used without timing); bad lines are rejected,
+    good ones kept; save writes back what was loaded."""
+    lib = F.lib()
+    lib.fy_gpu_tuning_clear()
+    src = tmp_path / "tune.txt"
+    src.write_text("# comment\n"
+                   "fp32 r1 s2 grid 8x8x8 region 16x16x16 u8\t2\t1\n"
+                   "fp32 r2 s2 grid 9x9x9 region 18x18x18 fp32\t0\t-1\n"
+                   "broken line without tabs\n"
+                   "fp32 r1 s2 grid 4x4x4 region 8x8x8 u8\t7\t0\n")
+    assert lib.fy_gpu_tuning_load(str(src).encode()) == 1          # two bad lines
+    rep = F.tuning_report()
+    assert "grid 8x8x8 region 16x16x16 u8: coef zchunk #2, upsample planes #1 (from a tuning file)" in rep
+    assert "grid 9x9x9 region 18x18x18 fp32: coef zchunk #0, upsample planes #0 (untuned) (from a tuning file)" \
+        in rep
+    assert "grid 4x4x4" not in rep
+    out = tmp_path / "saved.txt"
+    assert lib.fy_gpu_tuning_save(str(out).encode()) == 0
+    lines = [l for l in out.read_text().splitlines() if not l.startswith("#")]
+    assert sorted(lines) == sorted(["fp32 r1 s2 grid 8x8x8 region 16x16x16 u8\t2\t1",
+                                    "fp32 r2 s2 grid 9x9x9 region 18x18x18 fp32\t0\t-1"])
+    assert lib.fy_gpu_tuning_load(str(tmp_path / "missing.txt").encode()) == 1
+    lib.fy_gpu_tuning_clear()
+    assert F.tuning_report() == ""

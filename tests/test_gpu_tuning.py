@@ -68,3 +68,38 @@ def test_first_call_in_place():
         lib.fy_gpu_tuning(old)
     assert np.array_equal(d.cpu().numpy(), d_ref.cpu().numpy())
     assert "region 36x30x42 fp32: coef zchunk" in F.tuning_report()
+
+
+def test_tuned_shapes_persist(tmp_path):
+    """Shapes tuned in one process can be saved and loaded by the next: a
+    loaded shape runs its recorded candidates without timing them, with the
+    same output."""
+    shape = (28, 34, 30)
+    print(f"tuning persist shape={shape}", flush=True)
+    lib = F.lib()
+    rng = np.random.default_rng(3)
+    f = (phantom.sheets(shape, rng, n_sheets=4) + 0.05 * rng.standard_normal(shape)).astype(np.float32)
+    d_in = torch.from_numpy(f).cuda()
+
+    def run():
+        d_out = torch.empty_like(d_in)
+        torch.cuda.synchronize()
+        assert lib.fy_gpu_guided(C.c_void_p(d_in.data_ptr()), C.c_void_p(d_out.data_ptr()), *shape, 2, 2e-3, 2,
+                                 None) == 0
+        assert lib.fy_gpu_sync(None) == 0
+        return d_out.cpu().numpy()
+
+    old = lib.fy_gpu_tuning(1)
+    try:
+        first = run()                                   # tunes this shape
+        path = str(tmp_path / "tune.txt").encode()
+        assert lib.fy_gpu_tuning_save(path) == 0
+        lib.fy_gpu_tuning_clear()
+        assert lib.fy_gpu_tuning_load(path) == 0
+        key = "region 28x34x30 fp32"
+        assert any(key in l and "from a tuning file" in l for l in F.tuning_report().splitlines())
+        again = run()
+        assert any(key in l and "from a tuning file" in l for l in F.tuning_report().splitlines())
+    finally:
+        lib.fy_gpu_tuning(old)
+    assert np.array_equal(first, again)
