@@ -536,17 +536,38 @@ struct Cols {
     const V *tw;
 };
 
+// Lines longer than 256 leave a workgroup 8 fp32 columns (64 bytes of each
+// 128-byte row line; 4 fp64 columns past 256 in double), so two neighbouring
+// workgroups share every line.
+// Workgroups are dispatched to the 8 XCDs round-robin, each XCD with its own
+// L2; pair_block renumbers them so that the workgroups of logical groups 2k
+// and 2k+1 are dispatched 8 apart, i.e. together on one XCD, and the second
+// one's half of each line is an L2 hit.  Identity on a grid's last partial 16.
+__device__ __forceinline__ long pair_block(long h, long T) {
+    const long c = h >> 4, i = h & 15;
+    if ((c + 1) << 4 > T) return h;
+    return (c << 4) + 2 * (i & 7) + (i >> 3);
+}
+
 template <int L, class P>
 __global__ __launch_bounds__(256) void k_cfft_cols(Batch<Cols<P>> bt) {
     using V = typename Cx<P>::V;
+    constexpr bool PAIR = Shape<L, P>::NL_CM * 2 * sizeof(P) < 128;
+    long bx = blockIdx.x, by = blockIdx.y;
+    if (PAIR) {
+        const long T = (long)gridDim.x * gridDim.y;
+        const long h = pair_block(blockIdx.x + (long)gridDim.x * blockIdx.y, T);
+        bx = h % gridDim.x;
+        by = h / gridDim.x;
+    }
     const Cols<P> &a = bt.a[blockIdx.z];
     constexpr int NL = Shape<L, P>::NL_CM;
     __shared__ V buf[NL * Shape<L>::LD];
     __shared__ V tw_s[L];
     load_twiddles<L>(tw_s, a.tw);
     __syncthreads();
-    const long c0 = (long)blockIdx.x * NL;
-    V *base = a.data + (size_t)blockIdx.y * a.bstride + c0;
+    const long c0 = bx * NL;
+    V *base = a.data + (size_t)by * a.bstride + c0;
     const long nc = a.ncols - c0;
     const size_t es = (size_t)a.estride;
     // Columns past ncols (the last group of a row) load column nc-1 instead --
@@ -584,6 +605,7 @@ template <int L, class P>
 __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
     using V = typename Cx<P>::V;
     const ZFused<P> &a = bt.a[blockIdx.y];
+    const long bx = Shape<L, P>::NL_CM * 2 * sizeof(P) < 128 ? pair_block(blockIdx.x, gridDim.x) : (long)blockIdx.x;
     constexpr int NL = Shape<L, P>::NL_CM;
     constexpr int T = Shape<L>::T;
     using PF = Plan<L, false>;
@@ -595,7 +617,7 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
     __shared__ double s_fy[NL], s_fx[NL];
     __shared__ int s_ok[NL];
     load_twiddles<L>(tw_s, a.tw);
-    const long c0 = (long)blockIdx.x * NL;
+    const long c0 = bx * NL;
     if (threadIdx.x < NL) {
         long c = c0 + threadIdx.x;
         int y = (int)(c / a.hxp), k = (int)(c - (long)y * a.hxp);
