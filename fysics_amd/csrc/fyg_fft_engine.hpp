@@ -116,6 +116,9 @@ struct Shape {
     static constexpr int T = 256;
     static constexpr int Q = sizeof(P) / sizeof(float);
     static constexpr int NL_CM = (L <= 256 ? 16 : (4096 / L)) / Q < 1 ? 1 : (L <= 256 ? 16 : (4096 / L)) / Q;
+    // the fused Z pass keeps full 128-byte rows up to L = 512 (2 workgroups
+    // of 74 KB per CU): it measured faster than half rows at 4 per CU
+    static constexpr int NL_ZF = (L <= 512 ? 16 : (4096 / L)) / Q < 1 ? 1 : (L <= 512 ? 16 : (4096 / L)) / Q;
     static constexpr int RM0 = (4096 / Q) / L;
     static constexpr int NL_RM = RM0 > 256 ? 256 : RM0 < 1 ? 1 : RM0;
     static constexpr int LD = L + (L >> 4) + 1;   // padded line pitch in LDS (complex elements)
@@ -605,8 +608,8 @@ template <int L, class P>
 __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
     using V = typename Cx<P>::V;
     const ZFused<P> &a = bt.a[blockIdx.y];
-    const long bx = Shape<L, P>::NL_CM * 2 * sizeof(P) < 128 ? pair_block(blockIdx.x, gridDim.x) : (long)blockIdx.x;
-    constexpr int NL = Shape<L, P>::NL_CM;
+    const long bx = Shape<L, P>::NL_ZF * 2 * sizeof(P) < 128 ? pair_block(blockIdx.x, gridDim.x) : (long)blockIdx.x;
+    constexpr int NL = Shape<L, P>::NL_ZF;
     constexpr int T = Shape<L>::T;
     using PF = Plan<L, false>;
     constexpr int n = PF::n;
@@ -849,7 +852,7 @@ struct Launch {
     }
     static void zfused(fyg_stream *s, const ZFused<P> *a, int n) {
         long nc = (long)a[0].py * a[0].hxp;
-        unsigned g = (unsigned)((nc + Shape<L, P>::NL_CM - 1) / Shape<L, P>::NL_CM);
+        unsigned g = (unsigned)((nc + Shape<L, P>::NL_ZF - 1) / Shape<L, P>::NL_ZF);
         for (int i = 0; i < n; i += FYG_BATCH) {
             int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
             Batch<ZFused<P>> b{};
