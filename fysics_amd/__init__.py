@@ -233,6 +233,9 @@ SIGNATURES = {
                                    c_int_p, c_int_p]),
     "fy_gpu_export": (C.c_int, [_ZP, _ZP, C.POINTER(PipelineCfg), C.c_int, _L, _L, C.c_int, C.c_int,
                                 C.POINTER(ExportStats), _VP]),
+    "fy_calib_commit_eps": (None, [C.POINTER(PipelineCfg), c_double_p, c_double_p, C.c_int]),
+    "fy_aniso_ratio": (C.c_double, [C.c_double, C.c_double]),
+    "fy_calib_commit_aniso": (None, [C.POINTER(PipelineCfg), c_double_p, C.c_int]),
     "fy_gpu_tuning": (C.c_int, [C.c_int]),
     "fy_gpu_tuning_report": (C.c_int, [C.c_char_p, C.c_int]),
     "fy_gpu_tuning_save": (C.c_int, [C.c_char_p]),

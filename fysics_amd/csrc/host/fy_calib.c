@@ -11,6 +11,7 @@
  * fy_gpu_last_error() says why; fy_flat_noise then returns -1, a value no
  * successful call returns.
  */
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -92,6 +93,70 @@ double fy_valley_depth(const long hist[256], int *dark_out, int *light_out, int 
 static int cmp_double(const void *a, const void *b) {
     double x = *(const double *)a, y = *(const double *)b;
     return (x > y) - (x < y);
+}
+
+/* median of n values (the mean of the middle two for even n) of a sorted copy */
+static double median_of(const double *v, int n) {
+    if (n <= 0) return 0.0;
+    double *t = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!t) return 0.0;
+    memcpy(t, v, sizeof(double) * (size_t)n);
+    qsort(t, (size_t)n, sizeof(double), cmp_double);
+    double m = (n & 1) ? t[n / 2] : 0.5 * (t[n / 2 - 1] + t[n / 2]);
+    free(t);
+    return m;
+}
+
+/* pipeline.c:851-891: radial eps fit, median floor, guided eps commit */
+void fy_calib_commit_eps(fy_pipeline_cfg *cfg, const double *rads, const double *floors, int n) {
+    cfg->have_eps_r = 0;
+    if (n >= 8) {
+        double sr = 0, sf = 0, srr = 0, srf = 0, sff = 0;
+        for (int i = 0; i < n; i++) {
+            sr += rads[i];
+            sf += floors[i];
+            srr += rads[i] * rads[i];
+            srf += rads[i] * floors[i];
+            sff += floors[i] * floors[i];
+        }
+        const double nn = n, den = nn * srr - sr * sr;
+        if (den > 1e-9) {
+            const double b = (nn * srf - sr * sf) / den;
+            const double a = (sf - b * sr) / nn;
+            const double cden = sqrt((nn * srr - sr * sr) * (nn * sff - sf * sf));
+            const double corr = cden > 1e-12 ? (nn * srf - sr * sf) / cden : 0;
+            double rmax = 0;
+            for (int i = 0; i < n; i++)
+                if (rads[i] > rmax) rmax = rads[i];
+            const double f0 = a, f1 = a + b * rmax;
+            if (b > 0 && corr >= 0.5 && f0 > 0 && f1 / f0 >= 1.3) {
+                cfg->eps_fn_a = a;
+                cfg->eps_fn_b = b;
+                cfg->have_eps_r = 1;
+            }
+        }
+    }
+    const double med = n > 0 ? median_of(floors, n) : 0.015;
+    cfg->eps_fn_med = med;
+    cfg->noise_floor = med * 255.0;
+    if (cfg->guided_eps <= 0 && cfg->denoise_k >= 0) {
+        const double k = cfg->denoise_k > 0 ? cfg->denoise_k : 4.2;
+        cfg->guided_eps = (k * med) * (k * med);
+    }
+}
+
+/* pipeline.c:736-739 */
+double fy_aniso_ratio(double arz, double arxy) {
+    if (!(arz > 0.05 && arz < 0.95 && arxy > 0.05 && arxy < 0.95)) return -1.0;
+    const double ratio = sqrt(log(1.0 / arxy) / log(1.0 / arz));
+    return (ratio > 0.5 && ratio < 3.0) ? ratio : -1.0;
+}
+
+/* pipeline.c:894-900 */
+void fy_calib_commit_aniso(fy_pipeline_cfg *cfg, const double *ratios, int n) {
+    if (n < 5) return;
+    const double r = median_of(ratios, n);
+    if (r >= 1.05) cfg->psf_z_ratio = r < 1.6 ? r : 1.6;
 }
 
 /* the 10th percentile (element m/10 of the ascending order) of the kept stds */

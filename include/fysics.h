@@ -227,6 +227,25 @@ typedef struct {
     long occ_cz, occ_cy, occ_cx;
 } fy_pipeline_cfg;
 
+/* Pass-1 commits of fy_run_pipeline as functions (pipeline.c:735-740,851-900;
+ * no reference counterpart: the reference runs them inline).  Host fp64.
+ * fy_calib_commit_eps: from n flat-noise floors of sample sub-blocks and the
+ * distances of their centres to the rotation axis, the gated least-squares fit
+ * fn(r) = a + b*r (needs n >= 8, b > 0, correlation >= 0.5, a > 0 and
+ * fn(rmax)/a >= 1.3; sets have_eps_r, eps_fn_a, eps_fn_b, or clears
+ * have_eps_r), eps_fn_med = the median floor (0.015 with no samples),
+ * noise_floor = 255 * median, and, when guided_eps <= 0 and denoise_k >= 0,
+ * guided_eps = (k * median)^2 with k = denoise_k (4.2 when 0).  The inputs
+ * are not modified. */
+void   fy_calib_commit_eps(fy_pipeline_cfg *cfg, const double *rads, const double *floors, int n);
+/* sigma_z/sigma_xy of one sample from fy_noise_aniso's lag-1 correlations:
+ * sqrt(ln(1/arxy)/ln(1/arz)) when both lie in (0.05, 0.95) and the ratio in
+ * (0.5, 3), else -1 */
+double fy_aniso_ratio(double arz, double arxy);
+/* psf_z_ratio = min(median of n ratios, 1.6) when n >= 5 and the median is
+ * >= 1.05; otherwise cfg is left as it is */
+void   fy_calib_commit_aniso(fy_pipeline_cfg *cfg, const double *ratios, int n);
+
 /* Process one inner tile from an already-read halo'd u8 buffer (fysics.h:507-511;
  * pipeline.c:467-510): u8 -> f01 -> dering -> z-drift -> (deconv chain) -> guided
  * (per-tile eps) -> air-zero + despeckle (do_air_zero) -> recenter -> dithered

@@ -1133,3 +1133,64 @@ int orc_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
     return orc_process_buffer_f(cfg, u8buf, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx,
                                 vol_y, vol_x, out, all_air, NULL);
 }
+
+/* ------------------------------------------------------------------ pass-1 commits */
+/* median of a copy, insertion-sorted (pipeline.c:74-78 sorts in place with qsort) */
+static double orc_median(const double *v, int n) {
+    if (n <= 0) return 0.0;
+    double *t = malloc(sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        int j = i;
+        while (j > 0 && t[j - 1] > v[i]) { t[j] = t[j - 1]; j--; }
+        t[j] = v[i];
+    }
+    double m = (n & 1) ? t[n / 2] : 0.5 * (t[n / 2 - 1] + t[n / 2]);
+    free(t);
+    return m;
+}
+
+void orc_calib_commit_eps(fy_pipeline_cfg *cfg, const double *rads, const double *floors, int n) {
+    cfg->have_eps_r = 0;                                       /* pipeline.c:856-877 */
+    if (n >= 8) {
+        double sr = 0, sf2 = 0, srr = 0, srf = 0, sff = 0;
+        for (int i = 0; i < n; i++) {
+            sr += rads[i]; sf2 += floors[i];
+            srr += rads[i] * rads[i]; srf += rads[i] * floors[i]; sff += floors[i] * floors[i];
+        }
+        double nn = n, den = nn * srr - sr * sr;
+        if (den > 1e-9) {
+            double b = (nn * srf - sr * sf2) / den;
+            double a = (sf2 - b * sr) / nn;
+            double cden = sqrt((nn * srr - sr * sr) * (nn * sff - sf2 * sf2));
+            double corr = cden > 1e-12 ? (nn * srf - sr * sf2) / cden : 0;
+            double rmax = 0;
+            for (int i = 0; i < n; i++) if (rads[i] > rmax) rmax = rads[i];
+            double f0 = a, f1 = a + b * rmax;
+            if (b > 0 && corr >= 0.5 && f0 > 0 && f1 / f0 >= 1.3) {
+                cfg->eps_fn_a = a; cfg->eps_fn_b = b; cfg->have_eps_r = 1;
+            }
+        }
+    }
+    double med = n ? orc_median(floors, n) : 0.015;            /* pipeline.c:878-891 */
+    cfg->eps_fn_med = med;
+    cfg->noise_floor = med * 255.0;
+    if (cfg->guided_eps <= 0 && cfg->denoise_k >= 0) {
+        double k = cfg->denoise_k > 0 ? cfg->denoise_k : 4.2;
+        cfg->guided_eps = (k * med) * (k * med);
+    }
+}
+
+double orc_aniso_ratio(double arz, double arxy) {              /* pipeline.c:736-739 */
+    if (arz > 0.05 && arz < 0.95 && arxy > 0.05 && arxy < 0.95) {
+        double ratio = sqrt(log(1.0 / arxy) / log(1.0 / arz));
+        if (ratio > 0.5 && ratio < 3.0) return ratio;
+    }
+    return -1.0;
+}
+
+void orc_calib_commit_aniso(fy_pipeline_cfg *cfg, const double *ratios, int n) {   /* pipeline.c:894-900 */
+    if (n >= 5) {
+        double ratio = orc_median(ratios, n);
+        if (ratio >= 1.05) cfg->psf_z_ratio = ratio < 1.6 ? ratio : 1.6;
+    }
+}

@@ -108,6 +108,14 @@ void   orc_zdrift_u8_sub(const unsigned char *pb, long dz, long dy, long dx, lon
                          double *sums, long *counts);
 double orc_flat_noise(const float *v, int nz, int ny, int nx, int blk);
 
+/* pipeline.c:735-740,851-900 pass-1 commits (radial eps fit, median floor,
+ * guided eps; per-sample anisotropy ratio and its median commit).  Pinned by
+ * closed forms in tests/test_oracle.py: exact linear data returns its own
+ * line, each gate fails on its own, even/odd medians, the k = 4.2 default. */
+void   orc_calib_commit_eps(fy_pipeline_cfg *cfg, const double *rads, const double *floors, int n);
+double orc_aniso_ratio(double arz, double arxy);
+void   orc_calib_commit_aniso(fy_pipeline_cfg *cfg, const double *ratios, int n);
+
 /* stream.c valley depth; pipeline.c air-zero stage (d) and CC despeckle */
 double orc_valley_depth(const long hist[256], int *dark_out, int *light_out, int *valley_out);
 void   orc_air_cc_despeckle(float *f, int nz, int ny, int nx, int min_size);

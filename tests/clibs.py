@@ -155,6 +155,9 @@ _SIGS = {
                                  C.POINTER(C.c_long), C.c_float]),
     "zdrift_finalize": (None, [c_double_p, C.POINTER(C.c_long), C.c_int, c_float_p]),
     "flat_noise": (C.c_double, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "calib_commit_eps": (None, [C.POINTER(PipelineCfg), c_double_p, c_double_p, C.c_int]),
+    "aniso_ratio": (C.c_double, [C.c_double, C.c_double]),
+    "calib_commit_aniso": (None, [C.POINTER(PipelineCfg), c_double_p, C.c_int]),
     "valley_depth": (C.c_double, [C.POINTER(C.c_long), c_int_p, c_int_p, c_int_p]),
     "phase_correlate": (C.c_int, [c_float_p, c_float_p, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p,
                                   C.c_int]),

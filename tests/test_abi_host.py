@@ -162,3 +162,29 @@ def test_tuning_file_round_trip(tmp_path):
     assert lib.fy_gpu_tuning_load(str(tmp_path / "missing.txt").encode()) == 1
     lib.fy_gpu_tuning_clear()
     assert F.tuning_report() == ""
+
+
+def test_pass1_commits_match_oracle(fy, orc):
+    """fy_calib_commit_eps / fy_aniso_ratio / fy_calib_commit_aniso are host
+    fp64: bit-identical to the oracle on random samples and on the gate cases."""
+    rng = np.random.default_rng(8)
+    dp = lambda a: np.ascontiguousarray(a, np.float64).ctypes.data_as(C.POINTER(C.c_double))
+    for trial in range(40):
+        n = int(rng.integers(0, 60))
+        r = rng.uniform(0, 1500, n)
+        f = np.abs(rng.uniform(0.005, 0.02, n) + rng.uniform(-1e-5, 4e-5) * r)
+        kw = dict(guided_eps=float(rng.choice([0.0, 1e-3])), denoise_k=float(rng.choice([-1.0, 0.0, 3.5])))
+        a, b = F.PipelineCfg(), clibs.PipelineCfg()
+        for k, v in kw.items():
+            setattr(a, k, v)
+            setattr(b, k, v)
+        fy.fy_calib_commit_eps(C.byref(a), dp(r), dp(f), n)
+        orc.orc_calib_commit_eps(C.byref(b), dp(r), dp(f), n)
+        for k in ("have_eps_r", "eps_fn_a", "eps_fn_b", "eps_fn_med", "noise_floor", "guided_eps"):
+            assert getattr(a, k) == getattr(b, k), (trial, k)
+        ratios = rng.uniform(0.9, 1.8, int(rng.integers(0, 9)))
+        fy.fy_calib_commit_aniso(C.byref(a), dp(ratios), len(ratios))
+        orc.orc_calib_commit_aniso(C.byref(b), dp(ratios), len(ratios))
+        assert a.psf_z_ratio == b.psf_z_ratio
+        rz, rxy = rng.uniform(0, 1, 2)
+        assert fy.fy_aniso_ratio(rz, rxy) == orc.orc_aniso_ratio(rz, rxy)

@@ -719,3 +719,82 @@ def test_phase_correlate_circular_shift_exact(orc):
         rc, sh, pk = _pc(orc, "orc_", a, np.roll(a, s, (0, 1, 2)).astype(np.float32), 0)
         want = [((si + n // 2) % n) - n // 2 if ((si % n) > n // 2) else si % n for si, n in zip(s, a.shape)]
         assert rc == 0 and np.allclose(sh, want, atol=1e-4) and abs(pk - 1.0) < 1e-4
+
+
+# ---- pass-1 commits (pipeline.c:735-740,851-900), pinned by closed forms
+def _pcfg(**kw):
+    c = clibs.PipelineCfg()
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def _dp(a):
+    return np.ascontiguousarray(a, np.float64).ctypes.data_as(clibs.c_double_p)
+
+
+def test_commit_eps_fit_recovers_a_line(orc):
+    r = np.linspace(10.0, 900.0, 37)
+    f = 0.01 + 2.5e-5 * r                      # f(rmax)/f(0) = 3.25
+    c = _pcfg(guided_eps=0.0, denoise_k=0.0)
+    orc.orc_calib_commit_eps(C.byref(c), _dp(r), _dp(f), len(r))
+    assert c.have_eps_r == 1
+    assert abs(c.eps_fn_a - 0.01) < 1e-12 and abs(c.eps_fn_b - 2.5e-5) < 1e-15
+    med = np.median(f)
+    assert abs(c.eps_fn_med - med) < 1e-15
+    assert abs(c.noise_floor - 255 * med) < 1e-12
+    assert abs(c.guided_eps - (4.2 * med) ** 2) < 1e-15          # k = 4.2 default
+
+
+@pytest.mark.parametrize("case", ["few", "neg_slope", "low_corr", "flat_span", "neg_intercept"])
+def test_commit_eps_gates(orc, case):
+    rng = np.random.default_rng(4)
+    r = np.linspace(10.0, 900.0, 40)
+    f = 0.01 + 2.5e-5 * r
+    if case == "few":
+        r, f = r[:7], f[:7]
+    elif case == "neg_slope":
+        f = 0.05 - 2.5e-5 * r
+    elif case == "low_corr":
+        f = 0.02 + 0.02 * rng.random(r.size) + 1e-6 * r
+    elif case == "flat_span":
+        f = 0.01 + 2e-6 * r                    # f1/f0 = 1.18 < 1.3
+    elif case == "neg_intercept":
+        f = -0.005 + 2.5e-5 * r
+    c = _pcfg(guided_eps=2e-3, denoise_k=4.0, eps_fn_a=7.0, eps_fn_b=8.0, have_eps_r=1)
+    orc.orc_calib_commit_eps(C.byref(c), _dp(r), _dp(f), len(r))
+    assert c.have_eps_r == 0 and c.eps_fn_a == 7.0 and c.eps_fn_b == 8.0
+    assert c.guided_eps == 2e-3                                   # preset eps kept
+
+
+def test_commit_eps_medians_and_denoise_off(orc):
+    f = np.array([0.3, 0.1, 0.2, 0.4])
+    r = np.zeros(4)
+    c = _pcfg(guided_eps=0.0, denoise_k=-1.0)
+    orc.orc_calib_commit_eps(C.byref(c), _dp(r), _dp(f), 4)
+    assert c.eps_fn_med == 0.25 and c.guided_eps == 0.0           # even n: mean of the middle two
+    c = _pcfg(guided_eps=0.0, denoise_k=3.0)
+    orc.orc_calib_commit_eps(C.byref(c), _dp(r[:3]), _dp(f[:3]), 3)
+    assert c.eps_fn_med == 0.2 and abs(c.guided_eps - (3.0 * 0.2) ** 2) < 1e-15
+    c = _pcfg(guided_eps=0.0, denoise_k=0.0)
+    orc.orc_calib_commit_eps(C.byref(c), None, None, 0)
+    assert c.eps_fn_med == 0.015
+
+
+def test_aniso_ratio_and_commit(orc):
+    # rho = exp(-1/(2 s^2)) for a Gaussian correlation of width s: ratio = s_z/s_xy
+    sz, sxy = 1.4, 1.0
+    rz, rxy = np.exp(-1 / (2 * sz ** 2)), np.exp(-1 / (2 * sxy ** 2))
+    assert abs(orc.orc_aniso_ratio(rz, rxy) - sz / sxy) < 1e-12
+    assert orc.orc_aniso_ratio(0.99, 0.5) == -1.0 and orc.orc_aniso_ratio(0.5, 0.04) == -1.0
+    assert orc.orc_aniso_ratio(0.94, 0.06) == -1.0                # ratio > 3 is rejected
+    c = _pcfg(psf_z_ratio=1.0)
+    orc.orc_calib_commit_aniso(C.byref(c), _dp([1.2, 1.3, 1.25, 1.1, 1.4]), 5)
+    assert c.psf_z_ratio == 1.25
+    c = _pcfg(psf_z_ratio=1.0)
+    orc.orc_calib_commit_aniso(C.byref(c), _dp([2.0] * 6), 6)
+    assert c.psf_z_ratio == 1.6                                   # clamped
+    for vals in ([1.2] * 4, [1.0, 1.01, 1.04, 1.02, 1.03]):       # too few / below 1.05
+        c = _pcfg(psf_z_ratio=1.0)
+        orc.orc_calib_commit_aniso(C.byref(c), _dp(vals), len(vals))
+        assert c.psf_z_ratio == 1.0
