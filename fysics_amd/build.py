@@ -15,6 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfysics_amd.so")
+ARCHIVE = os.path.join(HERE, "libfysics_amd.a")
 ARCH = os.environ.get("FYSICS_OFFLOAD_ARCH", "gfx950")
 
 HIP_SRC = ["fyg_runtime.hip", "fyg_fft.hip", "fyg_fft64.hip", "fyg_guided.hip", "fyg_elem.hip",
@@ -94,7 +95,7 @@ def build(force=False, jobs=None, verbose=True):
                 failed.append(src)
     if failed:
         raise RuntimeError("compile failed: " + ", ".join(os.path.relpath(f, ROOT) for f in failed))
-    if force or work or not os.path.exists(LIB):
+    if force or work or not os.path.exists(LIB) or not os.path.exists(ARCHIVE):
         tmp = LIB + ".tmp"
         cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + \
               ["-Wl,-Bsymbolic", "-lpthread", "-lm"]
@@ -102,6 +103,15 @@ def build(force=False, jobs=None, verbose=True):
         if r.returncode != 0:
             raise RuntimeError("link failed:\n" + r.stderr)
         os.replace(tmp, LIB)
+        # the static archive of the same objects, for callers that link the
+        # reference's static libfysics.a (CMakeLists.txt:55-62)
+        atmp = ARCHIVE + ".tmp"
+        if os.path.exists(atmp):
+            os.remove(atmp)
+        r = subprocess.run(["ar", "rcs", atmp] + objs, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("ar failed:\n" + r.stderr)
+        os.replace(atmp, ARCHIVE)
     if verbose:
         print(f"[build] {os.path.relpath(LIB, ROOT)} ({len(work)} units compiled)")
     return LIB

@@ -188,3 +188,43 @@ def test_pass1_commits_match_oracle(fy, orc):
         assert a.psf_z_ratio == b.psf_z_ratio
         rz, rxy = rng.uniform(0, 1, 2)
         assert fy.fy_aniso_ratio(rz, rxy) == orc.orc_aniso_ratio(rz, rxy)
+
+
+LINK_PROG = r"""
+#include <stdio.h>
+#include "fysics.h"
+#include "fysics_gpu.h"
+int main(void) {
+    fy_physics p = {300.0, 110.0, 1200.0, 4.3, 0.8, 0.5, 0.0, 0.0};
+    printf("%d %d %d %.17g %.17g\n", fy_next_fft_size(160), fy_next_fft_size(96), fy_is_pow2(64),
+           fy_paganin_transfer(0.25, &p), fy_guided_eps_for_noise(0.01));
+    return 0;
+}
+"""
+
+
+@pytest.mark.parametrize("kind", ["shared", "static"])
+def test_c_program_links_unchanged(tmp_path, kind):
+    """A C caller of the reference API compiles against include/ and links the
+    shared library or the static archive (the reference ships both,
+    CMakeLists.txt:55-74) with no source change."""
+    src = tmp_path / "prog.c"
+    src.write_text(LINK_PROG)
+    exe = tmp_path / "prog"
+    libdir = os.path.join(ROOT, "fysics_amd")
+    if kind == "shared":
+        link = ["-L" + libdir, "-lfysics_amd", "-Wl,-rpath," + libdir]
+    else:
+        link = [os.path.join(libdir, "libfysics_amd.a"), "-L/opt/rocm/lib", "-lamdhip64",
+                "-Wl,-rpath,/opt/rocm/lib", "-lstdc++", "-lpthread", "-lm"]
+    r = subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)] + link,
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    v = out.stdout.split()
+    p = F.physics(delta_beta=300.0, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3, unsharp_sigma=0.8,
+                  unsharp_coeff=0.5)
+    assert v[:3] == ["192", "96", "1"]
+    assert float(v[3]) == F.lib().fy_paganin_transfer(0.25, C.byref(p))
+    assert float(v[4]) == F.lib().fy_guided_eps_for_noise(0.01)
