@@ -17,8 +17,7 @@ __global__ __launch_bounds__(256) void k_cfft_rows(Rows a) {
     constexpr int NL = Shape<L>::NL_RM;
     __shared__ float2 buf[NL * Shape<L>::LD];
     __shared__ float2 tw_s[L];
-    load_twiddles<L>(tw_s, a.tw);
-    __syncthreads();
+    const TwPrefetch<L, float2> twp(a.tw);
     const long g0 = (long)blockIdx.x * NL;
     auto first = [&](int l, int p0, int off) -> float2 {
         long g = g0 + l;
@@ -28,10 +27,11 @@ __global__ __launch_bounds__(256) void k_cfft_rows(Rows a) {
         long g = g0 + l;
         if (g < a.nlines) a.data[(size_t)g * L + p0 + off] = v;
     };
+    auto hook = [&] { twp.store(tw_s); };
     if (a.sg < 0)
-        run_from<L, NL, false, false, 0>(-1.0f, tw_s, buf, first, last);
+        run_from<L, NL, false, false, 0>(-1.0f, tw_s, buf, first, last, hook);
     else
-        run_from<L, NL, false, false, 0>(1.0f, tw_s, buf, first, last);
+        run_from<L, NL, false, false, 0>(1.0f, tw_s, buf, first, last, hook);
 }
 
 __global__ void k_interleave(const float *re, const float *im, float2 *c, size_t n) {

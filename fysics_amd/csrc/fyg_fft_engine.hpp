@@ -116,9 +116,9 @@ struct Shape {
     static constexpr int T = 256;
     static constexpr int Q = sizeof(P) / sizeof(float);
     static constexpr int NL_CM = (L <= 256 ? 16 : (4096 / L)) / Q < 1 ? 1 : (L <= 256 ? 16 : (4096 / L)) / Q;
-    // the fused Z pass keeps full 128-byte rows up to L = 512 (2 workgroups
+    // the fused Z pass keeps full 128-byte rows at L = 512 too (2 workgroups
     // of 74 KB per CU): it measured faster than half rows at 4 per CU
-    static constexpr int NL_ZF = (L <= 512 ? 16 : (4096 / L)) / Q < 1 ? 1 : (L <= 512 ? 16 : (4096 / L)) / Q;
+    static constexpr int NL_ZF = L == 512 ? (16 / Q < 1 ? 1 : 16 / Q) : NL_CM;
     static constexpr int RM0 = (4096 / Q) / L;
     static constexpr int NL_RM = RM0 > 256 ? 256 : RM0 < 1 ? 1 : RM0;
     static constexpr int LD = L + (L >> 4) + 1;   // padded line pitch in LDS (complex elements)
@@ -324,20 +324,32 @@ struct LdsWr {
     __device__ void operator()(int l, int p0, int off, V v) const { buf[l * Shape<L>::LD + lds_pos(p0 + off)] = v; }
 };
 
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
 // stages [S, END) of a plan (END < 0: all); stage 0 reads `first`, the final
-// stage writes `last`, stages between go through LDS
-template <int L, int NL, bool CM, bool REV, int S, int END = -1, class V, class First, class Last>
-__device__ __forceinline__ void run_from(scal_t<V> sg, const V *tw, V *buf, First first, Last last) {
+// stage writes `last`, stages between go through LDS.  `hook` runs after the
+// first stage's stores and before its barrier: stage 0 applies no twiddles
+// (span 1), so the kernels store their twiddle table there and its load does
+// not delay the first data loads.
+template <int L, int NL, bool CM, bool REV, int S, int END = -1, class V, class First, class Last,
+          class Hook = NoHook>
+__device__ __forceinline__ void run_from(scal_t<V> sg, const V *tw, V *buf, First first, Last last,
+                                         Hook hook = Hook{}) {
     using PL = Plan<L, REV>;
     constexpr int E = END < 0 ? PL::n : END;
     constexpr int T = Shape<L>::T;
     if constexpr (S < E) {
         constexpr int R = PL::R(S), NS = PL::NS(S);
         constexpr bool FIRST = S == 0, FINAL = S == E - 1;
+        static_assert(!FIRST || NS == 1, "the first stage applies no twiddles");
         if constexpr (FIRST && FINAL) {
             stage<L, NL, T, CM, R, NS, false>(sg, tw, first, last);
+            hook();
         } else if constexpr (FIRST) {
             stage<L, NL, T, CM, R, NS, false>(sg, tw, first, LdsWr<L, NL, V>{buf});
+            hook();
             __syncthreads();
             run_from<L, NL, CM, REV, S + 1, END>(sg, tw, buf, first, last);
         } else if constexpr (FINAL) {
@@ -357,6 +369,35 @@ template <int L, class V>
 __device__ __forceinline__ void load_twiddles(V *tw_s, const V *__restrict__ tw) {
     for (int i = threadIdx.x; i < L; i += Shape<L>::T) tw_s[i] = tw[i];
 }
+
+// a twiddle table loaded into registers at kernel start (L <= 1024: at most 4
+// per thread) and stored to LDS later, as run_from's hook
+template <int L, class V>
+struct TwPrefetch {
+    static constexpr int T = Shape<L>::T, K = L <= 1024 ? (L + T - 1) / T : 0;
+    V r[K > 0 ? K : 1];
+    const V *src;
+    __device__ explicit TwPrefetch(const V *__restrict__ tw) : src(tw) {
+        if constexpr (K > 0) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int i = threadIdx.x + k * T;
+                if (i < L) r[k] = tw[i];
+            }
+        }
+    }
+    __device__ void store(V *tw_s) const {
+        if constexpr (K > 0) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int i = threadIdx.x + k * T;
+                if (i < L) tw_s[i] = r[k];
+            }
+        } else {
+            load_twiddles<L>(tw_s, src);
+        }
+    }
+};
 
 __device__ __forceinline__ int reflect_idx(int i, int n) { return i < n ? i : 2 * n - 1 - i; }
 
@@ -454,7 +495,7 @@ __global__ __launch_bounds__(256) void k_rfft_x(Batch<XFwd<P>> bt) {
     __shared__ V buf[NL * Shape<L>::LD];
     __shared__ V tw_s[L];
     __shared__ XLine meta[NL];
-    load_twiddles<L>(tw_s, a.tw);
+    const TwPrefetch<L, V> twp(a.tw);
     const long g0 = (long)blockIdx.x * NL;
     int s0 = 0;
     if (RING) s0 = ring_slab(a.pro, a.pro.gz0);
@@ -506,7 +547,7 @@ __global__ __launch_bounds__(256) void k_rfft_x(Batch<XFwd<P>> bt) {
         // as it would separate zeros, and B is not stored.
         return Cx<P>::make((P)va, (P)vb);
     };
-    run_from<L, NL, false, false, 0>((P)-1, tw_s, buf, first, LdsWr<L, NL, V>{buf});
+    run_from<L, NL, false, false, 0>((P)-1, tw_s, buf, first, LdsWr<L, NL, V>{buf}, [&] { twp.store(tw_s); });
     __syncthreads();
     if (U8 && a.flag) {
         if (__syncthreads_or(any) && threadIdx.x == 0) *a.flag = *a.gen;
@@ -567,8 +608,7 @@ __global__ __launch_bounds__(256) void k_cfft_cols(Batch<Cols<P>> bt) {
     constexpr int NL = Shape<L, P>::NL_CM;
     __shared__ V buf[NL * Shape<L>::LD];
     __shared__ V tw_s[L];
-    load_twiddles<L>(tw_s, a.tw);
-    __syncthreads();
+    const TwPrefetch<L, V> twp(a.tw);
     const long c0 = bx * NL;
     V *base = a.data + (size_t)by * a.bstride + c0;
     const long nc = a.ncols - c0;
@@ -583,10 +623,11 @@ __global__ __launch_bounds__(256) void k_cfft_cols(Batch<Cols<P>> bt) {
     auto last = [&](int l, int p0, int off, V v) {
         if (l < nc) base[(size_t)p0 * es + l + (size_t)off * es] = v;
     };
+    auto hook = [&] { twp.store(tw_s); };
     if (a.sg < 0)
-        run_from<L, NL, true, false, 0>((P)-1, tw_s, buf, first, last);
+        run_from<L, NL, true, false, 0>((P)-1, tw_s, buf, first, last, hook);
     else
-        run_from<L, NL, true, false, 0>((P)1, tw_s, buf, first, last);
+        run_from<L, NL, true, false, 0>((P)1, tw_s, buf, first, last, hook);
 }
 
 // the LUT holds fp32 weights for P = float (the reference table) and the same
@@ -619,6 +660,8 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
     __shared__ V tw_s[L];
     __shared__ double s_fy[NL], s_fx[NL];
     __shared__ int s_ok[NL];
+    // the table's loads overlap the per-column metadata loads that the barrier
+    // below waits for anyway
     load_twiddles<L>(tw_s, a.tw);
     const long c0 = bx * NL;
     if (threadIdx.x < NL) {
@@ -766,7 +809,7 @@ __global__ __launch_bounds__(256) void k_irfft_x(Batch<XInv<P>> bt) {
     __shared__ V buf[NL * Shape<L>::LD];
     __shared__ V tw_s[L];
     __shared__ XInvLine meta[NL];
-    load_twiddles<L>(tw_s, a.tw);
+    const TwPrefetch<L, V> twp(a.tw);
     const long g0 = (long)blockIdx.x * NL;
     for (int l = threadIdx.x; l < NL; l += Shape<L>::T) {
         XInvLine m;
@@ -802,7 +845,7 @@ __global__ __launch_bounds__(256) void k_irfft_x(Batch<XInv<P>> bt) {
         a.out[m.oo + x] = epilogue<P>(a.epi, v.x * a.invn);
         if (m.pair) a.out[m.oo + a.nx + x] = epilogue<P>(a.epi, v.y * a.invn);
     };
-    run_from<L, NL, false, false, 0>((P)1, tw_s, buf, first, last);
+    run_from<L, NL, false, false, 0>((P)1, tw_s, buf, first, last, [&] { twp.store(tw_s); });
 }
 
 unsigned grid1d(size_t n) {
