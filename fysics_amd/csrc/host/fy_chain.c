@@ -132,7 +132,20 @@ static size_t air_bytes(long hz, long hy, long hx) {
     return 2 * al256(sizeof(float) * hn) + al256(fyg_air_ws_bytes((int)hz, (int)hy, (int)hx));
 }
 
-static int plan_cube(const fy_cube *c, cube_plan *pl, fyg_stream *s) {
+/* the transfer table a cube's deconvolution uses; 0: the cube has none */
+static int deconv_req(const fy_cube *c, fyh_table_req *r) {
+    const fy_cube_params *P = c->params;
+    if (!P || !P->deconv || c->hz <= 0 || c->hy <= 0 || c->hx <= 0) return 0;
+    int matched = P->deconv == 2;
+    double param = P->deconv_param;
+    if (param <= 0) param = matched ? 0.05 : fy_auto_reg(&P->phys, 0.015);
+    r->nz = (int)c->hz; r->ny = (int)c->hy; r->nx = (int)c->hx;
+    r->kind = matched; r->f64 = P->fp64 != 0; r->p = &P->phys;
+    r->param = param; r->cap = P->deconv_cap; r->fz2_scale = P->fz2_scale > 0 ? P->fz2_scale : 1.0;
+    return 1;
+}
+
+static int plan_cube(const fy_cube *c, cube_plan *pl, const fyh_lut_memo *memo, fyg_stream *s) {
     memset(pl, 0, sizeof *pl);
     pl->c = c;
     const fy_cube_params *P = c->params;
@@ -172,13 +185,9 @@ static int plan_cube(const fy_cube *c, cube_plan *pl, fyg_stream *s) {
 
     if (P->deconv) {                                            /* (b) */
         pl->deconv = 1;
-        int matched = P->deconv == 2;
-        double param = P->deconv_param;
-        if (param <= 0) param = matched ? 0.05 : fy_auto_reg(&P->phys, 0.015);
+        fyh_table_req rq;
         fyh_tables t;
-        if (fyh_tables_get((int)hz, (int)hy, (int)hx, &P->phys, param, matched, P->deconv_cap,
-                           P->fz2_scale > 0 ? P->fz2_scale : 1.0, pl->f64, s, &t))
-            return 1;
+        if (!deconv_req(c, &rq) || fyh_tables_get_m(&rq, memo, s, &t)) return 1;
         if (!fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px)) return 1;
         fyg_deconv_args *a = &pl->d;
         a->nz = (int)hz; a->ny = (int)hy; a->nx = (int)hx;
@@ -372,10 +381,20 @@ static int run_chunk(const fy_cube *cubes, int n, int *d_flags, const int *gen, 
     int *gi = (int *)malloc(sizeof(int) * (size_t)n);
     int **gf = (int **)malloc(sizeof(int *) * (size_t)n);
     int rc = (!pl || !grp || !gi || !gf) ? 1 : 0;
+    /* transfer tables of cubes with their own physics: built ahead, in parallel */
+    fyh_lut_memo *memo = NULL;
+    fyh_table_req *rq = rc || n < 2 ? NULL : (fyh_table_req *)malloc(sizeof *rq * (size_t)n);
+    if (rq) {
+        int nr = 0;
+        for (int i = 0; i < n; i++) nr += deconv_req(&cubes[i], &rq[nr]);
+        if (nr > 1) memo = fyh_tables_prefill(rq, nr, s);
+        free(rq);
+    }
     for (int i = 0; i < n && !rc; i++) {
-        rc = plan_cube(&cubes[i], &pl[i], s);
+        rc = plan_cube(&cubes[i], &pl[i], memo, s);
         gi[i] = -1;
     }
+    fyh_lut_memo_free(memo);
     for (int i = 0; i < n && !rc; i++) {
         if (gi[i] >= 0 || pl[i].skip) continue;
         int m = 0;

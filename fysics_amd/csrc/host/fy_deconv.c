@@ -10,6 +10,7 @@
  */
 #define _GNU_SOURCE
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -48,16 +49,18 @@ typedef struct {
     double phys[8], param, cap, fz2_scale;
 } table_key;
 
-int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, int kind,
-                   double cap, double fz2_scale, int f64, fyg_stream *s, fyh_tables *t) {
-    int pz = fy_next_fft_size(nz), py = fy_next_fft_size(ny), px = fy_next_fft_size(nx);
+/* padded shape, frequency axes (fftfreq convention, fz^2 scaled), LUT scale
+ * and cache key of one table; *axes holds fz2, fy2, fx2 (caller frees) */
+static int table_shape(const fyh_table_req *r, fyh_tables *t, double **axes, table_key *key, size_t *lut_bytes,
+                       size_t *bytes) {
+    int pz = fy_next_fft_size(r->nz), py = fy_next_fft_size(r->ny), px = fy_next_fft_size(r->nx);
     int hx = px / 2 + 1;
-    if (fz2_scale <= 0) fz2_scale = 1.0;
+    double fz2_scale = r->fz2_scale > 0 ? r->fz2_scale : 1.0;
     t->pz = pz; t->py = py; t->px = px; t->hx = hx; t->hxp = fyg_deconv_hxp(px);
     /* layout of the device block: lut[4097] (padded to 4100 entries; fp32, or the
      * same weights in fp64 for the all-FP64 variant), fz2, fy2, fx2 */
-    size_t lut_bytes = (f64 ? sizeof(double) : sizeof(float)) * 4100;
-    size_t bytes = lut_bytes + sizeof(double) * (size_t)(pz + py + hx);
+    *lut_bytes = (r->f64 ? sizeof(double) : sizeof(float)) * 4100;
+    *bytes = *lut_bytes + sizeof(double) * (size_t)(pz + py + hx);
     double *fz2 = (double *)malloc(sizeof(double) * (size_t)(pz + py + hx));
     if (!fz2) return 1;
     double *fy2 = fz2 + pz, *fx2 = fy2 + py;
@@ -77,39 +80,131 @@ int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, in
         fx2[i] = f * f;
         if (fx2[i] > mx) mx = fx2[i];
     }
-    double us = FYH_LUT_N / (mz + my + mx);
-    t->us = us;
+    t->us = FYH_LUT_N / (mz + my + mx);
+    memset(key, 0, sizeof *key);
+    memcpy(key->tag, r->f64 ? "L64" : "LUT", 4);
+    key->kind = r->kind; key->pz = pz; key->py = py; key->px = px;
+    memcpy(key->phys, r->p, sizeof(double) * 8);
+    key->param = r->param; key->cap = r->cap; key->fz2_scale = fz2_scale;
+    *axes = fz2;
+    return 0;
+}
 
-    table_key key;
-    memset(&key, 0, sizeof key);
-    memcpy(key.tag, f64 ? "L64" : "LUT", 4);
-    key.kind = kind; key.pz = pz; key.py = py; key.px = px;
-    memcpy(key.phys, p, sizeof(double) * 8);
-    key.param = param; key.cap = cap; key.fz2_scale = fz2_scale;
-    /* per-stream slots first (no device wait on a miss), the process-wide cache
-     * for tables too large for a slot */
+/* per-stream slots first (no device wait on a miss), the process-wide cache
+ * for tables too large for a slot; data == NULL: lookup only */
+static const void *table_put(const table_key *key, const void *data, size_t bytes, fyg_stream *s) {
     int slot = bytes <= FYG_ARENA_SLOT && !fyg_stream_persistent(s);
-    const unsigned char *dev = (const unsigned char *)(slot ? fyg_stream_table(s, &key, sizeof key, NULL, 0)
-                                                            : fyg_table_find(&key, sizeof key));
+    if (slot) return fyg_stream_table(s, key, sizeof *key, data, bytes);
+    return data ? fyg_table(key, sizeof *key, data, bytes) : fyg_table_find(key, sizeof *key);
+}
+
+/* the 4097 weights over u = i/us, evaluated in double (paganin.c:212-230),
+ * plus three zero pad entries */
+static void lut_fill(void *lut, const fy_physics *p, double param, int kind, double cap, int f64, double us) {
+    float *l32 = (float *)lut;
+    double *l64 = (double *)lut;
+    for (int i = 0; i <= FYH_LUT_N; i++) {
+        double fr = sqrt((double)i / us);
+        double w = kind == 0 ? wiener_weight(fr, p, param) : matched_weight(fr, p, param);
+        if (cap > 0 && w > cap) w = cap;
+        if (f64) l64[i] = w;
+        else l32[i] = (float)w;
+    }
+    for (int i = FYH_LUT_N + 1; i < 4100; i++) {
+        if (f64) l64[i] = 0.0;
+        else l32[i] = 0.0f;
+    }
+}
+
+/* weights built ahead of a planning pass */
+typedef struct {
+    table_key key;
+    fy_physics phys;
+    double us;
+    int f64;
+    void *lut;
+} memo_ent;
+
+struct fyh_lut_memo {
+    int n, next;
+    memo_ent *e;
+    pthread_mutex_t mu;
+};
+
+static void *memo_worker(void *arg) {
+    fyh_lut_memo *m = (fyh_lut_memo *)arg;
+    for (;;) {
+        pthread_mutex_lock(&m->mu);
+        int i = m->next < m->n ? m->next++ : -1;
+        pthread_mutex_unlock(&m->mu);
+        if (i < 0) return NULL;
+        memo_ent *e = &m->e[i];
+        lut_fill(e->lut, &e->phys, e->key.param, e->key.kind, e->key.cap, e->f64, e->us);
+    }
+}
+
+/* Each table costs 4097 transcendental weight evaluations on the host; a batch
+ * of cubes that each bring their own physics (a serving wave) would otherwise
+ * evaluate them one table after another on the calling thread.  The tables
+ * not cached yet are built here, one per worker at a time. */
+fyh_lut_memo *fyh_tables_prefill(const fyh_table_req *req, int n, fyg_stream *s) {
+    fyh_lut_memo *m = (fyh_lut_memo *)calloc(1, sizeof *m);
+    if (!m) return NULL;
+    m->e = (memo_ent *)calloc((size_t)(n > 0 ? n : 1), sizeof *m->e);
+    if (!m->e) { free(m); return NULL; }
+    pthread_mutex_init(&m->mu, NULL);
+    for (int i = 0; i < n; i++) {
+        fyh_tables t;
+        double *axes = NULL;
+        table_key key;
+        size_t lut_bytes, bytes;
+        if (table_shape(&req[i], &t, &axes, &key, &lut_bytes, &bytes)) continue;
+        free(axes);
+        if (table_put(&key, NULL, bytes, s)) continue;               /* cached */
+        int dup = 0;
+        for (int j = 0; j < m->n && !dup; j++) dup = !memcmp(&m->e[j].key, &key, sizeof key);
+        if (dup) continue;
+        memo_ent *e = &m->e[m->n];
+        e->lut = malloc(lut_bytes);
+        if (!e->lut) continue;
+        e->key = key; e->phys = *req[i].p; e->us = t.us; e->f64 = req[i].f64;
+        m->n++;
+    }
+    enum { MAXW = 8 };
+    pthread_t th[MAXW];
+    int nw = m->n / 2 < MAXW ? m->n / 2 : MAXW, started = 0;
+    for (int w = 0; w < nw; w++)
+        if (!pthread_create(&th[started], NULL, memo_worker, m)) started++;
+    memo_worker(m);
+    for (int w = 0; w < started; w++) pthread_join(th[w], NULL);
+    return m;
+}
+
+void fyh_lut_memo_free(fyh_lut_memo *m) {
+    if (!m) return;
+    for (int i = 0; i < m->n; i++) free(m->e[i].lut);
+    pthread_mutex_destroy(&m->mu);
+    free(m->e);
+    free(m);
+}
+
+int fyh_tables_get_m(const fyh_table_req *r, const fyh_lut_memo *m, fyg_stream *s, fyh_tables *t) {
+    double *fz2 = NULL;
+    table_key key;
+    size_t lut_bytes, bytes;
+    if (table_shape(r, t, &fz2, &key, &lut_bytes, &bytes)) return 1;
+    const int pz = t->pz, py = t->py, hx = t->hx;
+    const unsigned char *dev = (const unsigned char *)table_put(&key, NULL, bytes, s);
     if (!dev) {
         unsigned char *blk = (unsigned char *)malloc(bytes);
         if (!blk) { free(fz2); return 1; }
-        float *lut = (float *)blk;
-        double *lut64 = (double *)blk;
-        for (int i = 0; i <= FYH_LUT_N; i++) {
-            double fr = sqrt((double)i / us);
-            double w = kind == 0 ? wiener_weight(fr, p, param) : matched_weight(fr, p, param);
-            if (cap > 0 && w > cap) w = cap;
-            if (f64) lut64[i] = w;
-            else lut[i] = (float)w;
-        }
-        for (int i = 4097; i < 4100; i++) {
-            if (f64) lut64[i] = 0.0;
-            else lut[i] = 0.0f;
-        }
+        const memo_ent *e = NULL;
+        for (int j = 0; m && j < m->n && !e; j++)
+            if (!memcmp(&m->e[j].key, &key, sizeof key)) e = &m->e[j];
+        if (e) memcpy(blk, e->lut, lut_bytes);
+        else lut_fill(blk, r->p, r->param, r->kind, r->cap, r->f64, t->us);
         memcpy(blk + lut_bytes, fz2, sizeof(double) * (size_t)(pz + py + hx));
-        dev = (const unsigned char *)(slot ? fyg_stream_table(s, &key, sizeof key, blk, bytes)
-                                           : fyg_table(&key, sizeof key, blk, bytes));
+        dev = (const unsigned char *)table_put(&key, blk, bytes, s);
         free(blk);
     }
     free(fz2);
@@ -119,6 +214,12 @@ int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, in
     t->fy2 = t->fz2 + pz;
     t->fx2 = t->fy2 + py;
     return 0;
+}
+
+int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, int kind,
+                   double cap, double fz2_scale, int f64, fyg_stream *s, fyh_tables *t) {
+    const fyh_table_req r = {nz, ny, nx, kind, f64, p, param, cap, fz2_scale};
+    return fyh_tables_get_m(&r, NULL, s, t);
 }
 
 int fyh_deconv_dev(const float *d_in, float *d_out, int nz, int ny, int nx, const fy_physics *p,

@@ -27,6 +27,18 @@ typedef struct {
 int fyh_tables_get(int nz, int ny, int nx, const fy_physics *p, double param, int kind,
                    double cap, double fz2_scale, int f64, fyg_stream *s, fyh_tables *t);
 double fyh_matched_fz2_scale(const fy_physics *p);
+/* the arguments of one fyh_tables_get call */
+typedef struct {
+    int nz, ny, nx, kind, f64;
+    const fy_physics *p;
+    double param, cap, fz2_scale;
+} fyh_table_req;
+/* LUT weights of the not yet cached tables among req, built ahead on host
+ * threads; fyh_tables_get_m takes them from there instead of evaluating */
+typedef struct fyh_lut_memo fyh_lut_memo;
+fyh_lut_memo *fyh_tables_prefill(const fyh_table_req *req, int n, fyg_stream *s);
+void fyh_lut_memo_free(fyh_lut_memo *m);
+int fyh_tables_get_m(const fyh_table_req *r, const fyh_lut_memo *m, fyg_stream *s, fyh_tables *t);
 /* the physics fy_process_buffer deconvolves with (pipeline.c:89-104) */
 void fyh_scaled_physics(const fy_pipeline_cfg *c, fy_physics *p);
 
