@@ -676,10 +676,14 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
     V *base = a.data + c0;
     const size_t es = (size_t)a.estride;
     auto valid = [&](int l) { return s_ok[l] != 0; };
-    // Every column of a group lies inside its row's pitch, so loads need no
-    // guard: pitch-padding columns (k >= hx) are transformed and not stored.
+    // Pitch-padding columns (k >= hx) of a group are transformed and not
+    // stored.  When NL does not divide py*hxp (lines of 384 and 768), the last
+    // group reaches past the plane's columns: those lines load its last column.
+    const long left = (long)a.py * a.hxp - c0;
+    const int nv = left < NL ? (int)left : NL;
     auto first = [&](int l, int p0, int off) -> V {
-        return base[(size_t)p0 * es + l + (size_t)off * es];
+        const int lc = l < nv ? l : nv - 1;
+        return base[(size_t)p0 * es + lc + (size_t)off * es];
     };
     auto last = [&](int l, int p0, int off, V v) {
         if (valid(l)) base[(size_t)p0 * es + l + (size_t)off * es] = v;
