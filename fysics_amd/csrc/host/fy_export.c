@@ -124,7 +124,7 @@ typedef struct {
 } slot_t;
 
 typedef struct {
-    int kind;                   /* 0 read, 1 write, 2 stop */
+    int kind;                   /* 0 read, 1 write, 2 stop, 3 await a slot then queue its writes */
     int slot, k;
 } task_t;
 
@@ -211,8 +211,20 @@ static void *worker(void *arg) {
         e->qn--;
         pthread_mutex_unlock(&e->mu);
         if (t.kind == 2) break;
-        if (t.kind == 0) do_read(e, &e->sl[t.slot], t.k);
-        else do_write(e, &e->sl[t.slot], t.k);
+        if (t.kind == 0) {
+            do_read(e, &e->sl[t.slot], t.k);
+        } else if (t.kind == 3) {
+            /* one thread waits for a slot's copies back, then queues its chunk
+             * writes, so the other threads keep reading meanwhile */
+            slot_t *sl = &e->sl[t.slot];
+            (void)fyg_event_wait(sl->e_out);
+            pthread_mutex_lock(&e->mu);
+            for (int k = 0; k < sl->n; k++)
+                if (sl->cube_of[k] >= 0) push(e, (task_t){1, t.slot, k});
+            pthread_mutex_unlock(&e->mu);
+        } else {
+            do_write(e, &e->sl[t.slot], t.k);
+        }
     }
     return NULL;
 }
@@ -302,7 +314,7 @@ int fy_gpu_export(const fy_zarr *zin, const fy_zarr *zout, const fy_pipeline_cfg
     const long ext = tile + 2L * e->halo;
     e->cap = (size_t)lmin(ext, e->Z) * lmin(ext, e->Y) * lmin(ext, e->X);
     e->tcap = (size_t)tile * tile * tile;
-    e->qcap = NSLOT * batch * 2 + threads + 4;
+    e->qcap = NSLOT * batch * 2 + NSLOT + threads + 4;
     e->q = (task_t *)calloc((size_t)e->qcap, sizeof(task_t));
     e->zc = fyh_zcache_new((size_t)1 << 31);
     int rc = e->q == NULL || e->zc == NULL;
@@ -372,8 +384,7 @@ int fy_gpu_export(const fy_zarr *zin, const fy_zarr *zout, const fy_pipeline_cfg
         } else {
             s->state = S_SUBMITTED;
             s->writes_left = nc;
-            for (int k = 0; k < s->n; k++)
-                if (s->cube_of[k] >= 0) push(e, (task_t){1, (int)(b % NSLOT), k});
+            push(e, (task_t){3, (int)(b % NSLOT), 0});
         }
         pthread_cond_broadcast(&e->cv_main);
         pthread_mutex_unlock(&e->mu);
