@@ -72,6 +72,36 @@ MEASURED_TRAFFIC_256 = {
 }
 
 
+def roofline_of(prof, n, p):
+    """Roofline entry of the dominant kernel from per-launch-site HIP-event
+    totals {site: (ms, launches)}.  The dominant kernel is the kernel function
+    with the largest share of the step, as the kernel trace ranks them: the Y
+    pass (k_cfft_cols) runs twice per cube, forward and inverse, so its two
+    launch sites count together.  achieved = algorithmic bytes / event time."""
+    funcs = {"cfft_y_fwd": "cfft_y", "cfft_y_inv": "cfft_y"}
+    agg = {}
+    for name, (ms, cnt) in prof.items():
+        t = agg.setdefault(funcs.get(name, name), {"ms": 0.0, "cnt": 0, "bytes": 0, "sites": []})
+        b = kernel_bytes(name, n, p)
+        t["ms"] += ms
+        t["cnt"] += cnt
+        t["bytes"] = None if b is None or t["bytes"] is None else t["bytes"] + b * cnt
+        t["sites"].append(name)
+    dom = max(agg, key=lambda k: agg[k]["ms"])
+    d = agg[dom]
+    dom_bytes = d["bytes"] / max(d["cnt"], 1) if d["bytes"] else None
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["bytes"] else None
+    traffic = None
+    if n == 256 and all(x in MEASURED_TRAFFIC_256 for x in d["sites"]):
+        traffic = round(sum(MEASURED_TRAFFIC_256[x] for x in d["sites"]) / len(d["sites"]))
+    return {"kernel": dom, "launch_sites": d["sites"], "bound": "hbm",
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": traffic, "algorithmic_bytes": round(dom_bytes) if dom_bytes else None,
+            "avg_us": round(1000.0 * d["ms"] / max(d["cnt"], 1), 2)}
+
+
 def build_workload(n, seed):
     from fysics_amd import phantom
     t0 = time.time()
@@ -212,32 +242,7 @@ def main():
         b = kernel_bytes(name, n, p)
         kernels[name] = {"avg_us": round(avg_us, 2), "launches": cnt,
                          "GBps": round(b / (avg_us * 1e-6) / 1e9, 1) if b else None}
-    # the dominant kernel is the kernel function with the largest share of the
-    # step, as the kernel trace ranks them: the Y pass (k_cfft_cols) runs twice
-    # per cube, forward and inverse, so its two launch sites count together
-    funcs = {"cfft_y_fwd": "cfft_y", "cfft_y_inv": "cfft_y"}
-    agg = {}
-    for name, (ms, cnt) in prof.items():
-        t = agg.setdefault(funcs.get(name, name), {"ms": 0.0, "cnt": 0, "bytes": 0, "sites": []})
-        b = kernel_bytes(name, n, p)
-        t["ms"] += ms
-        t["cnt"] += cnt
-        t["bytes"] = None if b is None or t["bytes"] is None else t["bytes"] + b * cnt
-        t["sites"].append(name)
-    dom = max(agg, key=lambda k: agg[k]["ms"])
-    d = agg[dom]
-    dom_us = round(1000.0 * d["ms"] / max(d["cnt"], 1), 2)
-    dom_bytes = d["bytes"] / max(d["cnt"], 1) if d["bytes"] else None
-    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["bytes"] else None
-    traffic = None
-    if n == 256 and all(x in MEASURED_TRAFFIC_256 for x in d["sites"]):
-        traffic = round(sum(MEASURED_TRAFFIC_256[x] for x in d["sites"]) / len(d["sites"]))
-    roofline = {"kernel": dom, "launch_sites": d["sites"], "bound": "hbm",
-                "achieved": round(achieved, 1) if achieved else None,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                "traffic": traffic, "algorithmic_bytes": round(dom_bytes) if dom_bytes else None,
-                "avg_us": dom_us}
+    roofline = roofline_of(prof, n, p)
 
     # plain drop-in with host buffers (PCIe-inclusive; reported beside, never `value`)
     cfgc = F.pipeline_cfg(delta_beta=300.0, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
