@@ -661,14 +661,14 @@ def export_bench(F, vol, eps, tile=128, threads=16, batch=8):
         src = os.path.join(work, "in.zarr").encode()
         shp = (C.c_long * 3)(Z, Z, Z)
         ch = (C.c_long * 3)(tile, tile, tile)
-        assert lib.fy_zarr_create(C.byref(zi), src, shp, ch) == 0
+        assert lib.fy_gpu_zarr_create(C.byref(zi), src, shp, ch) == 0
         nt = Z // tile
         for cz in range(nt):
             for cy in range(nt):
                 for cx in range(nt):
                     blk = np.ascontiguousarray(big[cz * tile:(cz + 1) * tile, cy * tile:(cy + 1) * tile,
                                                    cx * tile:(cx + 1) * tile])
-                    assert lib.fy_zarr_write_chunk(C.byref(zi), cz, cy, cx, blk.ctypes.data, tile, tile, tile) == 0
+                    assert lib.fy_gpu_zarr_write_chunk(C.byref(zi), cz, cy, cx, blk.ctypes.data, tile, tile, tile) == 0
         cfg = F.pipeline_cfg(delta_beta=300.0, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
                              unsharp_sigma=0.8, unsharp_coeff=0.5, window_lo=-0.1, window_hi=1.2,
                              do_deconv=1, guided_eps=eps, halo=8)
@@ -676,7 +676,7 @@ def export_bench(F, vol, eps, tile=128, threads=16, batch=8):
 
         def export(name, t1=0):
             zo = F.Zarr()
-            assert lib.fy_zarr_create(C.byref(zo), os.path.join(work, name).encode(), shp, ch) == 0
+            assert lib.fy_gpu_zarr_create(C.byref(zo), os.path.join(work, name).encode(), shp, ch) == 0
             t = time.perf_counter()
             rc = lib.fy_gpu_export(C.byref(zi), C.byref(zo), C.byref(cfg), tile, 0, t1, threads, batch,
                                    C.byref(stats), None)

@@ -225,11 +225,14 @@ SIGNATURES = {
     "fy_calib_reset": (C.c_int, [_VP]),
     "fy_calib_destroy": (None, [_VP]),
     "fy_gpu_flat_noise": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, c_double_p, _VP]),
-    "fy_zarr_open": (C.c_int, [_ZP, C.c_char_p]),
-    "fy_zarr_create": (C.c_int, [_ZP, C.c_char_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
-    "fy_zarr_read": (C.c_int, [_ZP, _L, _L, _L, _L, _L, _L, _VP]),
-    "fy_zarr_write_chunk": (C.c_int, [_ZP, _L, _L, _L, _VP, _L, _L, _L]),
-    "fy_process_chunk": (C.c_int, [_ZP, C.POINTER(PipelineCfg), _L, _L, _L, C.c_int, _VP, c_int_p, c_int_p,
+    "fy_gpu_zarr_open": (C.c_int, [_ZP, C.c_char_p]),
+    "fy_gpu_zarr_create": (C.c_int, [_ZP, C.c_char_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    "fy_gpu_zarr_read": (C.c_int, [_ZP, _L, _L, _L, _L, _L, _L, _VP]),
+    "fy_gpu_zarr_write_chunk": (C.c_int, [_ZP, _L, _L, _L, _VP, _L, _L, _L]),
+    "fy_gpu_zarr_cache_new": (C.c_void_p, [C.c_size_t]),
+    "fy_gpu_zarr_cache_free": (None, [C.c_void_p]),
+    "fy_gpu_zarr_read_cached": (C.c_int, [_ZP, C.c_void_p, _L, _L, _L, _L, _L, _L, _VP]),
+    "fy_gpu_process_chunk": (C.c_int, [_ZP, C.POINTER(PipelineCfg), _L, _L, _L, C.c_int, _VP, c_int_p, c_int_p,
                                    c_int_p, c_int_p]),
     "fy_gpu_export": (C.c_int, [_ZP, _ZP, C.POINTER(PipelineCfg), C.c_int, _L, _L, C.c_int, C.c_int,
                                 C.POINTER(ExportStats), _VP]),
@@ -242,6 +245,7 @@ SIGNATURES = {
     "fy_gpu_tuning_load": (C.c_int, [C.c_char_p]),
     "fy_gpu_tuning_clear": (None, []),
     "fy_gpu_sync": (C.c_int, [_VP]),
+    "fy_gpu_table_cache": (C.c_long, [C.POINTER(C.c_long), C.c_long]),
     "fy_gpu_profile": (None, [C.c_int]),
     "fy_gpu_profile_read": (C.c_int, [C.c_char_p, C.c_int, c_double_p, C.POINTER(C.c_long),
                                       C.c_int]),

@@ -6,6 +6,7 @@ the C host code, linked into one shared library next to this file.
 import argparse
 import concurrent.futures as cf
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -38,6 +39,30 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
 C_FLAGS = ["-std=gnu11", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+
+
+HEADERS = [os.path.join(ROOT, "include", "fysics.h"), os.path.join(ROOT, "include", "fysics_gpu.h")]
+
+
+def exported_names():
+    """Every function the public headers declare: the shared library exports
+    these and nothing else (the kernels, the fyg_* runtime and the fyh_* host
+    helpers stay internal)."""
+    names = set()
+    for h in HEADERS:
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        src = re.sub(r"typedef struct \{.*?\}\s*\w+;", "", src, flags=re.S)
+        names.update(m.group(1) for m in re.finditer(r"\b(fy_\w+)\s*\(", src))
+    return sorted(names)
+
+
+def _version_script():
+    path = os.path.join(OBJ, "exports.map")
+    body = "{\n  global:\n" + "".join(f"    {n};\n" for n in exported_names()) + "  local:\n    *;\n};\n"
+    with open(path, "w") as f:
+        f.write(body)
+    return path
 
 
 def _deps(src):
@@ -98,7 +123,7 @@ def build(force=False, jobs=None, verbose=True):
     if force or work or not os.path.exists(LIB) or not os.path.exists(ARCHIVE):
         tmp = LIB + ".tmp"
         cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + \
-              ["-Wl,-Bsymbolic", "-lpthread", "-lm"]
+              ["-Wl,-Bsymbolic", "-Wl,--version-script=" + _version_script(), "-lpthread", "-lm"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError("link failed:\n" + r.stderr)

@@ -76,10 +76,28 @@ const char *fyg_error_string(void);
 void fyg_set_error(const char *msg);            /* host-side failures (I/O, arguments) */
 
 /* process-wide cache of small read-only device tables (twiddles, transfer
- * LUTs).  Returns the device copy for key, uploading data on a miss.  Thread
- * safe; entries live until process end (bounded LRU for transfer tables). */
+ * LUTs, per-volume blobs).  Returns the device copy for key, uploading data on
+ * a miss.  Thread safe.  Twiddles ('T' keys) live until process end; the other
+ * tables form a bounded cache that evicts the least recently used entry that
+ * no hold covers, and frees its memory only after the events of all its
+ * releases have completed. */
 const void *fyg_table(const void *key, size_t keylen, const void *data, size_t bytes);
 const void *fyg_table_find(const void *key, size_t keylen);
+/* Holds: every table the calling thread finds or makes between fyg_hold_begin
+ * and fyg_hold_end (or release) is held -- never evicted -- until
+ * fyg_hold_release, which records an event on s after the work queued so far
+ * (s NULL or capturing: no event) and frees the hold.  Holds nest (LIFO); each
+ * open hold holds the tables.  A captured graph keeps its hold until it is
+ * destroyed. */
+typedef struct fyg_hold fyg_hold;
+fyg_hold *fyg_hold_begin(void);
+void fyg_hold_end(fyg_hold *h);
+int  fyg_hold_release(fyg_hold *h, fyg_stream *s);
+/* st[0..6]: entries, evictable entries, held entries, evicted entries not yet
+ * freed, evictions so far, frees so far, the cache limit */
+void fyg_table_stats(long *st);
+/* sets the limit on evictable entries (n >= 1; 0: unchanged); returns the old one */
+size_t fyg_table_limit(size_t n);
 /* per-stream table slots for tables that change from call to call (per-request
  * transfer LUTs): on a miss the host data is staged into pinned memory and
  * copied on the stream, so no call waits on the device.  FYG_ARENA_SLOTS slots

@@ -747,8 +747,7 @@ __global__ void k_box_pass(PassArgs p) {
 struct PointArgs {
     int mode;               // 0: (I-c0),(I-c0)^2 from decimated source; 1: moments -> a,b; 2: apply
     Grid g;
-    const float *in;
-    float c0;
+    const float *in;        // the guide; c0 = in[0], the centring constant, is read here on the device
     double eps;
     const float *m0, *m1;
     float *o0, *o1;
@@ -758,12 +757,14 @@ struct PointArgs {
 __global__ void k_point(PointArgs p) {
     const Grid &g = p.g;
     size_t n = (size_t)g.nz * g.ny * g.nx;
+    // modes 0 and 1 write only workspace, so the guide's first voxel is intact
+    const float c0 = p.mode < 2 ? p.in[0] : 0.0f;
     for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
         int x = (int)(i % g.nx);
         size_t t = i / g.nx;
         int y = (int)(t % g.ny), z = (int)(t / g.ny);
         if (p.mode == 0) {
-            float v = p.in[((size_t)z * g.s * g.fny + (size_t)y * g.s) * g.fnx + (size_t)x * g.s] - p.c0;
+            float v = p.in[((size_t)z * g.s * g.fny + (size_t)y * g.s) * g.fnx + (size_t)x * g.s] - c0;
             p.o0[i] = v;
             p.o1[i] = v * v;
         } else if (p.mode == 1) {
@@ -772,7 +773,7 @@ __global__ void k_point(PointArgs p) {
             if (var < 0.0f) var = 0.0f;
             float ai = var / (var + (float)p.eps);
             p.o0[i] = ai;
-            p.o1[i] = (m + p.c0) * (1.0f - ai);
+            p.o1[i] = (m + c0) * (1.0f - ai);
         } else {
             emit(p.em, g.ny, g.nx, z, y, x, p.m0[i] * p.in[i] + p.m1[i]);
         }
@@ -1026,17 +1027,13 @@ extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
             box2(s, b, r);
         } else {
             float *T0 = ws + 2 * n, *T1 = ws + 3 * n, *O0 = ws + 4 * n, *O1 = ws + 5 * n;
-            PointArgs p0{0, g, a->in, 0.0f, a->eps, nullptr, nullptr, O0, O1, a->em};
-            // c0 = first voxel, read on device inside k_point via a one-element copy
-            float c0 = 0.0f;
-            if (fyg_d2h(&c0, a->in, sizeof(float), s) || fyg_sync(s)) return 1;
-            p0.c0 = c0;
+            PointArgs p0{0, g, a->in, a->eps, nullptr, nullptr, O0, O1, a->em};
             fyg_launch(s, "box_point", k_point, dim3(grid1d(n)), dim3(NT), 0, p0);
             box_global(s, g, r, 2, O0, O1, T0, T1, A, B);   // means in T0, T1
-            PointArgs p1{1, g, a->in, c0, a->eps, T0, T1, A, B, a->em};
+            PointArgs p1{1, g, a->in, a->eps, T0, T1, A, B, a->em};
             fyg_launch(s, "box_point", k_point, dim3(grid1d(n)), dim3(NT), 0, p1);
             box_global(s, g, r, 2, A, B, O0, O1, T0, T1);   // means in O0, O1
-            PointArgs p2{2, g, a->in, 0.0f, a->eps, O0, O1, nullptr, nullptr, a->em};
+            PointArgs p2{2, g, a->in, a->eps, O0, O1, nullptr, nullptr, a->em};
             fyg_launch(s, "box_point", k_point, dim3(grid1d(n)), dim3(NT), 0, p2);
         }
         return fyg_last_error();
@@ -1049,12 +1046,10 @@ extern "C" int fyg_guided_run(const fyg_guided_args *a, fyg_stream *s) {
     Grid gl{lz, ly, lx, sd, ny, nx};
     {
         float *T0 = ws + 4 * nl, *T1 = ws + 5 * nl, *O0 = ws + 6 * nl, *O1 = ws + 7 * nl;
-        float c0 = 0.0f;
-        if (fyg_d2h(&c0, a->in, sizeof(float), s) || fyg_sync(s)) return 1;
-        PointArgs p0{0, gl, a->in, c0, a->eps, nullptr, nullptr, O0, O1, a->em};
+        PointArgs p0{0, gl, a->in, a->eps, nullptr, nullptr, O0, O1, a->em};
         fyg_launch(s, "box_point", k_point, dim3(grid1d(nl)), dim3(NT), 0, p0);
         box_global(s, gl, rl, 2, O0, O1, T0, T1, A, B);
-        PointArgs p1{1, gl, a->in, c0, a->eps, T0, T1, A, B, a->em};
+        PointArgs p1{1, gl, a->in, a->eps, T0, T1, A, B, a->em};
         fyg_launch(s, "box_point", k_point, dim3(grid1d(nl)), dim3(NT), 0, p1);
         box_global(s, gl, rl, 2, A, B, O0, O1, T0, T1);
         if (fyg_d2d(MA, O0, sizeof(float) * nl, s) || fyg_d2d(MB, O1, sizeof(float) * nl, s)) return 1;

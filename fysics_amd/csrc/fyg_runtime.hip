@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <list>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -269,40 +270,173 @@ extern "C" int fyg_last_error(void) {
 }
 
 // ------------------------------------------------------------------ table cache
+// Entries are held while a caller may still queue work that reads them: a
+// hold (fyg_hold_begin .. fyg_hold_release) collects every table its thread
+// finds or makes in between, and a captured graph keeps its hold until it is
+// destroyed.  Releasing records an event on the stream the work went to; a
+// transfer table ('T' keys, the twiddles, are never evicted) that is neither
+// held nor the most recently used leaves the map when the cache is full, and
+// its memory is freed once every event of its releases has completed.  No call
+// waits for the device.
 namespace {
+struct Ev {
+    hipEvent_t e = nullptr;
+    ~Ev() {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
 struct TableEntry {
+    std::string key;
     void *dev = nullptr;
     size_t bytes = 0;
+    int holders = 0;
+    std::vector<std::shared_ptr<Ev>> pending;   // releases whose work may still read the table
+    std::list<std::shared_ptr<TableEntry>>::iterator lru;
+    bool in_lru = false;
 };
+using EntryP = std::shared_ptr<TableEntry>;
 std::mutex g_tab_mu;
-std::unordered_map<std::string, TableEntry> g_tab;
-std::list<std::string> g_tab_lru;       // transfer tables only (keys not starting with 'T')
-const size_t kMaxTables = 512;
+std::unordered_map<std::string, EntryP> g_tab;
+std::list<EntryP> g_tab_lru;            // transfer tables only, least recently used first
+std::vector<EntryP> g_tab_dead;         // evicted, memory freed once their events complete
+size_t g_tab_max = 512;
+long g_tab_evicted = 0, g_tab_freed = 0;
+
+bool events_done(TableEntry &t) {
+    for (auto &ev : t.pending) {
+        if (!ev || !ev->e) continue;
+        if (hipEventQuery(ev->e) != hipSuccess) return false;
+    }
+    return true;
+}
+
+// frees the evicted entries whose last readers have finished (lock held)
+void sweep_dead() {
+    size_t k = 0;
+    for (size_t i = 0; i < g_tab_dead.size(); i++) {
+        EntryP &t = g_tab_dead[i];
+        if (t->holders == 0 && events_done(*t)) {
+            (void)hipFree(t->dev);
+            t->dev = nullptr;
+            g_tab_freed++;
+        } else {
+            g_tab_dead[k++] = t;
+        }
+    }
+    g_tab_dead.resize(k);
+}
+
+// evicts least recently used, unheld transfer tables while the cache is full (lock held)
+void evict_to_fit() {
+    auto it = g_tab_lru.begin();
+    while (g_tab_lru.size() >= g_tab_max && it != g_tab_lru.end()) {
+        EntryP t = *it;
+        if (t->holders > 0) {
+            ++it;
+            continue;
+        }
+        it = g_tab_lru.erase(it);
+        t->in_lru = false;
+        g_tab.erase(t->key);
+        g_tab_dead.push_back(t);
+        g_tab_evicted++;
+    }
+    sweep_dead();
+}
 }  // namespace
+
+struct fyg_hold {
+    std::vector<EntryP> held;
+    fyg_hold *prev = nullptr;
+    bool collecting = true;
+};
+static thread_local fyg_hold *tls_hold = nullptr;
+
+// the entry of key was found or made: move it to the recent end, hold it (lock held)
+static void touch(const EntryP &t) {
+    if (t->in_lru) g_tab_lru.splice(g_tab_lru.end(), g_tab_lru, t->lru);
+    if (t->key[0] == 'T') return;
+    for (fyg_hold *h = tls_hold; h; h = h->prev) {      // nested holds each hold it
+        if (!h->collecting) continue;
+        bool have = false;
+        for (auto &x : h->held)
+            if (x == t) {
+                have = true;
+                break;
+            }
+        if (!have) {
+            t->holders++;
+            h->held.push_back(t);
+        }
+    }
+}
+
+extern "C" fyg_hold *fyg_hold_begin(void) {
+    fyg_hold *h = new fyg_hold();
+    h->prev = tls_hold;
+    tls_hold = h;
+    return h;
+}
+
+extern "C" void fyg_hold_end(fyg_hold *h) {
+    if (!h || !h->collecting) return;
+    h->collecting = false;
+    if (tls_hold == h) tls_hold = h->prev;
+}
+
+extern "C" int fyg_hold_release(fyg_hold *h, fyg_stream *s) {
+    if (!h) return 0;
+    fyg_hold_end(h);
+    int rc = 0;
+    std::shared_ptr<Ev> ev;
+    if (s && !s->capturing && !h->held.empty()) {
+        ev = std::make_shared<Ev>();
+        rc = note(hipEventCreateWithFlags(&ev->e, hipEventDisableTiming), "hipEventCreate(hold)");
+        if (!rc) rc = note(hipEventRecord(ev->e, s->s), "hipEventRecord(hold)");
+        if (rc) {
+            // the work's completion cannot be tracked: keep the tables held for good
+            delete h;
+            return 1;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tab_mu);
+        for (auto &t : h->held) {
+            t->holders--;
+            if (ev) {
+                // drop events of earlier releases that have completed
+                size_t k = 0;
+                for (size_t i = 0; i < t->pending.size(); i++)
+                    if (hipEventQuery(t->pending[i]->e) != hipSuccess) t->pending[k++] = t->pending[i];
+                t->pending.resize(k);
+                t->pending.push_back(ev);
+            }
+        }
+        sweep_dead();
+    }
+    delete h;
+    return rc;
+}
 
 extern "C" const void *fyg_table_find(const void *key, size_t keylen) {
     std::string k((const char *)key, keylen);
     std::lock_guard<std::mutex> lk(g_tab_mu);
     auto it = g_tab.find(k);
-    return it == g_tab.end() ? nullptr : it->second.dev;
+    if (it == g_tab.end()) return nullptr;
+    touch(it->second);
+    return it->second->dev;
 }
 
 extern "C" const void *fyg_table(const void *key, size_t keylen, const void *data, size_t bytes) {
     std::string k((const char *)key, keylen);
     std::lock_guard<std::mutex> lk(g_tab_mu);
     auto it = g_tab.find(k);
-    if (it != g_tab.end()) return it->second.dev;
-    if (!data) return nullptr;
-    if (k[0] != 'T' && g_tab_lru.size() >= kMaxTables) {
-        // evict the oldest transfer table; wait for all queued users first
-        (void)hipDeviceSynchronize();
-        auto old = g_tab.find(g_tab_lru.front());
-        if (old != g_tab.end()) {
-            (void)hipFree(old->second.dev);
-            g_tab.erase(old);
-        }
-        g_tab_lru.pop_front();
+    if (it != g_tab.end()) {
+        touch(it->second);
+        return it->second->dev;
     }
+    if (!data) return nullptr;
+    if (k[0] != 'T') evict_to_fit();
     void *dev = nullptr;
     if (note(hipMalloc(&dev, bytes), "hipMalloc(table)")) return nullptr;
     // synchronous upload: the table must be valid before any stream uses it
@@ -310,9 +444,40 @@ extern "C" const void *fyg_table(const void *key, size_t keylen, const void *dat
         (void)hipFree(dev);
         return nullptr;
     }
-    g_tab[k] = TableEntry{dev, bytes};
-    if (k[0] != 'T') g_tab_lru.push_back(k);
+    EntryP t = std::make_shared<TableEntry>();
+    t->key = k;
+    t->dev = dev;
+    t->bytes = bytes;
+    if (k[0] != 'T') {
+        g_tab_lru.push_back(t);
+        t->lru = std::prev(g_tab_lru.end());
+        t->in_lru = true;
+    }
+    g_tab[k] = t;
+    touch(t);
     return dev;
+}
+
+extern "C" void fyg_table_stats(long *st) {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    sweep_dead();
+    long held = 0;
+    for (auto &kv : g_tab)
+        if (kv.second->holders > 0) held++;
+    st[0] = (long)g_tab.size();
+    st[1] = (long)g_tab_lru.size();
+    st[2] = held;
+    st[3] = (long)g_tab_dead.size();
+    st[4] = g_tab_evicted;
+    st[5] = g_tab_freed;
+    st[6] = (long)g_tab_max;
+}
+
+extern "C" size_t fyg_table_limit(size_t n) {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    size_t old = g_tab_max;
+    if (n >= 1) g_tab_max = n;
+    return old;
 }
 
 // ------------------------------------------------------------------ per-stream table slots

@@ -376,6 +376,10 @@ static int run_group(cube_plan *pl, int n, int **flags, const int *gen, gen_step
  * launches per stage as FYG_BATCH and FYH_GROUP_BYTES allow.  Results do not
  * depend on the grouping: every cube has its own workspace slice and flag. */
 static int run_chunk(const fy_cube *cubes, int n, int *d_flags, const int *gen, gen_step *st, fyg_stream *s) {
+    /* the cached tables the plans point at (ring and z-drift blobs, transfer
+     * tables of the process-wide cache) stay held until the chunk's launches
+     * are queued and their completion is tracked by an event */
+    fyg_hold *hold = fyg_hold_begin();
     cube_plan *pl = (cube_plan *)malloc(sizeof *pl * (size_t)n);
     cube_plan *grp = (cube_plan *)malloc(sizeof *grp * (size_t)n);
     int *gi = (int *)malloc(sizeof(int) * (size_t)n);
@@ -414,6 +418,7 @@ static int run_chunk(const fy_cube *cubes, int n, int *d_flags, const int *gen, 
             rc = run_group(grp + k, m - k < part ? m - k : part, gf + k, gen, st, s);
     }
     free(pl); free(grp); free(gi); free(gf);
+    if (fyg_hold_release(hold, s)) rc = 1;
     return rc;
 }
 
@@ -471,6 +476,7 @@ int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream) {
 struct fy_graph {
     fyg_stream *s;
     void *exec;
+    fyg_hold *hold;             /* the tables the recorded launches read */
 };
 
 fy_graph *fy_gpu_graph_cubes(const fy_cube *cubes, int n, void *stream) {
@@ -483,6 +489,7 @@ fy_graph *fy_gpu_graph_cubes(const fy_cube *cubes, int n, void *stream) {
     int rc = !g->s || !fyg_flags_reserve(g->s, n, &dflags, &dgen, &nflag) ||
              run_cubes(cubes, n, dflags, dgen, nflag, g->s) || fyg_sync(g->s);
     if (!rc) {
+        g->hold = fyg_hold_begin();
         rc = fyg_capture_begin(g->s);
         if (!rc) {
             int r1 = !fyg_flags_reserve(g->s, n, &dflags, &dgen, &nflag) ||
@@ -490,6 +497,7 @@ fy_graph *fy_gpu_graph_cubes(const fy_cube *cubes, int n, void *stream) {
             int r2 = fyg_capture_end(g->s, &g->exec);
             rc = r1 || r2;
         }
+        fyg_hold_end(g->hold);
     }
     if (rc) {
         fy_graph_destroy(g);
@@ -503,6 +511,9 @@ int fy_graph_launch(fy_graph *g) { return !g || fyg_graph_launch(g->exec, g->s);
 void fy_graph_destroy(fy_graph *g) {
     if (!g) return;
     fyg_graph_free(g->exec);
+    /* replays still queued on the stream read the tables: the release's event
+     * follows them */
+    (void)fyg_hold_release(g->hold, g->s);
     fyg_stream_private_free(g->s);
     free(g);
 }

@@ -4,6 +4,7 @@
  */
 #define _GNU_SOURCE
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -265,6 +266,14 @@ int fy_gpu_phase_correlate(const float *d_fixed, const float *d_moving, int nz, 
 }
 
 /* ---------------------------------------------------------------- 2x LOD (downscale.c) */
+/* input bytes per slab of fy_downscale2x (FYSICS_DS_SLAB_BYTES overrides: a
+ * positive byte count, at least two planes are taken) */
+static size_t ds_slab_bytes(void) {
+    const char *e = getenv("FYSICS_DS_SLAB_BYTES");
+    long long v = e ? atoll(e) : 0;
+    return v > 0 ? (size_t)v : ((size_t)256 << 20);
+}
+
 void fy_downscale2x(const unsigned char *in, int nx, int ny, int nz, unsigned char *out, int *oxp, int *oyp,
                     int *ozp, fy_ds_method method, float alpha) {
     int ox = (nx + 1) / 2, oy = (ny + 1) / 2, oz = (nz + 1) / 2;
@@ -275,15 +284,35 @@ void fy_downscale2x(const unsigned char *in, int nx, int ny, int nz, unsigned ch
     if (oyp) *oyp = oy;
     if (ozp) *ozp = oz;
     if (nx < 1 || ny < 1 || nz < 1) return;
+    /* Slabs of output planes through a bounded device workspace (each output
+     * plane reads its two parent planes, so slabs are independent).  The
+     * reference cannot fail here (downscale.c:303-320 returns void); if the
+     * device does, out is zeroed and fy_gpu_last_error() says why. */
+    const size_t plane_in = (size_t)ny * nx, plane_out = (size_t)oy * ox;
+    long per = (long)(ds_slab_bytes() / (2 * plane_in));
+    if (per < 1) per = 1;
+    if (per > oz) per = oz;
     fyg_stream *s = fyg_thread_stream();
-    size_t n = (size_t)nx * ny * nz, no = (size_t)ox * oy * oz;
-    unsigned char *d = s ? (unsigned char *)fyg_ws(s, WS_U8, n) : NULL;
-    unsigned char *o = s ? (unsigned char *)fyg_ws(s, WS_OUT, no) : NULL;
-    if (!d || !o) return;
-    if (fyg_h2d(d, in, n, s) || fyg_downscale2x_run(d, nx, ny, nz, o, (int)method, alpha, s) ||
-        fyg_d2h(out, o, no, s))
-        return;
-    fyg_sync(s);
+    unsigned char *d = s ? (unsigned char *)fyg_ws(s, WS_U8, 2 * plane_in * (size_t)per) : NULL;
+    unsigned char *o = s ? (unsigned char *)fyg_ws(s, WS_OUT, plane_out * (size_t)per) : NULL;
+    int rc = !d || !o;
+    for (long z0 = 0; z0 < oz && !rc; z0 += per) {
+        long nzs = oz - z0 < per ? oz - z0 : per;
+        long zin = 2 * z0, nin = 2 * nzs < nz - zin ? 2 * nzs : nz - zin;
+        rc = fyg_h2d(d, in + (size_t)zin * plane_in, (size_t)nin * plane_in, s) ||
+             fyg_downscale2x_run(d, nx, ny, (int)nin, o, (int)method, alpha, s) ||
+             fyg_d2h(out + (size_t)z0 * plane_out, o, (size_t)nzs * plane_out, s);
+    }
+    /* copies already queued must end before out is handed back (or zeroed) */
+    if (s && fyg_sync(s)) rc = 1;
+    if (rc) {
+        const char *why = fyg_error_string();
+        char msg[200];
+        snprintf(msg, sizeof msg, "fy_downscale2x: output zeroed after a device failure (%s)",
+                 why && *why ? why : "no device");
+        memset(out, 0, (size_t)ox * oy * oz);
+        fyg_set_error(msg);
+    }
 }
 
 int fy_gpu_downscale2x(const unsigned char *d_in, int nx, int ny, int nz, unsigned char *d_out, fy_ds_method method,
@@ -308,3 +337,9 @@ int fy_gpu_profile_read(char *names, int names_len, double *ms, long *count, int
 }
 void fy_gpu_profile_reset(void) { fyg_prof_reset(); }
 const char *fy_gpu_last_error(void) { return fyg_error_string(); }
+
+long fy_gpu_table_cache(long *stats, long limit) {
+    size_t old = fyg_table_limit(limit > 0 ? (size_t)limit : 0);
+    if (stats) fyg_table_stats(stats);
+    return (long)old;
+}

@@ -226,10 +226,14 @@ int fyh_deconv_dev(const float *d_in, float *d_out, int nz, int ny, int nx, cons
                    double param, int kind, double cap, double fz2_scale, fyg_stream *s) {
     if (nz <= 0 || ny <= 0 || nx <= 0) return 0;
     fyh_tables t;
-    if (fyh_tables_get(nz, ny, nx, p, param, kind, cap, fz2_scale, 0, s, &t)) return 1;
-    if (!fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px)) return 1;
-    void *spec = fyg_ws(s, WS_SPEC, fyg_deconv_spec_bytes(t.pz, t.py, t.px));
-    if (!spec) return 1;
+    fyg_hold *hold = fyg_hold_begin();          /* tables of the process-wide cache stay until queued */
+    int rc = fyh_tables_get(nz, ny, nx, p, param, kind, cap, fz2_scale, 0, s, &t) ||
+             !fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px);
+    void *spec = rc ? NULL : fyg_ws(s, WS_SPEC, fyg_deconv_spec_bytes(t.pz, t.py, t.px));
+    if (!spec) {
+        fyg_hold_release(hold, s);
+        return 1;
+    }
     fyg_deconv_args a;
     memset(&a, 0, sizeof a);
     a.nz = nz; a.ny = ny; a.nx = nx;
@@ -238,7 +242,9 @@ int fyh_deconv_dev(const float *d_in, float *d_out, int nz, int ny, int nx, cons
     a.in = d_in; a.in_u8 = 0;
     a.out = d_out;
     a.spec = spec;
-    return fyg_deconv_run(&a, s);
+    rc = fyg_deconv_run(&a, s);
+    if (fyg_hold_release(hold, s)) rc = 1;
+    return rc;
 }
 
 static int deconv_host(const float *in, float *out, int nz, int ny, int nx, const fy_physics *p,
@@ -297,10 +303,14 @@ int fy_gpu_deconv_radial_f64(const float *d_in, double *d_out, int nz, int ny, i
     fyg_stream *s = fyg_stream_wrap(stream);
     if (!s) return 1;
     fyh_tables t;
-    if (fyh_tables_get(nz, ny, nx, p, param, kind, cap, fz2_scale, 1, s, &t)) return 1;
-    if (!fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px)) return 1;
-    void *spec = fyg_ws(s, WS_SPEC, fyg_deconv_spec_bytes64(t.pz, t.py, t.px));
-    if (!spec) return 1;
+    fyg_hold *hold = fyg_hold_begin();
+    int rc = fyh_tables_get(nz, ny, nx, p, param, kind, cap, fz2_scale, 1, s, &t) ||
+             !fyg_fft_supported(t.pz) || !fyg_fft_supported(t.py) || !fyg_fft_supported(t.px);
+    void *spec = rc ? NULL : fyg_ws(s, WS_SPEC, fyg_deconv_spec_bytes64(t.pz, t.py, t.px));
+    if (!spec) {
+        fyg_hold_release(hold, s);
+        return 1;
+    }
     fyg_deconv_args a;
     memset(&a, 0, sizeof a);
     a.nz = nz; a.ny = ny; a.nx = nx;
@@ -310,5 +320,7 @@ int fy_gpu_deconv_radial_f64(const float *d_in, double *d_out, int nz, int ny, i
     a.out = d_out;
     a.spec = spec;
     a.f64 = 1;
-    return fyg_deconv_run(&a, s);
+    rc = fyg_deconv_run(&a, s);
+    if (fyg_hold_release(hold, s)) rc = 1;
+    return rc;
 }

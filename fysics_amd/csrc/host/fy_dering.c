@@ -208,16 +208,21 @@ void fy_dering_apply(const fy_dering *d, float *f, long z0, long y0, long x0, lo
     if (!n) return;
     fyg_stream *s = fyg_thread_stream();
     if (!s) return;
+    fyg_hold *hold = fyg_hold_begin();
     const float *ring = (const float *)fyh_const_blob('R', d->ring, sizeof(float) * (size_t)d->nslab * d->nr);
     float *df = (float *)fyg_ws(s, WS_IN, sizeof(float) * n);
-    if (!ring || !df) return;
+    if (!ring || !df) {
+        fyg_hold_release(hold, s);
+        return;
+    }
     fyg_prologue pro;
     memset(&pro, 0, sizeof pro);
     pro.dering = 1; pro.ring = ring; pro.nr = d->nr; pro.nslab = d->nslab; pro.slab_z = d->slab_z;
     pro.cy = d->cy; pro.cx = d->cx; pro.scale = (float)scale;
     pro.gz0 = z0; pro.gy0 = y0; pro.gx0 = x0;
     pro.ring_fast = fyh_ring_fast(d->cy, d->cx, y0, x0, dy, dx);
-    if (fyg_h2d(df, f, sizeof(float) * n, s) || fyg_dering_apply_run(df, dz, dy, dx, &pro, s)) return;
-    if (fyg_d2h(f, df, sizeof(float) * n, s)) return;
-    fyg_sync(s);
+    int rc = fyg_h2d(df, f, sizeof(float) * n, s) || fyg_dering_apply_run(df, dz, dy, dx, &pro, s) ||
+             fyg_d2h(f, df, sizeof(float) * n, s);
+    fyg_hold_release(hold, s);
+    if (!rc) fyg_sync(s);
 }

@@ -1,6 +1,6 @@
 /* fy_export.c -- feeding zarr tiles through the GPU chain.
  *
- * fy_process_chunk (pipeline.c:518-538): one tile, synchronous, its halo'd
+ * fy_gpu_process_chunk (fy_process_chunk, pipeline.c:518-538): one tile, synchronous, its halo'd
  * region read into the thread's pinned staging buffer.
  *
  * fy_gpu_export: the pass-2 tile loop of fy_run_pipeline (pipeline.c:1000-1070)
@@ -8,8 +8,8 @@
  * (read, chain, quantise, write).  Here host threads only do I/O -- they read
  * halo'd regions straight into pinned slots and write finished chunks -- while
  * the calling thread moves slots through three GPU stages on three streams.
- * The reads share one mapping of each input chunk (fyh_zcache): the up to 27
- * chunks a halo'd tile touches are each mapped once per export, not once per
+ * The reads share one mapping of each input chunk (fy_gpu_zarr_cache): the up
+ * to 27 chunks a halo'd tile touches are each mapped once per export, not once per
  * neighbouring tile.
  *
  *   reads (threads) -> H2D (copy stream) -> fy_gpu_process_cubes (the caller's
@@ -81,8 +81,8 @@ static void geom(long Z, long Y, long X, long nty, long ntx, int tile, int halo,
     g->hx = lmin(X, g->x0 + g->tx + halo) - g->rx0;
 }
 
-int fy_process_chunk(const fy_zarr *zin, const fy_pipeline_cfg *cfg, long z0, long y0, long x0, int tile,
-                     unsigned char *out, int *out_tz, int *out_ty, int *out_tx, int *all_air) {
+int fy_gpu_process_chunk(const fy_zarr *zin, const fy_pipeline_cfg *cfg, long z0, long y0, long x0, int tile,
+                         unsigned char *out, int *out_tz, int *out_ty, int *out_tx, int *all_air) {
     const long Z = zin->shape[0], Y = zin->shape[1], X = zin->shape[2];
     const int halo = cfg->halo > 0 ? cfg->halo : 8;
     const long tz = lmin(tile, Z - z0), ty = lmin(tile, Y - y0), tx = lmin(tile, X - x0);
@@ -91,7 +91,7 @@ int fy_process_chunk(const fy_zarr *zin, const fy_pipeline_cfg *cfg, long z0, lo
     if (out_tx) *out_tx = (int)tx;
     if (all_air) *all_air = 1;
     if (tile < 1 || z0 < 0 || y0 < 0 || x0 < 0 || tz <= 0 || ty <= 0 || tx <= 0) {
-        fyg_set_error("fy_process_chunk: tile outside the volume");
+        fyg_set_error("fy_gpu_process_chunk: tile outside the volume");
         return 1;
     }
     const long rz0 = lmax(0, z0 - halo), ry0 = lmax(0, y0 - halo), rx0 = lmax(0, x0 - halo);
@@ -101,7 +101,7 @@ int fy_process_chunk(const fy_zarr *zin, const fy_pipeline_cfg *cfg, long z0, lo
     if (!s) return 1;
     unsigned char *u8 = (unsigned char *)fyg_pinned(s, 0, (size_t)hz * hy * hx);
     if (!u8) return 1;
-    if (fy_zarr_read(zin, rz0, ry0, rx0, hz, hy, hx, u8)) return 1;
+    if (fy_gpu_zarr_read(zin, rz0, ry0, rx0, hz, hy, hx, u8)) return 1;
     return fy_process_buffer(cfg, u8, rz0, ry0, rx0, hz, hy, hx, z0, y0, x0, tz, ty, tx, Y, X, out, all_air);
 }
 
@@ -135,7 +135,7 @@ typedef struct {
     long Z, Y, X, nty, ntx;
     size_t cap, tcap;
     slot_t sl[NSLOT];
-    fyh_zcache *zc;             /* input chunk mappings shared by the reads */
+    fy_gpu_zarr_cache *zc;             /* input chunk mappings shared by the reads */
     task_t *q;
     int qcap, qhead, qn;
     pthread_mutex_t mu;
@@ -161,7 +161,7 @@ static void do_read(exporter *e, slot_t *s, int k) {
     unsigned char *dst = s->hin + (size_t)k * e->cap;
     const size_t hn = (size_t)g->hz * g->hy * g->hx;
     double t = now_s();
-    int bad = fyh_zarr_read_cached(e->zin, e->zc, g->rz0, g->ry0, g->rx0, g->hz, g->hy, g->hx, dst);
+    int bad = fy_gpu_zarr_read_cached(e->zin, e->zc, g->rz0, g->ry0, g->rx0, g->hz, g->hy, g->hx, dst);
     int air = bad ? 1 : !any_nonzero(dst, hn);
     t = now_s() - t;
     pthread_mutex_lock(&e->mu);
@@ -183,7 +183,7 @@ static void do_write(exporter *e, slot_t *s, int k) {
     int bad = fyg_event_wait(s->e_out) != 0;
     double t = now_s();
     if (!bad)
-        bad = fy_zarr_write_chunk(e->zout, g->iz, g->iy, g->ix, s->hout + (size_t)k * e->tcap, g->tz, g->ty,
+        bad = fy_gpu_zarr_write_chunk(e->zout, g->iz, g->iy, g->ix, s->hout + (size_t)k * e->tcap, g->tz, g->ty,
                                   g->tx) != 0;
     t = now_s() - t;
     pthread_mutex_lock(&e->mu);
@@ -316,7 +316,7 @@ int fy_gpu_export(const fy_zarr *zin, const fy_zarr *zout, const fy_pipeline_cfg
     e->tcap = (size_t)tile * tile * tile;
     e->qcap = NSLOT * batch * 2 + NSLOT + threads + 4;
     e->q = (task_t *)calloc((size_t)e->qcap, sizeof(task_t));
-    e->zc = fyh_zcache_new((size_t)1 << 31);
+    e->zc = fy_gpu_zarr_cache_new((size_t)1 << 31);
     int rc = e->q == NULL || e->zc == NULL;
     for (int i = 0; i < NSLOT && !rc; i++) {
         slot_t *s = &e->sl[i];
@@ -425,7 +425,7 @@ int fy_gpu_export(const fy_zarr *zin, const fy_zarr *zout, const fy_pipeline_cfg
     }
     free(th);
     free(e->q);
-    fyh_zcache_free(e->zc);
+    fy_gpu_zarr_cache_free(e->zc);
     if (e->io_fail) rc = 1;
     if (rc && e->err[0]) fyg_set_error(e->err);
     if (stats) {

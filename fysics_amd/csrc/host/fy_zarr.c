@@ -11,7 +11,9 @@
  * pages of the sliver it copies) and copy rows straight into the caller's
  * buffer, which the pass-2 feeder allocates pinned so the bytes go to the GPU
  * without another copy.  S3 roots (zarr_io.c:20-36, built with FYSICS_S3) are
- * not supported: fy_zarr_open fails on them.
+ * not supported: fy_gpu_zarr_open fails on them.  These functions keep the
+ * reference's argument meaning under their own names (fy_gpu_zarr_*), so a
+ * program that links both libraries keeps the reference's fy_zarr_* with S3.
  */
 #define _GNU_SOURCE
 #include <errno.h>
@@ -64,7 +66,7 @@ static int json_triple(const char *v, long out[3]) {
 
 static int json_is(const char *v, const char *lit) { return v && strncmp(v, lit, strlen(lit)) == 0; }
 
-int fy_zarr_open(fy_zarr *z, const char *root) {
+int fy_gpu_zarr_open(fy_zarr *z, const char *root) {
     memset(z, 0, sizeof *z);
     if (!root || strncmp(root, "s3://", 5) == 0) return zerr("S3 roots are not supported:", root ? root : "");
     if (strlen(root) >= sizeof z->root) return zerr("root path too long:", root);
@@ -108,7 +110,7 @@ static int make_parents(const char *path) {
     return 0;
 }
 
-int fy_zarr_create(fy_zarr *z, const char *root, const long shape[3], const long chunk[3]) {
+int fy_gpu_zarr_create(fy_zarr *z, const char *root, const long shape[3], const long chunk[3]) {
     memset(z, 0, sizeof *z);
     if (!root || strlen(root) >= sizeof z->root) return zerr("bad root", root ? root : "");
     snprintf(z->root, sizeof z->root, "%s", root);
@@ -217,7 +219,7 @@ typedef struct {
 
 #define ZC_ENTRIES 256
 
-struct fyh_zcache {
+struct fy_gpu_zarr_cache {
     pthread_mutex_t mu;
     pthread_cond_t cv;
     zc_entry e[ZC_ENTRIES];
@@ -231,8 +233,8 @@ static void zc_release(chunk_bytes *cb) {
     memset(cb, 0, sizeof *cb);
 }
 
-fyh_zcache *fyh_zcache_new(size_t budget) {
-    fyh_zcache *c = (fyh_zcache *)calloc(1, sizeof *c);
+fy_gpu_zarr_cache *fy_gpu_zarr_cache_new(size_t budget) {
+    fy_gpu_zarr_cache *c = (fy_gpu_zarr_cache *)calloc(1, sizeof *c);
     if (!c) return NULL;
     c->budget = budget;
     pthread_mutex_init(&c->mu, NULL);
@@ -240,7 +242,7 @@ fyh_zcache *fyh_zcache_new(size_t budget) {
     return c;
 }
 
-void fyh_zcache_free(fyh_zcache *c) {
+void fy_gpu_zarr_cache_free(fy_gpu_zarr_cache *c) {
     if (!c) return;
     for (int i = 0; i < ZC_ENTRIES; i++) zc_release(&c->e[i].cb);
     pthread_mutex_destroy(&c->mu);
@@ -251,7 +253,7 @@ void fyh_zcache_free(fyh_zcache *c) {
 /* the ready entry of chunk (cz,cy,cx), mapped whole and held until zc_put;
  * NULL on error.  A reader holds one entry at a time, so with fewer reading
  * threads than entries a free or unheld entry always turns up. */
-static zc_entry *zc_get(fyh_zcache *c, const fy_zarr *z, long cz, long cy, long cx, size_t cap) {
+static zc_entry *zc_get(fy_gpu_zarr_cache *c, const fy_zarr *z, long cz, long cy, long cx, size_t cap) {
     pthread_mutex_lock(&c->mu);
     zc_entry *slot;
     for (;;) {
@@ -316,7 +318,7 @@ static zc_entry *zc_get(fyh_zcache *c, const fy_zarr *z, long cz, long cy, long 
     return bad ? NULL : slot;
 }
 
-static void zc_put(fyh_zcache *c, zc_entry *e) {
+static void zc_put(fy_gpu_zarr_cache *c, zc_entry *e) {
     pthread_mutex_lock(&c->mu);
     e->refs--;
     pthread_cond_broadcast(&c->cv);
@@ -353,11 +355,11 @@ static void copy_rows(unsigned char *dst, size_t dpitch, const unsigned char *sr
     }
 }
 
-int fy_zarr_read(const fy_zarr *z, long z0, long y0, long x0, long dz, long dy, long dx, unsigned char *out) {
-    return fyh_zarr_read_cached(z, NULL, z0, y0, x0, dz, dy, dx, out);
+int fy_gpu_zarr_read(const fy_zarr *z, long z0, long y0, long x0, long dz, long dy, long dx, unsigned char *out) {
+    return fy_gpu_zarr_read_cached(z, NULL, z0, y0, x0, dz, dy, dx, out);
 }
 
-int fyh_zarr_read_cached(const fy_zarr *z, fyh_zcache *zc, long z0, long y0, long x0, long dz, long dy, long dx,
+int fy_gpu_zarr_read_cached(const fy_zarr *z, fy_gpu_zarr_cache *zc, long z0, long y0, long x0, long dz, long dy, long dx,
                          unsigned char *out) {
     if (dz <= 0 || dy <= 0 || dx <= 0) return 0;
     const long C[3] = {z->chunk[0], z->chunk[1], z->chunk[2]};
@@ -425,7 +427,7 @@ int fyh_zarr_read_cached(const fy_zarr *z, fyh_zcache *zc, long z0, long y0, lon
     return rc;
 }
 
-int fy_zarr_write_chunk(const fy_zarr *z, long cz, long cy, long cx, const unsigned char *buf, long bz, long by,
+int fy_gpu_zarr_write_chunk(const fy_zarr *z, long cz, long cy, long cx, const unsigned char *buf, long bz, long by,
                         long bx) {
     char path[1100];
     snprintf(path, sizeof path, "%s/0/%ld/%ld/%ld", z->root, cz, cy, cx);

@@ -60,12 +60,4 @@ int fyh_chain(const fy_pipeline_cfg *cfg, const unsigned char *d_u8,
 int fyh_deconv_dev(const float *d_in, float *d_out, int nz, int ny, int nx, const fy_physics *p,
                    double param, int kind, double cap, double fz2_scale, fyg_stream *s);
 
-/* chunk mappings shared by the reads of one tile loop (fy_zarr.c) */
-typedef struct fyh_zcache fyh_zcache;
-fyh_zcache *fyh_zcache_new(size_t budget_bytes);
-void fyh_zcache_free(fyh_zcache *c);
-/* fy_zarr_read through the cache (zc == NULL: uncached) */
-int fyh_zarr_read_cached(const fy_zarr *z, fyh_zcache *zc, long z0, long y0, long x0, long dz, long dy, long dx,
-                         unsigned char *out);
-
 #endif

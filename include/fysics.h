@@ -11,10 +11,10 @@
  * Only the hot-path subset is provided (FFT, transfers, Wiener deconvolution,
  * guided filter, u8/physical conversions, ring removal, noise fit, the pass-1
  * calibration statistics -- histogram, z-drift sums, flat-block noise floor --
- * the per-tile chain with its air-zero stage, local zarr tile I/O and
- * fy_process_chunk, the 2x LOD downscale and phase correlation).  The rest of
- * registration, S3 zarr, MUSICA and the whole-volume calibrate/run drivers are
- * not part of this library.
+ * the per-tile chain with its air-zero stage, the 2x LOD downscale and phase
+ * correlation).  Registration, zarr I/O (local zarr I/O lives beside, under
+ * fy_gpu_ names), fy_process_chunk, MUSICA and the whole-volume calibrate/run
+ * drivers are not part of it.
  *
  * Layout: row-major, x fastest, idx = (z*ny + y)*nx + x (fysics.h:120).
  */
@@ -227,25 +227,6 @@ typedef struct {
     long occ_cz, occ_cy, occ_cx;
 } fy_pipeline_cfg;
 
-/* Pass-1 commits of fy_run_pipeline as functions (pipeline.c:735-740,851-900;
- * no reference counterpart: the reference runs them inline).  Host fp64.
- * fy_calib_commit_eps: from n flat-noise floors of sample sub-blocks and the
- * distances of their centres to the rotation axis, the gated least-squares fit
- * fn(r) = a + b*r (needs n >= 8, b > 0, correlation >= 0.5, a > 0 and
- * fn(rmax)/a >= 1.3; sets have_eps_r, eps_fn_a, eps_fn_b, or clears
- * have_eps_r), eps_fn_med = the median floor (0.015 with no samples),
- * noise_floor = 255 * median, and, when guided_eps <= 0 and denoise_k >= 0,
- * guided_eps = (k * median)^2 with k = denoise_k (4.2 when 0).  The inputs
- * are not modified. */
-void   fy_calib_commit_eps(fy_pipeline_cfg *cfg, const double *rads, const double *floors, int n);
-/* sigma_z/sigma_xy of one sample from fy_noise_aniso's lag-1 correlations:
- * sqrt(ln(1/arxy)/ln(1/arz)) when both lie in (0.05, 0.95) and the ratio in
- * (0.5, 3), else -1 */
-double fy_aniso_ratio(double arz, double arxy);
-/* psf_z_ratio = min(median of n ratios, 1.6) when n >= 5 and the median is
- * >= 1.05; otherwise cfg is left as it is */
-void   fy_calib_commit_aniso(fy_pipeline_cfg *cfg, const double *ratios, int n);
-
 /* Process one inner tile from an already-read halo'd u8 buffer (fysics.h:507-511;
  * pipeline.c:467-510): u8 -> f01 -> dering -> z-drift -> (deconv chain) -> guided
  * (per-tile eps) -> air-zero + despeckle (do_air_zero) -> recenter -> dithered
@@ -257,32 +238,16 @@ int fy_process_buffer(const fy_pipeline_cfg *cfg, const unsigned char *u8buf,
                       long vol_y, long vol_x,
                       unsigned char *out, int *all_air);
 
-/* Local zarr v2 raw-u8 volumes (zarr_io.c; fysics.h:375-389, same struct
- * layout): <root>/0/.zarray plus one uncompressed C-order file per chunk at
- * <root>/0/cz/cy/cx; chunks on the far faces hold their true extent; a
- * missing chunk (or the part past a short file's end) reads as fill_value.
- * fy_zarr_open fails on S3 roots, compressed chunks and dtypes other than u1.
- * fy_zarr_read fills voxels outside the volume with fill_value. */
+/* The zarr volume handle (fysics.h:376-381, same layout).  This library reads
+ * and writes local zarr volumes under its own names (fy_gpu_zarr_*,
+ * fy_gpu_process_chunk in fysics_gpu.h); the reference's fy_zarr_* and
+ * fy_process_chunk, which also reach S3, are not replaced. */
 typedef struct {
     char root[1024];
     long shape[3];      /* z,y,x voxels */
     long chunk[3];      /* chunk dims */
     unsigned char fill; /* fill_value */
 } fy_zarr;
-
-int fy_zarr_open(fy_zarr *z, const char *root);
-int fy_zarr_create(fy_zarr *z, const char *root, const long shape[3], const long chunk[3]);
-int fy_zarr_read(const fy_zarr *z, long z0, long y0, long x0,
-                 long dz, long dy, long dx, unsigned char *out);
-int fy_zarr_write_chunk(const fy_zarr *z, long cz, long cy, long cx,
-                        const unsigned char *buf, long bz, long by, long bx);
-
-/* fy_process_buffer of the inner tile^3 at (z0,y0,x0), clamped to the volume,
- * with its halo (cfg->halo, 8 if <= 0) read from zin (pipeline.c:518-538).
- * Writes tz*ty*tx u8 to out and the clamped sizes to out_t* (any may be NULL). */
-int fy_process_chunk(const fy_zarr *zin, const fy_pipeline_cfg *cfg,
-                     long z0, long y0, long x0, int tile,
-                     unsigned char *out, int *out_tz, int *out_ty, int *out_tx, int *all_air);
 
 #ifdef __cplusplus
 }

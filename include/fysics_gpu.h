@@ -134,6 +134,25 @@ void fy_serve_destroy(fy_server *sv);
 int fy_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long ty, long tx,
                             long vol_y, long vol_x, fy_cube_params *out);
 
+/* Pass-1 commits of fy_run_pipeline as functions (pipeline.c:735-740,851-900;
+ * no reference counterpart: the reference runs them inline).  Host fp64.
+ * fy_calib_commit_eps: from n flat-noise floors of sample sub-blocks and the
+ * distances of their centres to the rotation axis, the gated least-squares fit
+ * fn(r) = a + b*r (needs n >= 8, b > 0, correlation >= 0.5, a > 0 and
+ * fn(rmax)/a >= 1.3; sets have_eps_r, eps_fn_a, eps_fn_b, or clears
+ * have_eps_r), eps_fn_med = the median floor (0.015 with no samples),
+ * noise_floor = 255 * median, and, when guided_eps <= 0 and denoise_k >= 0,
+ * guided_eps = (k * median)^2 with k = denoise_k (4.2 when 0).  The inputs
+ * are not modified. */
+void   fy_calib_commit_eps(fy_pipeline_cfg *cfg, const double *rads, const double *floors, int n);
+/* sigma_z/sigma_xy of one sample from fy_noise_aniso's lag-1 correlations:
+ * sqrt(ln(1/arxy)/ln(1/arz)) when both lie in (0.05, 0.95) and the ratio in
+ * (0.5, 3), else -1 */
+double fy_aniso_ratio(double arz, double arxy);
+/* psf_z_ratio = min(median of n ratios, 1.6) when n >= 5 and the median is
+ * >= 1.05; otherwise cfg is left as it is */
+void   fy_calib_commit_aniso(fy_pipeline_cfg *cfg, const double *ratios, int n);
+
 /* ---- pass-1 calibration sweep on device tiles (pipeline.c:576-701).
  * A sweep holds device accumulators for the volume's u8 histogram and the
  * per-slice z-drift sums of Z slices.  fy_calib_accumulate_u8 queues the
@@ -166,6 +185,38 @@ int  fy_gpu_phase_correlate(const float *d_fixed, const float *d_moving, int nz,
 /* fy_flat_noise of a device volume (stream.c:174-200) */
 int  fy_gpu_flat_noise(const float *d_v, int nz, int ny, int nx, int blk, double *out, void *stream);
 
+/* ---- local zarr v2 raw-u8 volumes (the format of zarr_io.c:1-9; the
+ * reference's fy_zarr_open / _create / _read / _write_chunk, fysics.h:375-389,
+ * restricted to local roots and kept under names of their own so that a program
+ * linking both libraries keeps the reference's S3-capable functions).
+ * <root>/0/.zarray plus one uncompressed C-order file per chunk at
+ * <root>/0/cz/cy/cx; chunks on the far faces hold their true extent; a missing
+ * chunk (or the part past a short file's end) reads as fill_value.
+ * fy_gpu_zarr_open fails on S3 roots, compressed chunks and dtypes other than
+ * u1; fy_gpu_zarr_read fills voxels outside the volume with fill_value.
+ * Volumes written by either library read back identically in the other. */
+int fy_gpu_zarr_open(fy_zarr *z, const char *root);
+int fy_gpu_zarr_create(fy_zarr *z, const char *root, const long shape[3], const long chunk[3]);
+int fy_gpu_zarr_read(const fy_zarr *z, long z0, long y0, long x0,
+                     long dz, long dy, long dx, unsigned char *out);
+int fy_gpu_zarr_write_chunk(const fy_zarr *z, long cz, long cy, long cx,
+                            const unsigned char *buf, long bz, long by, long bx);
+/* chunk mappings shared by the reads of one tile loop: each chunk file a read
+ * touches is mapped once and kept while the mapped bytes stay under
+ * budget_bytes; fy_gpu_zarr_read_cached reads through it (zc NULL: uncached) */
+typedef struct fy_gpu_zarr_cache fy_gpu_zarr_cache;
+fy_gpu_zarr_cache *fy_gpu_zarr_cache_new(size_t budget_bytes);
+void fy_gpu_zarr_cache_free(fy_gpu_zarr_cache *c);
+int fy_gpu_zarr_read_cached(const fy_zarr *z, fy_gpu_zarr_cache *zc, long z0, long y0, long x0,
+                            long dz, long dy, long dx, unsigned char *out);
+/* fy_process_chunk (pipeline.c:518-538) on a local zarr: fy_process_buffer of
+ * the inner tile^3 at (z0,y0,x0), clamped to the volume, with its halo
+ * (cfg->halo, 8 if <= 0) read from zin.  Writes tz*ty*tx u8 to out and the
+ * clamped sizes to out_t* (any may be NULL). */
+int fy_gpu_process_chunk(const fy_zarr *zin, const fy_pipeline_cfg *cfg,
+                         long z0, long y0, long x0, int tile,
+                         unsigned char *out, int *out_tz, int *out_ty, int *out_tx, int *all_air);
+
 /* ---- pass-2 export through the GPU: the tile loop of fy_run_pipeline
  * (pipeline.c:1000-1070) fed from zarr.  Tiles t in [t_begin, t_end) of the
  * pass-2 grid (t = (iz*nty + iy)*ntx + ix; t_end <= 0: all) are read with
@@ -177,7 +228,7 @@ int  fy_gpu_flat_noise(const float *d_v, int nz, int ny, int nx, int blk, double
  * directions, kernels and writes overlap.  All-air tiles are skipped (their
  * chunks stay absent and read as 0), as in the reference; failed reads and
  * writes are counted and make the call return 1.  Output chunks are
- * bit-identical to fy_process_chunk's.  No reference counterpart (the
+ * bit-identical to fy_gpu_process_chunk's.  No reference counterpart (the
  * reference runs this loop on host threads). */
 typedef struct {
     long tiles;                     /* tiles in the range */
@@ -209,6 +260,15 @@ int fy_gpu_tuning_load(const char *path);
 void fy_gpu_tuning_clear(void);
 
 int fy_gpu_sync(void *stream);
+
+/* the process-wide cache of device tables (transfer tables of captured and
+ * large deconvolutions, ring and z-drift blobs): stats (may be NULL) receives
+ * 7 counts -- entries, evictable entries, entries held by queued work or a
+ * live graph, evicted entries whose memory waits for their last readers,
+ * evictions and frees so far, and the limit on evictable entries.  limit > 0
+ * sets that limit (default 512).  Returns the limit before the call.  No
+ * reference counterpart. */
+long fy_gpu_table_cache(long *stats, long limit);
 
 /* per-kernel timing: HIP events recorded around every launch on its own stream */
 void fy_gpu_profile(int on);

@@ -1079,6 +1079,85 @@ int orc_chain_cube(const fy_cube_params *P, const unsigned char *u8buf,
     return 0;
 }
 
+/* orc_chain_cube with the deconvolution and the guided filter evaluated in
+ * fp64 (the same LUT semantics; orc_deconv_radial_f64, orc_guided_*_f64): the
+ * reference to hold the fp32 chain's float tile to relative L2 1e-5.  The u8
+ * prologue (u8/255, ring subtraction, z-drift, dewindow) stays fp32, as the
+ * reference's input to the deconvolution is fp32.  ftile gets the double tile
+ * after the guided filter (before recenter); out the inner tile quantised from
+ * it (recenter and dither as orc_chain_cube).  Air-zero is not covered (1). */
+int orc_chain_cube_f64(const fy_cube_params *P, const unsigned char *u8buf,
+                       long rz0, long ry0, long rx0, long hz, long hy, long hx,
+                       long z0, long y0, long x0, long tz, long ty, long tx,
+                       unsigned char *out, double *ftile) {
+    if (P->air_zero) return 1;
+    size_t hn = (size_t)hz * hy * hx;
+    int any = 0;
+    for (size_t i = 0; i < hn && !any; i++) any = u8buf[i] != 0;
+    if (!any) {
+        memset(out, 0, (size_t)tz * ty * tx);
+        if (ftile) memset(ftile, 0, sizeof(double) * hn);
+        return 0;
+    }
+    float *f = malloc(sizeof(float) * hn);
+    double *d = malloc(sizeof(double) * hn), *d1 = malloc(sizeof(double) * hn);
+    if (!f || !d || !d1) { free(f); free(d); free(d1); return 1; }
+    orc_u8_to_float(u8buf, f, hn);
+    if (P->dering && P->dering->have_rings) orc_dering_apply(P->dering, f, rz0, ry0, rx0, hz, hy, hx, 1.0 / 255.0);
+    if (P->zdrift_factor) orc_zdrift_apply(f, (int)hz, (int)hy, (int)hx, (int)rz0, P->zdrift_factor);
+    int nz = (int)hz, ny = (int)hy, nx = (int)hx, rc = 0;
+    if (P->deconv) {
+        int win = P->window_hi > P->window_lo;
+        float wlo = (float)P->window_lo, wspan = (float)(P->window_hi - P->window_lo);
+        if (win) for (size_t i = 0; i < hn; i++) f[i] = wlo + f[i] * wspan;
+        int matched = P->deconv == 2;
+        double param = P->deconv_param;
+        if (param <= 0) param = matched ? 0.05 : 0.015;
+        rc = orc_deconv_radial_f64(f, d, nz, ny, nx, &P->phys, param, matched, P->deconv_cap,
+                                   P->fz2_scale > 0 ? P->fz2_scale : 1.0);
+        if (!rc) {
+            double lo = win ? P->window_lo : 0.0, k = win ? 1.0 / (P->window_hi - P->window_lo) : 1.0;
+            int dec = P->have_dec_range && P->dec_hi - P->dec_lo > 1e-6;
+            for (size_t i = 0; i < hn; i++) {
+                double v = (d[i] - lo) * k;
+                if (dec) v = (v - P->dec_lo) / (P->dec_hi - P->dec_lo);
+                d[i] = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+            }
+        }
+    } else {
+        for (size_t i = 0; i < hn; i++) d[i] = f[i];
+    }
+    if (!rc && P->guided_eps > 0) {
+        int gs = P->guided_s == 0 ? 2 : P->guided_s;
+        int r = P->guided_radius < 1 ? 2 : P->guided_radius;
+        rc = gs <= 1 ? orc_guided_exact_f64(d, d1, nz, ny, nx, r, P->guided_eps)
+                     : orc_guided_fast_f64(d, d1, nz, ny, nx, r, P->guided_eps, gs);
+        if (!rc) memcpy(d, d1, sizeof(double) * hn);
+    }
+    if (!rc) {
+        if (ftile) memcpy(ftile, d, sizeof(double) * hn);
+        float mid = 0.5f * (float)(P->norm_lo + P->norm_hi) / 255.0f;
+        float k = P->norm_hi > P->norm_lo ? (2.0f * 0.375f) * 255.0f / (float)(P->norm_hi - P->norm_lo) : 0.0f;
+        int rec = P->recenter && P->norm_hi > P->norm_lo;
+        long iz = z0 - rz0, iy = y0 - ry0, ix = x0 - rx0, o = 0;
+        for (long z = iz; z < iz + tz; z++)
+            for (long y = iy; y < iy + ty; y++)
+                for (long x = ix; x < ix + tx; x++) {
+                    float v = (float)d[((size_t)z * hy + y) * hx + x];
+                    if (rec) {
+                        if (v <= 0.0f) v = 0.0f;
+                        else {
+                            v = 0.5f + (v - mid) * k;
+                            v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+                        }
+                    }
+                    out[o++] = orc_quant_u8(v, rz0 + z, ry0 + y, rx0 + x, P->no_dither);
+                }
+    }
+    free(f); free(d); free(d1);
+    return rc;
+}
+
 /* fy_cube_params derived from a pipeline cfg as fy_process_buffer does
  * (pipeline.c:89-104,264-297,447-456) */
 int orc_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long ty, long tx,

@@ -143,6 +143,12 @@ int    orc_chain_cube(const fy_cube_params *P, const unsigned char *u8buf,
                       long rz0, long ry0, long rx0, long hz, long hy, long hx,
                       long z0, long y0, long x0, long tz, long ty, long tx,
                       unsigned char *out, int *all_air, float *ftile);
+/* the same chain with deconvolution and guided filter in fp64: the double
+ * float tile after the filter (ftile, may be NULL) and the u8 inner tile */
+int    orc_chain_cube_f64(const fy_cube_params *P, const unsigned char *u8buf,
+                          long rz0, long ry0, long rx0, long hz, long hy, long hx,
+                          long z0, long y0, long x0, long tz, long ty, long tx,
+                          unsigned char *out, double *ftile);
 int    orc_cube_params_from_cfg(const fy_pipeline_cfg *cfg, long y0, long x0, long ty, long tx,
                                 long vol_y, long vol_x, fy_cube_params *o);
 /* same chain, also returning the float tile after the guided filter (hz*hy*hx) */

@@ -198,6 +198,7 @@ _ORC_ONLY = {
                                       C.c_long, C.c_long, C.c_long, C.c_long]),
     "orc_chain_cube": (C.c_int, [C.POINTER(CubeParams), c_u8_p] + [C.c_long] * 12 +
                        [c_u8_p, c_int_p, c_float_p]),
+    "orc_chain_cube_f64": (C.c_int, [C.POINTER(CubeParams), c_u8_p] + [C.c_long] * 12 + [c_u8_p, c_double_p]),
     "orc_zdrift_u8_sub": (None, [c_u8_p, C.c_long, C.c_long, C.c_long, C.c_long, c_double_p,
                                  C.POINTER(C.c_long)]),
     "orc_air_cc_despeckle": (None, [c_float_p, C.c_int, C.c_int, C.c_int, C.c_int]),

@@ -314,6 +314,25 @@ def main():
                                  300, 400, clibs.up(q), C.byref(air)) == 0
     out["chain_u8"], out["chain_out"] = u8, q
 
+    # ---- fy_float_to_u8 (stream.c:31-36): the rounding edges of v*255 + 0.5
+    k = np.arange(256, dtype=np.float32)
+    edges = np.concatenate([k / np.float32(255.0), (k + np.float32(0.5)) / np.float32(255.0),
+                            np.nextafter((k + np.float32(0.5)) / np.float32(255.0), np.float32(-1)),
+                            np.nextafter((k + np.float32(0.5)) / np.float32(255.0), np.float32(2))])
+    extra = np.array([-1e9, -1.0, -0.5 / 255, -1e-7, 0.0, 1.0, 1.0 + 1e-7, 254.5 / 255, 255.5 / 255, 1.5, 7.0,
+                      1e9], np.float32)
+    fu = np.concatenate([edges, extra, rng.uniform(-0.2, 1.2, 4000).astype(np.float32)]).astype(np.float32)
+    fq = np.empty(fu.size, np.uint8)
+    ref.fy_float_to_u8(clibs.fp(fu), clibs.up(fq), fu.size)
+    out["f2u8_in"], out["f2u8_out"] = fu, fq
+
+    # ---- fy_zdrift_apply (zdrift.c:64-71): a slab at z0 = 5 of a 19-slice volume
+    zc = rng.uniform(0.0, 1.0, (7, 9, 11)).astype(np.float32)
+    zf = rng.uniform(0.8, 1.25, 19).astype(np.float32)
+    za = zc.copy()
+    ref.fy_zdrift_apply(clibs.fp(za), 7, 9, 11, 5, clibs.fp(zf))
+    out["zapply_in"], out["zapply_factor"], out["zapply_out"] = zc, zf, za
+
     np.savez_compressed(os.path.join(HERE, "reference_vectors.npz"), **out)
     print(f"wrote {len(out)} arrays")
 

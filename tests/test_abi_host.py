@@ -39,6 +39,36 @@ def test_library_exports_every_declared_symbol():
     assert not unbound, unbound
 
 
+def _declared_in(path):
+    src = open(path).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"typedef struct \{.*?\}\s*\w+;", "", src, flags=re.S)
+    return {m.group(1) for m in re.finditer(r"\b(fy_\w+)\s*\(", src)}
+
+
+def _interposed_list():
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    body = doc.split("<!-- interposed-symbols -->", 1)[1].split("<!-- /interposed-symbols -->", 1)[0]
+    return {w for w in body.split() if w}
+
+
+def test_exports_are_the_declared_set():
+    """The shared library exports the functions the headers declare and nothing
+    else; the ones with reference names (include/fysics.h) are exactly the list
+    INTEGRATION.md documents, and, where the reference header is present, every
+    exported name the reference also declares is on that list -- so no
+    narrowed variant takes a reference symbol's place by accident."""
+    out = subprocess.run(["nm", "-D", "--defined-only", F.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if ln.split() and ln.split()[-2] in ("T", "W")}
+    assert exported == set(declared_functions())
+    interposed = _declared_in(HEADERS[0])
+    assert _interposed_list() == interposed
+    assert not (interposed & _declared_in(HEADERS[1]))
+    ref_h = "/root/reference/fysics.h"
+    if os.path.exists(ref_h):
+        assert exported & _declared_in(ref_h) == interposed
+
+
 def test_no_oracle_or_reference_symbols_in_product():
     out = subprocess.run(["nm", "-D", "--defined-only", F.LIB_PATH], capture_output=True, text=True).stdout
     assert "orc_" not in out and "ref_chain" not in out

@@ -3,8 +3,9 @@ parameters, one launch per stage per shape group and FYG_BATCH cubes).
 
 Bars: a cube's u8 output is bit-identical whether it runs alone or inside a
 mixed batch (result invariance: no state is shared between cubes); against the
-oracle's chain (orc_chain_cube) the u8 output is within +-1 LSB and the fp32
-tile after the guided filter within relative L2 1e-5.
+oracle's chain (orc_chain_cube, and its fp64 twin orc_chain_cube_f64) the u8
+output is within +-1 LSB and the fp32 tile after the guided filter within
+relative L2 1e-5.
 """
 import ctypes as C
 
@@ -79,6 +80,14 @@ class Case:
                       self.d_out.data_ptr(), self.d_float.data_ptr() if self.d_float is not None else None,
                       C.pointer(self.P))
 
+    def oracle64(self, orc):
+        want = np.empty(self.inner, np.uint8)
+        ftile = np.empty(self.u8.shape, np.float64)
+        op = as_oracle(self.P)
+        assert orc.orc_chain_cube_f64(C.byref(op), clibs.up(self.u8), *self.origin, *self.u8.shape, *self.inner_o,
+                                      *self.inner, clibs.up(want), clibs.dp(ftile)) == 0
+        return want, ftile
+
     def oracle(self, orc):
         want = np.empty(self.inner, np.uint8)
         ftile = np.empty(self.u8.shape, np.float32)
@@ -144,9 +153,14 @@ def test_batch_float_tiles_vs_oracle(orc):
         want, ftile, air = c.oracle(orc)
         if i == 5:
             continue
-        assert rel_l2(c.d_float.cpu().numpy(), ftile) <= 1e-5, f"cube {i}"
+        gf = c.d_float.cpu().numpy()
+        assert rel_l2(gf, ftile) <= 1e-5, f"cube {i}"
         diff = np.abs(got[i].astype(int) - want.astype(int))
         assert diff.max() <= 1, f"cube {i}"
+        want64, f64 = c.oracle64(orc)
+        assert rel_l2(gf, f64) <= 1e-5, f"cube {i} vs the fp64 oracle"
+        d64 = np.abs(got[i].astype(int) - want64.astype(int))
+        assert d64.max() <= 1, f"cube {i} vs the fp64 oracle"
 
 
 def test_batch_order_invariance():

@@ -119,6 +119,15 @@ def test_zdrift_finalize_and_correct(fy, orc):
     orc.orc_zdrift_finalize(clibs.dp(s), clibs.lp(c), nz, clibs.fp(f))
     want = vol * f[:, None, None]
     assert np.allclose(got, want, rtol=2e-7, atol=0)
+    # the correction itself is one fp32 multiply per voxel: with the factors
+    # finalized from the device's own sums the result is bit-exact
+    sg, cg = np.zeros(nz), np.zeros(nz, np.int64)
+    fy.fy_zdrift_accumulate(vol.ctypes.data_as(F.c_float_p), nz, ny, nx, 0, sg.ctypes.data_as(F.c_double_p),
+                            cg.ctypes.data_as(C.POINTER(C.c_long)), 0.2)
+    assert np.array_equal(cg, c) and np.allclose(sg, s, rtol=1e-12, atol=0)
+    fg = np.empty(nz, np.float32)
+    orc.orc_zdrift_finalize(clibs.dp(sg), clibs.lp(cg), nz, clibs.fp(fg))
+    assert np.array_equal(got, vol * fg[:, None, None])
 
 
 @pytest.mark.parametrize("blk", [8, 5, 3, 16, 41])

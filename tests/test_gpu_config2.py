@@ -4,7 +4,8 @@ the oracle on the same inputs.
 
 Bars: ring detection sums, counts, votes and radii bit-exact (the finalized
 ring tables equal); the float tile after the guided filter within relL2 1e-5
-of the oracle's; the dithered u8 within +-1 on < 0.1 % of voxels.
+of the oracle's fp32 chain and of its fp64 twin (deconvolution and guided
+filter in double); the dithered u8 within +-1 on < 0.1 % of voxels of both.
 """
 import ctypes as C
 
@@ -79,9 +80,18 @@ def test_config2_chain_vs_oracle(orc):
                               C.byref(air), clibs.fp(want_f)) == 0
     r = np.linalg.norm((got_f - want_f).astype(np.float64)) / np.linalg.norm(want_f.astype(np.float64))
     diff = np.abs(got_q.astype(int) - want_q.astype(int))
-    print(f"config 2: rings {n1}, float relL2 {r:.2e}, u8 off by one {(diff > 0).mean():.2e}", flush=True)
-    assert r <= 1e-5
+    # the fp64 oracle: deconvolution and guided filter in double (same LUT)
+    q64 = np.empty(u8.shape, np.uint8)
+    f64 = np.empty(u8.shape, np.float64)
+    assert orc.orc_chain_cube_f64(C.byref(op), clibs.up(u8), 0, 0, 0, n, n, n, 0, 0, 0, n, n, n, clibs.up(q64),
+                                  clibs.dp(f64)) == 0
+    r64 = np.linalg.norm(got_f - f64) / np.linalg.norm(f64)
+    d64 = np.abs(got_q.astype(int) - q64.astype(int))
+    print(f"config 2: rings {n1}, float relL2 {r:.2e} (fp32 oracle) {r64:.2e} (fp64 oracle), "
+          f"u8 off by one {(diff > 0).mean():.2e} / {(d64 > 0).mean():.2e}", flush=True)
+    assert r <= 1e-5 and r64 <= 1e-5
     assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+    assert d64.max() <= 1 and (d64 > 0).mean() < 1e-3
     for x in (t, d):
         lib.fy_dering_free(C.byref(x))
     for x in (ot, od):

@@ -44,7 +44,7 @@ def volume(tmp_path_factory):
     root = str(tmp_path_factory.mktemp("exp") / "in.zarr")
     write_volume(root, vol, IN_CHUNK)
     z = F.Zarr()
-    assert F.lib().fy_zarr_open(C.byref(z), root.encode()) == 0
+    assert F.lib().fy_gpu_zarr_open(C.byref(z), root.encode()) == 0
     return vol, z, root
 
 
@@ -59,7 +59,7 @@ def tiles():
 def process_chunk(z, cfg, org):
     out = np.empty(TILE ** 3, np.uint8)
     tz, ty, tx, air = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-    rc = F.lib().fy_process_chunk(C.byref(z), C.byref(cfg), *org, TILE, out.ctypes.data, C.byref(tz), C.byref(ty),
+    rc = F.lib().fy_gpu_process_chunk(C.byref(z), C.byref(cfg), *org, TILE, out.ctypes.data, C.byref(tz), C.byref(ty),
                                   C.byref(tx), C.byref(air))
     assert rc == 0, F.lib().fy_gpu_last_error()
     shp = (tz.value, ty.value, tx.value)
@@ -99,7 +99,7 @@ def chunk_bytes(root, idx):
 def run_export(z, root_out, cfg, t0=0, t1=0, threads=4, batch=3):
     lib = F.lib()
     zo = F.Zarr()
-    assert lib.fy_zarr_create(C.byref(zo), root_out.encode(), (C.c_long * 3)(*SHAPE), (C.c_long * 3)(TILE, TILE, TILE)) == 0
+    assert lib.fy_gpu_zarr_create(C.byref(zo), root_out.encode(), (C.c_long * 3)(*SHAPE), (C.c_long * 3)(TILE, TILE, TILE)) == 0
     st = F.ExportStats()
     rc = lib.fy_gpu_export(C.byref(z), C.byref(zo), C.byref(cfg), TILE, t0, t1, threads, batch, C.byref(st), None)
     return rc, st

@@ -63,3 +63,75 @@ def test_graph_replays_match_eager():
                 assert not got[5].any() and got[2].any()
     finally:
         lib.fy_graph_destroy(C.c_void_p(g))
+
+
+def _cache_stats():
+    st = (C.c_long * 7)()
+    F.lib().fy_gpu_table_cache(st, 0)
+    return list(st)
+
+
+def test_graph_tables_survive_cache_eviction():
+    """Regression test of the table cache: a captured graph keeps the device
+    tables its launches read (transfer tables, ring and z-drift blobs) while
+    later calls fill the cache far past its limit and evict; every replay still
+    equals the eager result bit for bit, and the evicted entries' memory is
+    freed once their readers have finished."""
+    lib = F.lib()
+    rng = np.random.default_rng(31)
+    cases = make_cases(rng)[:6]
+    zf = [rng.uniform(0.9, 1.1, 64).astype(np.float32) for _ in cases]
+    for c, f in zip(cases, zf):                       # z-drift blobs held by the graph too
+        c.P.zdrift_factor = f.ctypes.data_as(F.c_float_p)
+        c.P.vol_z = 64
+    arr = (F.Cube * len(cases))(*[c.cube() for c in cases])
+    stream = torch.cuda.Stream()
+    old_limit = lib.fy_gpu_table_cache(None, 8)
+    g = None
+    try:
+        torch.cuda.synchronize()
+        g = lib.fy_gpu_graph_cubes(arr, len(cases), C.c_void_p(stream.cuda_stream))
+        assert g, lib.fy_gpu_last_error()
+        stream.synchronize()
+        held = _cache_stats()[2]
+        assert held >= len(cases)                     # per cube: its transfer table and its z-drift blob
+        # 40 more tables through eager calls on a tiny cube: each brings a new z-drift blob
+        tiny = phantom.noisy_u8(phantom.forward(phantom.sheets((12, 14, 16), rng, 2), **phantom.CONFIG1_PHYS), rng)
+        d_t = torch.from_numpy(tiny).cuda()
+        d_o = torch.empty_like(d_t)
+        before = _cache_stats()
+        for k in range(40):
+            P = F.CubeParams()
+            P.phys = F.physics(delta_beta=300.0, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
+                               unsharp_sigma=0.8, unsharp_coeff=0.5)
+            P.deconv, P.deconv_param, P.deconv_cap = 1, 0.015, 8.0
+            P.guided_eps, P.guided_radius, P.guided_s = 1e-3, 2, 2
+            f = np.full(12, 1.0 + 0.001 * (k + 1), np.float32)
+            P.zdrift_factor = f.ctypes.data_as(F.c_float_p)
+            P.vol_z = 12
+            cube = F.Cube(d_t.data_ptr(), 0, 0, 0, 12, 14, 16, 0, 0, 0, 12, 14, 16, d_o.data_ptr(), None,
+                          C.pointer(P))
+            assert lib.fy_gpu_process_cubes(C.byref(cube), 1, None) == 0, lib.fy_gpu_last_error()
+        assert lib.fy_gpu_sync(None) == 0
+        after = _cache_stats()
+        print(f"table cache before {before} after {after}", flush=True)
+        assert after[4] - before[4] >= 30              # evictions happened
+        assert after[1] <= 8 + held                    # bounded: held entries are kept beyond the limit
+        for replay in range(2):
+            refill(cases, rng, replay)
+            for c in cases:
+                c.d_out.zero_()
+            torch.cuda.synchronize()
+            assert lib.fy_graph_launch(C.c_void_p(g)) == 0
+            stream.synchronize()
+            got = [c.d_out.cpu().numpy().copy() for c in cases]
+            want = run(cases)
+            for i, (a, b) in enumerate(zip(got, want)):
+                assert np.array_equal(a, b), f"replay {replay} cube {i}"
+    finally:
+        if g:
+            lib.fy_graph_destroy(C.c_void_p(g))
+        torch.cuda.synchronize()
+        lib.fy_gpu_table_cache(None, old_limit)
+    end = _cache_stats()
+    assert end[3] == 0                                 # every evicted entry's memory was freed

@@ -71,6 +71,19 @@ def test_fft3d_roundtrip_and_numpy(orc, shape):
     assert np.abs(a - re).max() < 1e-4
 
 
+@pytest.mark.parametrize("shape", [(8, 8, 8), (12, 8, 24), (6, 4, 3), (3, 16, 1)])
+def test_fft3d_f64_numpy(orc, shape):
+    """orc_fft3d_f64: sign -1 is numpy's fftn, sign +1 the unnormalised inverse
+    (N * ifftn), on complex data, so a conjugated axis or a skipped axis fails."""
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    n = int(np.prod(shape))
+    for sign, want in ((-1, np.fft.fftn(x)), (1, np.fft.ifftn(x) * n)):
+        a, b = np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag)
+        orc.orc_fft3d_f64(clibs.dp(a), clibs.dp(b), *shape, sign)
+        assert rel_l2(a + 1j * b, want) < 1e-14
+
+
 def test_next_fft_size(orc):
     cases = {176: 192, 128: 128, 160: 192, 97: 128, 1: 1, 2: 2, 3: 3, 5: 6, 64: 64, 96: 96,
              256: 256, 512: 512, 158: 192, 200: 256, 300: 384}
@@ -302,6 +315,48 @@ def test_conversions_roundtrip(orc):
     assert abs(orc.orc_phys_to_u8_level(0.0, -0.03, 0.145) - orc.orc_phys_to_u8_level(0.0, -0.03, 0.21)) > 1.0
 
 
+def test_float_to_u8_golden_and_properties(orc):
+    """orc_float_to_u8 against the reference's fy_float_to_u8 (stream.c:31-36) on
+    the rounding edges of every level, out-of-range and random values; and what
+    the rounding fixes: u8 -> float -> u8 is the identity, and the map is
+    monotone, 0 below 0.5/255 and 255 from 254.5/255 up."""
+    fu, fq = GOLD["f2u8_in"], GOLD["f2u8_out"]
+    got = np.empty(fu.size, np.uint8)
+    orc.orc_float_to_u8(clibs.fp(fu), clibs.up(got), fu.size)
+    assert np.array_equal(got, fq)
+    u = np.arange(256, dtype=np.uint8)
+    f = np.empty(256, np.float32)
+    orc.orc_u8_to_float(clibs.up(u), clibs.fp(f), 256)
+    back = np.empty(256, np.uint8)
+    orc.orc_float_to_u8(clibs.fp(f), clibs.up(back), 256)
+    assert np.array_equal(back, u)
+    xs = np.sort(np.concatenate([fu, np.linspace(-0.1, 1.1, 5001, dtype=np.float32)])).astype(np.float32)
+    q = np.empty(xs.size, np.uint8)
+    orc.orc_float_to_u8(clibs.fp(xs), clibs.up(q), xs.size)
+    assert np.all(np.diff(q.astype(int)) >= 0)
+    assert np.all(q[xs < 0.49 / 255] == 0) and np.all(q[xs > 254.51 / 255] == 255)
+
+
+def test_zdrift_apply_golden_and_properties(orc):
+    """orc_zdrift_apply against the reference's fy_zdrift_apply (zdrift.c:64-71):
+    slab z of the chunk is scaled by factor[z0 + z]; unit factors leave it
+    unchanged and a factor of 2 doubles exactly (an index off by one or a
+    dropped z0 fails the golden)."""
+    zc, zf, za = GOLD["zapply_in"], GOLD["zapply_factor"], GOLD["zapply_out"]
+    a = zc.copy()
+    orc.orc_zdrift_apply(clibs.fp(a), *zc.shape, 5, clibs.fp(zf))
+    assert np.array_equal(a, za)
+    one = np.ones(zf.size, np.float32)
+    b = zc.copy()
+    orc.orc_zdrift_apply(clibs.fp(b), *zc.shape, 5, clibs.fp(one))
+    assert np.array_equal(b, zc)
+    two = np.ones(zf.size, np.float32)
+    two[5 + 3] = 2.0
+    c = zc.copy()
+    orc.orc_zdrift_apply(clibs.fp(c), *zc.shape, 5, clibs.fp(two))
+    assert np.array_equal(c[3], 2 * zc[3]) and np.array_equal(np.delete(c, 3, 0), np.delete(zc, 3, 0))
+
+
 # ------------------------------------------------------------------ rings (dering.c)
 def test_dering_golden(orc):
     vol = GOLD["dering_vol"]
@@ -397,6 +452,40 @@ CHAIN_CFGS = [dict(), dict(no_dither=1), dict(do_deconv=0), dict(guided_subsampl
               dict(use_matched_deconv=1, psf_sigma_vox=1.0, psf_z_ratio=1.2),
               dict(have_norm=1, norm_lo=30, norm_hi=210), dict(have_dec_range=1, dec_lo=0.05, dec_hi=0.8),
               dict(have_eps_r=1, eps_fn_a=0.01, eps_fn_b=0.0001, eps_fn_med=0.012)]
+
+
+@pytest.mark.parametrize("i", [0, 2, 3, 4, 5, 6])
+def test_chain_f64_vs_f32_chain(orc, i):
+    """orc_chain_cube_f64 (deconvolution and guided filter in fp64) against the
+    fp32 chain, which the goldens and the live reference pin: the float tiles
+    agree to the fp32 rounding level (relL2 <= 2e-6, far below what a dropped
+    term, a wrong sign or index changes) and the u8 tiles to +-1, with a
+    z-drift table and an inner tile offset in the region."""
+    from fysics_amd import phantom
+    rng = np.random.default_rng(40 + i)
+    u8 = phantom.noisy_u8(phantom.forward(phantom.sheets((26, 28, 30), rng, 5), **phantom.CONFIG1_PHYS), rng)
+    cfg = _cfg(**CHAIN_CFGS[i])
+    P = clibs.CubeParams()
+    assert orc.orc_cube_params_from_cfg(C.byref(cfg), 24, 34, 20, 22, 400, 500, C.byref(P)) == 0
+    zf = rng.uniform(0.9, 1.1, 64).astype(np.float32)
+    P.zdrift_factor = zf.ctypes.data_as(clibs.c_float_p)
+    P.vol_z = 64
+    args = (10, 20, 30, *u8.shape, 13, 24, 34, 18, 20, 22)
+    q32 = np.empty((18, 20, 22), np.uint8)
+    f32 = np.empty(u8.shape, np.float32)
+    air = C.c_int(0)
+    assert orc.orc_chain_cube(C.byref(P), clibs.up(u8), *args, clibs.up(q32), C.byref(air), clibs.fp(f32)) == 0
+    q64 = np.empty_like(q32)
+    f64 = np.empty(u8.shape, np.float64)
+    assert orc.orc_chain_cube_f64(C.byref(P), clibs.up(u8), *args, clibs.up(q64), clibs.dp(f64)) == 0
+    assert rel_l2(f32, f64) <= 2e-6
+    du = np.abs(q32.astype(int) - q64.astype(int))
+    assert du.max() <= 1 and (du > 0).mean() < 2e-3
+    # without the z-drift table the tile changes: the factor is applied
+    P.zdrift_factor = None
+    f0 = np.empty(u8.shape, np.float64)
+    assert orc.orc_chain_cube_f64(C.byref(P), clibs.up(u8), *args, clibs.up(q64), clibs.dp(f0)) == 0
+    assert rel_l2(f0, f64) > 1e-3
 
 
 @pytest.mark.parametrize("i", range(len(CHAIN_CFGS)))

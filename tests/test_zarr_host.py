@@ -1,10 +1,11 @@
 """Zarr v2 raw-u8 I/O on the host (zarr_io.c): the format the pass-2 tile
 loop reads and writes.
 
-A volume is written chunk by chunk with fy_zarr_create / fy_zarr_write_chunk
-(far-face chunks at their true extent, zarr_io.c:280-304), some chunks are then
-deleted (sparse: read as fill) or truncated (the tail reads as fill), and
-regions read back with fy_zarr_read must equal the same assembly done with
+A volume is written chunk by chunk with fy_gpu_zarr_create /
+fy_gpu_zarr_write_chunk (the reference's fy_zarr_create / fy_zarr_write_chunk
+for local roots; far-face chunks at their true extent, zarr_io.c:280-304), some
+chunks are then deleted (sparse: read as fill) or truncated (the tail reads as
+fill), and regions read back with fy_gpu_zarr_read must equal the same assembly done with
 numpy slicing -- and, where the reference is built (oracle/_ref), the
 reference's own fy_zarr_read on the same files, byte for byte.  Errors: a
 missing header, an S3 root, compressed chunks.
@@ -26,14 +27,14 @@ def write_volume(root, vol, chunk, drop=(), trunc=()):
     z = F.Zarr()
     shape = (C.c_long * 3)(*vol.shape)
     ch = (C.c_long * 3)(*chunk)
-    assert lib.fy_zarr_create(C.byref(z), root.encode(), shape, ch) == 0
+    assert lib.fy_gpu_zarr_create(C.byref(z), root.encode(), shape, ch) == 0
     nc = [(n + c - 1) // c for n, c in zip(vol.shape, chunk)]
     for cz in range(nc[0]):
         for cy in range(nc[1]):
             for cx in range(nc[2]):
                 blk = np.ascontiguousarray(vol[cz * chunk[0]:(cz + 1) * chunk[0], cy * chunk[1]:(cy + 1) * chunk[1],
                                                cx * chunk[2]:(cx + 1) * chunk[2]])
-                assert lib.fy_zarr_write_chunk(C.byref(z), cz, cy, cx, blk.ctypes.data, *blk.shape) == 0
+                assert lib.fy_gpu_zarr_write_chunk(C.byref(z), cz, cy, cx, blk.ctypes.data, *blk.shape) == 0
     expect = vol.copy()
     for (cz, cy, cx) in drop:
         os.unlink(os.path.join(root, "0", str(cz), str(cy), str(cx)))
@@ -54,7 +55,7 @@ def write_volume(root, vol, chunk, drop=(), trunc=()):
 
 def read(lib, z, org, shp, zcls=None):
     out = np.full(shp, 0xAB, np.uint8)
-    assert lib.fy_zarr_read(C.byref(z), *org, *shp, out.ctypes.data) == 0
+    assert lib.fy_gpu_zarr_read(C.byref(z), *org, *shp, out.ctypes.data) == 0
     return out
 
 
@@ -76,7 +77,7 @@ def test_round_trip_sparse_and_truncated(tmp_path):
                                                                                   ((2, 0, 0), 7)])
     lib = F.lib()
     zz = F.Zarr()
-    assert lib.fy_zarr_open(C.byref(zz), root.encode()) == 0
+    assert lib.fy_gpu_zarr_open(C.byref(zz), root.encode()) == 0
     assert tuple(zz.shape) == SHAPE and tuple(zz.chunk) == CHUNK and zz.fill == 0
     for org, shp in regions(rng, SHAPE, 40):
         got = read(lib, zz, org, shp)
@@ -94,7 +95,7 @@ def test_matches_reference_reader(tmp_path, ref):
     write_volume(root, vol, CHUNK, drop=[(0, 2, 1)], trunc=[((1, 1, 1), 333)])
     lib = F.lib()
     zo, zr = F.Zarr(), clibs.Zarr()
-    assert lib.fy_zarr_open(C.byref(zo), root.encode()) == 0
+    assert lib.fy_gpu_zarr_open(C.byref(zo), root.encode()) == 0
     assert ref.fy_zarr_open(C.byref(zr), root.encode()) == 0
     assert tuple(zr.shape) == tuple(zo.shape) and tuple(zr.chunk) == tuple(zo.chunk) and zr.fill == zo.fill
     for org, shp in regions(rng, SHAPE, 60):
@@ -115,7 +116,7 @@ def test_fill_value_and_outside_volume(tmp_path):
     (root / "0" / "0" / "0" / "0").write_bytes(blk.tobytes())
     lib = F.lib()
     z = F.Zarr()
-    assert lib.fy_zarr_open(C.byref(z), str(root).encode()) == 0
+    assert lib.fy_gpu_zarr_open(C.byref(z), str(root).encode()) == 0
     assert z.fill == 7
     got = read(lib, z, (0, 0, 0), (8, 6, 4))
     want = np.full((8, 6, 4), 7, np.uint8)
@@ -135,15 +136,15 @@ def test_open_rejects(tmp_path, header, why):
     if header is not None:
         (root / "0" / ".zarray").write_text(header)
     z = F.Zarr()
-    assert F.lib().fy_zarr_open(C.byref(z), str(root).encode()) == 1, why
-    assert F.lib().fy_zarr_open(C.byref(z), b"s3://bucket/vol.zarr") == 1
+    assert F.lib().fy_gpu_zarr_open(C.byref(z), str(root).encode()) == 1, why
+    assert F.lib().fy_gpu_zarr_open(C.byref(z), b"s3://bucket/vol.zarr") == 1
 
 
 def test_header_is_readable_json(tmp_path):
     import json
     z = F.Zarr()
     root = str(tmp_path / "h.zarr")
-    assert F.lib().fy_zarr_create(C.byref(z), root.encode(), (C.c_long * 3)(3, 4, 5), (C.c_long * 3)(2, 2, 2)) == 0
+    assert F.lib().fy_gpu_zarr_create(C.byref(z), root.encode(), (C.c_long * 3)(3, 4, 5), (C.c_long * 3)(2, 2, 2)) == 0
     h = json.load(open(os.path.join(root, "0", ".zarray")))
     assert h["shape"] == [3, 4, 5] and h["chunks"] == [2, 2, 2] and h["dtype"] == "|u1"
     assert h["compressor"] is None and h["fill_value"] == 0 and h["dimension_separator"] == "/"
@@ -161,14 +162,9 @@ def test_shared_chunk_mappings_under_threads(tmp_path):
     root = str(tmp_path / "c.zarr")
     write_volume(root, vol, CHUNK, drop=[(1, 1, 1)], trunc=[((0, 0, 2), 100)])
     lib = F.lib()
-    lib.fyh_zcache_new.restype = C.c_void_p
-    lib.fyh_zcache_new.argtypes = [C.c_size_t]
-    lib.fyh_zcache_free.argtypes = [C.c_void_p]
-    lib.fyh_zarr_read_cached.restype = C.c_int
-    lib.fyh_zarr_read_cached.argtypes = [C.POINTER(F.Zarr), C.c_void_p] + [C.c_long] * 6 + [C.c_void_p]
     z = F.Zarr()
-    assert lib.fy_zarr_open(C.byref(z), root.encode()) == 0
-    cache = lib.fyh_zcache_new(3 * 16 * 20 * 12)
+    assert lib.fy_gpu_zarr_open(C.byref(z), root.encode()) == 0
+    cache = lib.fy_gpu_zarr_cache_new(3 * 16 * 20 * 12)
     regs = list(regions(rng, SHAPE, 200))
     errors = []
 
@@ -176,8 +172,8 @@ def test_shared_chunk_mappings_under_threads(tmp_path):
         for org, shp in regs[part::8]:
             a = np.empty(shp, np.uint8)
             b = np.empty(shp, np.uint8)
-            if lib.fyh_zarr_read_cached(C.byref(z), cache, *org, *shp, a.ctypes.data) != 0 or \
-                    lib.fy_zarr_read(C.byref(z), *org, *shp, b.ctypes.data) != 0 or not np.array_equal(a, b):
+            if lib.fy_gpu_zarr_read_cached(C.byref(z), cache, *org, *shp, a.ctypes.data) != 0 or \
+                    lib.fy_gpu_zarr_read(C.byref(z), *org, *shp, b.ctypes.data) != 0 or not np.array_equal(a, b):
                 errors.append((org, shp))
 
     ts = [threading.Thread(target=work, args=(i,)) for i in range(8)]
@@ -185,5 +181,5 @@ def test_shared_chunk_mappings_under_threads(tmp_path):
         t.start()
     for t in ts:
         t.join()
-    lib.fyh_zcache_free(cache)
+    lib.fy_gpu_zarr_cache_free(cache)
     assert not errors, errors[:3]
