@@ -21,7 +21,7 @@ ARCH = os.environ.get("FYSICS_OFFLOAD_ARCH", "gfx950")
 
 HIP_SRC = ["fyg_runtime.hip", "fyg_fft.hip", "fyg_fft64.hip", "fyg_guided.hip", "fyg_elem.hip",
            "fyg_noise.hip", "fyg_calib.hip", "fyg_air.hip",
-           "fyg_phasecorr.hip"]
+           "fyg_phasecorr.hip", "fyg_noalloc.hip"]
 C_SRC = ["host/fy_core.c", "host/fy_deconv.c", "host/fy_guided.c", "host/fy_dering.c",
          "host/fy_chain.c", "host/fy_noise.c", "host/fy_serve.c", "host/fy_calib.c",
          "host/fy_zarr.c", "host/fy_export.c"]

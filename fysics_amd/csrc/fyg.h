@@ -175,6 +175,7 @@ typedef struct {
     int *step_gen;                   /* optional (a[0] of a batch): generation word to step before
                                         anything reads it -- folded into the first ring-image launch */
     int step_nflag;                  /* flags that follow *step_gen */
+    const void *twx, *twy, *twz;     /* optional twiddle tables for px, py, pz (NULL: the cache's) */
 } fyg_deconv_args;
 int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s);
 /* n deconvolutions of one shape (same nz..hxp, in_u8 and pro.dering) */
@@ -183,6 +184,16 @@ size_t fyg_deconv_spec_bytes(int pz, int py, int px);
 size_t fyg_deconv_spec_bytes64(int pz, int py, int px);
 int fyg_deconv_run_batch_f64(const fyg_deconv_args *a, int n, fyg_stream *s);
 int fyg_deconv_hxp(int px);
+
+/* the deconvolution with nothing allocated: spectrum, transfer tables and
+ * twiddles live in a file-scope device array (fyg_noalloc.hip) of
+ * FYG_NOALLOC_SPEC_MAX bytes of spectrum plus a table area, shared by every
+ * call on a device; the tables are computed on the device each call.  phys8
+ * is fy_physics as 8 doubles; us the LUT index scale.  Wiener weight of the
+ * recon transfer, fz2 scale 1. */
+#define FYG_NOALLOC_SPEC_MAX ((size_t)80 << 20)
+int fyg_noalloc_deconv(const float *in, float *out, int nz, int ny, int nx, int pz, int py, int px,
+                       const double *phys8, double reg, double cap, double us, fyg_stream *s);
 
 /* complex FFT of split re/im device arrays (fy_fft1d/fy_fft3d) */
 int fyg_fft3d_run(float *d_re, float *d_im, int nz, int ny, int nx, int sign,

@@ -940,7 +940,9 @@ int deconv_batch(const fyg_deconv_args *a, int n, fyg_stream *s) {
             (b.in_u8 && (b.pro.dering != 0) != (h.pro.dering != 0)) || (b.f64 != 0) != (h.f64 != 0))
             return 1;
     }
-    const V *twx = twiddles<P>(h.px), *twy = twiddles<P>(h.py), *twz = twiddles<P>(h.pz);
+    const V *twx = h.twx ? (const V *)h.twx : twiddles<P>(h.px);
+    const V *twy = h.twy ? (const V *)h.twy : twiddles<P>(h.py);
+    const V *twz = h.twz ? (const V *)h.twz : twiddles<P>(h.pz);
     if (!twx || !twy || !twz) return 1;
     const long plane = (long)h.py * h.hxp;
     std::vector<XFwd<P>> xf(n);

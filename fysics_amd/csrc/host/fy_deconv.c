@@ -11,6 +11,7 @@
 #define _GNU_SOURCE
 #include <math.h>
 #include <pthread.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -88,6 +89,29 @@ static int table_shape(const fyh_table_req *r, fyh_tables *t, double **axes, tab
     key->param = r->param; key->cap = r->cap; key->fz2_scale = fz2_scale;
     *axes = fz2;
     return 0;
+}
+
+/* the LUT index scale of a padded shape: 4096 over the largest
+ * fz^2*scale + fy^2 + fx^2 (the maxima table_shape's axes reach) */
+static double lut_us(int pz, int py, int px, double fz2_scale) {
+    int hx = px / 2 + 1;
+    double mz = 0, my = 0, mx = 0;
+    for (int i = 0; i < pz; i++) {
+        double f = (i <= pz / 2) ? (double)i / pz : (double)(i - pz) / pz;
+        double v = f * f * fz2_scale;
+        if (v > mz) mz = v;
+    }
+    for (int i = 0; i < py; i++) {
+        double f = (i <= py / 2) ? (double)i / py : (double)(i - py) / py;
+        double v = f * f;
+        if (v > my) my = v;
+    }
+    for (int i = 0; i < hx; i++) {
+        double f = (double)i / px;
+        double v = f * f;
+        if (v > mx) mx = v;
+    }
+    return FYH_LUT_N / (mz + my + mx);
 }
 
 /* per-stream slots first (no device wait on a miss), the process-wide cache
@@ -323,4 +347,38 @@ int fy_gpu_deconv_radial_f64(const float *d_in, double *d_out, int nz, int ny, i
     rc = fyg_deconv_run(&a, s);
     if (fyg_hold_release(hold, s)) rc = 1;
     return rc;
+}
+
+/* fy_gpu_deconvolve with nothing allocated: the spectrum and the tables live
+ * in the library's per-GPU scratch array (fyg_noalloc.hip); sizes whose
+ * spectrum exceeds it are refused */
+int fy_gpu_deconvolve_noalloc(const float *d_in, float *d_out, int nz, int ny, int nx, const fy_physics *p,
+                              double reg, void *stream) {
+    if (nz <= 0 || ny <= 0 || nx <= 0) return 0;
+    if (!p || !d_in || !d_out) {
+        fyg_set_error("fy_gpu_deconvolve_noalloc: NULL argument");
+        return 1;
+    }
+    int pz = fy_next_fft_size(nz), py = fy_next_fft_size(ny), px = fy_next_fft_size(nx);
+    if (!fyg_fft_supported(pz) || !fyg_fft_supported(py) || !fyg_fft_supported(px)) {
+        fyg_set_error("fy_gpu_deconvolve_noalloc: padded axis longer than 4096");
+        return 1;
+    }
+    size_t need = fyg_deconv_spec_bytes(pz, py, px);
+    if (need > FYG_NOALLOC_SPEC_MAX) {
+        char msg[256];
+        snprintf(msg, sizeof msg,
+                 "fy_gpu_deconvolve_noalloc: %dx%dx%d pads to %dx%dx%d, whose spectrum needs %zu bytes; "
+                 "the scratch holds %zu (use fy_gpu_deconvolve)",
+                 nz, ny, nx, pz, py, px, need, (size_t)FYG_NOALLOC_SPEC_MAX);
+        fyg_set_error(msg);
+        return 1;
+    }
+    fyg_stream *s = fyg_stream_wrap(stream);
+    if (!s) return 1;
+    if (reg <= 0.0) reg = fy_auto_reg(p, 0.015);
+    const double phys8[8] = {p->delta_beta, p->energy_kev, p->distance_mm, p->pixel_um,
+                             p->unsharp_sigma, p->unsharp_coeff, p->psf_sigma_vox, p->psf_sigma_z_vox};
+    return fyg_noalloc_deconv(d_in, d_out, nz, ny, nx, pz, py, px, phys8, reg, FYH_MAX_GAIN,
+                              lut_us(pz, py, px, 1.0), s);
 }

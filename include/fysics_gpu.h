@@ -31,6 +31,20 @@ int fy_gpu_deconv_radial(const float *d_in, float *d_out, int nz, int ny, int nx
                          const fy_physics *p, double param, int kind, double cap,
                          double fz2_scale, void *stream);
 
+/* fy_gpu_deconvolve that allocates nothing.  No reference counterpart: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The half
+ * spectrum, the weight table, the frequency axes and the twiddles live in one
+ * device array per GPU that the library holds until the process ends (80 MiB
+ * of spectrum plus 256 KiB of tables); the tables are computed on the device
+ * at each call.  Limit: the padded volume's half spectrum,
+ * pz*py*roundup(px/2+1, 16)*8 bytes, must fit 80 MiB (256^3 fits, 384^3 does
+ * not); larger sizes return 1 and fy_gpu_last_error() says why.  Calls that
+ * overlap on one GPU share the array and race: serialise them (one stream, or
+ * the caller's own ordering).  d_out may equal d_in. */
+int fy_gpu_deconvolve_noalloc(const float *d_in, float *d_out, int nz, int ny, int nx,
+                              const fy_physics *p, double reg, void *stream);
+
 /* the all-FP64 variant of fy_gpu_deconv_radial: fp32 input, spectrum, weights
  * and output in double (the speed reference of the precision study; no
  * reference counterpart) */
