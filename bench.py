@@ -155,7 +155,7 @@ def main():
     ap.add_argument("--no-air-zero", action="store_true", help="skip the chain-with-air-zero timing")
     ap.add_argument("--no-lod", action="store_true", help="skip the downscale / phase-correlation timing")
     ap.add_argument("--no-export", action="store_true", help="skip the zarr -> GPU -> zarr export timing")
-    ap.add_argument("--served-requests", type=int, default=2000)
+    ap.add_argument("--served-requests", type=int, default=20000)
     ap.add_argument("--served-rate", type=float, default=400.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference point that scales the "
@@ -262,9 +262,12 @@ def main():
 
     batched = None if args.no_batched else batched_bench(F)
     served = None if args.no_served else served_bench(F, args.served_requests, args.served_rate)
-    # the same request mix offered far above the one-cube-at-a-time capacity:
-    # where continuous batching pays
-    stress = None if args.no_served else served_bench(F, 4000, 12000.0, seed=8)
+    # capacity: the same request mix offered far above what either loop sustains
+    # (throughput then is the loop's capacity), host-resident and HBM-resident
+    stress = None
+    if not args.no_served:
+        stress = {"host": served_bench(F, 6000, 60000.0, seed=8, host=True),
+                  "hbm": served_bench(F, 6000, 60000.0, seed=9, host=False)}
     precision = None if args.no_precision else precision_bench(F)
     calib = None if args.no_calibration else calibration_bench(F, vol)
     airz = None if args.no_air_zero else air_zero_bench(F, vol, eps)
@@ -290,7 +293,7 @@ def main():
         "host_api_mvox_s": round(host_rate, 1),
         "batched": batched,
         "served": served,
-        "served_overload": stress,
+        "served_capacity": stress,
         "precision_512": precision,
         "calibration": calib,
         "air_zero": airz,
@@ -304,11 +307,8 @@ def main():
 
 def _generate(fn, shapes, seeds):
     """Synthetic cubes in worker processes (spawned: they import numpy only)."""
-    import concurrent.futures as cf
-    import multiprocessing as mp
-    workers = max(1, min(8, (os.cpu_count() or 2) - 1))
-    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
-        return list(ex.map(fn, shapes, seeds))
+    from fysics_amd import phantom
+    return phantom.generate_many(fn, shapes, seeds)
 
 
 def _cube_params(F, phys, z_ratio, eps):
@@ -381,14 +381,20 @@ def batched_bench(F, reps=5, seed=100):
     return out
 
 
-def served_bench(F, n_req=2000, rate=400.0, seed=7, pool=4, eps=2e-3):
+_SERVE_POOL = {}
+
+
+def served_bench(F, n_req=20000, rate=400.0, seed=7, pool=4, eps=2e-3, host=True, slots=48, modes=None):
     """The served workload: open-loop Poisson arrivals at `rate` req/s, sizes 40%
     64^3, 30% 128^3, 20% 192^3, 10% 256^3, per-request physics (delta/beta, E, D,
-    pixel, unsharp) so every request brings its own transfer table; inputs from a
-    pool of `pool` resident phantoms per size.  Latency = completion - scheduled
-    arrival.  'served' is the request loop with continuous batching (waves of up
-    to 64 requests, 2 in flight, 200 GB budget); 'plain' is one cube per launch
-    sequence as each request arrives."""
+    pixel, unsharp, sigma_z/sigma_xy) so every request brings its own transfer
+    table; inputs from a pool of `pool` phantoms per size.  host=True: each
+    request's input and output are pinned host buffers, so its latency includes
+    both PCIe transfers; host=False: they are resident in HBM.  Latency =
+    completion seen by the polling thread - scheduled arrival.  'served' is the
+    request loop with continuous batching (waves of up to 16 requests on 3
+    lanes -- streams whose waves overlap -- 200 GB budget); 'plain' is one cube
+    at a time (waves of one on one lane), the plain one-cube-at-a-time path."""
     import torch
     from fysics_amd import phantom
     lib = F.lib()
@@ -396,39 +402,56 @@ def served_bench(F, n_req=2000, rate=400.0, seed=7, pool=4, eps=2e-3):
     sizes = [n for _, n in phantom.SERVE_SIZES]
     probs = [p for p, _ in phantom.SERVE_SIZES]
     jobs = [(n, n, n) for n in sizes for _ in range(pool)]
-    made = _generate(phantom.batch_cube, jobs, [seed * 100 + k for k in range(len(jobs))])
+    if pool not in _SERVE_POOL:             # the same phantoms for every served measurement
+        _SERVE_POOL[pool] = _generate(phantom.batch_cube, jobs, [700 + k for k in range(len(jobs))])
+    made = _SERVE_POOL[pool]
     inputs = {n: [] for n in sizes}
     for shape, (u8, _, _) in zip(jobs, made):
-        inputs[shape[0]].append(torch.from_numpy(u8).cuda())
-    slots = 256
-    outs = {n: torch.empty((slots, n, n, n), dtype=torch.uint8, device="cuda") for n in sizes}
+        t = torch.from_numpy(u8)
+        inputs[shape[0]].append(t.pin_memory() if host else t.cuda())
+    if host:
+        outs = {n: torch.empty((slots, n, n, n), dtype=torch.uint8).pin_memory() for n in sizes}
+    else:
+        outs = {n: torch.empty((slots, n, n, n), dtype=torch.uint8, device="cuda") for n in sizes}
     kinds = rng.choice(len(sizes), size=n_req, p=probs)
     arrive = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
     params = []
     for i in range(n_req):
         P = _cube_params(F, phantom.serve_physics(rng), float(rng.uniform(1.0, 1.3)), eps)
         params.append(P)
-    cubes = []
+    cubes, hin, hout = [], [], []
     for i in range(n_req):
         n = sizes[kinds[i]]
-        d_in = inputs[n][i % pool]
+        src = inputs[n][i % pool]
+        dst = outs[n][i % slots]
         z = (0, 0, 0)
-        cubes.append(F.Cube(d_in.data_ptr(), *z, n, n, n, *z, n, n, n, outs[n][i % slots].data_ptr(), None,
-                            C.pointer(params[i])))
+        if host:
+            cubes.append(F.Cube(None, *z, n, n, n, *z, n, n, n, None, None, C.pointer(params[i])))
+            hin.append(C.c_void_p(src.data_ptr()))
+            hout.append(C.c_void_p(dst.data_ptr()))
+        else:
+            cubes.append(F.Cube(src.data_ptr(), *z, n, n, n, *z, n, n, n, dst.data_ptr(), None, C.pointer(params[i])))
     nvox = float(sum(sizes[k] ** 3 for k in kinds))
+    torch.cuda.synchronize()
 
-    def run(max_batch, max_waves):
+    def submit(sv, i, ticket):
+        if host:
+            return lib.fy_serve_submit_host(sv, C.byref(cubes[i]), hin[i], hout[i], ticket)
+        return lib.fy_serve_submit(sv, C.byref(cubes[i]), ticket)
+
+    def run(max_batch, lanes):
         stream = torch.cuda.Stream()
-        sv = lib.fy_serve_create(200 << 30, max_batch, max_waves, C.c_void_p(stream.cuda_stream))
+        sv = lib.fy_serve_create(200 << 30, max_batch, lanes, C.c_void_p(stream.cuda_stream))
         if not sv:
             raise RuntimeError("fy_serve_create")
-        # warm the per-stream workspaces with one request of each size
-        for n in sizes:
-            j = int(np.flatnonzero(kinds == sizes.index(n))[0])
-            lib.fy_serve_submit(sv, C.byref(cubes[j]), -1)
         buf = (C.c_long * 4096)()
+        # warm every lane's workspaces with a full wave of the largest size
+        big = [j for j in range(n_req) if kinds[j] == len(sizes) - 1][:max_batch * lanes]
+        for j in big + [int(np.flatnonzero(kinds == k)[0]) for k in range(len(sizes))]:
+            submit(sv, j, -1)
         while lib.fy_serve_outstanding(sv):
-            lib.fy_serve_poll(sv, buf, 4096)
+            if lib.fy_serve_poll(sv, buf, 4096) < 0:
+                raise RuntimeError(lib.fy_gpu_last_error().decode())
         done_t = np.zeros(n_req)
         t0 = time.perf_counter()
         i = nd = 0
@@ -437,7 +460,7 @@ def served_bench(F, n_req=2000, rate=400.0, seed=7, pool=4, eps=2e-3):
             while i < n_req and arrive[i] <= now:
                 if i >= slots and done_t[i - slots] == 0:
                     break                   # the output slot is still in flight
-                if lib.fy_serve_submit(sv, C.byref(cubes[i]), i):
+                if submit(sv, i, i):
                     raise RuntimeError("fy_serve_submit")
                 i += 1
             m = lib.fy_serve_poll(sv, buf, 4096)
@@ -451,13 +474,19 @@ def served_bench(F, n_req=2000, rate=400.0, seed=7, pool=4, eps=2e-3):
         lat = np.sort(done_t - arrive) * 1000.0
         span = done_t.max() - arrive[0]
         return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                "mean_ms": round(float(lat.mean()), 3), "mvox_s": round(nvox / span / 1e6, 1)}
+                "mean_ms": round(float(lat.mean()), 3), "mvox_s": round(nvox / span / 1e6, 1),
+                "wall_s": round(float(span), 2)}
 
     res = {"workload": f"{n_req} requests, Poisson {rate:g} req/s, sizes 40% 64^3 / 30% 128^3 / 20% 192^3 / "
-                       "10% 256^3, per-request physics", "requests": n_req, "rate_req_s": rate,
+                       "10% 256^3, per-request physics; " +
+                       ("inputs and outputs in pinned host memory (latency includes H2D and D2H)" if host
+                        else "inputs and outputs resident in HBM"),
+           "requests": n_req, "rate_req_s": rate, "host_resident": host,
            "offered_mvox_s": round(nvox / (arrive[-1] - arrive[0]) / 1e6, 1)}
-    res["served"] = run(64, 2)
-    res["plain"] = run(1, 1 << 16)
+    for name, (mb, lanes) in (modes or {"served": (16, 3), "plain": (1, 1)}).items():
+        res[name] = run(mb, lanes)
+    if "served" in res and "plain" in res:
+        res["served_over_plain"] = round(res["served"]["mvox_s"] / res["plain"]["mvox_s"], 3)
     return res
 
 

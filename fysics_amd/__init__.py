@@ -214,6 +214,7 @@ SIGNATURES = {
     "fy_cube_workspace_bytes": (C.c_size_t, [C.POINTER(Cube)]),
     "fy_serve_create": (_VP, [C.c_size_t, C.c_int, C.c_int, _VP]),
     "fy_serve_submit": (C.c_int, [_VP, C.POINTER(Cube), _L]),
+    "fy_serve_submit_host": (C.c_int, [_VP, C.POINTER(Cube), _VP, _VP, C.c_long]),
     "fy_serve_poll": (C.c_int, [_VP, C.POINTER(C.c_long), C.c_int]),
     "fy_serve_outstanding": (C.c_int, [_VP]),
     "fy_serve_destroy": (None, [_VP]),

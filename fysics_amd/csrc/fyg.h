@@ -31,6 +31,8 @@ void *fyg_stream_handle(fyg_stream *s);          /* the hipStream_t */
 fyg_stream *fyg_stream_private(void *hip_stream);
 void  fyg_stream_private_free(fyg_stream *s);
 int   fyg_stream_persistent(fyg_stream *s);
+/* waits for s, then frees its workspaces (the next fyg_ws regrows them) */
+int   fyg_stream_trim(fyg_stream *s);
 /* an owned stream object on a new non-blocking hipStream_t; fyg_stream_free
  * waits for it, frees its buffers and destroys the stream */
 fyg_stream *fyg_stream_new(void);

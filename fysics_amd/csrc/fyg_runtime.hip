@@ -79,6 +79,8 @@ extern "C" fyg_stream *fyg_stream_private(void *hip_stream) {
     return st;
 }
 
+static void fyg_arena_free(fyg_stream *st);
+
 extern "C" void fyg_stream_private_free(fyg_stream *st) {
     if (!st) return;
     (void)hipStreamSynchronize(st->s);
@@ -87,10 +89,23 @@ extern "C" void fyg_stream_private_free(fyg_stream *st) {
     for (int i = 0; i < 2; i++)
         if (st->pin[i]) (void)hipHostFree(st->pin[i]);
     if (st->flag) (void)hipFree(st->flag);
+    fyg_arena_free(st);
     delete st;
 }
 
 extern "C" int fyg_stream_persistent(fyg_stream *s) { return s && s->persist; }
+
+// waits for the stream, then frees its workspaces (they regrow on demand)
+extern "C" int fyg_stream_trim(fyg_stream *s) {
+    if (!s) return 1;
+    if (note(hipStreamSynchronize(s->s), "hipStreamSynchronize")) return 1;
+    for (int i = 0; i < FYG_WS_SLOTS; i++) {
+        if (s->ws[i]) (void)hipFree(s->ws[i]);
+        s->ws[i] = nullptr;
+        s->wcap[i] = 0;
+    }
+    return 0;
+}
 
 // an owned stream object on a new non-blocking hipStream_t (default priority)
 extern "C" fyg_stream *fyg_stream_new(void) {
@@ -490,6 +505,18 @@ struct Arena {
     std::unordered_map<std::string, int> where;
     long clock = 0;
 };
+
+// the stream's table slots (the stream has been synchronised)
+static void fyg_arena_free(fyg_stream *st) {
+    Arena *a = st->arena;
+    if (!a) return;
+    for (auto e : a->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (a->dev) (void)hipFree(a->dev);
+    if (a->pin) (void)hipHostFree(a->pin);
+    delete a;
+    st->arena = nullptr;
+}
 
 extern "C" const void *fyg_stream_table(fyg_stream *s, const void *key, size_t keylen, const void *data,
                                         size_t bytes) {

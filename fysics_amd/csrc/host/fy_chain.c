@@ -460,13 +460,17 @@ size_t fy_cube_workspace_bytes(const fy_cube *c) {
     return b;
 }
 
-int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream) {
+int fyh_process_cubes(const fy_cube *cubes, int n, fyg_stream *s) {
     if (n <= 0) return 0;
-    fyg_stream *s = fyg_stream_wrap(stream);
     if (!s) return 1;
     int *dflags = NULL, *dgen = NULL, nflag = 0;
     if (!fyg_flags_reserve(s, n, &dflags, &dgen, &nflag)) return 1;
     return run_cubes(cubes, n, dflags, dgen, nflag, s);
+}
+
+int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream) {
+    if (n <= 0) return 0;
+    return fyh_process_cubes(cubes, n, fyg_stream_wrap(stream));
 }
 
 /* ---- captured batched form: the launches of one fy_gpu_process_cubes call

@@ -56,6 +56,9 @@ int fyh_chain(const fy_pipeline_cfg *cfg, const unsigned char *d_u8,
               long vol_y, long vol_x, unsigned char *d_out, int *d_flag, int *gen, int nflag,
               fyg_stream *s);
 
+/* fy_gpu_process_cubes on a stream object */
+int fyh_process_cubes(const fy_cube *cubes, int n, fyg_stream *s);
+
 /* deconvolution on device buffers (fp32 in/out) */
 int fyh_deconv_dev(const float *d_in, float *d_out, int nz, int ny, int nx, const fy_physics *p,
                    double param, int kind, double cap, double fz2_scale, fyg_stream *s);

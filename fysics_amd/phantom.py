@@ -153,3 +153,24 @@ def serve_physics(rng):
     return dict(delta_beta=float(rng.uniform(100.0, 1000.0)), energy_kev=float(rng.uniform(80.0, 120.0)),
                 distance_mm=float(rng.uniform(500.0, 2000.0)), pixel_um=float(rng.uniform(2.4, 7.9)),
                 unsharp_coeff=float(rng.uniform(0.3, 0.7)), unsharp_sigma=float(rng.uniform(0.6, 1.2)))
+
+
+def generate_many(fn, shapes, seeds, workers=8):
+    """fn(shape, seed) for each pair, in spawned worker processes (they import
+    numpy only), in order."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    import os
+    workers = max(1, min(workers, (os.cpu_count() or 2) - 1, len(shapes)))
+    if workers == 1:
+        return [fn(sh, sd) for sh, sd in zip(shapes, seeds)]
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(fn, shapes, seeds))
+
+
+def batch_set(seed=100):
+    """The 96 cubes of the batched workload (BATCH_CLASSES order), cube k with
+    seed + k: a list of (shape, u8, physics dict, z_ratio)."""
+    jobs = [shape for count, shape in BATCH_CLASSES for _ in range(count)]
+    made = generate_many(batch_cube, jobs, [seed + k for k in range(len(jobs))])
+    return [(shape, u8, phys, zr) for shape, (u8, phys, zr) in zip(jobs, made)]

@@ -130,16 +130,25 @@ void fy_graph_destroy(fy_graph *g);
 /* device bytes one cube holds while it runs (workspace admission) */
 size_t fy_cube_workspace_bytes(const fy_cube *cube);
 
-/* ---- served form: a request loop over one stream (no reference counterpart).
- * fy_serve_submit queues a request (its buffers and params must stay valid
- * until it is retired).  fy_serve_poll retires finished waves, then, while
- * fewer than max_waves waves are in flight, launches the pending requests that
- * fit the HBM budget next to the running ones (FIFO, at most max_batch) as one
- * wave: continuous batching.  Returns the number of tickets written to done
- * (retired in submission order), or -1 on error. */
+/* ---- served form: a request loop with continuous batching over `lanes`
+ * streams of its own (no reference counterpart).  fy_serve_submit queues a
+ * request whose region and output are in device memory (its buffers and
+ * params must stay valid until it is retired); fy_serve_submit_host queues
+ * one whose input (hz*hy*hx u8) and output (tz*ty*tx u8) are host buffers --
+ * the cube's d_in/d_out are ignored, the wave copies on its lane's stream
+ * (pinned buffers make the copies asynchronous).  fy_serve_poll retires the
+ * waves that have finished, in the order they finish, then, for each idle
+ * lane, launches the pending requests that fit the HBM budget next to what the
+ * other lanes hold (FIFO, at most max_batch) as one wave: waves on different
+ * lanes overlap on the device.  A wave waits for the work queued on `stream`
+ * (NULL: the calling thread's stream) before the poll that launched it.
+ * Returns the number of tickets written to done, or -1 on error.
+ * fy_serve_submit* return 1 if the request alone exceeds the budget. */
 typedef struct fy_server fy_server;
-fy_server *fy_serve_create(size_t hbm_budget, int max_batch, int max_waves, void *stream);
-int  fy_serve_submit(fy_server *sv, const fy_cube *cube, long ticket);   /* 1: exceeds the budget */
+fy_server *fy_serve_create(size_t hbm_budget, int max_batch, int lanes, void *stream);
+int  fy_serve_submit(fy_server *sv, const fy_cube *cube, long ticket);
+int  fy_serve_submit_host(fy_server *sv, const fy_cube *cube, const unsigned char *h_in,
+                          unsigned char *h_out, long ticket);
 int  fy_serve_poll(fy_server *sv, long *done, int max_done);
 int  fy_serve_outstanding(const fy_server *sv);                          /* queued + in flight */
 void fy_serve_destroy(fy_server *sv);

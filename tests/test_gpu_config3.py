@@ -1,0 +1,121 @@
+"""The batched workload at its real sizes: 96 cubes (32x64^3, 32x96^3,
+16x128^3, 12x160^3 padded to 192^3, 4x192x256x256), each with its own phantom
+and physics (delta/beta U[100,1000], pixel {2.4,4.3,7.9} um, sigma_z/sigma_xy
+U[1.0,1.3]), u8 -> dewindow -> Wiener deconvolution -> guided r=2 s=2 ->
+dithered u8, through fy_gpu_process_cubes and through its captured graph.
+
+Bars: every cube's u8 output is bit-identical alone, in the 96-cube mix and in
+the graph replay; for one cube of each size class the float tile after the
+guided filter is within relative L2 1e-5 of the fp64 oracle
+(orc_chain_cube_f64: deconvolution and guided filter in double, same LUT) and
+the u8 tile within +-1 LSB of the oracle's on < 0.1 % of voxels.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import clibs
+import fysics_amd as F
+from fysics_amd import phantom
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def cube_params(phys, zr):
+    P = F.CubeParams()
+    P.phys = F.physics(**phys)
+    P.deconv, P.deconv_param, P.deconv_cap, P.fz2_scale = 1, 0.015, 8.0, zr ** 2
+    P.window_lo, P.window_hi = -0.1, 1.2
+    P.guided_eps, P.guided_radius, P.guided_s = 2e-3, 2, 2
+    return P
+
+
+@pytest.fixture(scope="module")
+def batch():
+    print("config 3: generating the 96 cubes", flush=True)
+    items = phantom.batch_set(100)
+    out = []
+    for shape, u8, phys, zr in items:
+        P = cube_params(phys, zr)
+        d_in = torch.from_numpy(u8).cuda()
+        out.append(dict(shape=shape, u8=u8, P=P, d_in=d_in, mix=torch.zeros_like(d_in),
+                        alone=torch.zeros_like(d_in), graph=torch.zeros_like(d_in)))
+    torch.cuda.synchronize()
+    return out
+
+
+def _cube(b, out, d_float=None):
+    z = (0, 0, 0)
+    return F.Cube(b["d_in"].data_ptr(), *z, *b["shape"], *z, *b["shape"], out.data_ptr(),
+                  d_float.data_ptr() if d_float is not None else None, C.pointer(b["P"]))
+
+
+def test_config3_mix_alone_and_graph(batch):
+    lib = F.lib()
+    assert len(batch) == 96 and sum(int(np.prod(b["shape"])) for b in batch) == 169738240
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+    arr = (F.Cube * 96)(*[_cube(b, b["mix"]) for b in batch])
+    print("config 3: 96 cubes in one call", flush=True)
+    assert lib.fy_gpu_process_cubes(arr, 96, sh) == 0, lib.fy_gpu_last_error()
+    stream.synchronize()
+    print("config 3: each cube alone", flush=True)
+    for b in batch:
+        c = _cube(b, b["alone"])
+        assert lib.fy_gpu_process_cubes(C.byref(c), 1, sh) == 0, lib.fy_gpu_last_error()
+    stream.synchronize()
+    print("config 3: captured graph, two replays", flush=True)
+    garr = (F.Cube * 96)(*[_cube(b, b["graph"]) for b in batch])
+    g = lib.fy_gpu_graph_cubes(garr, 96, sh)
+    assert g, lib.fy_gpu_last_error()
+    try:
+        for b in batch:
+            b["graph"].zero_()
+        torch.cuda.synchronize()
+        for _ in range(2):
+            assert lib.fy_graph_launch(C.c_void_p(g)) == 0
+        stream.synchronize()
+    finally:
+        lib.fy_graph_destroy(C.c_void_p(g))
+    for k, b in enumerate(batch):
+        mix = b["mix"].cpu().numpy()
+        assert mix.any(), f"cube {k}"
+        assert np.array_equal(mix, b["alone"].cpu().numpy()), f"cube {k} {b['shape']}: mix != alone"
+        assert np.array_equal(mix, b["graph"].cpu().numpy()), f"cube {k} {b['shape']}: graph != mix"
+
+
+@pytest.mark.parametrize("cls", range(len(phantom.BATCH_CLASSES)))
+def test_config3_class_vs_fp64_oracle(batch, orc, cls):
+    k = sum(cnt for cnt, _ in phantom.BATCH_CLASSES[:cls])      # the class's first cube
+    b = batch[k]
+    shape = b["shape"]
+    lib = F.lib()
+    d_q = torch.zeros_like(b["d_in"])
+    d_f = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    c = _cube(b, d_q, d_f)
+    print(f"config 3: cube {k} {shape} with its float tile", flush=True)
+    assert lib.fy_gpu_process_cubes(C.byref(c), 1, None) == 0, lib.fy_gpu_last_error()
+    assert lib.fy_gpu_sync(None) == 0
+    got_q, got_f = d_q.cpu().numpy(), d_f.cpu().numpy()
+    if b["mix"].any():
+        assert np.array_equal(got_q, b["mix"].cpu().numpy())
+    op = clibs.CubeParams()
+    C.memmove(C.byref(op), C.byref(b["P"]), C.sizeof(op))
+    q64 = np.empty(shape, np.uint8)
+    f64 = np.empty(shape, np.float64)
+    z = (0, 0, 0)
+    assert orc.orc_chain_cube_f64(C.byref(op), clibs.up(b["u8"]), *z, *shape, *z, *shape, clibs.up(q64),
+                                  clibs.dp(f64)) == 0
+    r = rel_l2(got_f, f64)
+    d = np.abs(got_q.astype(int) - q64.astype(int))
+    print(f"config 3 {shape}: float relL2 vs fp64 oracle {r:.2e}, u8 off by one {(d > 0).mean():.2e}", flush=True)
+    assert r <= 1e-5
+    assert d.max() <= 1 and (d > 0).mean() < 1e-3
