@@ -33,6 +33,15 @@ void  fyg_stream_private_free(fyg_stream *s);
 int   fyg_stream_persistent(fyg_stream *s);
 /* waits for s, then frees its workspaces (the next fyg_ws regrows them) */
 int   fyg_stream_trim(fyg_stream *s);
+/* Fork / join: helper i (< FYG_SUBS) of s is a stream object of its own
+ * (workspaces, default priority; persistent when s is), created on first use
+ * and freed with s.  fyg_fork makes helpers 0..n-1 wait for the work queued
+ * on s so far; fyg_join makes s wait for everything queued on them.  Both are
+ * event dependencies, so they also hold inside a graph capture of s. */
+#define FYG_SUBS 3
+fyg_stream *fyg_stream_sub(fyg_stream *s, int i);
+int   fyg_fork(fyg_stream *s, int n);
+int   fyg_join(fyg_stream *s, int n);
 /* an owned stream object on a new non-blocking hipStream_t; fyg_stream_free
  * waits for it, frees its buffers and destroys the stream */
 fyg_stream *fyg_stream_new(void);

@@ -518,29 +518,35 @@ struct UpArgs {
 };
 
 constexpr int UPX = 4;               // consecutive x outputs per thread
-constexpr int UPK = 64 * UPX / 2 + 4;  // low-res columns one 256-wide chunk touches (s >= 2)
-constexpr int URY = 8, UPY = 6;          // output rows of one workgroup; low-res rows they touch (s >= 2)
-// URZ output planes per workgroup: RPT rows per thread (4 row groups), and the
-// UPZ low-res planes they touch (s >= 2)
-template <int URZ>
+constexpr int URY = 8;               // output rows of one workgroup at the full chunk width
+// Shape of one upsample workgroup: LPR lanes per row chunk (64, 32 or 16), so a
+// chunk is XW = LPR*UPX voxels wide and a wave covers 64/LPR rows; narrow
+// volumes (64-128 voxels in x) then keep every lane busy instead of leaving a
+// 256-wide chunk three quarters empty.  The workgroup covers URZ planes x RY
+// rows x XW voxels; each thread owns RPT rows of UPX voxels.  The low-res
+// patch these rows touch (s >= 2): UPZ planes x UPY rows x UPK columns.
+template <int URZ, int LPR>
 struct UpShape {
-    static constexpr int RPT = URZ * URY / 4, UPZ = URZ / 2 + 2;
+    static constexpr int SW = 64 / LPR, XW = LPR * UPX, RY = URY * SW;
+    static constexpr int RPT = URZ * URY / 4;
+    static constexpr int UPZ = URZ / 2 + 2, UPY = RY / 2 + 2, UPK = XW / 2 + 4, UPH = UPK / 2;
 };
 // patch rows store even columns first, then odd ones: a wave's reads of
 // columns 2q+e (s = 2) then hit consecutive words instead of every other one
-constexpr int UPH = UPK / 2;
+template <int UPH>
 __device__ __forceinline__ int up_col(int c) { return (c & 1) * UPH + (c >> 1); }
 
 // trilinear upsample of (mean_a, mean_b) from the decimated grid and apply
 // out = a*I + b (guided.c:188-228): u = c/s, i0 = min((int)u, l-2), f = min(u-i0, 1).
-// A workgroup produces URZ planes x 8 rows x 256 voxels.  Those rows share a
-// patch of at most UPZ x 6 low-res rows, which is loaded into LDS once.  As in
+// A workgroup produces URZ planes x RY rows x XW voxels.  Those rows share a
+// patch of at most UPZ x UPY low-res rows, which is loaded into LDS once.  As in
 // the reference, each output row blends its four (z,y) low-res rows first and
 // then lerps along x; a thread blends only the three low-res columns its four
-// consecutive voxels need.  Thread (q, g) owns x quad q of rows g, g+4, g+8, ...
+// consecutive voxels need.  Thread (q, g) -- q the lane within its row chunk,
+// g its sub-wave -- owns x quad q of rows g, g+4*SW, g+8*SW, ...
 // P2: s = 2^sh, so c/s is exactly c*(1/s) and needs no division; the knot
 // is then c >> sh and the fraction (c - (knot << sh)) / s, both exact, so the
-// row arithmetic (uniform across a wave) stays on the scalar unit.
+// row arithmetic (uniform across a wave when LPR = 64) stays on the scalar unit.
 template <bool P2, class P = float>
 struct UpCoord {
     P fs, inv;
@@ -558,22 +564,25 @@ struct UpCoord {
 };
 
 // grid: (x chunks, y blocks, cube * nzb + z block)
-template <bool P2, class P, int URZ>
+template <bool P2, class P, int URZ, int LPR>
 __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
+    using S = UpShape<URZ, LPR>;
+    constexpr int RPT = S::RPT, UPZ = S::UPZ, UPY = S::UPY, UPK = S::UPK, SW = S::SW, RY = S::RY;
     const UpArgs<P> &p = bt.a[blockIdx.z / nzb];
-    constexpr int RPT = UpShape<URZ>::RPT, UPZ = UpShape<URZ>::UPZ;
     __shared__ P pa[UPZ][UPY][UPK], pb[UPZ][UPY][UPK];
     const UpCoord<P2, P> U{(P)p.s, (P)1 / (P)p.s, P2 ? __builtin_ctz((unsigned)p.s) : 0};
-    const int Z0 = p.z0 + (int)(blockIdx.z % nzb) * URZ, Y0 = p.y0 + blockIdx.y * URY;
-    const int zl = min(Z0 + URZ, p.z0 + p.dz) - 1, yl = min(Y0 + URY, p.y0 + p.dy) - 1;
-    const int xc0 = p.x0 + blockIdx.x * 64 * UPX;
-    const int xc1 = min(xc0 + 64 * UPX, p.x0 + p.dx) - 1;
+    const int Z0 = p.z0 + (int)(blockIdx.z % nzb) * URZ, Y0 = p.y0 + blockIdx.y * RY;
+    const int zl = min(Z0 + URZ, p.z0 + p.dz) - 1, yl = min(Y0 + RY, p.y0 + p.dy) - 1;
+    const int xc0 = p.x0 + blockIdx.x * S::XW;
+    const int xc1 = min(xc0 + S::XW, p.x0 + p.dx) - 1;
     const int kb = U.i0(xc0, p.lx), nk = U.i0(xc1, p.lx) - kb + 2;
     const int zlo = U.i0(Z0, p.lz), nzp = U.i0(zl, p.lz) - zlo + 2;
     const int ylo = U.i0(Y0, p.ly), nyp = U.i0(yl, p.ly) - ylo + 2;
-    // the wave index is uniform across a wave: row and patch-row arithmetic below
-    // runs on the scalar unit, only column offsets are per lane
-    const int q = threadIdx.x & 63, g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // with whole-wave row chunks the row index is uniform across a wave: row and
+    // patch-row arithmetic below then runs on the scalar unit
+    const int q = threadIdx.x % LPR;
+    int g = threadIdx.x / LPR;
+    if constexpr (SW == 1) g = __builtin_amdgcn_readfirstlane(g);
     const int xb = xc0 + q * UPX;
     const bool xin = xb <= xc1, full = xb + UPX - 1 <= xc1;
 
@@ -581,7 +590,7 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
     P iv[RPT][UPX];
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
-        int r = g + 4 * j, z = Z0 + r / URY, y = Y0 + r % URY;
+        int r = g + 4 * SW * j, z = Z0 + r / RY, y = Y0 + r % RY;
         const bool rok = z <= zl && y <= yl;
         bool ok = xin && rok;
         const P *irow = p.in + ((size_t)(rok ? z : Z0) * p.ny + (rok ? y : Y0)) * p.nx;
@@ -598,49 +607,43 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
             for (int e = 0; e < UPX; e++) iv[j][e] = (ok && xb + e <= xc1) ? irow[xb + e] : 0.0f;
         }
     }
-    // low-res patch: wave g loads patch rows g, g+4, ..; lanes split the columns
+    // low-res patch: the workgroup's threads split the patch's (row, column)
+    // cells; every load is issued before the first store
     {
-        constexpr int RW = (UPZ * UPY + 3) / 4, CW = (UPK + 63) / 64;
+        constexpr int CELLS = UPZ * UPY * UPK, K = (CELLS + NT - 1) / NT;
         const int nrow = nzp * nyp;
-        P va[RW][CW], vb[RW][CW];
+        P va[K], vb[K];
 #pragma unroll
-        for (int i = 0; i < RW; i++) {
-            const int rr = g + 4 * i;
+        for (int k = 0; k < K; k++) {
+            const int i = threadIdx.x + NT * k;
+            const int col = i % UPK, rr = i / UPK;
             int pz = 0;
 #pragma unroll
-            for (int k = 1; k < UPZ; k++) pz += rr >= k * nyp;
+            for (int t = 1; t < UPZ; t++) pz += rr >= t * nyp;
             const int py = rr - pz * nyp;
-            const bool rok = rr < nrow;
-            const size_t ro = rok ? ((size_t)(zlo + pz) * p.ly + (ylo + py)) * p.lx + kb : (size_t)zlo * p.ly * p.lx;
-            const P *ra = p.ma + ro, *rb = p.mb + ro;
-#pragma unroll
-            for (int c = 0; c < CW; c++) {
-                int col = q + 64 * c;
-                int cc = (rok && col < nk) ? col : 0;
-                va[i][c] = ra[cc];
-                vb[i][c] = rb[cc];
-            }
+            const bool ok = i < CELLS && rr < nrow && col < nk;
+            const size_t ro = ok ? ((size_t)(zlo + pz) * p.ly + (ylo + py)) * p.lx + kb + col
+                                 : (size_t)zlo * p.ly * p.lx + kb;
+            va[k] = p.ma[ro];
+            vb[k] = p.mb[ro];
         }
 #pragma unroll
-        for (int i = 0; i < RW; i++) {
-            int rr = g + 4 * i;
+        for (int k = 0; k < K; k++) {
+            const int i = threadIdx.x + NT * k;
+            const int col = i % UPK, rr = i / UPK;
             int pz = 0;
 #pragma unroll
-            for (int k = 1; k < UPZ; k++) pz += rr >= k * nyp;
-            int py = rr - pz * nyp;
-#pragma unroll
-            for (int c = 0; c < CW; c++) {
-                int col = q + 64 * c;
-                if (rr < nrow && col < nk) {
-                    pa[pz][py][up_col(col)] = va[i][c];
-                    pb[pz][py][up_col(col)] = vb[i][c];
-                }
+            for (int t = 1; t < UPZ; t++) pz += rr >= t * nyp;
+            const int py = rr - pz * nyp;
+            if (i < CELLS && rr < nrow && col < nk) {
+                pa[pz][py][up_col<S::UPH>(col)] = va[k];
+                pb[pz][py][up_col<S::UPH>(col)] = vb[k];
             }
         }
     }
     __syncthreads();
     if (!xin) return;
-    // x-dependent weights, shared by the four rows
+    // x-dependent weights, shared by the thread's rows
     const int k0 = U.i0(xb, p.lx) - kb;
     P fxs[UPX];
     bool dsel[UPX];
@@ -651,11 +654,12 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
         fxs[e] = fx > 1.0f ? 1.0f : fx;
         dsel[e] = x0 - kb - k0 != 0;            // 0 or 1
     }
-    const int c0 = up_col(min(k0, nk - 1)), c1 = up_col(min(k0 + 1, nk - 1)), c2 = up_col(min(k0 + 2, nk - 1));
+    const int c0 = up_col<S::UPH>(min(k0, nk - 1)), c1 = up_col<S::UPH>(min(k0 + 1, nk - 1)),
+              c2 = up_col<S::UPH>(min(k0 + 2, nk - 1));
     const bool keep = emit_keep(p.em);
 #pragma unroll
     for (int j = 0; j < RPT; j++) {
-        int r = g + 4 * j, z = Z0 + r / URY, y = Y0 + r % URY;
+        int r = g + 4 * SW * j, z = Z0 + r / RY, y = Y0 + r % RY;
         if (z > zl || y > yl) continue;
         int z0 = U.i0(z, p.lz), y0 = U.i0(y, p.ly);
         P zf = U.frac(z, z0);
@@ -968,16 +972,40 @@ static void launch_coef(fyg_stream *s, int v, const Batch<CoefArgs<P>> &b, int m
     }
 }
 
-template <class P, int URZ>
-static void up_go(fyg_stream *s, const Batch<UpArgs<P>> &b, int m, int sd) {
+template <class P, int URZ, int LPR>
+static void up_go_w(fyg_stream *s, const Batch<UpArgs<P>> &b, int m, int sd) {
+    using S = UpShape<URZ, LPR>;
     const UpArgs<P> &u = b.a[0];
     const int nzb = (u.dz + URZ - 1) / URZ;
-    dim3 grid((unsigned)((u.dx + 64 * UPX - 1) / (64 * UPX)), (unsigned)((u.dy + URY - 1) / URY),
-              (unsigned)(nzb * m));
+    dim3 grid((unsigned)((u.dx + S::XW - 1) / S::XW), (unsigned)((u.dy + S::RY - 1) / S::RY), (unsigned)(nzb * m));
     if ((sd & (sd - 1)) == 0)
-        fyg_launch(s, "guided_upsample", k_upsample<true, P, URZ>, grid, dim3(NT), 0, b, nzb);
+        fyg_launch(s, "guided_upsample", k_upsample<true, P, URZ, LPR>, grid, dim3(NT), 0, b, nzb);
     else
-        fyg_launch(s, "guided_upsample", k_upsample<false, P, URZ>, grid, dim3(NT), 0, b, nzb);
+        fyg_launch(s, "guided_upsample", k_upsample<false, P, URZ, LPR>, grid, dim3(NT), 0, b, nzb);
+}
+
+// the row chunk width that wastes the fewest lanes on a region dx wide (ties:
+// the wider chunk)
+static int up_lanes(int dx) {
+    int best = 64;
+    long waste = -1;
+    for (int lpr = 64; lpr >= 16; lpr /= 2) {
+        long xw = (long)lpr * UPX, w = (dx + xw - 1) / xw * xw - dx;
+        if (waste < 0 || w < waste) {
+            waste = w;
+            best = lpr;
+        }
+    }
+    return best;
+}
+
+template <class P, int URZ>
+static void up_go(fyg_stream *s, const Batch<UpArgs<P>> &b, int m, int sd) {
+    switch (up_lanes(b.a[0].dx)) {
+    case 16: up_go_w<P, URZ, 16>(s, b, m, sd); break;
+    case 32: up_go_w<P, URZ, 32>(s, b, m, sd); break;
+    default: up_go_w<P, URZ, 64>(s, b, m, sd); break;
+    }
 }
 
 template <class P>

@@ -18,6 +18,9 @@ struct fyg_stream {
     int persist = 0;         // tables from the process-wide cache (stable addresses for graphs)
     int capturing = 0;       // launches are being recorded into a graph: no timing events
     struct Arena *arena = nullptr;   // per-stream table slots (fyg_stream_table)
+    fyg_stream *sub[FYG_SUBS] = {};  // helper streams of a fork (own workspaces), created on first use
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[FYG_SUBS] = {};
 };
 
 namespace fyg_prof {

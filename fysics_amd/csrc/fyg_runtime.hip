@@ -90,10 +90,56 @@ extern "C" void fyg_stream_private_free(fyg_stream *st) {
         if (st->pin[i]) (void)hipHostFree(st->pin[i]);
     if (st->flag) (void)hipFree(st->flag);
     fyg_arena_free(st);
+    for (int i = 0; i < FYG_SUBS; i++) {
+        if (st->sub[i]) fyg_stream_free(st->sub[i]);
+        if (st->join_ev[i]) (void)hipEventDestroy(st->join_ev[i]);
+    }
+    if (st->fork_ev) (void)hipEventDestroy(st->fork_ev);
     delete st;
 }
 
 extern "C" int fyg_stream_persistent(fyg_stream *s) { return s && s->persist; }
+
+extern "C" fyg_stream *fyg_stream_sub(fyg_stream *s, int i) {
+    if (!s || i < 0 || i >= FYG_SUBS) return nullptr;
+    if (s->sub[i]) return s->sub[i];
+    fyg_stream *st = fyg_stream_new();
+    if (!st) return nullptr;
+    st->persist = s->persist;
+    if (note(hipEventCreateWithFlags(&s->join_ev[i], hipEventDisableTiming), "hipEventCreate(join)")) {
+        fyg_stream_free(st);
+        return nullptr;
+    }
+    s->sub[i] = st;
+    return st;
+}
+
+extern "C" int fyg_fork(fyg_stream *s, int n) {
+    if (!s || n < 0 || n > FYG_SUBS) return 1;
+    if (!s->fork_ev && note(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming), "hipEventCreate(fork)"))
+        return 1;
+    if (note(hipEventRecord(s->fork_ev, s->s), "hipEventRecord(fork)")) return 1;
+    for (int i = 0; i < n; i++) {
+        fyg_stream *h = fyg_stream_sub(s, i);
+        if (!h) return 1;
+        h->capturing = s->capturing;
+        if (note(hipStreamWaitEvent(h->s, s->fork_ev, 0), "hipStreamWaitEvent(fork)")) return 1;
+    }
+    return 0;
+}
+
+extern "C" int fyg_join(fyg_stream *s, int n) {
+    if (!s || n < 0 || n > FYG_SUBS) return 1;
+    for (int i = 0; i < n; i++) {
+        fyg_stream *h = s->sub[i];
+        if (!h) return 1;
+        if (note(hipEventRecord(s->join_ev[i], h->s), "hipEventRecord(join)") ||
+            note(hipStreamWaitEvent(s->s, s->join_ev[i], 0), "hipStreamWaitEvent(join)"))
+            return 1;
+        h->capturing = 0;
+    }
+    return 0;
+}
 
 // waits for the stream, then frees its workspaces (they regrow on demand)
 extern "C" int fyg_stream_trim(fyg_stream *s) {

@@ -364,6 +364,10 @@ static int run_group(cube_plan *pl, int n, int **flags, const int *gen, gen_step
     return 0;
 }
 
+/* set on a thread whose caller runs several streams itself (the served form's
+ * lanes): its groups then run one after another on the one stream */
+static __thread int no_fork;
+
 /* device workspace one group may hold at once; larger groups run in parts */
 #define FYH_GROUP_BYTES ((size_t)8 << 30)
 
@@ -399,14 +403,20 @@ static int run_chunk(const fy_cube *cubes, int n, int *d_flags, const int *gen, 
         gi[i] = -1;
     }
     fyh_lut_memo_free(memo);
+    /* the launch groups: cubes of one shape and stage set, in parts that fit
+     * FYH_GROUP_BYTES; grp is filled group by group */
+    typedef struct { int first, m; double work; int lane; } part_t;
+    part_t *parts = rc ? NULL : (part_t *)malloc(sizeof(part_t) * (size_t)n);
+    int np = 0, ng = 0;
+    if (!rc && !parts) rc = 1;
     for (int i = 0; i < n && !rc; i++) {
         if (gi[i] >= 0 || pl[i].skip) continue;
         int m = 0;
         for (int j = i; j < n; j++) {
             if (gi[j] < 0 && !pl[j].skip && same_group(&pl[i], &pl[j])) {
                 gi[j] = i;
-                grp[m] = pl[j];
-                gf[m] = d_flags + j;
+                grp[ng + m] = pl[j];
+                gf[ng + m] = d_flags + j;
                 m++;
             }
         }
@@ -414,9 +424,51 @@ static int run_chunk(const fy_cube *cubes, int n, int *d_flags, const int *gen, 
                      2 * al256(pl[i].img_bytes) + al256(pl[i].air_bytes);
         int part = (int)(FYH_GROUP_BYTES / (per ? per : 1));
         if (part < 1) part = 1;
-        for (int k = 0; k < m && !rc; k += part)
-            rc = run_group(grp + k, m - k < part ? m - k : part, gf + k, gen, st, s);
+        for (int k = 0; k < m; k += part) {
+            int mk = m - k < part ? m - k : part;
+            parts[np].first = ng + k;
+            parts[np].m = mk;
+            parts[np].work = (double)mk * (double)(per ? per : 1);
+            parts[np].lane = 0;
+            np++;
+        }
+        ng += m;
     }
+    /* Several groups run side by side on helper streams: the small cubes' short
+     * grids then share the chip with the large ones instead of leaving it
+     * half idle one group after another.  Each lane has its own workspaces;
+     * results do not depend on the lane. */
+    int lanes = no_fork ? 1 : (np < 1 + FYG_SUBS ? np : 1 + FYG_SUBS);
+    if (!rc && lanes > 1) {
+        double load[1 + FYG_SUBS] = {0};
+        for (int a = 0; a < np; a++) {                  /* largest part first, to the least loaded lane */
+            int best = -1;
+            for (int b = 0; b < np; b++)
+                if (parts[b].lane == 0 && parts[b].work >= 0 && (best < 0 || parts[b].work > parts[best].work))
+                    best = b;
+            int l = 0;
+            for (int k = 1; k < lanes; k++)
+                if (load[k] < load[l]) l = k;
+            load[l] += parts[best].work;
+            parts[best].lane = l + 1;                   /* 1-based while assigning */
+            parts[best].work = -1.0;
+        }
+        for (int a = 0; a < np; a++) parts[a].lane -= 1;
+        /* the generation word is stepped before any lane reads it */
+        if (st->gen) {
+            rc = fyg_gen_step(st->gen, st->nflag, s);
+            st->gen = NULL;
+        }
+        if (!rc) rc = fyg_fork(s, lanes - 1);
+    }
+    for (int l = 0; l < (lanes > 1 ? lanes : 1) && !rc; l++) {
+        fyg_stream *ls = l == 0 ? s : fyg_stream_sub(s, l - 1);
+        if (!ls) rc = 1;
+        for (int a = 0; a < np && !rc; a++)
+            if (parts[a].lane == l) rc = run_group(grp + parts[a].first, parts[a].m, gf + parts[a].first, gen, st, ls);
+    }
+    if (lanes > 1 && fyg_join(s, lanes - 1)) rc = 1;
+    free(parts);
     free(pl); free(grp); free(gi); free(gf);
     if (fyg_hold_release(hold, s)) rc = 1;
     return rc;
@@ -466,6 +518,14 @@ int fyh_process_cubes(const fy_cube *cubes, int n, fyg_stream *s) {
     int *dflags = NULL, *dgen = NULL, nflag = 0;
     if (!fyg_flags_reserve(s, n, &dflags, &dgen, &nflag)) return 1;
     return run_cubes(cubes, n, dflags, dgen, nflag, s);
+}
+
+int fyh_process_cubes_serial(const fy_cube *cubes, int n, fyg_stream *s) {
+    int old = no_fork;
+    no_fork = 1;
+    int rc = fyh_process_cubes(cubes, n, s);
+    no_fork = old;
+    return rc;
 }
 
 int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream) {

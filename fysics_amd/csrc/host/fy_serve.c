@@ -185,7 +185,7 @@ static int launch(fy_server *sv, lane *l, int m, size_t bytes) {
             if (fyg_h2d((void *)l->batch[i].d_in, r->h_in, hn, l->s)) return 1;
         }
     }
-    if (fyh_process_cubes(l->batch, m, l->s)) return 1;
+    if (fyh_process_cubes_serial(l->batch, m, l->s)) return 1;   /* the lanes are the concurrency */
     for (int i = 0; i < m; i++) {
         request *r = &sv->q[(sv->qh + i) % sv->qcap];
         if (r->h_out &&

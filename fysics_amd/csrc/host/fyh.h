@@ -58,6 +58,8 @@ int fyh_chain(const fy_pipeline_cfg *cfg, const unsigned char *d_u8,
 
 /* fy_gpu_process_cubes on a stream object */
 int fyh_process_cubes(const fy_cube *cubes, int n, fyg_stream *s);
+/* the same with every launch group on s itself (no helper streams) */
+int fyh_process_cubes_serial(const fy_cube *cubes, int n, fyg_stream *s);
 
 /* deconvolution on device buffers (fp32 in/out) */
 int fyh_deconv_dev(const float *d_in, float *d_out, int nz, int ny, int nx, const fy_physics *p,
