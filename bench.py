@@ -215,7 +215,7 @@ def main():
 
     # per-cube latency distribution (events around each step on the same stream)
     lat = []
-    for _ in range(min(args.steps, 50)):
+    for _ in range(200):
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         a.record(stream)
@@ -268,7 +268,7 @@ def main():
     if not args.no_served:
         stress = {"host": served_bench(F, 6000, 60000.0, seed=8, host=True),
                   "hbm": served_bench(F, 6000, 60000.0, seed=9, host=False)}
-    precision = None if args.no_precision else precision_bench(F)
+    precision = None if args.no_precision else precision_bench(F, vol)
     calib = None if args.no_calibration else calibration_bench(F, vol)
     airz = None if args.no_air_zero else air_zero_bench(F, vol, eps)
     lod = None if args.no_lod else lod_phasecorr_bench(F, vol)
@@ -277,6 +277,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(vol, phys, eps, ring, args.cpu_threads)
+        # the plain CPU reference path of the 128^3 config-1 volume, beside the drop-in
+        cpu["config1_128"] = config1_bench(F)
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mvoxels/s", "n_gpus": 1,
@@ -490,21 +492,31 @@ def served_bench(F, n_req=20000, rate=400.0, seed=7, pool=4, eps=2e-3, host=True
     return res
 
 
-def precision_bench(F, n=512, reps=5, seed=3):
-    """The 512^3 precision configuration timed two ways: the fp32 chain (the
-    product path) and its all-FP64 variant (deconvolution spectrum, weights and
-    the fast guided filter in double), both u8 in HBM -> u8 in HBM.  The accuracy
-    of both against the fp64 oracle is the subject of tests/test_gpu_precision.py."""
+def precision_bench(F, vol, reps=5):
+    """The 512^3 precision configuration (the workload cube tiled 2x2x2): the
+    fp32 chain (the product path) and its all-FP64 variant (deconvolution
+    spectrum, weights and the fast guided filter in double), both u8 in HBM ->
+    u8 in HBM, timed; and, outside the timed region, the fp32 results against
+    the all-FP64 ones: the deconvolved tile (fy_gpu_deconv_radial vs
+    fy_gpu_deconv_radial_f64 on the dewindowed cube), the guided tile and the
+    u8 output of the chain.  The FP64 path itself agrees with the fp64 CPU
+    oracle to ~1e-12 (deconvolution) and 3.5e-8 (its fp32-emitted guided tile),
+    which tests/test_gpu_precision.py checks at this size."""
     import torch
     lib = F.lib()
-    rng = np.random.default_rng(seed)
-    d_in = torch.from_numpy(rng.integers(0, 256, size=(n, n, n), dtype=np.uint8)).cuda()
+    big = np.tile(vol, (2, 2, 2))
+    n = big.shape[0]
+    d_in = torch.from_numpy(big).cuda()
+    del big
     d_out = torch.empty_like(d_in)
+    d_out64 = torch.empty_like(d_in)
     phys = F.physics(delta_beta=300.0 * 0.35, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3,
                      unsharp_sigma=0.8, unsharp_coeff=0.5)
     stream = torch.cuda.Stream()
     sh = C.c_void_p(stream.cuda_stream)
-    out = {"workload": f"{n}^3 u8 cube: dewindow -> Wiener deconv (sigma ratio 1.17) -> guided r=2 s=2 -> u8"}
+    out = {"workload": f"{n}^3 u8 cube (the workload cube tiled 2x2x2): dewindow -> Wiener deconv (sigma ratio "
+                       "1.17) -> guided r=2 s=2 -> u8"}
+    prm = {}
     for name, fp64 in (("fp32", 0), ("fp64", 1)):
         P = F.CubeParams()
         P.phys = phys
@@ -512,8 +524,10 @@ def precision_bench(F, n=512, reps=5, seed=3):
         P.window_lo, P.window_hi = -0.1, 1.2
         P.guided_eps, P.guided_radius, P.guided_s = 2e-3, 2, 2
         P.fp64 = fp64
+        prm[name] = P
         z = (0, 0, 0)
-        cube = F.Cube(d_in.data_ptr(), *z, n, n, n, *z, n, n, n, d_out.data_ptr(), None, C.pointer(P))
+        dst = d_out64 if fp64 else d_out
+        cube = F.Cube(d_in.data_ptr(), *z, n, n, n, *z, n, n, n, dst.data_ptr(), None, C.pointer(P))
         if lib.fy_gpu_process_cubes(C.byref(cube), 1, sh):
             raise RuntimeError(lib.fy_gpu_last_error().decode())
         stream.synchronize()
@@ -523,6 +537,48 @@ def precision_bench(F, n=512, reps=5, seed=3):
         stream.synchronize()
         dt = (time.perf_counter() - t0) / reps
         out[name] = {"ms_per_cube": round(1000 * dt, 3), "mvox_s": round(n ** 3 / dt / 1e6, 1)}
+
+    def rel_max(a, b):
+        a = a.double()
+        b = b.double()
+        return (float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)),
+                float((a - b).abs().max()))
+
+    # errors, untimed: deconvolution alone (the chain's dewindowed input, fp32 vs fp64 spectrum)
+    f = d_in.float() * np.float32(1.0 / 255.0)
+    f = f * np.float32(1.2 - (-0.1)) + np.float32(-0.1)
+    d32 = torch.empty_like(f)
+    d64 = torch.empty(f.shape, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    rc = lib.fy_gpu_deconv_radial(C.c_void_p(f.data_ptr()), C.c_void_p(d32.data_ptr()), n, n, n, C.byref(phys),
+                                  0.015, 0, 8.0, 1.17 ** 2, sh)
+    rc |= lib.fy_gpu_deconv_radial_f64(C.c_void_p(f.data_ptr()), C.c_void_p(d64.data_ptr()), n, n, n,
+                                       C.byref(phys), 0.015, 0, 8.0, 1.17 ** 2, sh)
+    stream.synchronize()
+    if rc:
+        raise RuntimeError(lib.fy_gpu_last_error().decode())
+    e_dec = rel_max(d32, d64)
+    del f, d32, d64
+    # the guided tile of the chain (fp32 tile out) and the u8 output
+    gt = {}
+    for name in ("fp32", "fp64"):
+        t = torch.empty(d_in.shape, dtype=torch.float32, device="cuda")
+        z = (0, 0, 0)
+        cube = F.Cube(d_in.data_ptr(), *z, n, n, n, *z, n, n, n, d_out.data_ptr() if name == "fp32" else
+                      d_out64.data_ptr(), t.data_ptr(), C.pointer(prm[name]))
+        if lib.fy_gpu_process_cubes(C.byref(cube), 1, sh):
+            raise RuntimeError(lib.fy_gpu_last_error().decode())
+        gt[name] = t
+    stream.synchronize()
+    e_gui = rel_max(gt["fp32"], gt["fp64"])
+    du = (d_out.int() - d_out64.int()).abs()
+    out["errors_vs_fp64_gpu"] = {
+        "deconv": {"rel_l2": e_dec[0], "max_abs": e_dec[1]},
+        "guided": {"rel_l2": e_gui[0], "max_abs": e_gui[1]},
+        "u8": {"max_abs": int(du.max()), "frac_differing": float((du > 0).float().mean())},
+        "reference": "the all-FP64 GPU path (within ~1e-12 / 3.5e-8 of the fp64 CPU oracle, "
+                     "tests/test_gpu_precision.py)"}
+    out["fp32_over_fp64_speed"] = round(out["fp32"]["mvox_s"] / out["fp64"]["mvox_s"], 3)
     return out
 
 
@@ -759,14 +815,40 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
 
 
+def _host_flags():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+def _ref_library():
+    """The reference compiled from its sources (oracle/_ref): the AVX-512 build
+    where the host CPU has AVX-512 F/BW/CD/DQ/VL (what the reference's own
+    -march=native would use there), else the AVX2+FMA build.  Returns
+    (CDLL, march) or (None, reason)."""
+    v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+    cand = [("libfysics_ref_v4.so", "x86-64-v4 (AVX-512)")] if v4 <= _host_flags() else []
+    cand.append(("libfysics_ref.so", "x86-64-v3 (AVX2+FMA)"))
+    for so, march in cand:
+        path = os.path.join(ROOT, "oracle", "_ref", so)
+        if os.path.exists(path):
+            return C.CDLL(path), march
+    return None, "oracle/_ref not built on this host"
+
+
 def cpu_baseline(vol, phys, eps, ring, threads):
-    """The reference's own chain (compiled from its sources, oracle/_ref) on the
-    host cores: one 256^3 cube per OpenMP thread, one warm-up round, median of 3."""
-    ref_so = os.path.join(ROOT, "oracle", "_ref", "libfysics_ref.so")
-    if not os.path.exists(ref_so):
-        return {"value": None, "note": "oracle/_ref not built on this host"}
+    """The reference's own chain (compiled from its sources, oracle/_ref, with its
+    -Ofast flags) on the host cores: one 256^3 cube per OpenMP thread, one warm-up
+    round, median of 3."""
+    ref, march = _ref_library()
+    if ref is None:
+        return {"value": None, "note": march}
     import fysics_amd as F
-    ref = C.CDLL(ref_so)
     ref.ref_chain_many.restype = C.c_double
     ref.ref_chain_many.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_long, C.c_long,
                                    C.c_long, C.c_long, C.POINTER(F.Physics), C.c_int, C.c_double, C.c_double,
@@ -795,9 +877,67 @@ def cpu_baseline(vol, phys, eps, ring, threads):
         times.append(t)
     med = statistics.median(times[1:])
     return {"value": round(threads * n ** 3 / 1e6 / med, 1), "unit": "Mvoxels/s", "cores": threads,
-            "kind": "reference",
+            "kind": "reference", "march": march,
             "sample": f"{threads} x {n}^3 cubes per round (one per OpenMP thread), warm-up + median of 3 "
-                      f"rounds, {med:.2f} s/round; CPU: {cpu_name}"}
+                      f"rounds, {med:.2f} s/round; CPU: {cpu_name}; built -Ofast -ffast-math -funroll-loops "
+                      f"-march={march.split()[0]} -fopenmp"}
+
+
+def config1_bench(F):
+    """The plain CPU reference path on one 128^3 volume (the config-1 phantom:
+    40 sheets, Paganin+unsharp forward model, noise var = 0.02 I + 4, u8 through
+    [-0.1, 1.2]) restored in the order a reference user calls it: fy_u8_to_phys
+    -> fy_deconvolve(reg 0.015) -> fy_estimate_noise(win 5, 10th percentile,
+    ref 0.5) -> fy_guided_eps_for_noise -> fy_guided_denoise(r 2), host buffers
+    in and out.  Timed for the reference compiled from its sources (its own
+    OpenMP where it has any) and for this library's drop-in entry points of the
+    same names (PCIe included); median of 3 after a warm-up."""
+    from fysics_amd import phantom
+    ref, march = _ref_library()
+    u8 = phantom.config1_volume(128, seed=1)
+    shape, n = u8.shape, u8.size
+    lo, hi = phantom.WINDOW
+    ph = F.physics(**phantom.CONFIG1_PHYS)
+    libs = [("gpu", F.lib())]
+    if ref is not None:
+        for name, (res, args) in F.SIGNATURES.items():
+            if name in ("fy_u8_to_phys", "fy_deconvolve", "fy_estimate_noise", "fy_guided_eps_for_noise",
+                        "fy_guided_denoise") and hasattr(ref, name):
+                fn = getattr(ref, name)
+                fn.restype, fn.argtypes = res, args
+        libs.append(("reference_cpu", ref))
+
+    def chain(lib):
+        f = np.empty(shape, np.float32)
+        d = np.empty(shape, np.float32)
+        g = np.empty(shape, np.float32)
+        t0 = time.perf_counter()
+        lib.fy_u8_to_phys(u8.ctypes.data_as(F.c_u8_p), F._fp(f), n, lo, hi)
+        if lib.fy_deconvolve(F._fp(f), F._fp(d), *shape, C.byref(ph), 0.015):
+            raise RuntimeError("fy_deconvolve")
+        nm = F.NoiseModel()
+        if lib.fy_estimate_noise(F._fp(d), *shape, 5, 10.0, 0.5, C.byref(nm)):
+            raise RuntimeError("fy_estimate_noise")
+        eps = lib.fy_guided_eps_for_noise(nm.noise_ref)
+        if lib.fy_guided_denoise(F._fp(d), F._fp(g), *shape, 2, eps):
+            raise RuntimeError("fy_guided_denoise")
+        return time.perf_counter() - t0, g, eps
+
+    out = {"workload": "128^3 config-1 phantom: dewindow -> fy_deconvolve(0.015) -> fy_estimate_noise -> "
+                       "guided r=2 exact, host buffers"}
+    res = {}
+    for name, lib in libs:
+        chain(lib)
+        runs = [chain(lib) for _ in range(3)]
+        t = statistics.median(r[0] for r in runs)
+        res[name] = runs[0][1]
+        out[name] = {"ms": round(1000 * t, 2), "mvox_s": round(n / t / 1e6, 1), "eps": runs[0][2]}
+    if "reference_cpu" in out:
+        out["reference_cpu"]["march"] = march
+        a, b = res["gpu"].astype(np.float64), res["reference_cpu"].astype(np.float64)
+        out["gpu_vs_reference_rel_l2"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        out["speedup"] = round(out["reference_cpu"]["ms"] / out["gpu"]["ms"], 1)
+    return out
 
 
 if __name__ == "__main__":
