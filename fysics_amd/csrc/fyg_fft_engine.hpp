@@ -489,7 +489,7 @@ struct XLine {
 template <int L, int MODE, class P>
 __global__ __launch_bounds__(256) void k_rfft_x(Batch<XFwd<P>> bt) {
     using V = typename Cx<P>::V;
-    const XFwd<P> &a = bt.a[blockIdx.y];
+    const XFwd<P> a = bt.a[blockIdx.y];             // into registers once
     constexpr int NL = Shape<L, P>::NL_RM;
     constexpr bool U8 = MODE != 0, RING = MODE == 2;
     __shared__ V buf[NL * Shape<L>::LD];
@@ -808,7 +808,7 @@ struct XInvLine {
 template <int L, class P>
 __global__ __launch_bounds__(256) void k_irfft_x(Batch<XInv<P>> bt) {
     using V = typename Cx<P>::V;
-    const XInv<P> &a = bt.a[blockIdx.y];
+    const XInv<P> a = bt.a[blockIdx.y];             // into registers once
     constexpr int NL = Shape<L, P>::NL_RM;
     __shared__ V buf[NL * Shape<L>::LD];
     __shared__ V tw_s[L];

@@ -357,7 +357,7 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
     const int T = gridDim.x * gridDim.y * gridDim.z;
     const int L = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), T);
     const int tx = L % gridDim.x, ty = (L / gridDim.x) % gridDim.y, tz = L / (gridDim.x * gridDim.y);
-    const CoefArgs<P> &p = bt.a[tz / nzc];
+    const CoefArgs<P> p = bt.a[tz / nzc];          // into registers once
     const int zchunk = tz % nzc;
     static_assert(M::AW * M::AW == NT, "one thread per a, b column");
     static_assert(M::SW <= M::SP, "sample row fits its pitch");
@@ -568,7 +568,7 @@ template <bool P2, class P, int URZ, int LPR>
 __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
     using S = UpShape<URZ, LPR>;
     constexpr int RPT = S::RPT, UPZ = S::UPZ, UPY = S::UPY, UPK = S::UPK, SW = S::SW, RY = S::RY;
-    const UpArgs<P> &p = bt.a[blockIdx.z / nzb];
+    const UpArgs<P> p = bt.a[blockIdx.z / nzb];     // into registers once, not re-read per use
     __shared__ P pa[UPZ][UPY][UPK], pb[UPZ][UPY][UPK];
     const UpCoord<P2, P> U{(P)p.s, (P)1 / (P)p.s, P2 ? __builtin_ctz((unsigned)p.s) : 0};
     const int Z0 = p.z0 + (int)(blockIdx.z % nzb) * URZ, Y0 = p.y0 + blockIdx.y * RY;
@@ -607,37 +607,45 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
             for (int e = 0; e < UPX; e++) iv[j][e] = (ok && xb + e <= xc1) ? irow[xb + e] : 0.0f;
         }
     }
-    // low-res patch: the workgroup's threads split the patch's (row, column)
-    // cells; every load is issued before the first store
+    // low-res patch: wave w loads patch rows w, w+4, ..; its lanes split the
+    // columns.  The row index is wave-uniform, so row arithmetic is scalar.
     {
-        constexpr int CELLS = UPZ * UPY * UPK, K = (CELLS + NT - 1) / NT;
+        constexpr int RW = (UPZ * UPY + 3) / 4, CW = (UPK + 63) / 64;
+        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
         const int nrow = nzp * nyp;
-        P va[K], vb[K];
+        P va[RW][CW], vb[RW][CW];
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int i = threadIdx.x + NT * k;
-            const int col = i % UPK, rr = i / UPK;
+        for (int i = 0; i < RW; i++) {
+            const int rr = wv + 4 * i;
             int pz = 0;
 #pragma unroll
-            for (int t = 1; t < UPZ; t++) pz += rr >= t * nyp;
+            for (int k = 1; k < UPZ; k++) pz += rr >= k * nyp;
             const int py = rr - pz * nyp;
-            const bool ok = i < CELLS && rr < nrow && col < nk;
-            const size_t ro = ok ? ((size_t)(zlo + pz) * p.ly + (ylo + py)) * p.lx + kb + col
-                                 : (size_t)zlo * p.ly * p.lx + kb;
-            va[k] = p.ma[ro];
-            vb[k] = p.mb[ro];
+            const bool rok = rr < nrow;
+            const size_t ro = rok ? ((size_t)(zlo + pz) * p.ly + (ylo + py)) * p.lx + kb : (size_t)zlo * p.ly * p.lx;
+            const P *ra = p.ma + ro, *rb = p.mb + ro;
+#pragma unroll
+            for (int c = 0; c < CW; c++) {
+                int col = lane + 64 * c;
+                int cc = (rok && col < nk) ? col : 0;
+                va[i][c] = ra[cc];
+                vb[i][c] = rb[cc];
+            }
         }
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int i = threadIdx.x + NT * k;
-            const int col = i % UPK, rr = i / UPK;
+        for (int i = 0; i < RW; i++) {
+            const int rr = wv + 4 * i;
             int pz = 0;
 #pragma unroll
-            for (int t = 1; t < UPZ; t++) pz += rr >= t * nyp;
+            for (int k = 1; k < UPZ; k++) pz += rr >= k * nyp;
             const int py = rr - pz * nyp;
-            if (i < CELLS && rr < nrow && col < nk) {
-                pa[pz][py][up_col<S::UPH>(col)] = va[k];
-                pb[pz][py][up_col<S::UPH>(col)] = vb[k];
+#pragma unroll
+            for (int c = 0; c < CW; c++) {
+                int col = lane + 64 * c;
+                if (rr < nrow && col < nk && col < UPK) {
+                    pa[pz][py][up_col<S::UPH>(col)] = va[i][c];
+                    pb[pz][py][up_col<S::UPH>(col)] = vb[i][c];
+                }
             }
         }
     }
