@@ -206,6 +206,13 @@ int fyg_deconv_hxp(int px);
 int fyg_noalloc_deconv(const float *in, float *out, int nz, int ny, int nx, int pz, int py, int px,
                        const double *phys8, double reg, double cap, double us, fyg_stream *s);
 
+/* pass structure of the deconvolution for cubes whose padded X and Y axes are
+ * one length of 64 or 96: 1 (default) fused X+Y plane kernels, 0 the separate
+ * X and Y passes (the same transforms; fp32 rounding may differ in the last
+ * bits); returns the previous mode */
+int fyg_fft_plane_mode(void);
+int fyg_fft_plane_set(int mode);
+
 /* complex FFT of split re/im device arrays (fy_fft1d/fy_fft3d) */
 int fyg_fft3d_run(float *d_re, float *d_im, int nz, int ny, int nx, int sign,
                   void *work /* 8*nz*ny*nx bytes */, fyg_stream *s);

@@ -1,6 +1,8 @@
 // fyg_fft.hip -- the fp32 instantiation of the deconvolution passes
 // (fyg_fft_engine.hpp; deconv_radial, paganin.c:152-293) and the complex FFTs of
 // fft.c (fy_fft1d / fy_fft3d, fft.c:170-228) behind the C ABI of fyg.h.
+#include <atomic>
+
 #include "fyg_fft_engine.hpp"
 
 namespace {
@@ -51,6 +53,10 @@ __global__ void k_scale(float *d, size_t n, float k) {
 }
 
 }  // namespace
+
+static std::atomic<int> g_plane_mode{1};
+extern "C" int fyg_fft_plane_mode(void) { return g_plane_mode.load(std::memory_order_relaxed); }
+extern "C" int fyg_fft_plane_set(int mode) { return g_plane_mode.exchange(mode ? 1 : 0); }
 
 extern "C" int fyg_fft_supported(int L) {
     if (L < 1 || L > FYG_FFT_MAX) return 0;

@@ -857,6 +857,178 @@ unsigned grid1d(size_t n) {
     return (unsigned)(b < 8192 ? (b ? b : 1) : 8192);
 }
 
+// ------------------------------------------------------------------ fused planes (short axes)
+// For cubes whose padded X and Y axes are one length L of 64 or 96 (fp32), a z
+// plane's half spectrum (L rows x L/2+1 columns, at most 37.6 KB) fits in LDS,
+// so the X and Y transforms of a plane run in one workgroup and the spectrum
+// visits HBM once between them instead of twice:
+//   k_rfft_xy : u8 gather + prologue, R2C along x in chunks of NLX row pairs,
+//               untangled into the plane held column by column, then the
+//               column FFTs along y in place; the last stage writes the plane.
+//   k_irfft_yx: the inverse: column FFTs of a plane (first stage read from HBM),
+//               then the Hermitian repack and C2R along x of rows y < ny with
+//               the epilogue, stored as the tile.
+// Every line's arithmetic (radix order, twiddles, untangle, repack) is that
+// of k_rfft_x + k_cfft_cols and k_cfft_cols + k_irfft_x, so the results are
+// bit-identical to the separate passes.
+template <int L>
+struct PlaneShape {
+    static constexpr int HX = L / 2 + 1;                    // half-spectrum columns
+    static constexpr int NLX = L >= 96 ? 8 : 16;            // X row pairs per chunk (LDS: 2-3 groups per CU)
+    // 128 fits too (81 KB, two groups per CU) but measured slower than the
+    // separate passes: eight chunk barriers per plane at 8 waves per CU
+    static constexpr bool OK = L == 64 || L == 96;
+};
+
+template <int L>
+__device__ __forceinline__ float2 *plane_col(float2 *col, int k, int y) {
+    return col + k * Shape<L>::LD + lds_pos(y);
+}
+
+// grid: (pz planes, cube)
+template <int L, int MODE>
+__global__ __launch_bounds__(256) void k_rfft_xy(Batch<XFwd<float>> bt) {
+    using V = float2;
+    using PS = PlaneShape<L>;
+    constexpr int HX = PS::HX, NLX = PS::NLX, LD = Shape<L>::LD;
+    constexpr bool U8 = MODE != 0, RING = MODE == 2;
+    const XFwd<float> a = bt.a[blockIdx.y];
+    __shared__ V col[HX * LD];
+    __shared__ V xb[NLX * LD];
+    __shared__ V tw_s[L];
+    __shared__ XLine meta[NLX];
+    const TwPrefetch<L, V> twp(a.tw);                       // px == py: one table
+    const int z = blockIdx.x;
+    const int rz = reflect_idx(z, a.nz);
+    const long gz = a.pro.gz0 + rz;
+    const float zf = (U8 && a.pro.zdrift) ? a.pro.zfactor[gz] : 1.0f;
+    long cs = 0;
+    if (RING) cs = (long)(ring_slab(a.pro, gz) - ring_slab(a.pro, a.pro.gz0)) * a.ny;
+    int any = 0;
+    for (int c0 = 0; c0 < L / 2; c0 += NLX) {
+        for (int l = threadIdx.x; l < NLX; l += Shape<L>::T) {
+            XLine m;
+            const int y0 = 2 * (c0 + l);
+            const int ry0 = reflect_idx(y0, a.ny), ry1 = reflect_idx(y0 + 1, a.ny);
+            m.valid = 1;
+            m.pair = 1;
+            m.a = ((long)rz * a.ny + ry0) * a.nx;
+            m.b = ((long)rz * a.ny + ry1) * a.nx;
+            m.so = 0;
+            m.zf = zf;
+            m.ca = (cs + ry0) * a.nx;
+            m.cb = (cs + ry1) * a.nx;
+            meta[l] = m;
+        }
+        __syncthreads();
+        auto first = [&](int l, int p0, int off) -> V {
+            const XLine &m = meta[l];
+            int rx = reflect_idx(p0 + off, a.nx);
+            float va, vb;
+            if (U8) {
+                const unsigned char *u = (const unsigned char *)a.in;
+                unsigned char ua = u[m.a + rx], ub = u[m.b + rx];
+                float ca = 0.0f, cb = 0.0f;
+                if (RING) {
+                    ca = a.pro.cimg[m.ca + rx];
+                    cb = a.pro.cimg[m.cb + rx];
+                }
+                any |= ua | ub;
+                va = prologue(a.pro, ua, ca, m.zf);
+                vb = prologue(a.pro, ub, cb, m.zf);
+            } else {
+                const float *f = (const float *)a.in;
+                va = f[m.a + rx];
+                vb = f[m.b + rx];
+            }
+            return make_float2(va, vb);
+        };
+        const bool c_first = c0 == 0;
+        run_from<L, NLX, false, false, 0>(-1.0f, tw_s, xb, first, LdsWr<L, NLX, V>{xb}, [&] {
+            if (c_first) twp.store(tw_s);
+        });
+        __syncthreads();
+        // untangle A(k) = (X(k) + conj X(-k))/2 -> row y0, B(k) = (X(k) - conj X(-k))/(2i) -> row y0+1
+        for (int i = threadIdx.x; i < NLX * HX; i += Shape<L>::T) {
+            const int l = i / HX, k = i - l * HX;
+            const int nk = k ? L - k : 0;
+            const V p = xb[l * LD + lds_pos(k)];
+            const V q = xb[l * LD + lds_pos(nk)];
+            const int y0 = 2 * (c0 + l);
+            *plane_col<L>(col, k, y0) = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
+            *plane_col<L>(col, k, y0 + 1) = make_float2(0.5f * (p.y + q.y), 0.5f * (q.x - p.x));
+        }
+        __syncthreads();
+    }
+    if (U8 && a.flag) {
+        if (__syncthreads_or(any) && threadIdx.x == 0) *a.flag = *a.gen;
+    }
+    // column FFTs along y, in place: the first stage with a barrier between its
+    // reads and writes, then the remaining stages; the last one writes the plane
+    using PL = Plan<L, false>;
+    stage<L, HX, Shape<L>::T, true, PL::R(0), 1, true>(-1.0f, tw_s, LdsRd<L, HX, V>{col}, LdsWr<L, HX, V>{col});
+    __syncthreads();
+    V *plane = a.spec + (size_t)z * a.py * a.hxp;
+    const long hxp = a.hxp;
+    auto last = [&](int l, int p0, int off, V v) { plane[(size_t)(p0 + off) * hxp + l] = v; };
+    if constexpr (PL::n > 1) run_from<L, HX, true, false, 1>(-1.0f, tw_s, col, LdsRd<L, HX, V>{col}, last);
+    else
+        for (int i = threadIdx.x; i < HX * L; i += Shape<L>::T) last(i % HX, i / HX, 0, col[(i % HX) * LD + lds_pos(i / HX)]);
+}
+
+// grid: (nz planes, cube)
+template <int L>
+__global__ __launch_bounds__(256) void k_irfft_yx(Batch<XInv<float>> bt) {
+    using V = float2;
+    using PS = PlaneShape<L>;
+    constexpr int HX = PS::HX, NLX = PS::NLX, LD = Shape<L>::LD;
+    const XInv<float> a = bt.a[blockIdx.y];
+    __shared__ V col[HX * LD];
+    __shared__ V xb[NLX * LD];
+    __shared__ V tw_s[L];
+    __shared__ XInvLine meta[NLX];
+    const TwPrefetch<L, V> twp(a.tw);
+    const int z = blockIdx.x;
+    const V *plane = a.spec + (size_t)z * a.py * a.hxp;
+    const long hxp = a.hxp;
+    // column FFTs of the plane (k < hx), inverse sign, result kept in LDS
+    auto firstg = [&](int l, int p0, int off) -> V { return plane[(size_t)(p0 + off) * hxp + l]; };
+    run_from<L, HX, true, false, 0>(1.0f, tw_s, col, firstg, LdsWr<L, HX, V>{col}, [&] { twp.store(tw_s); });
+    __syncthreads();
+    const int npairs = (a.ny + 1) / 2;
+    for (int c0 = 0; c0 < npairs; c0 += NLX) {
+        for (int l = threadIdx.x; l < NLX; l += Shape<L>::T) {
+            XInvLine m;
+            const int g = c0 + l;
+            m.valid = g < npairs;
+            const int y0 = 2 * (m.valid ? g : 0);
+            m.pair = y0 + 1 < a.ny;
+            m.so = y0;                          // plane rows of y0 and y0+1 (y0+1 < L: L is even)
+            m.sb = y0 + 1;
+            m.oo = ((long)z * a.ny + y0) * a.nx;
+            meta[l] = m;
+        }
+        __syncthreads();
+        auto first = [&](int l, int p0, int off) -> V {
+            const XInvLine &m = meta[l];
+            const int x = p0 + off;
+            const int k = x < HX ? x : L - x;
+            const V A = *plane_col<L>(col, k, (int)m.so);
+            const V B = *plane_col<L>(col, k, (int)m.sb);
+            return x < HX ? make_float2(A.x - B.y, A.y + B.x) : make_float2(A.x + B.y, B.x - A.y);
+        };
+        auto last = [&](int l, int p0, int off, V v) {
+            const XInvLine &m = meta[l];
+            const int x = p0 + off;
+            if (x >= a.nx || !m.valid) return;
+            a.out[m.oo + x] = epilogue<float>(a.epi, v.x * a.invn);
+            if (m.pair) a.out[m.oo + a.nx + x] = epilogue<float>(a.epi, v.y * a.invn);
+        };
+        run_from<L, NLX, false, false, 0>(1.0f, tw_s, xb, first, last);
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ dispatch over L
 template <class F>
 bool dispatch(int L, F f) {
@@ -905,6 +1077,33 @@ struct Launch {
             Batch<ZFused<P>> b{};
             for (int j = 0; j < m; j++) b.a[j] = a[i + j];
             fyg_launch(s, "zfused", k_zfused<L, P>, dim3(g, m), dim3(256), 0, b);
+        }
+    }
+    // fused X+Y passes of whole planes (fp32, px == py == L in {64, 96, 128})
+    static void rfft_xy(fyg_stream *s, const XFwd<P> *a, int n, bool u8) {
+        if constexpr (PlaneShape<L>::OK && std::is_same<P, float>::value) {
+            for (int i = 0; i < n; i += FYG_BATCH) {
+                int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+                Batch<XFwd<P>> b{};
+                for (int j = 0; j < m; j++) b.a[j] = a[i + j];
+                dim3 g((unsigned)a[0].pz, (unsigned)m);
+                if (u8 && a[0].pro.dering)
+                    fyg_launch(s, "rfft_xy_u8", k_rfft_xy<L, 2>, g, dim3(256), 0, b);
+                else if (u8)
+                    fyg_launch(s, "rfft_xy_u8", k_rfft_xy<L, 1>, g, dim3(256), 0, b);
+                else
+                    fyg_launch(s, "rfft_xy", k_rfft_xy<L, 0>, g, dim3(256), 0, b);
+            }
+        }
+    }
+    static void irfft_yx(fyg_stream *s, const XInv<P> *a, int n) {
+        if constexpr (PlaneShape<L>::OK && std::is_same<P, float>::value) {
+            for (int i = 0; i < n; i += FYG_BATCH) {
+                int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+                Batch<XInv<P>> b{};
+                for (int j = 0; j < m; j++) b.a[j] = a[i + j];
+                fyg_launch(s, "irfft_yx", k_irfft_yx<L>, dim3((unsigned)a[0].nz, (unsigned)m), dim3(256), 0, b);
+            }
         }
     }
     static void irfft_x(fyg_stream *s, const XInv<P> *a, int n) {
@@ -986,11 +1185,21 @@ int deconv_batch(const fyg_deconv_args *a, int n, fyg_stream *s) {
         }
     }
     const bool u8 = h.in_u8 != 0;
-    dispatch(h.px, [&](auto c) { Launch<decltype(c)::value, P>::rfft_x(s, xf.data(), n, u8); });
-    dispatch(h.py, [&](auto c) { Launch<decltype(c)::value, P>::cols(s, yf.data(), n, h.pz, "cfft_y_fwd"); });
+    const bool planes = std::is_same<P, float>::value && h.px == h.py && (h.px == 64 || h.px == 96) &&
+                        fyg_fft_plane_mode() != 0;
+    if (planes)
+        dispatch(h.px, [&](auto c) { Launch<decltype(c)::value, P>::rfft_xy(s, xf.data(), n, u8); });
+    else {
+        dispatch(h.px, [&](auto c) { Launch<decltype(c)::value, P>::rfft_x(s, xf.data(), n, u8); });
+        dispatch(h.py, [&](auto c) { Launch<decltype(c)::value, P>::cols(s, yf.data(), n, h.pz, "cfft_y_fwd"); });
+    }
     dispatch(h.pz, [&](auto c) { Launch<decltype(c)::value, P>::zfused(s, zf.data(), n); });
-    dispatch(h.py, [&](auto c) { Launch<decltype(c)::value, P>::cols(s, yi.data(), n, h.nz, "cfft_y_inv"); });
-    dispatch(h.px, [&](auto c) { Launch<decltype(c)::value, P>::irfft_x(s, xi.data(), n); });
+    if (planes)
+        dispatch(h.px, [&](auto c) { Launch<decltype(c)::value, P>::irfft_yx(s, xi.data(), n); });
+    else {
+        dispatch(h.py, [&](auto c) { Launch<decltype(c)::value, P>::cols(s, yi.data(), n, h.nz, "cfft_y_inv"); });
+        dispatch(h.px, [&](auto c) { Launch<decltype(c)::value, P>::irfft_x(s, xi.data(), n); });
+    }
     return fyg_last_error();
 }
 

@@ -343,3 +343,5 @@ long fy_gpu_table_cache(long *stats, long limit) {
     if (stats) fyg_table_stats(stats);
     return (long)old;
 }
+
+int fy_gpu_fft_planes(int mode) { return fyg_fft_plane_set(mode); }

@@ -284,6 +284,15 @@ void fy_gpu_tuning_clear(void);
 
 int fy_gpu_sync(void *stream);
 
+/* pass structure of the deconvolution for cubes whose padded X and Y axes
+ * are one length of 64 or 96 (fp32): mode 1 (default) transforms whole z
+ * planes along X and Y in one kernel each way, so the half spectrum makes one
+ * HBM round trip fewer per direction; mode 0 runs the separate X and Y passes.
+ * Both compute the same transforms; the fp32 rounding may differ in the last
+ * bits (relative L2 ~1e-7 between them).  Returns the previous mode.  No
+ * reference counterpart. */
+int fy_gpu_fft_planes(int mode);
+
 /* the process-wide cache of device tables (transfer tables of captured and
  * large deconvolutions, ring and z-drift blobs): stats (may be NULL) receives
  * 7 counts -- entries, evictable entries, entries held by queued work or a

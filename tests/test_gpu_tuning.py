@@ -103,3 +103,59 @@ def test_tuned_shapes_persist(tmp_path):
     finally:
         lib.fy_gpu_tuning(old)
     assert np.array_equal(first, again)
+
+
+@pytest.mark.parametrize("shape,rings", [((64, 64, 64), False), ((96, 96, 96), False), ((40, 60, 64), True),
+                                         ((70, 90, 96), True), ((60, 64, 64), False)])
+def test_fft_planes_bit_identical(shape, rings):
+    """The fused X+Y plane kernels (px == py in {64, 96}) against the
+    separate X and Y passes: the same transforms, compiled into other kernels,
+    so fp32 rounding may differ where the compiler fuses a multiply and an add
+    differently.  Bars: float tile and fp32-input deconvolution
+    (fy_gpu_deconvolve) within relative L2 1e-6 of each other, u8 output
+    within +-1 on < 0.1 % of voxels."""
+    from test_gpu_batch import ring_model
+    lib = F.lib()
+    rng = np.random.default_rng(sum(shape))
+    if rings:
+        ring, u8 = ring_model(shape, rng)
+        assert ring.have_rings
+    else:
+        ring = None
+        u8 = phantom.noisy_u8(phantom.forward(phantom.sheets(shape, rng, 6), **phantom.CONFIG1_PHYS), rng)
+    P = F.CubeParams()
+    P.phys = F.physics(delta_beta=400.0, energy_kev=100.0, distance_mm=900.0, pixel_um=4.3, unsharp_sigma=0.8,
+                       unsharp_coeff=0.5)
+    P.deconv, P.deconv_param, P.deconv_cap, P.fz2_scale = 1, 0.015, 8.0, 1.2 ** 2
+    P.window_lo, P.window_hi = -0.1, 1.2
+    P.guided_eps, P.guided_radius, P.guided_s = 2e-3, 2, 2
+    if ring is not None:
+        P.dering = C.pointer(ring)
+    d_in = torch.from_numpy(u8).cuda()
+    f_in = (d_in.float() / 255.0).contiguous()
+    outs = {}
+    old = lib.fy_gpu_fft_planes(1)
+    try:
+        for mode in (1, 0):
+            lib.fy_gpu_fft_planes(mode)
+            d_q = torch.zeros_like(d_in)
+            d_f = torch.zeros(shape, dtype=torch.float32, device="cuda")
+            d_d = torch.zeros(shape, dtype=torch.float32, device="cuda")
+            z = (0, 0, 0)
+            c = F.Cube(d_in.data_ptr(), *z, *shape, *z, *shape, d_q.data_ptr(), d_f.data_ptr(), C.pointer(P))
+            print(f"fft planes mode {mode}: {shape} rings={rings}", flush=True)
+            assert lib.fy_gpu_process_cubes(C.byref(c), 1, None) == 0, lib.fy_gpu_last_error()
+            assert lib.fy_gpu_deconvolve(C.c_void_p(f_in.data_ptr()), C.c_void_p(d_d.data_ptr()), *shape,
+                                         C.byref(P.phys), 0.015, None) == 0
+            assert lib.fy_gpu_sync(None) == 0
+            outs[mode] = (d_q.cpu().numpy(), d_f.cpu().numpy(), d_d.cpu().numpy())
+    finally:
+        lib.fy_gpu_fft_planes(old)
+    (q1, f1, d1), (q0, f0, d0) = outs[1], outs[0]
+    for a, b in ((f1, f0), (d1, d0)):
+        r = np.linalg.norm((a - b).astype(np.float64)) / np.linalg.norm(b.astype(np.float64))
+        print(f"  relL2 planes vs passes {r:.2e}", flush=True)
+        assert r <= 1e-6
+    du = np.abs(q1.astype(int) - q0.astype(int))
+    assert du.max() <= 1 and (du > 0).mean() < 1e-3
+    assert q1.any()
