@@ -78,6 +78,19 @@ def test_downscale_device_256_and_slabs(fy, orc):
         assert np.array_equal(o.cpu().numpy(), ds_oracle(orc, v, 1, 0.5))
 
 
+@pytest.mark.parametrize("shape,slab", [((37, 40, 52), 4096), ((30, 24, 24), 1), ((9, 14, 17), 1000)])
+def test_downscale_host_slabs(fy, orc, shape, slab, monkeypatch):
+    """fy_downscale2x works through slabs of output planes (FYSICS_DS_SLAB_BYTES
+    of input per slab, at least two planes): small slabs -- odd plane counts,
+    a last slab with one parent plane -- give the oracle's bytes exactly."""
+    monkeypatch.setenv("FYSICS_DS_SLAB_BYTES", str(slab))
+    print(f"downscale slabs shape={shape} slab={slab}", flush=True)
+    rng = np.random.default_rng(sum(shape) + slab)
+    v = np.where(rng.random(shape) < 0.3, 0, rng.integers(0, 256, shape)).astype(np.uint8)
+    for method, alpha in ((0, 0.0), (1, 0.5)):
+        assert np.array_equal(ds_host(fy, v, method, alpha), ds_oracle(orc, v, method, alpha))
+
+
 PC_ATOL = {0: 5e-2, 1: 1e-4, 2: 5e-2, 3: 1e-4, 4: 1e-4, 5: 0.0}
 
 
