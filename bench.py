@@ -159,12 +159,18 @@ def main():
     ap.add_argument("--no-export", action="store_true", help="skip the zarr -> GPU -> zarr export timing")
     ap.add_argument("--served-requests", type=int, default=20000)
     ap.add_argument("--served-rate", type=float, default=400.0)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only run the 96-cube batched measurement and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference point that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters into HBM bytes)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.batched_only:
+        import fysics_amd as F
+        print(json.dumps(batched_bench(F, reps=args.steps)))
         return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
