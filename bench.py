@@ -170,7 +170,9 @@ def main():
         return
     if args.batched_only:
         import fysics_amd as F
-        print(json.dumps(batched_bench(F, reps=args.steps)))
+        out = batched_bench(F, reps=args.steps)
+        out["fft_radix_plans"] = [l for l in F.tuning_report().splitlines() if l.startswith("fft ")]
+        print(json.dumps(out))
         return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
@@ -320,6 +322,7 @@ def main():
         "lod_phasecorr": lod,
         "export": export,
         "cpu_baseline": cpu,
+        "fft_radix_plans": [l for l in F.tuning_report().splitlines() if l.startswith("fft ")],
         "workload_gen_s": round(gen_s, 1),
     }
     print(json.dumps(line))

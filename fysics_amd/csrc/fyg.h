@@ -187,6 +187,8 @@ typedef struct {
                                         anything reads it -- folded into the first ring-image launch */
     int step_nflag;                  /* flags that follow *step_gen */
     const void *twx, *twy, *twz;     /* optional twiddle tables for px, py, pz (NULL: the cache's) */
+    int no_tune;                     /* never time radix plans here (callers that may not allocate):
+                                        lengths not decided yet take plan 0 */
 } fyg_deconv_args;
 int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s);
 /* n deconvolutions of one shape (same nz..hxp, in_u8 and pro.dering) */
@@ -212,6 +214,22 @@ int fyg_noalloc_deconv(const float *in, float *out, int nz, int ny, int nx, int 
  * bits); returns the previous mode */
 int fyg_fft_plane_mode(void);
 int fyg_fft_plane_set(int mode);
+
+/* Radix plans of the fp32 deconvolution passes.  Lengths 64..512 with two
+ * plans (radix-16 stages, or radix-8 stages) take one per pass kind (X, Y, Z,
+ * fused XY planes), decided once per process and length: mode 0 plan 0
+ * everywhere, 2 plan 1 everywhere, 1 (default) time both on a synthetic
+ * problem of the length the first time a deconvolution that may tune meets it
+ * (not under capture, not no_tune) or take the tuning file's choice.
+ * Returns the previous mode. */
+int fyg_fft_radix_mode(int mode);
+void fyg_fft_radix_clear(void);
+/* "fft\t<kind>\t<length>\t<plan>" lines of every decided (kind, length) */
+int fyg_fft_radix_lines(char *buf, int len);
+/* one such line (without "fft\t"): 0 ok, 1 malformed */
+int fyg_fft_radix_load_line(const char *line);
+/* human-readable decisions with the timings they came from */
+int fyg_fft_radix_report(char *buf, int len);
 
 /* complex FFT of split re/im device arrays (fy_fft1d/fy_fft3d) */
 int fyg_fft3d_run(float *d_re, float *d_im, int nz, int ny, int nx, int sign,

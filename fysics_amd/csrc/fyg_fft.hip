@@ -2,6 +2,12 @@
 // (fyg_fft_engine.hpp; deconv_radial, paganin.c:152-293) and the complex FFTs of
 // fft.c (fy_fft1d / fy_fft3d, fft.c:170-228) behind the C ABI of fyg.h.
 #include <atomic>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
 
 #include "fyg_fft_engine.hpp"
 
@@ -77,12 +83,217 @@ extern "C" size_t fyg_deconv_spec_bytes64(int pz, int py, int px) {
     return sizeof(double2) * (size_t)pz * py * fyg_deconv_hxp(px);
 }
 
+// ---- radix plans of the fp32 deconvolution passes.  A (kind, length) with two
+// plans is decided once per process: by the mode, by a tuning file, or by
+// timing both plans of that pass on a synthetic problem of about 16 M voxels
+// at that length (zero data: the time of a pass does not depend on the values).
+// The choice changes rounding (both plans are exact factorizations), so it is
+// made before the first result of the length and kept.
+namespace {
+
+enum RadixKind { RK_X, RK_Y, RK_Z, RK_XY, RK_N };
+const char *const kRadixKind[RK_N] = {"x", "y", "z", "xy"};
+
+struct RadixRec {
+    int pv = 0;
+    float us[2] = {0.0f, 0.0f};
+    bool loaded = false;
+};
+
+std::mutex g_rad_mu;
+std::map<std::pair<int, int>, RadixRec> g_rad;
+std::atomic<int> g_rad_mode{1};
+
+// the supported FFT length nearest (in ratio) to x
+int nearest_length(double x) {
+    int best = 16;
+    double bd = 1e30;
+    for (int L = 16; L <= 1024; L++) {
+        if (!fyg_fft_supported(L)) continue;
+        double d = std::fabs(std::log((double)L / x));
+        if (d < bd) {
+            bd = d;
+            best = L;
+        }
+    }
+    return best;
+}
+
+float elapsed_us(hipEvent_t a, hipEvent_t b) {
+    float ms = 1e30f;
+    if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) return 1e30f;
+    return 1000.0f * ms;
+}
+
+// both plans of one pass kind at length L: us[pv] = best of three timed runs
+// after one untimed run (which also loads the kernels' code).  Returns 0 when
+// timed, 1 when the synthetic problem could not be set up.
+int time_plans(int kind, int L, fyg_stream *s, float us[2]) {
+    int nz, ny, nx;
+    if (kind == RK_XY) {
+        nz = (1 << 24) / (L * L);
+        ny = nx = L;
+    } else {
+        const int M = nearest_length(std::sqrt((double)(1 << 24) / L));
+        nz = ny = nx = M;
+        if (kind == RK_X) nx = L;
+        if (kind == RK_Y) ny = L;
+        if (kind == RK_Z) nz = L;
+    }
+    const int hx = nx / 2 + 1, hxp = fyg_deconv_hxp(nx);
+    const size_t nvox = (size_t)nz * ny * nx;
+    const size_t b_in = sizeof(float) * nvox, b_spec = fyg_deconv_spec_bytes(nz, ny, nx);
+    const int nf = nz > ny ? (nz > hx ? nz : hx) : (ny > hx ? ny : hx);
+    const size_t b_lut = sizeof(float) * 4100, b_f2 = sizeof(double) * (size_t)nf;
+    const size_t total = 2 * b_in + b_spec + b_lut + b_f2 + 4 * 256;
+    char *base = nullptr;
+    if (hipMalloc((void **)&base, total) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    char *p = base;
+    float *in = (float *)p;
+    p += up(b_in);
+    float *out = (float *)p;
+    p += up(b_in);
+    void *spec = p;
+    p += up(b_spec);
+    float *lut = (float *)p;
+    p += up(b_lut);
+    double *f2 = (double *)p;
+    int rc = hipMemsetAsync(base, 0, total, s->s) != hipSuccess;
+    fyg_deconv_args a;
+    memset(&a, 0, sizeof a);
+    a.nz = nz; a.ny = ny; a.nx = nx;
+    a.pz = nz; a.py = ny; a.px = nx; a.hx = hx; a.hxp = hxp;
+    a.lut = lut; a.us = 1.0; a.fz2 = f2; a.fy2 = f2; a.fx2 = f2;
+    a.in = in; a.out = out; a.spec = spec;
+    Passes<float> ps;
+    rc |= ps.build(&a, 1);
+    hipEvent_t e[4] = {};
+    for (int i = 0; i < 4 && !rc; i++) rc |= hipEventCreate(&e[i]) != hipSuccess;
+    for (int pv = 0; pv < 2 && !rc; pv++) {
+        auto run = [&] {
+            switch (kind) {
+            case RK_X: ps.x_fwd(s, pv); ps.x_inv(s, pv); break;
+            case RK_Y: ps.y_fwd(s, pv); ps.y_inv(s, pv); break;
+            case RK_Z: ps.z(s, pv); break;
+            default: ps.xy_fwd(s, pv); ps.yx_inv(s, pv); break;
+            }
+        };
+        run();
+        float best = 1e30f;
+        for (int r = 0; r < 3; r++) {
+            (void)hipEventRecord(e[0], s->s);
+            run();
+            (void)hipEventRecord(e[1], s->s);
+            float t = elapsed_us(e[0], e[1]);
+            if (t < best) best = t;
+        }
+        us[pv] = best;
+    }
+    rc |= fyg_last_error();
+    (void)hipStreamSynchronize(s->s);
+    for (int i = 0; i < 4; i++)
+        if (e[i]) (void)hipEventDestroy(e[i]);
+    (void)hipFree(base);
+    (void)hipGetLastError();
+    return rc;
+}
+
+// plan variant of (kind, L) for a deconvolution on s
+int radix_plan(int kind, int L, fyg_stream *s, bool may_tune) {
+    if (!plan_alt(L)) return 0;
+    const int mode = g_rad_mode.load(std::memory_order_relaxed);
+    if (mode == 0) return 0;
+    if (mode == 2) return 1;
+    std::lock_guard<std::mutex> lk(g_rad_mu);
+    auto it = g_rad.find({kind, L});
+    if (it != g_rad.end()) return it->second.pv;
+    if (!may_tune) return 0;
+    RadixRec r;
+    if (time_plans(kind, L, s, r.us) == 0)
+        r.pv = r.us[1] < 0.98f * r.us[0] ? 1 : 0;      // plan 1 only when clearly faster
+    else
+        r.us[0] = r.us[1] = -1.0f;                      // not timed: plan 0
+    g_rad[{kind, L}] = r;
+    return r.pv;
+}
+
+}  // namespace
+
+extern "C" int fyg_fft_radix_mode(int mode) { return g_rad_mode.exchange(mode < 0 ? 1 : mode > 2 ? 2 : mode); }
+
+extern "C" void fyg_fft_radix_clear(void) {
+    std::lock_guard<std::mutex> lk(g_rad_mu);
+    g_rad.clear();
+}
+
+extern "C" int fyg_fft_radix_lines(char *buf, int len) {
+    std::lock_guard<std::mutex> lk(g_rad_mu);
+    std::string out;
+    char line[96];
+    for (auto &kv : g_rad) {
+        snprintf(line, sizeof line, "fft\t%s\t%d\t%d\n", kRadixKind[kv.first.first], kv.first.second,
+                 kv.second.pv);
+        out += line;
+    }
+    if (buf && len > 0) snprintf(buf, (size_t)len, "%s", out.c_str());
+    return (int)out.size();
+}
+
+extern "C" int fyg_fft_radix_load_line(const char *line) {
+    char kind[8];
+    int L = 0, pv = -1;
+    if (sscanf(line, "%7s %d %d", kind, &L, &pv) != 3) return 1;
+    int k = 0;
+    while (k < RK_N && strcmp(kind, kRadixKind[k])) k++;
+    if (k == RK_N || !plan_alt(L) || pv < 0 || pv > 1) return 1;
+    RadixRec r;
+    r.pv = pv;
+    r.loaded = true;
+    std::lock_guard<std::mutex> lk(g_rad_mu);
+    g_rad[{k, L}] = r;
+    return 0;
+}
+
+extern "C" int fyg_fft_radix_report(char *buf, int len) {
+    std::lock_guard<std::mutex> lk(g_rad_mu);
+    std::string out;
+    char line[160];
+    for (auto &kv : g_rad) {
+        const RadixRec &r = kv.second;
+        const char *k = kRadixKind[kv.first.first];
+        const int L = kv.first.second;
+        if (r.loaded)
+            snprintf(line, sizeof line, "fft %s %d: plan %d (from a tuning file)\n", k, L, r.pv);
+        else if (r.us[0] < 0)
+            snprintf(line, sizeof line, "fft %s %d: plan 0 (not timed: no memory for the problem)\n", k, L);
+        else
+            snprintf(line, sizeof line, "fft %s %d: plan %d (radix-16 %.1f us, radix-8 %.1f us)\n", k, L, r.pv,
+                     r.us[0], r.us[1]);
+        out += line;
+    }
+    if (buf && len > 0) snprintf(buf, (size_t)len, "%s", out.c_str());
+    return (int)out.size();
+}
+
 extern "C" int fyg_deconv_run_batch(const fyg_deconv_args *a, int n, fyg_stream *s) {
     if (n < 1) return 0;
     const fyg_deconv_args &h = a[0];
     if (!fyg_fft_supported(h.px) || !fyg_fft_supported(h.py) || !fyg_fft_supported(h.pz)) return 1;
     if (h.f64) return fyg_deconv_run_batch_f64(a, n, s);
-    return deconv_batch<float>(a, n, s);
+    const bool may_tune = s && !s->capturing && !h.no_tune;
+    PassPlans pp;
+    if (use_planes<float>(h.px, h.py))
+        pp.xy = radix_plan(RK_XY, h.px, s, may_tune);
+    else {
+        pp.x = radix_plan(RK_X, h.px, s, may_tune);
+        pp.y = radix_plan(RK_Y, h.py, s, may_tune);
+    }
+    pp.z = radix_plan(RK_Z, h.pz, s, may_tune);
+    return deconv_batch<float>(a, n, s, pp);
 }
 
 extern "C" int fyg_deconv_run(const fyg_deconv_args *a, fyg_stream *s) { return fyg_deconv_run_batch(a, 1, s); }

@@ -73,7 +73,13 @@ constexpr int ilog2c(int x) {
     return l;
 }
 
-constexpr Radices make_plan(int L) {
+// The radix plan of a length: a factor 3 first when there is one, then
+//   pv 0: radix-16 stages behind one smaller radix (2, 4, 8, or 4 and 8 where
+//         a lone 2 would be left);
+//   pv 1: radix-8 stages behind one radix 2 or 4.
+// Both are exact factorizations; they round differently, so one length's plan
+// is fixed before its first use in a process.
+constexpr Radices make_plan(int L, int pv = 0) {
     Radices p{};
     p.n = 0;
     int m = L;
@@ -82,23 +88,39 @@ constexpr Radices make_plan(int L) {
         m /= 3;
     }
     int lg = ilog2c(m);
-    int q = lg / 4, rem = lg % 4;
-    if (rem == 1 && q >= 1) {
-        p.r[p.n++] = 4;
-        p.r[p.n++] = 8;
-        q -= 1;
-    } else if (rem) {
-        p.r[p.n++] = 1 << rem;
+    if (pv == 1) {
+        if (lg % 3) p.r[p.n++] = 1 << (lg % 3);
+        for (int i = 0; i < lg / 3; i++) p.r[p.n++] = 8;
+    } else {
+        int q = lg / 4, rem = lg % 4;
+        if (rem == 1 && q >= 1) {
+            p.r[p.n++] = 4;
+            p.r[p.n++] = 8;
+            q -= 1;
+        } else if (rem) {
+            p.r[p.n++] = 1 << rem;
+        }
+        for (int i = 0; i < q; i++) p.r[p.n++] = 16;
     }
-    for (int i = 0; i < q; i++) p.r[p.n++] = 16;
     if (p.n == 0) p.r[p.n++] = 1;
     return p;
 }
 
+constexpr bool same_plan(const Radices &a, const Radices &b) {
+    if (a.n != b.n) return false;
+    for (int i = 0; i < a.n; i++)
+        if (a.r[i] != b.r[i]) return false;
+    return true;
+}
+
+// lengths with a second plan to choose from (fp32 only): the cube sizes of the
+// batched and served forms; elsewhere pv 1 means pv 0
+constexpr bool plan_alt(int L) { return L >= 64 && L <= 512 && !same_plan(make_plan(L, 0), make_plan(L, 1)); }
+
 // stage s of the forward plan (REV=false) or of the reversed plan (REV=true)
-template <int L, bool REV>
+template <int L, bool REV, int PV = 0>
 struct Plan {
-    static constexpr Radices P = make_plan(L);
+    static constexpr Radices P = make_plan(L, PV);
     static constexpr int n = P.n;
     static constexpr int R(int s) { return REV ? P.r[P.n - 1 - s] : P.r[s]; }
     static constexpr int NS(int s) {
@@ -333,11 +355,11 @@ struct NoHook {
 // first stage's stores and before its barrier: stage 0 applies no twiddles
 // (span 1), so the kernels store their twiddle table there and its load does
 // not delay the first data loads.
-template <int L, int NL, bool CM, bool REV, int S, int END = -1, class V, class First, class Last,
+template <int L, int NL, bool CM, bool REV, int S, int END = -1, int PV = 0, class V, class First, class Last,
           class Hook = NoHook>
 __device__ __forceinline__ void run_from(scal_t<V> sg, const V *tw, V *buf, First first, Last last,
                                          Hook hook = Hook{}) {
-    using PL = Plan<L, REV>;
+    using PL = Plan<L, REV, PV>;
     constexpr int E = END < 0 ? PL::n : END;
     constexpr int T = Shape<L>::T;
     if constexpr (S < E) {
@@ -351,7 +373,7 @@ __device__ __forceinline__ void run_from(scal_t<V> sg, const V *tw, V *buf, Firs
             stage<L, NL, T, CM, R, NS, false>(sg, tw, first, LdsWr<L, NL, V>{buf});
             hook();
             __syncthreads();
-            run_from<L, NL, CM, REV, S + 1, END>(sg, tw, buf, first, last);
+            run_from<L, NL, CM, REV, S + 1, END, PV>(sg, tw, buf, first, last);
         } else if constexpr (FINAL) {
             // a final stage that stores back into the same LDS lines (a partial
             // plan, END < n) needs the barrier between its read and write phases
@@ -360,7 +382,7 @@ __device__ __forceinline__ void run_from(scal_t<V> sg, const V *tw, V *buf, Firs
         } else {
             stage<L, NL, T, CM, R, NS, true>(sg, tw, LdsRd<L, NL, V>{buf}, LdsWr<L, NL, V>{buf});
             __syncthreads();
-            run_from<L, NL, CM, REV, S + 1, END>(sg, tw, buf, first, last);
+            run_from<L, NL, CM, REV, S + 1, END, PV>(sg, tw, buf, first, last);
         }
     }
 }
@@ -486,7 +508,7 @@ struct XLine {
 };
 
 // MODE 0: fp32 input; 1: u8 input with the prologue; 2: as 1 plus rings
-template <int L, int MODE, class P>
+template <int L, int MODE, class P, int PV = 0>
 __global__ __launch_bounds__(256) void k_rfft_x(Batch<XFwd<P>> bt) {
     using V = typename Cx<P>::V;
     const XFwd<P> a = bt.a[blockIdx.y];             // into registers once
@@ -547,7 +569,7 @@ __global__ __launch_bounds__(256) void k_rfft_x(Batch<XFwd<P>> bt) {
         // as it would separate zeros, and B is not stored.
         return Cx<P>::make((P)va, (P)vb);
     };
-    run_from<L, NL, false, false, 0>((P)-1, tw_s, buf, first, LdsWr<L, NL, V>{buf}, [&] { twp.store(tw_s); });
+    run_from<L, NL, false, false, 0, -1, PV>((P)-1, tw_s, buf, first, LdsWr<L, NL, V>{buf}, [&] { twp.store(tw_s); });
     __syncthreads();
     if (U8 && a.flag) {
         if (__syncthreads_or(any) && threadIdx.x == 0) *a.flag = *a.gen;
@@ -593,7 +615,7 @@ __device__ __forceinline__ long pair_block(long h, long T) {
     return (c << 4) + 2 * (i & 7) + (i >> 3);
 }
 
-template <int L, class P>
+template <int L, class P, int PV = 0>
 __global__ __launch_bounds__(256) void k_cfft_cols(Batch<Cols<P>> bt) {
     using V = typename Cx<P>::V;
     constexpr bool PAIR = Shape<L, P>::NL_CM * 2 * sizeof(P) < 128;
@@ -625,9 +647,9 @@ __global__ __launch_bounds__(256) void k_cfft_cols(Batch<Cols<P>> bt) {
     };
     auto hook = [&] { twp.store(tw_s); };
     if (a.sg < 0)
-        run_from<L, NL, true, false, 0>((P)-1, tw_s, buf, first, last, hook);
+        run_from<L, NL, true, false, 0, -1, PV>((P)-1, tw_s, buf, first, last, hook);
     else
-        run_from<L, NL, true, false, 0>((P)1, tw_s, buf, first, last, hook);
+        run_from<L, NL, true, false, 0, -1, PV>((P)1, tw_s, buf, first, last, hook);
 }
 
 // the LUT holds fp32 weights for P = float (the reference table) and the same
@@ -645,14 +667,14 @@ struct ZFused {
 };
 
 // forward Z (plan order), weight, inverse Z (reversed plan), all in one kernel
-template <int L, class P>
+template <int L, class P, int PV = 0>
 __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
     using V = typename Cx<P>::V;
     const ZFused<P> &a = bt.a[blockIdx.y];
     const long bx = Shape<L, P>::NL_ZF * 2 * sizeof(P) < 128 ? pair_block(blockIdx.x, gridDim.x) : (long)blockIdx.x;
     constexpr int NL = Shape<L, P>::NL_ZF;
     constexpr int T = Shape<L>::T;
-    using PF = Plan<L, false>;
+    using PF = Plan<L, false, PV>;
     constexpr int n = PF::n;
     constexpr int RL = PF::R(n - 1);          // last forward radix == first inverse radix
     constexpr int NSL = PF::NS(n - 1);
@@ -724,7 +746,7 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
     // the spectrum makes a single LDS round trip around them.
     V v[BPT][RL];
     if constexpr (n > 1) {
-        run_from<L, NL, true, false, 0, n - 1>((P)-1, tw_s, buf, first, LdsWr<L, NL, V>{buf});
+        run_from<L, NL, true, false, 0, n - 1, PV>((P)-1, tw_s, buf, first, LdsWr<L, NL, V>{buf});
         __syncthreads();
     }
 #pragma unroll
@@ -770,7 +792,7 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
             }
         }
         __syncthreads();
-        run_from<L, NL, true, true, 1>((P)1, tw_s, buf, first, last);
+        run_from<L, NL, true, true, 1, -1, PV>((P)1, tw_s, buf, first, last);
     } else {
 #pragma unroll
         for (int b = 0; b < BPT; b++) {
@@ -805,7 +827,7 @@ struct XInvLine {
     int valid;
 };
 
-template <int L, class P>
+template <int L, class P, int PV = 0>
 __global__ __launch_bounds__(256) void k_irfft_x(Batch<XInv<P>> bt) {
     using V = typename Cx<P>::V;
     const XInv<P> a = bt.a[blockIdx.y];             // into registers once
@@ -849,7 +871,7 @@ __global__ __launch_bounds__(256) void k_irfft_x(Batch<XInv<P>> bt) {
         a.out[m.oo + x] = epilogue<P>(a.epi, v.x * a.invn);
         if (m.pair) a.out[m.oo + a.nx + x] = epilogue<P>(a.epi, v.y * a.invn);
     };
-    run_from<L, NL, false, false, 0>((P)1, tw_s, buf, first, last, [&] { twp.store(tw_s); });
+    run_from<L, NL, false, false, 0, -1, PV>((P)1, tw_s, buf, first, last, [&] { twp.store(tw_s); });
 }
 
 unsigned grid1d(size_t n) {
@@ -886,7 +908,7 @@ __device__ __forceinline__ float2 *plane_col(float2 *col, int k, int y) {
 }
 
 // grid: (pz planes, cube)
-template <int L, int MODE>
+template <int L, int MODE, int PV = 0>
 __global__ __launch_bounds__(256) void k_rfft_xy(Batch<XFwd<float>> bt) {
     using V = float2;
     using PS = PlaneShape<L>;
@@ -944,7 +966,7 @@ __global__ __launch_bounds__(256) void k_rfft_xy(Batch<XFwd<float>> bt) {
             return make_float2(va, vb);
         };
         const bool c_first = c0 == 0;
-        run_from<L, NLX, false, false, 0>(-1.0f, tw_s, xb, first, LdsWr<L, NLX, V>{xb}, [&] {
+        run_from<L, NLX, false, false, 0, -1, PV>(-1.0f, tw_s, xb, first, LdsWr<L, NLX, V>{xb}, [&] {
             if (c_first) twp.store(tw_s);
         });
         __syncthreads();
@@ -965,19 +987,19 @@ __global__ __launch_bounds__(256) void k_rfft_xy(Batch<XFwd<float>> bt) {
     }
     // column FFTs along y, in place: the first stage with a barrier between its
     // reads and writes, then the remaining stages; the last one writes the plane
-    using PL = Plan<L, false>;
+    using PL = Plan<L, false, PV>;
     stage<L, HX, Shape<L>::T, true, PL::R(0), 1, true>(-1.0f, tw_s, LdsRd<L, HX, V>{col}, LdsWr<L, HX, V>{col});
     __syncthreads();
     V *plane = a.spec + (size_t)z * a.py * a.hxp;
     const long hxp = a.hxp;
     auto last = [&](int l, int p0, int off, V v) { plane[(size_t)(p0 + off) * hxp + l] = v; };
-    if constexpr (PL::n > 1) run_from<L, HX, true, false, 1>(-1.0f, tw_s, col, LdsRd<L, HX, V>{col}, last);
+    if constexpr (PL::n > 1) run_from<L, HX, true, false, 1, -1, PV>(-1.0f, tw_s, col, LdsRd<L, HX, V>{col}, last);
     else
         for (int i = threadIdx.x; i < HX * L; i += Shape<L>::T) last(i % HX, i / HX, 0, col[(i % HX) * LD + lds_pos(i / HX)]);
 }
 
 // grid: (nz planes, cube)
-template <int L>
+template <int L, int PV = 0>
 __global__ __launch_bounds__(256) void k_irfft_yx(Batch<XInv<float>> bt) {
     using V = float2;
     using PS = PlaneShape<L>;
@@ -993,7 +1015,7 @@ __global__ __launch_bounds__(256) void k_irfft_yx(Batch<XInv<float>> bt) {
     const long hxp = a.hxp;
     // column FFTs of the plane (k < hx), inverse sign, result kept in LDS
     auto firstg = [&](int l, int p0, int off) -> V { return plane[(size_t)(p0 + off) * hxp + l]; };
-    run_from<L, HX, true, false, 0>(1.0f, tw_s, col, firstg, LdsWr<L, HX, V>{col}, [&] { twp.store(tw_s); });
+    run_from<L, HX, true, false, 0, -1, PV>(1.0f, tw_s, col, firstg, LdsWr<L, HX, V>{col}, [&] { twp.store(tw_s); });
     __syncthreads();
     const int npairs = (a.ny + 1) / 2;
     for (int c0 = 0; c0 < npairs; c0 += NLX) {
@@ -1024,7 +1046,7 @@ __global__ __launch_bounds__(256) void k_irfft_yx(Batch<XInv<float>> bt) {
             a.out[m.oo + x] = epilogue<float>(a.epi, v.x * a.invn);
             if (m.pair) a.out[m.oo + a.nx + x] = epilogue<float>(a.epi, v.y * a.invn);
         };
-        run_from<L, NLX, false, false, 0>(1.0f, tw_s, xb, first, last);
+        run_from<L, NLX, false, false, 0, -1, PV>(1.0f, tw_s, xb, first, last);
         __syncthreads();
     }
 }
@@ -1044,76 +1066,108 @@ bool dispatch(int L, F f) {
     }
 }
 
+// f(std::integral_constant<int, PV>) for the plan variant pv of length L:
+// lengths with one plan, and fp64, always take plan 0
+template <int L, class P, class F>
+void with_plan(int pv, F f) {
+    if constexpr (plan_alt(L) && std::is_same<P, float>::value) {
+        if (pv == 1) {
+            f(std::integral_constant<int, 1>{});
+            return;
+        }
+    }
+    f(std::integral_constant<int, 0>{});
+}
+
 template <int L, class P>
 struct Launch {
-    static void rfft_x(fyg_stream *s, const XFwd<P> *a, int n, bool u8) {
+    static void rfft_x(fyg_stream *s, const XFwd<P> *a, int n, bool u8, int pv = 0) {
         unsigned g = (unsigned)((a[0].nlines + Shape<L, P>::NL_RM - 1) / Shape<L, P>::NL_RM);
-        for (int i = 0; i < n; i += FYG_BATCH) {
-            int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
-            Batch<XFwd<P>> b{};
-            for (int j = 0; j < m; j++) b.a[j] = a[i + j];
-            if (u8 && a[0].pro.dering)
-                fyg_launch(s, "rfft_x_u8", k_rfft_x<L, 2, P>, dim3(g, m), dim3(256), 0, b);
-            else if (u8)
-                fyg_launch(s, "rfft_x_u8", k_rfft_x<L, 1, P>, dim3(g, m), dim3(256), 0, b);
-            else
-                fyg_launch(s, "rfft_x", k_rfft_x<L, 0, P>, dim3(g, m), dim3(256), 0, b);
-        }
-    }
-    static void cols(fyg_stream *s, const Cols<P> *a, int n, int nbatch, const char *name) {
-        unsigned g = (unsigned)((a[0].ncols + Shape<L, P>::NL_CM - 1) / Shape<L, P>::NL_CM);
-        for (int i = 0; i < n; i += FYG_BATCH) {
-            int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
-            Batch<Cols<P>> b{};
-            for (int j = 0; j < m; j++) b.a[j] = a[i + j];
-            fyg_launch(s, name, k_cfft_cols<L, P>, dim3(g, nbatch, m), dim3(256), 0, b);
-        }
-    }
-    static void zfused(fyg_stream *s, const ZFused<P> *a, int n) {
-        long nc = (long)a[0].py * a[0].hxp;
-        unsigned g = (unsigned)((nc + Shape<L, P>::NL_ZF - 1) / Shape<L, P>::NL_ZF);
-        for (int i = 0; i < n; i += FYG_BATCH) {
-            int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
-            Batch<ZFused<P>> b{};
-            for (int j = 0; j < m; j++) b.a[j] = a[i + j];
-            fyg_launch(s, "zfused", k_zfused<L, P>, dim3(g, m), dim3(256), 0, b);
-        }
-    }
-    // fused X+Y passes of whole planes (fp32, px == py == L in {64, 96, 128})
-    static void rfft_xy(fyg_stream *s, const XFwd<P> *a, int n, bool u8) {
-        if constexpr (PlaneShape<L>::OK && std::is_same<P, float>::value) {
+        with_plan<L, P>(pv, [&](auto c) {
+            constexpr int PV = decltype(c)::value;
             for (int i = 0; i < n; i += FYG_BATCH) {
                 int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
                 Batch<XFwd<P>> b{};
                 for (int j = 0; j < m; j++) b.a[j] = a[i + j];
-                dim3 g((unsigned)a[0].pz, (unsigned)m);
                 if (u8 && a[0].pro.dering)
-                    fyg_launch(s, "rfft_xy_u8", k_rfft_xy<L, 2>, g, dim3(256), 0, b);
+                    fyg_launch(s, "rfft_x_u8", k_rfft_x<L, 2, P, PV>, dim3(g, m), dim3(256), 0, b);
                 else if (u8)
-                    fyg_launch(s, "rfft_xy_u8", k_rfft_xy<L, 1>, g, dim3(256), 0, b);
+                    fyg_launch(s, "rfft_x_u8", k_rfft_x<L, 1, P, PV>, dim3(g, m), dim3(256), 0, b);
                 else
-                    fyg_launch(s, "rfft_xy", k_rfft_xy<L, 0>, g, dim3(256), 0, b);
+                    fyg_launch(s, "rfft_x", k_rfft_x<L, 0, P, PV>, dim3(g, m), dim3(256), 0, b);
             }
+        });
+    }
+    static void cols(fyg_stream *s, const Cols<P> *a, int n, int nbatch, const char *name, int pv = 0) {
+        unsigned g = (unsigned)((a[0].ncols + Shape<L, P>::NL_CM - 1) / Shape<L, P>::NL_CM);
+        with_plan<L, P>(pv, [&](auto c) {
+            constexpr int PV = decltype(c)::value;
+            for (int i = 0; i < n; i += FYG_BATCH) {
+                int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+                Batch<Cols<P>> b{};
+                for (int j = 0; j < m; j++) b.a[j] = a[i + j];
+                fyg_launch(s, name, k_cfft_cols<L, P, PV>, dim3(g, nbatch, m), dim3(256), 0, b);
+            }
+        });
+    }
+    static void zfused(fyg_stream *s, const ZFused<P> *a, int n, int pv = 0) {
+        long nc = (long)a[0].py * a[0].hxp;
+        unsigned g = (unsigned)((nc + Shape<L, P>::NL_ZF - 1) / Shape<L, P>::NL_ZF);
+        with_plan<L, P>(pv, [&](auto c) {
+            constexpr int PV = decltype(c)::value;
+            for (int i = 0; i < n; i += FYG_BATCH) {
+                int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+                Batch<ZFused<P>> b{};
+                for (int j = 0; j < m; j++) b.a[j] = a[i + j];
+                fyg_launch(s, "zfused", k_zfused<L, P, PV>, dim3(g, m), dim3(256), 0, b);
+            }
+        });
+    }
+    // fused X+Y passes of whole planes (fp32, px == py == L in {64, 96})
+    static void rfft_xy(fyg_stream *s, const XFwd<P> *a, int n, bool u8, int pv = 0) {
+        if constexpr (PlaneShape<L>::OK && std::is_same<P, float>::value) {
+            with_plan<L, P>(pv, [&](auto c) {
+                constexpr int PV = decltype(c)::value;
+                for (int i = 0; i < n; i += FYG_BATCH) {
+                    int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+                    Batch<XFwd<P>> b{};
+                    for (int j = 0; j < m; j++) b.a[j] = a[i + j];
+                    dim3 g((unsigned)a[0].pz, (unsigned)m);
+                    if (u8 && a[0].pro.dering)
+                        fyg_launch(s, "rfft_xy_u8", k_rfft_xy<L, 2, PV>, g, dim3(256), 0, b);
+                    else if (u8)
+                        fyg_launch(s, "rfft_xy_u8", k_rfft_xy<L, 1, PV>, g, dim3(256), 0, b);
+                    else
+                        fyg_launch(s, "rfft_xy", k_rfft_xy<L, 0, PV>, g, dim3(256), 0, b);
+                }
+            });
         }
     }
-    static void irfft_yx(fyg_stream *s, const XInv<P> *a, int n) {
+    static void irfft_yx(fyg_stream *s, const XInv<P> *a, int n, int pv = 0) {
         if constexpr (PlaneShape<L>::OK && std::is_same<P, float>::value) {
+            with_plan<L, P>(pv, [&](auto c) {
+                constexpr int PV = decltype(c)::value;
+                for (int i = 0; i < n; i += FYG_BATCH) {
+                    int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
+                    Batch<XInv<P>> b{};
+                    for (int j = 0; j < m; j++) b.a[j] = a[i + j];
+                    fyg_launch(s, "irfft_yx", k_irfft_yx<L, PV>, dim3((unsigned)a[0].nz, (unsigned)m), dim3(256),
+                               0, b);
+                }
+            });
+        }
+    }
+    static void irfft_x(fyg_stream *s, const XInv<P> *a, int n, int pv = 0) {
+        unsigned g = (unsigned)((a[0].nlines + Shape<L, P>::NL_RM - 1) / Shape<L, P>::NL_RM);
+        with_plan<L, P>(pv, [&](auto c) {
+            constexpr int PV = decltype(c)::value;
             for (int i = 0; i < n; i += FYG_BATCH) {
                 int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
                 Batch<XInv<P>> b{};
                 for (int j = 0; j < m; j++) b.a[j] = a[i + j];
-                fyg_launch(s, "irfft_yx", k_irfft_yx<L>, dim3((unsigned)a[0].nz, (unsigned)m), dim3(256), 0, b);
+                fyg_launch(s, "irfft_x", k_irfft_x<L, P, PV>, dim3(g, m), dim3(256), 0, b);
             }
-        }
-    }
-    static void irfft_x(fyg_stream *s, const XInv<P> *a, int n) {
-        unsigned g = (unsigned)((a[0].nlines + Shape<L, P>::NL_RM - 1) / Shape<L, P>::NL_RM);
-        for (int i = 0; i < n; i += FYG_BATCH) {
-            int m = n - i < FYG_BATCH ? n - i : FYG_BATCH;
-            Batch<XInv<P>> b{};
-            for (int j = 0; j < m; j++) b.a[j] = a[i + j];
-            fyg_launch(s, "irfft_x", k_irfft_x<L, P>, dim3(g, m), dim3(256), 0, b);
-        }
+        });
     }
 };
 
@@ -1125,45 +1179,107 @@ inline const float2 *twiddles<float>(int L) { return (const float2 *)fyg_twiddle
 template <>
 inline const double2 *twiddles<double>(int L) { return (const double2 *)fyg_twiddles64(L); }
 
+// plan variant per pass kind of a deconvolution: X (rfft_x / irfft_x), Y (the
+// column passes), Z (zfused), XY (the fused plane kernels)
+struct PassPlans {
+    int x = 0, y = 0, z = 0, xy = 0;
+};
+
+// the kernel descriptors of n deconvolutions of one shape in precision P
+template <class P>
+struct Passes {
+    using V = typename Cx<P>::V;
+    std::vector<XFwd<P>> xf;
+    std::vector<Cols<P>> yf, yi;
+    std::vector<ZFused<P>> zf;
+    std::vector<XInv<P>> xi;
+    int n = 0;
+    int px = 0, py = 0, pz = 0;
+    bool u8 = false;
+
+    // 0, or 1 when the cubes do not share one shape and kernel variant
+    int build(const fyg_deconv_args *a, int na) {
+        const fyg_deconv_args &h = a[0];
+        for (int i = 1; i < na; i++) {              // one shape and one kernel variant per batch
+            const fyg_deconv_args &b = a[i];
+            if (b.nz != h.nz || b.ny != h.ny || b.nx != h.nx || b.pz != h.pz || b.py != h.py || b.px != h.px ||
+                b.hx != h.hx || b.hxp != h.hxp || (b.in_u8 != 0) != (h.in_u8 != 0) ||
+                (b.in_u8 && (b.pro.dering != 0) != (h.pro.dering != 0)) || (b.f64 != 0) != (h.f64 != 0))
+                return 1;
+        }
+        const V *twx = h.twx ? (const V *)h.twx : twiddles<P>(h.px);
+        const V *twy = h.twy ? (const V *)h.twy : twiddles<P>(h.py);
+        const V *twz = h.twz ? (const V *)h.twz : twiddles<P>(h.pz);
+        if (!twx || !twy || !twz) return 1;
+        n = na;
+        px = h.px;
+        py = h.py;
+        pz = h.pz;
+        u8 = h.in_u8 != 0;
+        const long plane = (long)h.py * h.hxp;
+        xf.resize(n);
+        yf.resize(n);
+        yi.resize(n);
+        zf.resize(n);
+        xi.resize(n);
+        for (int i = 0; i < n; i++) {
+            const fyg_deconv_args *d = &a[i];
+            V *spec = (V *)d->spec;
+            xf[i] = XFwd<P>{d->in, d->nz, d->ny, d->nx, d->pz, d->py, d->hx, d->hxp,
+                            (long)d->pz * ((d->py + 1) / 2), (long)(d->py + 1) / 2, spec, twx, d->pro,
+                            d->nonzero_flag, d->nonzero_gen};
+            // columns k < hx: the pitch padding stays idle
+            yf[i] = Cols<P>{spec, d->hx, d->hxp, plane, (P)-1, twy};
+            zf[i] = ZFused<P>{spec, d->py, d->hx, d->hxp, plane, (const P *)d->lut, d->us, d->fz2, d->fy2, d->fx2,
+                              twz};
+            yi[i] = Cols<P>{spec, d->hx, d->hxp, plane, (P)1, twy};
+            // 1/(pz*py*px) as paganin.c:285 forms it in fp32; exact in fp64
+            P invn = sizeof(P) == sizeof(float) ? (P)(1.0f / ((float)d->pz * (float)d->py * (float)d->px))
+                                                : (P)(1.0 / ((double)d->pz * d->py * d->px));
+            xi[i] = XInv<P>{spec, (P *)d->out, d->nz, d->ny, d->nx, d->py, d->hx, d->hxp,
+                            (long)d->nz * ((d->ny + 1) / 2), (long)(d->ny + 1) / 2, invn, d->epi, twx};
+        }
+        return 0;
+    }
+    void x_fwd(fyg_stream *s, int pv) {
+        dispatch(px, [&](auto c) { Launch<decltype(c)::value, P>::rfft_x(s, xf.data(), n, u8, pv); });
+    }
+    void y_fwd(fyg_stream *s, int pv) {
+        dispatch(py, [&](auto c) { Launch<decltype(c)::value, P>::cols(s, yf.data(), n, pz, "cfft_y_fwd", pv); });
+    }
+    void z(fyg_stream *s, int pv) {
+        dispatch(pz, [&](auto c) { Launch<decltype(c)::value, P>::zfused(s, zf.data(), n, pv); });
+    }
+    void y_inv(fyg_stream *s, int pv) {
+        const int nz = xi[0].nz;
+        dispatch(py, [&](auto c) { Launch<decltype(c)::value, P>::cols(s, yi.data(), n, nz, "cfft_y_inv", pv); });
+    }
+    void x_inv(fyg_stream *s, int pv) {
+        dispatch(px, [&](auto c) { Launch<decltype(c)::value, P>::irfft_x(s, xi.data(), n, pv); });
+    }
+    void xy_fwd(fyg_stream *s, int pv) {
+        dispatch(px, [&](auto c) { Launch<decltype(c)::value, P>::rfft_xy(s, xf.data(), n, u8, pv); });
+    }
+    void yx_inv(fyg_stream *s, int pv) {
+        dispatch(px, [&](auto c) { Launch<decltype(c)::value, P>::irfft_yx(s, xi.data(), n, pv); });
+    }
+};
+
+// the fused plane kernels take cubes whose X and Y axes are one length of 64
+// or 96 (fp32, unless switched off)
+template <class P>
+bool use_planes(int px, int py) {
+    return std::is_same<P, float>::value && px == py && (px == 64 || px == 96) && fyg_fft_plane_mode() != 0;
+}
+
 // n deconvolutions of one shape in precision P: ring images, then the five
 // passes, each one launch per FYG_BATCH cubes
 template <class P>
-int deconv_batch(const fyg_deconv_args *a, int n, fyg_stream *s) {
-    using V = typename Cx<P>::V;
+int deconv_batch(const fyg_deconv_args *a, int n, fyg_stream *s, PassPlans pp = PassPlans{}) {
     if (n < 1) return 0;
     const fyg_deconv_args &h = a[0];
-    for (int i = 1; i < n; i++) {                   // one shape and one kernel variant per batch
-        const fyg_deconv_args &b = a[i];
-        if (b.nz != h.nz || b.ny != h.ny || b.nx != h.nx || b.pz != h.pz || b.py != h.py || b.px != h.px ||
-            b.hx != h.hx || b.hxp != h.hxp || (b.in_u8 != 0) != (h.in_u8 != 0) ||
-            (b.in_u8 && (b.pro.dering != 0) != (h.pro.dering != 0)) || (b.f64 != 0) != (h.f64 != 0))
-            return 1;
-    }
-    const V *twx = h.twx ? (const V *)h.twx : twiddles<P>(h.px);
-    const V *twy = h.twy ? (const V *)h.twy : twiddles<P>(h.py);
-    const V *twz = h.twz ? (const V *)h.twz : twiddles<P>(h.pz);
-    if (!twx || !twy || !twz) return 1;
-    const long plane = (long)h.py * h.hxp;
-    std::vector<XFwd<P>> xf(n);
-    std::vector<Cols<P>> yf(n), yi(n);
-    std::vector<ZFused<P>> zf(n);
-    std::vector<XInv<P>> xi(n);
-    for (int i = 0; i < n; i++) {
-        const fyg_deconv_args *d = &a[i];
-        V *spec = (V *)d->spec;
-        xf[i] = XFwd<P>{d->in, d->nz, d->ny, d->nx, d->pz, d->py, d->hx, d->hxp,
-                        (long)d->pz * ((d->py + 1) / 2), (long)(d->py + 1) / 2, spec, twx, d->pro, d->nonzero_flag,
-                        d->nonzero_gen};
-        // columns k < hx: the pitch padding stays idle
-        yf[i] = Cols<P>{spec, d->hx, d->hxp, plane, (P)-1, twy};
-        zf[i] = ZFused<P>{spec, d->py, d->hx, d->hxp, plane, (const P *)d->lut, d->us, d->fz2, d->fy2, d->fx2, twz};
-        yi[i] = Cols<P>{spec, d->hx, d->hxp, plane, (P)1, twy};
-        // 1/(pz*py*px) as paganin.c:285 forms it in fp32; exact in fp64
-        P invn = sizeof(P) == sizeof(float) ? (P)(1.0f / ((float)d->pz * (float)d->py * (float)d->px))
-                                            : (P)(1.0 / ((double)d->pz * d->py * d->px));
-        xi[i] = XInv<P>{spec, (P *)d->out, d->nz, d->ny, d->nx, d->py, d->hx, d->hxp,
-                        (long)d->nz * ((d->ny + 1) / 2), (long)(d->ny + 1) / 2, invn, d->epi, twx};
-    }
+    Passes<P> ps;
+    if (ps.build(a, n)) return 1;
     if (h.step_gen && !(h.in_u8 && h.pro.dering) && fyg_gen_step(h.step_gen, h.step_nflag, s)) return 1;
     if (h.in_u8 && h.pro.dering) {
         std::vector<RingImg> ri(n);
@@ -1184,21 +1300,19 @@ int deconv_batch(const fyg_deconv_args *a, int n, fyg_stream *s) {
                        (unsigned)(m * mslab)), dim3(256), 0, b, mslab, step, h.step_nflag);
         }
     }
-    const bool u8 = h.in_u8 != 0;
-    const bool planes = std::is_same<P, float>::value && h.px == h.py && (h.px == 64 || h.px == 96) &&
-                        fyg_fft_plane_mode() != 0;
+    const bool planes = use_planes<P>(h.px, h.py);
     if (planes)
-        dispatch(h.px, [&](auto c) { Launch<decltype(c)::value, P>::rfft_xy(s, xf.data(), n, u8); });
+        ps.xy_fwd(s, pp.xy);
     else {
-        dispatch(h.px, [&](auto c) { Launch<decltype(c)::value, P>::rfft_x(s, xf.data(), n, u8); });
-        dispatch(h.py, [&](auto c) { Launch<decltype(c)::value, P>::cols(s, yf.data(), n, h.pz, "cfft_y_fwd"); });
+        ps.x_fwd(s, pp.x);
+        ps.y_fwd(s, pp.y);
     }
-    dispatch(h.pz, [&](auto c) { Launch<decltype(c)::value, P>::zfused(s, zf.data(), n); });
+    ps.z(s, pp.z);
     if (planes)
-        dispatch(h.px, [&](auto c) { Launch<decltype(c)::value, P>::irfft_yx(s, xi.data(), n); });
+        ps.yx_inv(s, pp.xy);
     else {
-        dispatch(h.py, [&](auto c) { Launch<decltype(c)::value, P>::cols(s, yi.data(), n, h.nz, "cfft_y_inv"); });
-        dispatch(h.px, [&](auto c) { Launch<decltype(c)::value, P>::irfft_x(s, xi.data(), n); });
+        ps.y_inv(s, pp.y);
+        ps.x_inv(s, pp.x);
     }
     return fyg_last_error();
 }

@@ -1178,7 +1178,8 @@ static int fast_batch(const fyg_guided_args *a, int n, fyg_stream *s) {
 
 extern "C" int fyg_tune_mode(int mode) { return g_tune_mode.exchange(mode); }
 
-// Tuning file: one line per shape, "key<TAB>coef<TAB>up" (up -1: not tuned).
+// Tuning file: one line per shape, "key<TAB>coef<TAB>up" (up -1: not tuned),
+// then one line per decided FFT radix plan, "fft<TAB>kind<TAB>length<TAB>plan".
 // Saving writes every shape tuned or loaded so far; loading adds the file's
 // shapes, which are then used without timing (a later process starts tuned).
 extern "C" int fyg_tune_save(const char *path) {
@@ -1191,6 +1192,10 @@ extern "C" int fyg_tune_save(const char *path) {
         if (kv.second.coef_done)
             bad |= fprintf(f, "%s\t%d\t%d\n", kv.first.c_str(), kv.second.coef,
                            kv.second.up_done ? kv.second.up : -1) < 0;
+    const int nf = fyg_fft_radix_lines(nullptr, 0);
+    std::string fl((size_t)nf + 1, '\0');
+    fyg_fft_radix_lines(&fl[0], nf + 1);
+    bad |= fputs(fl.c_str(), f) < 0;
     bad |= fclose(f) != 0;
     if (!bad) bad = rename(tmp.c_str(), path) != 0;
     if (bad) remove(tmp.c_str());
@@ -1205,6 +1210,10 @@ extern "C" int fyg_tune_load(const char *path) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     while (fgets(line, sizeof line, f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
+        if (!strncmp(line, "fft\t", 4)) {
+            bad |= fyg_fft_radix_load_line(line + 4);
+            continue;
+        }
         char *t1 = strchr(line, '\t');
         char *t2 = t1 ? strchr(t1 + 1, '\t') : nullptr;
         if (!t1 || !t2) { bad = 1; continue; }
@@ -1224,8 +1233,11 @@ extern "C" int fyg_tune_load(const char *path) {
 }
 
 extern "C" void fyg_tune_clear(void) {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune.clear();
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune.clear();
+    }
+    fyg_fft_radix_clear();
 }
 
 // one line per tuned shape: key, chosen candidates and their times (us)
@@ -1247,6 +1259,10 @@ extern "C" int fyg_tune_report(char *buf, int len) {
                      "outputs overlap guides)\n", kv.first.c_str(), r.coef, r.coef_us[0], r.coef_us[1], r.coef_us[2]);
         out += line;
     }
+    const int nf = fyg_fft_radix_report(nullptr, 0);
+    std::string fl((size_t)nf + 1, '\0');
+    fyg_fft_radix_report(&fl[0], nf + 1);
+    out += fl.c_str();
     if (buf && len > 0) {
         snprintf(buf, (size_t)len, "%s", out.c_str());
     }

@@ -165,5 +165,6 @@ extern "C" int fyg_noalloc_deconv(const float *in, float *out, int nz, int ny, i
     a.out = out;
     a.spec = base;
     a.twx = t.twx; a.twy = t.twy; a.twz = t.twz;
+    a.no_tune = 1;                  /* timing the radix plans would allocate */
     return fyg_deconv_run(&a, s);
 }

@@ -293,6 +293,23 @@ int fy_gpu_sync(void *stream);
  * reference counterpart. */
 int fy_gpu_fft_planes(int mode);
 
+/* radix plans of the fp32 deconvolution passes.  Padded lengths 64..512 have
+ * two plans: radix-16 stages behind one smaller radix (plan 0), or radix-8
+ * stages behind one radix 2 or 4 (plan 1).  Each pass kind (X, Y, Z, the fused
+ * XY planes) and length takes one, decided once per process: mode 0 plan 0
+ * everywhere, 2 plan 1 everywhere, 1 (default) the tuner's choice -- the first
+ * deconvolution that meets the length times both plans of the pass on a
+ * synthetic problem of about 16 M voxels (a one-time ~200 MB allocation and a
+ * few ms; never under graph capture or in fy_gpu_deconvolve_noalloc, which
+ * take plan 0 for lengths not decided yet) and keeps plan 1 only when it is at
+ * least 2 % faster.  The plans round differently (relative L2 ~1e-7 between
+ * them), so a length keeps its plan for the process; the choices are saved and
+ * loaded with the tile shapes (fy_gpu_tuning_save / _load: "fft<TAB>kind<TAB>
+ * length<TAB>plan" lines) and listed by fy_gpu_tuning_report, and
+ * fy_gpu_tuning_clear forgets them.  Returns the previous mode.  No reference
+ * counterpart. */
+int fy_gpu_fft_radix(int mode);
+
 /* the process-wide cache of device tables (transfer tables of captured and
  * large deconvolutions, ring and z-drift blobs): stats (may be NULL) receives
  * 7 counts -- entries, evictable entries, entries held by queued work or a

@@ -159,3 +159,93 @@ def test_fft_planes_bit_identical(shape, rings):
     du = np.abs(q1.astype(int) - q0.astype(int))
     assert du.max() <= 1 and (du > 0).mean() < 1e-3
     assert q1.any()
+
+
+def _deconv(lib, x, shape, phys):
+    d_in = torch.from_numpy(x).cuda()
+    d_out = torch.empty_like(d_in)
+    torch.cuda.synchronize()
+    assert lib.fy_gpu_deconvolve(C.c_void_p(d_in.data_ptr()), C.c_void_p(d_out.data_ptr()), *shape, C.byref(phys),
+                                 0.015, None) == 0, lib.fy_gpu_last_error()
+    assert lib.fy_gpu_sync(None) == 0
+    return d_out.cpu().numpy()
+
+
+RADIX_PHYS = dict(delta_beta=300.0, energy_kev=110.0, distance_mm=1200.0, pixel_um=4.3, unsharp_sigma=0.8,
+                  unsharp_coeff=0.5)
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 128, 128), (100, 180, 250), (60, 64, 64), (200, 150, 96)])
+def test_fft_radix_plans(orc, shape):
+    """The two radix plans of the deconvolution passes (radix-16 stages, or
+    radix-8 stages) on lengths 64..512: padded X/Y/Z of 64 (fused planes),
+    128, 256/192/128 and 256/192/96 (96 has one plan).  Bars: the two results
+    within relative L2 1e-6 of each other, each within 1e-5 of the fp64
+    oracle (same LUT)."""
+    import clibs
+    lib = F.lib()
+    rng = np.random.default_rng(sum(shape))
+    x = rng.random(shape, dtype=np.float32)
+    phys = F.physics(**RADIX_PHYS)
+    outs = {}
+    old = lib.fy_gpu_fft_radix(0)
+    try:
+        for mode in (0, 2):
+            lib.fy_gpu_fft_radix(mode)
+            print(f"fft radix mode {mode}: {shape}", flush=True)
+            outs[mode] = _deconv(lib, x, shape, phys)
+    finally:
+        lib.fy_gpu_fft_radix(old)
+    want = np.empty(shape, np.float64)
+    op = clibs.physics(**RADIX_PHYS)
+    assert orc.orc_deconv_radial_f64(clibs.fp(x), clibs.dp(want), *shape, C.byref(op), 0.015, 0, 8.0, 1.0) == 0
+    a, b = outs[0].astype(np.float64), outs[2].astype(np.float64)
+    r = np.linalg.norm(a - b) / np.linalg.norm(a)
+    ra, rb = (np.linalg.norm(v - want) / np.linalg.norm(want) for v in (a, b))
+    print(f"  relL2 plan 1 vs plan 0 {r:.2e}; vs fp64 oracle {ra:.2e} / {rb:.2e}", flush=True)
+    assert r <= 1e-6
+    assert ra <= 1e-5 and rb <= 1e-5
+
+
+def test_fft_radix_tuned_and_persisted(tmp_path):
+    """Tuning mode: the first deconvolution of a length times both plans of each
+    pass and records the choice; later calls reuse it (bit-identical output), a
+    saved and reloaded choice is used without timing (bit-identical again), and
+    where every pass chose the same plan the output equals the run forced to
+    that plan."""
+    lib = F.lib()
+    shape = (128, 128, 128)
+    rng = np.random.default_rng(11)
+    x = rng.random(shape, dtype=np.float32)
+    phys = F.physics(**RADIX_PHYS)
+    before = str(tmp_path / "before.txt").encode()
+    assert lib.fy_gpu_tuning_save(before) == 0          # this process's decisions, restored below
+    old = lib.fy_gpu_fft_radix(1)
+    try:
+        lib.fy_gpu_tuning_clear()
+        print("fft radix tuning: 128^3", flush=True)
+        first = _deconv(lib, x, shape, phys)
+        lines = [l for l in F.tuning_report().splitlines() if l.startswith("fft ")]
+        print("\n".join(lines), flush=True)
+        plans = {}
+        for kind in ("x", "y", "z"):
+            l = [l for l in lines if l.startswith(f"fft {kind} 128:")]
+            assert len(l) == 1 and "radix-16" in l[0] and "radix-8" in l[0]
+            plans[kind] = int(l[0].split("plan ")[1].split()[0])
+        assert np.array_equal(first, _deconv(lib, x, shape, phys))
+        path = str(tmp_path / "tune.txt").encode()
+        assert lib.fy_gpu_tuning_save(path) == 0
+        saved = open(path.decode()).read()
+        for kind in ("x", "y", "z"):
+            assert f"fft\t{kind}\t128\t{plans[kind]}\n" in saved
+        lib.fy_gpu_tuning_clear()
+        assert lib.fy_gpu_tuning_load(path) == 0
+        assert all("from a tuning file" in l for l in F.tuning_report().splitlines() if l.startswith("fft "))
+        assert np.array_equal(first, _deconv(lib, x, shape, phys))
+        if len(set(plans.values())) == 1:
+            lib.fy_gpu_fft_radix(2 if plans["x"] else 0)
+            assert np.array_equal(first, _deconv(lib, x, shape, phys))
+    finally:
+        lib.fy_gpu_tuning_clear()
+        lib.fy_gpu_tuning_load(before)
+        lib.fy_gpu_fft_radix(old)
