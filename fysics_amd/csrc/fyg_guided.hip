@@ -542,8 +542,11 @@ __device__ __forceinline__ int up_col(int c) { return (c & 1) * UPH + (c >> 1); 
 // patch of at most UPZ x UPY low-res rows, which is loaded into LDS once.  As in
 // the reference, each output row blends its four (z,y) low-res rows first and
 // then lerps along x; a thread blends only the three low-res columns its four
-// consecutive voxels need.  Thread (q, g) -- q the lane within its row chunk,
-// g its sub-wave -- owns x quad q of rows g, g+4*SW, g+8*SW, ...
+// consecutive voxels need.  (Four voxels span at most two knots when s > 2, or
+// when s = 2 and the quad starts on an even x: up_args starts an odd region
+// one voxel early at s = 2, and that voxel is not emitted.)  Thread (q, g) --
+// q the lane within its row chunk, g its sub-wave -- owns x quad q of rows g,
+// g+4*SW, g+8*SW, ...
 // P2: s = 2^sh, so c/s is exactly c*(1/s) and needs no division; the knot
 // is then c >> sh and the fraction (c - (knot << sh)) / s, both exact, so the
 // row arithmetic (uniform across a wave when LPR = 64) stays on the scalar unit.
@@ -584,7 +587,9 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
     int g = threadIdx.x / LPR;
     if constexpr (SW == 1) g = __builtin_amdgcn_readfirstlane(g);
     const int xb = xc0 + q * UPX;
-    const bool xin = xb <= xc1, full = xb + UPX - 1 <= xc1;
+    // the first x emitted: a quantised region may start one voxel after x0
+    const int xe = p.em.quant ? (int)p.em.ix0 : p.x0;
+    const bool xin = xb <= xc1, full = xb >= xe && xb + UPX - 1 <= xc1;
 
     // input voxels of the thread's rows: issued first, used last
     P iv[RPT][UPX];
@@ -881,8 +886,16 @@ static UpArgs<P> up_args(const fyg_guided_args *a, const P *MA, const P *MB) {
         u.z0 = (int)a->em.iz0; u.y0 = (int)a->em.iy0; u.x0 = (int)a->em.ix0;
         u.dz = (int)a->em.tz; u.dy = (int)a->em.ty; u.dx = (int)a->em.tx;
     }
+    // At s = 2 four consecutive voxels from an odd x span three knots, one more
+    // than the upsample blends: start such a region one voxel early (its first
+    // voxel is computed and not emitted) so every quad starts on an even x.
+    const bool shifted = sd == 2 && (u.x0 & 1);
+    if (shifted) {
+        u.x0 -= 1;
+        u.dx += 1;
+    }
     u.vin = (nx % 4 == 0) && (u.x0 % 4 == 0) && ((uintptr_t)in % 16 == 0);
-    u.vout = a->em.quant ? (a->em.tx % 4 == 0 && (uintptr_t)a->em.q % 4 == 0)
+    u.vout = a->em.quant ? (!shifted && a->em.tx % 4 == 0 && (uintptr_t)a->em.q % 4 == 0)
                          : (u.vin && (uintptr_t)a->em.out % 16 == 0);
     return u;
 }
