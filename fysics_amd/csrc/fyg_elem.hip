@@ -65,6 +65,98 @@ __global__ void k_dering_apply(float *f, long dz, long dy, long dx, fyg_prologue
     }
 }
 
+// the same two kernels on (row z*ny + y, x block) grids: a voxel's z and y (and
+// its ring slab) are per-workgroup values instead of 64-bit divisions per voxel
+__global__ __launch_bounds__(NT) void k_dering_apply_rows(float *f, int dy, long dx, fyg_prologue P) {
+    const int row = blockIdx.x, z = row / dy, y = row - z * dy;
+    const long x = (long)blockIdx.y * NT + threadIdx.x;
+    if (x >= dx) return;
+    const size_t i = (size_t)row * dx + x;
+    f[i] = ring_sub(P, f[i], P.gz0 + z, P.gy0 + y, P.gx0 + x);
+}
+
+// ring_sub with the row's terms computed once: the y distance (squared, in the
+// precision ring_sub uses) and the slab's ring row.  Coordinates are converted
+// from int: the same values as from long for |c| < 2^31.
+struct RingRow {
+    const float *ring;                   // the slab's ring values (NULL: no ring removal)
+    float fy2, cxf;                      // fast path (float radius)
+    double dy2, cxd;                     // double radius
+};
+__device__ __forceinline__ RingRow ring_row(const fyg_prologue &P, int gz, int gy) {
+    RingRow R;
+    R.ring = nullptr;
+    if (!P.dering) return R;
+    int slab = gz / P.slab_z;
+    slab = slab < 0 ? 0 : (slab >= P.nslab ? P.nslab - 1 : slab);
+    R.ring = P.ring + (size_t)slab * P.nr;
+    const float fy = (float)gy - (float)P.cy;
+    R.fy2 = fy * fy;
+    R.cxf = (float)P.cx;
+    const double dy = (double)gy - P.cy;
+    R.dy2 = dy * dy;
+    R.cxd = P.cx;
+    return R;
+}
+__device__ __forceinline__ float ring_sub_row(const fyg_prologue &P, const RingRow &R, float f, int gx) {
+    #pragma clang fp contract(off)
+    if (f <= 0.0f) return f;
+    int ri;
+    if (P.ring_fast) {
+        const float fx = (float)gx - R.cxf;
+        ri = (int)(sqrtf(R.fy2 + fx * fx) + 0.5f);
+    } else {
+        const double dx = (double)gx - R.cxd;
+        ri = (int)(sqrt(R.dy2 + dx * dx) + 0.5);
+    }
+    if (ri >= P.nr) return f;
+    const float v = f - P.scale * R.ring[ri];
+    return v > 0.0f ? v : 0.0f;
+}
+
+// one row per wave (z*ny + y, NT/64 rows per workgroup), four consecutive
+// voxels per lane and step (u8 word loads and float4 stores when the rows allow
+// them: vec)
+__global__ __launch_bounds__(NT) void k_prologue_rows(const unsigned char *in, float *out, int nrows, int ny,
+                                                      int nx, fyg_prologue P, int *flag, const int *gen, int vec) {
+    #pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    const int z = row / ny, y = row - z * ny;
+    int any = 0;
+    for (int x0 = 4 * lane; row < nrows && x0 < nx; x0 += 256) {
+        const size_t i0 = (size_t)row * nx + x0;
+        const int gz = (int)(P.gz0 + z), gy = (int)(P.gy0 + y), gx0 = (int)(P.gx0 + x0);
+        const RingRow R = ring_row(P, gz, gy);
+        const float zf = P.zdrift ? P.zfactor[gz] : 1.0f;
+        const bool full = vec && x0 + 3 < nx;
+        unsigned char u[4];
+        if (full) {
+            const uchar4 w = *(const uchar4 *)(in + i0);
+            u[0] = w.x; u[1] = w.y; u[2] = w.z; u[3] = w.w;
+        } else {
+            for (int e = 0; e < 4; e++) u[e] = x0 + e < nx ? in[i0 + e] : 0;
+        }
+        float o[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            any |= u[e];
+            float f = (float)u[e] * (1.0f / 255.0f);
+            if (P.dering) f = ring_sub_row(P, R, f, gx0 + e);
+            if (P.zdrift) f *= zf;
+            if (P.window) f = P.wlo + f * P.wspan;
+            o[e] = f;
+        }
+        if (full) {
+            *(float4 *)(out + i0) = make_float4(o[0], o[1], o[2], o[3]);
+        } else {
+            for (int e = 0; e < 4; e++)
+                if (x0 + e < nx) out[i0 + e] = o[e];
+        }
+    }
+    if (flag && __syncthreads_or(any) && threadIdx.x == 0) fyg_flag_set(flag, gen);
+}
+
 __global__ void k_prologue(const unsigned char *in, float *out, int nz, int ny, int nx, fyg_prologue P,
                            int *flag, const int *gen) {
     #pragma clang fp contract(off)
@@ -82,7 +174,7 @@ __global__ void k_prologue(const unsigned char *in, float *out, int nz, int ny, 
         if (P.window) f = P.wlo + f * P.wspan;
         out[i] = f;
     }
-    if (flag && __syncthreads_or(any) && threadIdx.x == 0) *flag = *gen;
+    if (flag && __syncthreads_or(any) && threadIdx.x == 0) fyg_flag_set(flag, gen);
 }
 
 // Sector/radius histogram of nonzero u8 samples on the ss-subsampled (y,x) grid.
@@ -253,7 +345,12 @@ extern "C" int fyg_zdrift_run(float *f, int nz, int ny, int nx, const float *fac
 extern "C" int fyg_dering_apply_run(float *f, long dz, long dy, long dx, const fyg_prologue *pro, fyg_stream *s) {
     size_t n = (size_t)dz * dy * dx;
     if (!n) return 0;
-    fyg_launch(s, "dering_apply", k_dering_apply, dim3(grid1d(n)), dim3(NT), 0, f, dz, dy, dx, *pro);
+    const long rows = dz * dy, xb = (dx + NT - 1) / NT;
+    if (rows <= 0x7fffffffL && xb <= 65535)
+        fyg_launch(s, "dering_apply", k_dering_apply_rows, dim3((unsigned)rows, (unsigned)xb), dim3(NT), 0, f,
+                   (int)dy, dx, *pro);
+    else
+        fyg_launch(s, "dering_apply", k_dering_apply, dim3(grid1d(n)), dim3(NT), 0, f, dz, dy, dx, *pro);
     return fyg_last_error();
 }
 
@@ -261,7 +358,16 @@ extern "C" int fyg_prologue_run(const unsigned char *in, float *out, int nz, int
                                 const fyg_prologue *pro, int *flag, const int *gen, fyg_stream *s) {
     size_t n = (size_t)nz * ny * nx;
     if (!n) return 0;
-    fyg_launch(s, "prologue", k_prologue, dim3(grid1d(n)), dim3(NT), 0, in, out, nz, ny, nx, *pro, flag, gen);
+    const long rows = (long)nz * ny;
+    // global coordinates are int in the row kernel
+    const bool small = pro->gz0 + nz < 0x7fffffffL && pro->gy0 + ny < 0x7fffffffL && pro->gx0 + nx < 0x7fffffffL &&
+                       pro->gz0 >= 0 && pro->gy0 >= 0 && pro->gx0 >= 0;
+    const int vec = nx % 4 == 0 && (uintptr_t)in % 4 == 0 && (uintptr_t)out % 16 == 0;
+    if (rows <= 0x7fffffffL - NT && small)
+        fyg_launch(s, "prologue", k_prologue_rows, dim3((unsigned)((rows + NT / 64 - 1) / (NT / 64))), dim3(NT), 0,
+                   in, out, (int)rows, ny, nx, *pro, flag, gen, vec);
+    else
+        fyg_launch(s, "prologue", k_prologue, dim3(grid1d(n)), dim3(NT), 0, in, out, nz, ny, nx, *pro, flag, gen);
     return fyg_last_error();
 }
 

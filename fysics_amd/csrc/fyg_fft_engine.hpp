@@ -572,7 +572,7 @@ __global__ __launch_bounds__(256) void k_rfft_x(Batch<XFwd<P>> bt) {
     run_from<L, NL, false, false, 0, -1, PV>((P)-1, tw_s, buf, first, LdsWr<L, NL, V>{buf}, [&] { twp.store(tw_s); });
     __syncthreads();
     if (U8 && a.flag) {
-        if (__syncthreads_or(any) && threadIdx.x == 0) *a.flag = *a.gen;
+        if (__syncthreads_or(any) && threadIdx.x == 0) fyg_flag_set(a.flag, a.gen);
     }
     // untangle A(k) = (X(k) + conj X(-k))/2, B(k) = (X(k) - conj X(-k))/(2i)
     const int dl = Shape<L>::T / a.hxp, dk = Shape<L>::T - dl * a.hxp;
@@ -983,7 +983,7 @@ __global__ __launch_bounds__(256) void k_rfft_xy(Batch<XFwd<float>> bt) {
         __syncthreads();
     }
     if (U8 && a.flag) {
-        if (__syncthreads_or(any) && threadIdx.x == 0) *a.flag = *a.gen;
+        if (__syncthreads_or(any) && threadIdx.x == 0) fyg_flag_set(a.flag, a.gen);
     }
     // column FFTs along y, in place: the first stage with a barrier between its
     // reads and writes, then the remaining stages; the last one writes the plane

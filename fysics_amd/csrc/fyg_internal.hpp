@@ -63,3 +63,13 @@ struct Batch {
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+
+// A cube's "not all air" flag is set to the call's generation by every
+// workgroup that saw a nonzero voxel.  Tens of thousands of workgroups storing
+// to one word serialise on the L2 channel that holds it, so a workgroup stores
+// only when the word does not hold the generation yet (a stale read at worst
+// repeats the store).
+__device__ __forceinline__ void fyg_flag_set(int *flag, const int *gen) {
+    const int g = *gen;
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != g) *flag = g;
+}
