@@ -699,9 +699,17 @@ def air_zero_bench(F, vol, eps, reps=10):
              if k.startswith("air_") or k.startswith("cc_") or k in ("prologue", "emit", "guided_coef",
                                                                      "guided_upsample")}
     zeroed = float((d_out == 0).float().mean().item())
+    # stage (d) and the emit: kernel time per cube against the 6N bytes the
+    # stage needs at least (float tile in, u8 input in, u8 tile out)
+    d_ms = sum(ms for k, (ms, c) in prof.items()
+               if k.startswith("air_") or k.startswith("cc_") or k in ("prologue", "emit")) / 5
+    d_bytes = 6 * n ** 3
+    roof = {"bound": "hbm", "kernels_us": round(1000 * d_ms, 1), "algorithmic_bytes": d_bytes,
+            "achieved": round(d_bytes / (d_ms * 1e-3) / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(d_bytes / (d_ms * 1e-3) / 8e12, 4)}
     return {"workload": f"{n}^3 u8 cube, the chain plus air-zero (valley rule) and despeckle (< 500 voxels)",
             "ms_per_cube": round(1000 * dt, 3), "mvox_s": round(n ** 3 / dt / 1e6, 1),
-            "zeroed_fraction": round(zeroed, 4), "kernels_us": stage}
+            "zeroed_fraction": round(zeroed, 4), "kernels_us": stage, "stage_d_roofline": roof}
 
 
 def lod_phasecorr_bench(F, vol, reps=20):
