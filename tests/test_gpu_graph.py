@@ -135,3 +135,36 @@ def test_graph_tables_survive_cache_eviction():
         lib.fy_gpu_table_cache(None, old_limit)
     end = _cache_stats()
     assert end[3] == 0                                 # every evicted entry's memory was freed
+
+
+@pytest.mark.parametrize("radius,s", [(6, 2), (3, 1)])
+def test_graph_wide_guided_radius(radius, s):
+    """Guided radii beyond the LDS tiles (r/s > 2 on the fast path, r > 2 on
+    the exact path) take the global-memory box passes, whose centring constant
+    is read on the device: they capture like the rest, and every replay equals
+    an eager call bit for bit."""
+    rng = np.random.default_rng(30 + radius)
+    cases = make_cases(rng)[:6]
+    for c in cases:
+        c.P.guided_radius, c.P.guided_s = radius, s
+    arr = (F.Cube * len(cases))(*[c.cube() for c in cases])
+    stream = torch.cuda.Stream()
+    lib = F.lib()
+    torch.cuda.synchronize()
+    print(f"graph, guided r={radius} s={s}", flush=True)
+    g = lib.fy_gpu_graph_cubes(arr, len(cases), C.c_void_p(stream.cuda_stream))
+    assert g, lib.fy_gpu_last_error()
+    try:
+        for replay in range(2):
+            refill(cases, rng, 0)
+            for c in cases:
+                c.d_out.zero_()
+            torch.cuda.synchronize()
+            assert lib.fy_graph_launch(C.c_void_p(g)) == 0
+            stream.synchronize()
+            got = [c.d_out.cpu().numpy().copy() for c in cases]
+            want = run(cases)
+            for i, (a, b) in enumerate(zip(got, want)):
+                assert a.any() and np.array_equal(a, b), f"replay {replay} cube {i}"
+    finally:
+        lib.fy_graph_destroy(C.c_void_p(g))
