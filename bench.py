@@ -343,13 +343,17 @@ def _cube_params(F, phys, z_ratio, eps):
     return P
 
 
-def batched_bench(F, reps=5, seed=100):
+def batched_bench(F, reps=20, seed=100):
     """The batched workload: 96 cubes (32x64^3, 32x96^3, 16x128^3, 12x160^3 -> 192^3
     FFT, 4x192x256x256; 169.7 M voxels), each with its own phantom and physics,
     u8 in HBM -> deconv + guided -> u8 in HBM.  Timed as one fy_gpu_process_cubes
-    call over all 96 (grouped launches), as the replay of those launches captured
-    in a HIP graph (fy_gpu_graph_cubes / fy_graph_launch), and, beside them, one
-    call per cube."""
+    call over all 96 in each launch form (fy_gpu_batch_mix): one launch per
+    stage per cache-sized part of mixed shapes, parts on two streams
+    ("mixed_parts", the default), one launch per stage for the whole batch
+    ("mixed_whole"), one launch per stage per shape group ("grouped"); as the
+    replay of the default and the grouped form captured in a HIP graph
+    (fy_gpu_graph_cubes / fy_graph_launch); and, beside them, one call per
+    cube."""
     import torch
     from fysics_amd import phantom
     lib = F.lib()
@@ -370,7 +374,7 @@ def batched_bench(F, reps=5, seed=100):
     stream = torch.cuda.Stream()
     sh = C.c_void_p(stream.cuda_stream)
 
-    def grouped():
+    def batch():
         if lib.fy_gpu_process_cubes(arr, len(cubes), sh):
             raise RuntimeError(lib.fy_gpu_last_error().decode())
 
@@ -383,24 +387,38 @@ def batched_bench(F, reps=5, seed=100):
                        "per-cube phantom, delta/beta U[100,1000], pixel {2.4,4.3,7.9} um, sigma_z/sigma_xy "
                        "U[1,1.3]; u8 -> dewindow -> Wiener deconv -> guided r=2 s=2 -> dithered u8",
            "cubes": len(cubes), "voxels": int(nvox), "workload_gen_s": round(gen_s, 1)}
-    graph = lib.fy_gpu_graph_cubes(arr, len(cubes), sh)
-    if not graph:
-        raise RuntimeError(lib.fy_gpu_last_error().decode())
+    graphs = {}
 
-    def captured():
-        if lib.fy_graph_launch(C.c_void_p(graph)):
-            raise RuntimeError(lib.fy_gpu_last_error().decode())
+    def captured(mode):
+        def fn():
+            if lib.fy_graph_launch(C.c_void_p(graphs[mode])):
+                raise RuntimeError(lib.fy_gpu_last_error().decode())
+        return fn
 
-    for name, fn in (("grouped", grouped), ("graph", captured), ("one_call_per_cube", one_by_one)):
-        fn()
-        stream.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(reps):
-            fn()
-        stream.synchronize()
-        dt = (time.perf_counter() - t1) / reps
-        out[name] = {"ms_per_batch": round(1000 * dt, 3), "mvox_s": round(nvox / dt / 1e6, 1)}
-    lib.fy_graph_destroy(C.c_void_p(graph))
+    runs = (("mixed_parts", 1, batch), ("mixed_parts_graph", 1, captured(1)), ("mixed_whole", 2, batch),
+            ("grouped", 0, batch), ("grouped_graph", 0, captured(0)), ("one_call_per_cube", 1, one_by_one))
+    old = lib.fy_gpu_batch_mix(1)
+    try:
+        for mode in (1, 0):
+            lib.fy_gpu_batch_mix(mode)
+            graphs[mode] = lib.fy_gpu_graph_cubes(arr, len(cubes), sh)
+            if not graphs[mode]:
+                raise RuntimeError(lib.fy_gpu_last_error().decode())
+        for name, mode, fn in runs:
+            lib.fy_gpu_batch_mix(mode)
+            for _ in range(3):
+                fn()
+            stream.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            stream.synchronize()
+            dt = (time.perf_counter() - t1) / reps
+            out[name] = {"ms_per_batch": round(1000 * dt, 3), "mvox_s": round(nvox / dt / 1e6, 1)}
+    finally:
+        lib.fy_gpu_batch_mix(old)
+        for g in graphs.values():
+            lib.fy_graph_destroy(C.c_void_p(g))
     return out
 
 

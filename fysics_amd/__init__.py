@@ -248,6 +248,7 @@ SIGNATURES = {
     "fy_gpu_sync": (C.c_int, [_VP]),
     "fy_gpu_table_cache": (C.c_long, [C.POINTER(C.c_long), C.c_long]),
     "fy_gpu_fft_planes": (C.c_int, [C.c_int]),
+    "fy_gpu_batch_mix": (C.c_int, [C.c_int]),
     "fy_gpu_fft_radix": (C.c_int, [C.c_int]),
     "fy_gpu_deconvolve_noalloc": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_int, _P, C.c_double, _VP]),
     "fy_gpu_profile": (None, [C.c_int]),

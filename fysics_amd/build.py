@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libfysics_amd.so")
 ARCHIVE = os.path.join(HERE, "libfysics_amd.a")
 ARCH = os.environ.get("FYSICS_OFFLOAD_ARCH", "gfx950")
 
-HIP_SRC = ["fyg_runtime.hip", "fyg_fft.hip", "fyg_fft64.hip", "fyg_guided.hip", "fyg_elem.hip",
+HIP_SRC = ["fyg_runtime.hip", "fyg_fft.hip", "fyg_fft64.hip", "fyg_mix.hip", "fyg_guided.hip", "fyg_elem.hip",
            "fyg_noise.hip", "fyg_calib.hip", "fyg_air.hip",
            "fyg_phasecorr.hip", "fyg_noalloc.hip"]
 C_SRC = ["host/fy_core.c", "host/fy_deconv.c", "host/fy_guided.c", "host/fy_dering.c",
@@ -37,6 +37,10 @@ def hipcc():
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
              # packed fp32 VALU forms are kept out of every kernel
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # multiply-adds fuse only within one source expression, so a kernel
+             # body gives the same bits in every kernel it is inlined into (the
+             # per-shape kernels and the mixed-shape stage kernels)
+             "-ffp-contract=on",
              "-Wno-unused-command-line-argument"]
 C_FLAGS = ["-std=gnu11", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
@@ -110,7 +114,7 @@ def build(force=False, jobs=None, verbose=True):
             work.append(("c", src, obj))
     jobs = jobs or min(8, os.cpu_count() or 4)
     # the FFT unit is the long pole: start it first
-    work.sort(key=lambda j: 0 if "fft" in j[1] else 1)
+    work.sort(key=lambda j: 0 if ("fft" in j[1] or "mix" in j[1]) else 1)
     failed = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for src, rc, err in ex.map(_compile, work):

@@ -141,6 +141,11 @@ void fyg_prof_reset(void);
  * launches can be captured), and larger groups take several launches. */
 #define FYG_BATCH 16
 
+/* Cubes of mixed shapes run as one launch per stage for all of them: the
+ * launch's blocks are cut into one segment per cube (at most FYG_MIX_MAX),
+ * and the segments and descriptors travel in the kernel arguments. */
+#define FYG_MIX_MAX 128
+
 /* ------------------------------------------------------------------ FFT / deconv */
 int fyg_fft_supported(int L);                   /* 2^k or 3*2^k, L <= FYG_FFT_MAX */
 #define FYG_FFT_MAX 4096
@@ -224,6 +229,10 @@ int fyg_fft_plane_set(int mode);
  * Returns the previous mode. */
 int fyg_fft_radix_mode(int mode);
 void fyg_fft_radix_clear(void);
+/* the plan variant of pass kind (FYG_RK_*) at length L for a deconvolution on
+ * s, timed now when may_tune and not decided yet */
+enum { FYG_RK_X, FYG_RK_Y, FYG_RK_Z, FYG_RK_XY };
+int fyg_radix_plan(int kind, int L, fyg_stream *s, int may_tune);
 /* "fft\t<kind>\t<length>\t<plan>" lines of every decided (kind, length) */
 int fyg_fft_radix_lines(char *buf, int len);
 /* one such line (without "fft\t"): 0 ok, 1 malformed */
@@ -287,6 +296,22 @@ int fyg_tune_load(const char *path);
 void fyg_tune_clear(void);
 /* n guided filters; cubes of one shape on the fast path share launches */
 int fyg_guided_run_batch(const fyg_guided_args *a, int n, fyg_stream *s);
+/* 1 when a padded FFT length runs in mixed launches (64, 96, 128, 192, 256) */
+int fyg_mix_len_ok(int L);
+/* n deconvolutions (fp32, u8 input, all with or all without rings, lengths
+ * that fyg_mix_len_ok accepts): ring images (or the generation step), then
+ * X forward, Y forward, Z, Y inverse, X inverse, one launch each */
+int fyg_mix_deconv_run(const fyg_deconv_args *a, int n, fyg_stream *s);
+/* the fast guided filter of a cube can run in a mixed launch; n filters in
+ * one coefficient launch and one upsample launch */
+int fyg_mix_guided_ok(const fyg_guided_args *a);
+int fyg_mix_guided_run(const fyg_guided_args *a, int n, fyg_stream *s);
+/* batches of several shapes where every cube allows it: 1 (default) mixed
+ * launches over parts of the batch whose spectra fit the last-level cache,
+ * on two streams; 2 mixed launches over the whole batch; 0 one launch per
+ * stage per shape group.  Returns the previous mode. */
+int fyg_mix_mode(void);
+int fyg_mix_set(int mode);
 /* prologue-only path (deconv disabled): u8 -> f32 through the tile prologue */
 int fyg_prologue_run(const unsigned char *in, float *out, int nz, int ny, int nx,
                      const fyg_prologue *pro, int *nonzero_flag, const int *gen, fyg_stream *s);

@@ -91,7 +91,7 @@ extern "C" size_t fyg_deconv_spec_bytes64(int pz, int py, int px) {
 // made before the first result of the length and kept.
 namespace {
 
-enum RadixKind { RK_X, RK_Y, RK_Z, RK_XY, RK_N };
+enum RadixKind { RK_X = FYG_RK_X, RK_Y = FYG_RK_Y, RK_Z = FYG_RK_Z, RK_XY = FYG_RK_XY, RK_N };
 const char *const kRadixKind[RK_N] = {"x", "y", "z", "xy"};
 
 struct RadixRec {
@@ -222,6 +222,14 @@ int radix_plan(int kind, int L, fyg_stream *s, bool may_tune) {
 }
 
 }  // namespace
+
+extern "C" int fyg_radix_plan(int kind, int L, fyg_stream *s, int may_tune) {
+    return radix_plan(kind, L, s, may_tune != 0);
+}
+
+static std::atomic<int> g_mix_mode{1};
+extern "C" int fyg_mix_mode(void) { return g_mix_mode.load(std::memory_order_relaxed); }
+extern "C" int fyg_mix_set(int mode) { return g_mix_mode.exchange(mode < 0 ? 0 : mode > 2 ? 2 : mode); }
 
 extern "C" int fyg_fft_radix_mode(int mode) { return g_rad_mode.exchange(mode < 0 ? 1 : mode > 2 ? 2 : mode); }
 

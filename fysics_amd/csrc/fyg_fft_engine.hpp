@@ -457,14 +457,10 @@ struct RingImg {
     int ny, nx, s0, nimg;
 };
 
-// grid: (x / 256, y, cube * mslab + slab).  step: the call's generation word
-// to step (workgroup 0, before any thread returns), or null
-__global__ __launch_bounds__(256) void k_ring_image(Batch<RingImg> b, int mslab, int *step, int nflag) {
+// voxel x of row y of image si
+__device__ __forceinline__ void ring_image_at(const RingImg &r, int x, int y, int si) {
     #pragma clang fp contract(off)
-    if (step && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) fyg_gen_step_block(step, nflag);
-    const RingImg &r = b.a[blockIdx.z / mslab];
     const fyg_prologue &P = r.P;
-    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, si = blockIdx.z % mslab;
     if (x >= r.nx || y >= r.ny || si >= r.nimg) return;
     double dy = (double)(P.gy0 + y) - P.cy, dx = (double)(P.gx0 + x) - P.cx;
     double dy2 = dy * dy;
@@ -472,6 +468,13 @@ __global__ __launch_bounds__(256) void k_ring_image(Batch<RingImg> b, int mslab,
     float c = 0.0f;
     if (ri < P.nr) c = P.scale * P.ring[(long)(r.s0 + si) * P.nr + ri];
     P.cimg[((long)si * r.ny + y) * r.nx + x] = c;
+}
+
+// grid: (x / 256, y, cube * mslab + slab).  step: the call's generation word
+// to step (workgroup 0, before any thread returns), or null
+__global__ __launch_bounds__(256) void k_ring_image(Batch<RingImg> b, int mslab, int *step, int nflag) {
+    if (step && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) fyg_gen_step_block(step, nflag);
+    ring_image_at(b.a[blockIdx.z / mslab], blockIdx.x * 256 + threadIdx.x, blockIdx.y, blockIdx.z % mslab);
 }
 
 // rewindow, dec-range rescale, clamp (pipeline.c:273-283), in the working precision
@@ -507,18 +510,27 @@ struct XLine {
     int pair, valid;
 };
 
-// MODE 0: fp32 input; 1: u8 input with the prologue; 2: as 1 plus rings
-template <int L, int MODE, class P, int PV = 0>
-__global__ __launch_bounds__(256) void k_rfft_x(Batch<XFwd<P>> bt) {
+// LDS of one row-pass workgroup (X forward and X inverse)
+template <int L, class P, class Line>
+struct RowLds {
     using V = typename Cx<P>::V;
-    const XFwd<P> a = bt.a[blockIdx.y];             // into registers once
+    static constexpr int NL = Shape<L, P>::NL_RM;
+    V buf[NL * Shape<L>::LD];
+    V tw_s[L];
+    Line meta[NL];
+};
+
+// MODE 0: fp32 input; 1: u8 input with the prologue; 2: as 1 plus rings.
+// Workgroup bx of one cube's X forward pass.
+template <int L, int MODE, class P, int PV = 0>
+__device__ __forceinline__ void rfft_x_body(const XFwd<P> &a, long bx, RowLds<L, P, XLine> &sm) {
+    using V = typename Cx<P>::V;
     constexpr int NL = Shape<L, P>::NL_RM;
     constexpr bool U8 = MODE != 0, RING = MODE == 2;
-    __shared__ V buf[NL * Shape<L>::LD];
-    __shared__ V tw_s[L];
-    __shared__ XLine meta[NL];
+    V *buf = sm.buf, *tw_s = sm.tw_s;
+    XLine *meta = sm.meta;
     const TwPrefetch<L, V> twp(a.tw);
-    const long g0 = (long)blockIdx.x * NL;
+    const long g0 = bx * NL;
     int s0 = 0;
     if (RING) s0 = ring_slab(a.pro, a.pro.gz0);
     for (int l = threadIdx.x; l < NL; l += Shape<L>::T) {
@@ -593,6 +605,14 @@ __global__ __launch_bounds__(256) void k_rfft_x(Batch<XFwd<P>> bt) {
     }
 }
 
+// grid: (line groups, cube)
+template <int L, int MODE, class P, int PV = 0>
+__global__ __launch_bounds__(256) void k_rfft_x(Batch<XFwd<P>> bt) {
+    __shared__ RowLds<L, P, XLine> sm;
+    const XFwd<P> a = bt.a[blockIdx.y];             // into registers once
+    rfft_x_body<L, MODE, P, PV>(a, blockIdx.x, sm);
+}
+
 template <class P>
 struct Cols {
     using V = typename Cx<P>::V;
@@ -615,21 +635,21 @@ __device__ __forceinline__ long pair_block(long h, long T) {
     return (c << 4) + 2 * (i & 7) + (i >> 3);
 }
 
-template <int L, class P, int PV = 0>
-__global__ __launch_bounds__(256) void k_cfft_cols(Batch<Cols<P>> bt) {
+// LDS of one column-pass workgroup
+template <int L, class P>
+struct ColLds {
     using V = typename Cx<P>::V;
-    constexpr bool PAIR = Shape<L, P>::NL_CM * 2 * sizeof(P) < 128;
-    long bx = blockIdx.x, by = blockIdx.y;
-    if (PAIR) {
-        const long T = (long)gridDim.x * gridDim.y;
-        const long h = pair_block(blockIdx.x + (long)gridDim.x * blockIdx.y, T);
-        bx = h % gridDim.x;
-        by = h / gridDim.x;
-    }
-    const Cols<P> &a = bt.a[blockIdx.z];
+    static constexpr int NL = Shape<L, P>::NL_CM;
+    V buf[NL * Shape<L>::LD];
+    V tw_s[L];
+};
+
+// workgroup (bx, by) of one cube's column pass: column group bx of batch by
+template <int L, class P, int PV = 0>
+__device__ __forceinline__ void cols_body(const Cols<P> &a, long bx, long by, ColLds<L, P> &sm) {
+    using V = typename Cx<P>::V;
     constexpr int NL = Shape<L, P>::NL_CM;
-    __shared__ V buf[NL * Shape<L>::LD];
-    __shared__ V tw_s[L];
+    V *buf = sm.buf, *tw_s = sm.tw_s;
     const TwPrefetch<L, V> twp(a.tw);
     const long c0 = bx * NL;
     V *base = a.data + (size_t)by * a.bstride + c0;
@@ -652,6 +672,21 @@ __global__ __launch_bounds__(256) void k_cfft_cols(Batch<Cols<P>> bt) {
         run_from<L, NL, true, false, 0, -1, PV>((P)1, tw_s, buf, first, last, hook);
 }
 
+// grid: (column groups, batches, cube)
+template <int L, class P, int PV = 0>
+__global__ __launch_bounds__(256) void k_cfft_cols(Batch<Cols<P>> bt) {
+    constexpr bool PAIR = Shape<L, P>::NL_CM * 2 * sizeof(P) < 128;
+    long bx = blockIdx.x, by = blockIdx.y;
+    if (PAIR) {
+        const long T = (long)gridDim.x * gridDim.y;
+        const long h = pair_block(blockIdx.x + (long)gridDim.x * blockIdx.y, T);
+        bx = h % gridDim.x;
+        by = h / gridDim.x;
+    }
+    __shared__ ColLds<L, P> sm;
+    cols_body<L, P, PV>(bt.a[blockIdx.z], bx, by, sm);
+}
+
 // the LUT holds fp32 weights for P = float (the reference table) and the same
 // weights evaluated in double for P = double
 template <class P>
@@ -666,22 +701,31 @@ struct ZFused {
     const V *tw;
 };
 
-// forward Z (plan order), weight, inverse Z (reversed plan), all in one kernel
-template <int L, class P, int PV = 0>
-__global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
+// LDS of one fused Z workgroup
+template <int L, class P>
+struct ZLds {
     using V = typename Cx<P>::V;
-    const ZFused<P> &a = bt.a[blockIdx.y];
-    const long bx = Shape<L, P>::NL_ZF * 2 * sizeof(P) < 128 ? pair_block(blockIdx.x, gridDim.x) : (long)blockIdx.x;
+    static constexpr int NL = Shape<L, P>::NL_ZF;
+    V buf[NL * Shape<L>::LD];
+    V tw_s[L];
+    double s_fy[NL], s_fx[NL];
+    int s_ok[NL];
+};
+
+// forward Z (plan order), weight, inverse Z (reversed plan), all in one
+// workgroup: column group bx of one cube
+template <int L, class P, int PV = 0>
+__device__ __forceinline__ void zfused_body(const ZFused<P> &a, long bx, ZLds<L, P> &sm) {
+    using V = typename Cx<P>::V;
     constexpr int NL = Shape<L, P>::NL_ZF;
     constexpr int T = Shape<L>::T;
     using PF = Plan<L, false, PV>;
     constexpr int n = PF::n;
     constexpr int RL = PF::R(n - 1);          // last forward radix == first inverse radix
     constexpr int NSL = PF::NS(n - 1);
-    __shared__ V buf[NL * Shape<L>::LD];
-    __shared__ V tw_s[L];
-    __shared__ double s_fy[NL], s_fx[NL];
-    __shared__ int s_ok[NL];
+    V *buf = sm.buf, *tw_s = sm.tw_s;
+    double *s_fy = sm.s_fy, *s_fx = sm.s_fx;
+    int *s_ok = sm.s_ok;
     // the table's loads overlap the per-column metadata loads that the barrier
     // below waits for anyway
     load_twiddles<L>(tw_s, a.tw);
@@ -807,6 +851,14 @@ __global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
     }
 }
 
+// grid: (column groups, cube)
+template <int L, class P, int PV = 0>
+__global__ __launch_bounds__(256) void k_zfused(Batch<ZFused<P>> bt) {
+    const long bx = Shape<L, P>::NL_ZF * 2 * sizeof(P) < 128 ? pair_block(blockIdx.x, gridDim.x) : (long)blockIdx.x;
+    __shared__ ZLds<L, P> sm;
+    zfused_body<L, P, PV>(bt.a[blockIdx.y], bx, sm);
+}
+
 template <class P>
 struct XInv {
     using V = typename Cx<P>::V;
@@ -827,16 +879,15 @@ struct XInvLine {
     int valid;
 };
 
+// workgroup bx of one cube's X inverse pass
 template <int L, class P, int PV = 0>
-__global__ __launch_bounds__(256) void k_irfft_x(Batch<XInv<P>> bt) {
+__device__ __forceinline__ void irfft_x_body(const XInv<P> &a, long bx, RowLds<L, P, XInvLine> &sm) {
     using V = typename Cx<P>::V;
-    const XInv<P> a = bt.a[blockIdx.y];             // into registers once
     constexpr int NL = Shape<L, P>::NL_RM;
-    __shared__ V buf[NL * Shape<L>::LD];
-    __shared__ V tw_s[L];
-    __shared__ XInvLine meta[NL];
+    V *buf = sm.buf, *tw_s = sm.tw_s;
+    XInvLine *meta = sm.meta;
     const TwPrefetch<L, V> twp(a.tw);
-    const long g0 = (long)blockIdx.x * NL;
+    const long g0 = bx * NL;
     for (int l = threadIdx.x; l < NL; l += Shape<L>::T) {
         XInvLine m;
         long g = g0 + l;
@@ -874,6 +925,14 @@ __global__ __launch_bounds__(256) void k_irfft_x(Batch<XInv<P>> bt) {
     run_from<L, NL, false, false, 0, -1, PV>((P)1, tw_s, buf, first, last, [&] { twp.store(tw_s); });
 }
 
+// grid: (line groups, cube)
+template <int L, class P, int PV = 0>
+__global__ __launch_bounds__(256) void k_irfft_x(Batch<XInv<P>> bt) {
+    __shared__ RowLds<L, P, XInvLine> sm;
+    const XInv<P> a = bt.a[blockIdx.y];             // into registers once
+    irfft_x_body<L, P, PV>(a, blockIdx.x, sm);
+}
+
 unsigned grid1d(size_t n) {
     size_t b = (n + 255) / 256;
     return (unsigned)(b < 8192 ? (b ? b : 1) : 8192);
@@ -907,20 +966,25 @@ __device__ __forceinline__ float2 *plane_col(float2 *col, int k, int y) {
     return col + k * Shape<L>::LD + lds_pos(y);
 }
 
-// grid: (pz planes, cube)
+// LDS of one plane workgroup
+template <int L, class Line>
+struct PlaneLds {
+    float2 col[PlaneShape<L>::HX * Shape<L>::LD];
+    float2 xb[PlaneShape<L>::NLX * Shape<L>::LD];
+    float2 tw_s[L];
+    Line meta[PlaneShape<L>::NLX];
+};
+
+// plane z of one cube's fused X+Y forward pass
 template <int L, int MODE, int PV = 0>
-__global__ __launch_bounds__(256) void k_rfft_xy(Batch<XFwd<float>> bt) {
+__device__ __forceinline__ void rfft_xy_body(const XFwd<float> &a, int z, PlaneLds<L, XLine> &sm) {
     using V = float2;
     using PS = PlaneShape<L>;
     constexpr int HX = PS::HX, NLX = PS::NLX, LD = Shape<L>::LD;
     constexpr bool U8 = MODE != 0, RING = MODE == 2;
-    const XFwd<float> a = bt.a[blockIdx.y];
-    __shared__ V col[HX * LD];
-    __shared__ V xb[NLX * LD];
-    __shared__ V tw_s[L];
-    __shared__ XLine meta[NLX];
+    V *col = sm.col, *xb = sm.xb, *tw_s = sm.tw_s;
+    XLine *meta = sm.meta;
     const TwPrefetch<L, V> twp(a.tw);                       // px == py: one table
-    const int z = blockIdx.x;
     const int rz = reflect_idx(z, a.nz);
     const long gz = a.pro.gz0 + rz;
     const float zf = (U8 && a.pro.zdrift) ? a.pro.zfactor[gz] : 1.0f;
@@ -998,19 +1062,23 @@ __global__ __launch_bounds__(256) void k_rfft_xy(Batch<XFwd<float>> bt) {
         for (int i = threadIdx.x; i < HX * L; i += Shape<L>::T) last(i % HX, i / HX, 0, col[(i % HX) * LD + lds_pos(i / HX)]);
 }
 
-// grid: (nz planes, cube)
+// grid: (pz planes, cube)
+template <int L, int MODE, int PV = 0>
+__global__ __launch_bounds__(256) void k_rfft_xy(Batch<XFwd<float>> bt) {
+    __shared__ PlaneLds<L, XLine> sm;
+    const XFwd<float> a = bt.a[blockIdx.y];
+    rfft_xy_body<L, MODE, PV>(a, blockIdx.x, sm);
+}
+
+// plane z of one cube's fused Y+X inverse pass
 template <int L, int PV = 0>
-__global__ __launch_bounds__(256) void k_irfft_yx(Batch<XInv<float>> bt) {
+__device__ __forceinline__ void irfft_yx_body(const XInv<float> &a, int z, PlaneLds<L, XInvLine> &sm) {
     using V = float2;
     using PS = PlaneShape<L>;
     constexpr int HX = PS::HX, NLX = PS::NLX, LD = Shape<L>::LD;
-    const XInv<float> a = bt.a[blockIdx.y];
-    __shared__ V col[HX * LD];
-    __shared__ V xb[NLX * LD];
-    __shared__ V tw_s[L];
-    __shared__ XInvLine meta[NLX];
+    V *col = sm.col, *xb = sm.xb, *tw_s = sm.tw_s;
+    XInvLine *meta = sm.meta;
     const TwPrefetch<L, V> twp(a.tw);
-    const int z = blockIdx.x;
     const V *plane = a.spec + (size_t)z * a.py * a.hxp;
     const long hxp = a.hxp;
     // column FFTs of the plane (k < hx), inverse sign, result kept in LDS
@@ -1049,6 +1117,14 @@ __global__ __launch_bounds__(256) void k_irfft_yx(Batch<XInv<float>> bt) {
         run_from<L, NLX, false, false, 0, -1, PV>(1.0f, tw_s, xb, first, last);
         __syncthreads();
     }
+}
+
+// grid: (nz planes, cube)
+template <int L, int PV = 0>
+__global__ __launch_bounds__(256) void k_irfft_yx(Batch<XInv<float>> bt) {
+    __shared__ PlaneLds<L, XInvLine> sm;
+    const XInv<float> a = bt.a[blockIdx.y];
+    irfft_yx_body<L, PV>(a, blockIdx.x, sm);
 }
 
 // ------------------------------------------------------------------ dispatch over L

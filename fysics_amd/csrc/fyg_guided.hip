@@ -15,11 +15,13 @@
 // Exact path (guided.c:113-137):   k_moments(I) -> a,b ; k_box2(a,b) -> out = ma*I + mb
 // Fast path  (guided.c:152-231):   k_moments(I decimated by s) -> a,b (low-res);
 //                                  k_box2 -> ma, mb (low-res); k_upsample -> out
+#include <algorithm>
 #include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 
@@ -350,19 +352,23 @@ __device__ __forceinline__ int xcd_tile(int h, int T) {
     return (x < r ? x * (c + 1) : r * (c + 1) + (x - r) * c) + i;
 }
 
-// grid: (x blocks, y blocks, cube * nzc + z chunk), renumbered by xcd_tile
+// LDS of one coefficient-march workgroup
 template <int RHO, class P, int ZC>
-__global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
+struct CoefLds {
     using M = March<RHO, ZC>;
-    const int T = gridDim.x * gridDim.y * gridDim.z;
-    const int L = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), T);
-    const int tx = L % gridDim.x, ty = (L / gridDim.x) % gridDim.y, tz = L / (gridDim.x * gridDim.y);
-    const CoefArgs<P> p = bt.a[tz / nzc];          // into registers once
-    const int zchunk = tz % nzc;
+    P sv[M::SW][M::SP];
+    P sa[M::AW][M::AP], sb[M::AW][M::AP];
+};
+
+// column block (tx, ty), z chunk zchunk of one cube's coefficient march
+template <int RHO, class P, int ZC>
+__device__ __forceinline__ void coef_body(const CoefArgs<P> &p, int tx, int ty, int zchunk, CoefLds<RHO, P, ZC> &sm) {
+    using M = March<RHO, ZC>;
     static_assert(M::AW * M::AW == NT, "one thread per a, b column");
     static_assert(M::SW <= M::SP, "sample row fits its pitch");
-    __shared__ P sv[M::SW][M::SP];
-    __shared__ P sa[M::AW][M::AP], sb[M::AW][M::AP];
+    auto &sv = sm.sv;
+    auto &sa = sm.sa;
+    auto &sb = sm.sb;
     const Grid &g = p.g;
     const int bx = tx * M::OW, by = ty * M::OW, zs = zchunk * M::ZC;
     const int ze = min(zs + M::ZC, g.nz);
@@ -491,6 +497,17 @@ __global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
     }
 }
 
+// grid: (x blocks, y blocks, cube * nzc + z chunk), renumbered by xcd_tile
+template <int RHO, class P, int ZC>
+__global__ __launch_bounds__(NT) void k_coef(Batch<CoefArgs<P>> bt, int nzc) {
+    const int T = gridDim.x * gridDim.y * gridDim.z;
+    const int L = xcd_tile(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), T);
+    const int tx = L % gridDim.x, ty = (L / gridDim.x) % gridDim.y, tz = L / (gridDim.x * gridDim.y);
+    const CoefArgs<P> p = bt.a[tz / nzc];          // into registers once
+    __shared__ CoefLds<RHO, P, ZC> sm;
+    coef_body<RHO, P, ZC>(p, tx, ty, tz % nzc, sm);
+}
+
 struct Box1Args {
     Grid g;
     const float *in;
@@ -566,17 +583,24 @@ struct UpCoord {
     }
 };
 
-// grid: (x chunks, y blocks, cube * nzb + z block)
+// LDS of one upsample workgroup
+template <class P, int URZ, int LPR>
+struct UpLds {
+    using S = UpShape<URZ, LPR>;
+    P pa[S::UPZ][S::UPY][S::UPK], pb[S::UPZ][S::UPY][S::UPK];
+};
+
+// x chunk bx, y block by, z block bz of one cube's upsample
 template <bool P2, class P, int URZ, int LPR>
-__global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
+__device__ __forceinline__ void upsample_body(const UpArgs<P> &p, int bx, int by, int bz, UpLds<P, URZ, LPR> &sm) {
     using S = UpShape<URZ, LPR>;
     constexpr int RPT = S::RPT, UPZ = S::UPZ, UPY = S::UPY, UPK = S::UPK, SW = S::SW, RY = S::RY;
-    const UpArgs<P> p = bt.a[blockIdx.z / nzb];     // into registers once, not re-read per use
-    __shared__ P pa[UPZ][UPY][UPK], pb[UPZ][UPY][UPK];
+    auto &pa = sm.pa;
+    auto &pb = sm.pb;
     const UpCoord<P2, P> U{(P)p.s, (P)1 / (P)p.s, P2 ? __builtin_ctz((unsigned)p.s) : 0};
-    const int Z0 = p.z0 + (int)(blockIdx.z % nzb) * URZ, Y0 = p.y0 + blockIdx.y * RY;
+    const int Z0 = p.z0 + bz * URZ, Y0 = p.y0 + by * RY;
     const int zl = min(Z0 + URZ, p.z0 + p.dz) - 1, yl = min(Y0 + RY, p.y0 + p.dy) - 1;
-    const int xc0 = p.x0 + blockIdx.x * S::XW;
+    const int xc0 = p.x0 + bx * S::XW;
     const int xc1 = min(xc0 + S::XW, p.x0 + p.dx) - 1;
     const int kb = U.i0(xc0, p.lx), nk = U.i0(xc1, p.lx) - kb + 2;
     const int zlo = U.i0(Z0, p.lz), nzp = U.i0(zl, p.lz) - zlo + 2;
@@ -702,6 +726,14 @@ __global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
             for (int e = 0; e < UPX && xb + e <= xc1; e++) emit(p.em, p.ny, p.nx, z, y, xb + e, of[e]);
         }
     }
+}
+
+// grid: (x chunks, y blocks, cube * nzb + z block)
+template <bool P2, class P, int URZ, int LPR>
+__global__ __launch_bounds__(NT) void k_upsample(Batch<UpArgs<P>> bt, int nzb) {
+    __shared__ UpLds<P, URZ, LPR> sm;
+    const UpArgs<P> p = bt.a[blockIdx.z / nzb];     // into registers once, not re-read per use
+    upsample_body<P2, P, URZ, LPR>(p, blockIdx.x, blockIdx.y, blockIdx.z % nzb, sm);
 }
 
 struct EmitArgs {
@@ -1186,6 +1218,115 @@ static int fast_batch(const fyg_guided_args *a, int n, fyg_stream *s) {
         g_tune[key] = rec;
     }
     launch_upsample(s, u.data(), n, sd, rec.up, from);
+    return fyg_last_error();
+}
+
+// ---- the fast filter of cubes of mixed shapes: one launch for every cube's
+// coefficient march and one for every cube's upsample (segments as in
+// fyg_mix.hip).  The tile shapes are the first candidates (every candidate
+// gives the same bits); the upsample's row chunk follows the region's width
+// as in the grouped launches.
+namespace {
+// kind: the window radius on the decimated grid (1 or 2); nbx, nby: the
+// cube's x and y column blocks, nbt its workgroups (a segment starts on a
+// multiple of 8, so xcd_tile's XCD assignment holds within the cube)
+__global__ __launch_bounds__(NT) void k_mix_coef(MixBatch<CoefArgs<float>> mb) {
+    const MixSegs &t = mb.t;
+    const int sg = mix_seg(t, blockIdx.x);
+    const int l = mix_uniform((int)blockIdx.x - t.start[sg]), T = mix_uniform(t.nbt[sg]);
+    if (l >= T) return;
+    const int gx = mix_uniform(t.nbx[sg]), gy = mix_uniform(t.nby[sg]);
+    const int L = mix_uniform(xcd_tile(l, T));
+    const int tx = mix_uniform(L % gx), ty = mix_uniform((L / gx) % gy), tz = mix_uniform(L / (gx * gy));
+    const CoefArgs<float> p = mb.a[t.idx[sg]];      // into registers once
+    if (t.kind[sg] == 2) {
+        constexpr int ZC = CoefCand<2>::z[0];
+        coef_body<2, float, ZC>(p, tx, ty, tz, mix_lds_as<CoefLds<2, float, ZC>>());
+    } else {
+        constexpr int ZC = CoefCand<1>::z[0];
+        coef_body<1, float, ZC>(p, tx, ty, tz, mix_lds_as<CoefLds<1, float, ZC>>());
+    }
+}
+
+// kind: lanes per row chunk (64, 32 or 16); nbx, nby: x chunks and y blocks
+__global__ __launch_bounds__(NT) void k_mix_up(MixBatch<UpArgs<float>> mb) {
+    constexpr int URZ = kUpRZ[0];
+    const MixSegs &t = mb.t;
+    const int sg = mix_seg(t, blockIdx.x);
+    const int l = mix_uniform((int)blockIdx.x - t.start[sg]);
+    const int gx = mix_uniform(t.nbx[sg]), gy = mix_uniform(t.nby[sg]);
+    const int bx = mix_uniform(l % gx), by = mix_uniform((l / gx) % gy), bz = mix_uniform(l / (gx * gy));
+    // read from the arguments where used: a register copy of every field, live
+    // across the three bodies, exceeds the scalar registers and spills
+    const UpArgs<float> &p = mb.a[t.idx[sg]];
+    switch (t.kind[sg]) {
+    case 16: upsample_body<true, float, URZ, 16>(p, bx, by, bz, mix_lds_as<UpLds<float, URZ, 16>>()); break;
+    case 32: upsample_body<true, float, URZ, 32>(p, bx, by, bz, mix_lds_as<UpLds<float, URZ, 32>>()); break;
+    default: upsample_body<true, float, URZ, 64>(p, bx, by, bz, mix_lds_as<UpLds<float, URZ, 64>>()); break;
+    }
+}
+
+}  // namespace
+
+// the cube's fast filter can run in a mixed launch: fp32, s a power of two,
+// a decimated window radius of 1 or 2, a decimated grid of at least 2 per axis
+extern "C" int fyg_mix_guided_ok(const fyg_guided_args *a) {
+    const int sd = a->s, r = a->r < 1 ? 2 : a->r;
+    if (a->f64 || sd < 2 || (sd & (sd - 1)) || r / sd > 2) return 0;
+    return (a->nz + sd - 1) / sd >= 2 && (a->ny + sd - 1) / sd >= 2 && (a->nx + sd - 1) / sd >= 2;
+}
+
+extern "C" int fyg_mix_guided_run(const fyg_guided_args *a, int n, fyg_stream *s) {
+    if (n < 1) return 0;
+    if (n > FYG_MIX_MAX) return 1;
+    for (int i = 0; i < n; i++)
+        if (!fyg_mix_guided_ok(&a[i])) return 1;
+    std::vector<CoefArgs<float>> c(n);
+    std::vector<UpArgs<float>> u(n);
+    std::vector<int> rl(n), ord(n);
+    for (int i = 0; i < n; i++) {
+        const int sd = a[i].s, r = a[i].r < 1 ? 2 : a[i].r;
+        const int lz = (a[i].nz + sd - 1) / sd, ly = (a[i].ny + sd - 1) / sd, lx = (a[i].nx + sd - 1) / sd;
+        const size_t nl = (size_t)lz * ly * lx;
+        rl[i] = r / sd < 1 ? 1 : r / sd;
+        float *ws = (float *)a[i].ws;
+        float *MA = ws + 2 * nl, *MB = ws + 3 * nl;
+        c[i] = CoefArgs<float>{Grid{lz, ly, lx, sd, a[i].ny, a[i].nx}, guide<float>(&a[i]), a[i].eps, MA, MB};
+        u[i] = up_args<float>(&a[i], MA, MB);
+        ord[i] = i;
+    }
+    // largest volume first
+    std::stable_sort(ord.begin(), ord.end(), [&](int p, int q) {
+        return (size_t)a[p].nz * a[p].ny * a[p].nx > (size_t)a[q].nz * a[q].ny * a[q].nx;
+    });
+    MixStage sc, su;
+    for (int i : ord) {
+        const dim3 g = rl[i] == 2 ? march_grid<2, CoefCand<2>::z[0]>(c[i].g) : march_grid<1, CoefCand<1>::z[0]>(c[i].g);
+        const size_t lds = rl[i] == 2 ? sizeof(CoefLds<2, float, CoefCand<2>::z[0]>)
+                                      : sizeof(CoefLds<1, float, CoefCand<1>::z[0]>);
+        sc.add(i, rl[i], (long)g.x * g.y * g.z, lds, (int)g.x, (int)g.y, 8);
+        const UpArgs<float> &p = u[i];
+        if ((size_t)p.dz * p.dy * p.dx == 0) continue;
+        const int lpr = up_lanes(p.dx), urz = kUpRZ[0];
+        const int xw = lpr * UPX, ry = URY * (64 / lpr);
+        const int gx = (p.dx + xw - 1) / xw, gy = (p.dy + ry - 1) / ry, gz = (p.dz + urz - 1) / urz;
+        const size_t ul = lpr == 16 ? sizeof(UpLds<float, kUpRZ[0], 16>)
+                          : lpr == 32 ? sizeof(UpLds<float, kUpRZ[0], 32>) : sizeof(UpLds<float, kUpRZ[0], 64>);
+        su.add(i, lpr, (long)gx * gy * gz, ul, gx, gy);
+    }
+    if (sc.total > 0x7fffffff || su.total > 0x7fffffff) return 1;
+    if (sc.total) {
+        auto mb = std::make_unique<MixBatch<CoefArgs<float>>>();
+        mb->t = sc.t;
+        for (int i = 0; i < n; i++) mb->a[i] = c[i];
+        fyg_launch(s, "mix_coef", k_mix_coef, dim3((unsigned)sc.total), dim3(NT), sc.lds, *mb);
+    }
+    if (su.total) {
+        auto mb = std::make_unique<MixBatch<UpArgs<float>>>();
+        mb->t = su.t;
+        for (int i = 0; i < n; i++) mb->a[i] = u[i];
+        fyg_launch(s, "mix_up", k_mix_up, dim3((unsigned)su.total), dim3(NT), su.lds, *mb);
+    }
     return fyg_last_error();
 }
 

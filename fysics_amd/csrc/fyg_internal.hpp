@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "fyg.h"
 
 struct fyg_stream {
@@ -58,6 +60,77 @@ __device__ __forceinline__ void fyg_gen_step_block(int *gen, int nflag) {
 template <class T>
 struct Batch {
     T a[FYG_BATCH];
+};
+
+// segment table of a launch over cubes of mixed shapes: segment i covers
+// blocks [start[i], start[i+1]) and runs body kind[i] on descriptor idx[i];
+// nbx, nby, nbt carry the cube's grid shape where the body needs it
+struct MixSegs {
+    int n;
+    int start[FYG_MIX_MAX + 1];
+    unsigned short kind[FYG_MIX_MAX];
+    unsigned short idx[FYG_MIX_MAX];
+    int nbx[FYG_MIX_MAX], nby[FYG_MIX_MAX], nbt[FYG_MIX_MAX];
+};
+
+// the segment of block b: the last one that starts at or before it (the
+// starts sit in the kernel arguments; the search is uniform across the block)
+__device__ __forceinline__ int mix_seg(const MixSegs &t, int b) {
+    int lo = 0, hi = t.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t.start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// a block-uniform integer kept in a scalar register: values derived from the
+// segment (block offsets, quotients of the grid shape) would otherwise come
+// out of the integer-division expansion in vector registers and take the
+// bodies' per-row index arithmetic off the scalar unit
+__device__ __forceinline__ int mix_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// the dynamic LDS of a mixed launch, viewed as one body's LDS struct
+extern __shared__ __align__(16) unsigned char fyg_mix_lds[];
+template <class T>
+__device__ __forceinline__ T &mix_lds_as() {
+    return *reinterpret_cast<T *>(fyg_mix_lds);
+}
+
+// the arguments of a mixed launch: the segment table and every cube's
+// descriptor, by value (up to ~30 KB of kernel arguments), so the bodies see
+// the descriptors' pointers as global ones exactly as in the per-shape
+// kernels and nothing is uploaded before the launch
+template <class T>
+struct MixBatch {
+    MixSegs t;
+    T a[FYG_MIX_MAX];
+};
+
+// host side: appends segments in launch order; the largest dynamic LDS any
+// body of the launch needs
+struct MixStage {
+    MixSegs t;
+    size_t lds = 0;
+    long total = 0;
+    MixStage() { memset(&t, 0, sizeof t); }
+    // blocks: the cube's workgroups; align: its first block's index is a
+    // multiple of align (the blocks of the gap exit at once)
+    void add(int idx, int kind, long blocks, size_t lds_b, int nbx = 0, int nby = 0, int align = 1) {
+        if (blocks <= 0) return;
+        total = (total + align - 1) / align * align;
+        t.start[t.n] = (int)total;
+        t.kind[t.n] = (unsigned short)kind;
+        t.idx[t.n] = (unsigned short)idx;
+        t.nbx[t.n] = nbx;
+        t.nby[t.n] = nby;
+        t.nbt[t.n] = (int)blocks;
+        t.n++;
+        total += blocks;
+        t.start[t.n] = (int)total;
+        if (lds_b > lds) lds = lds_b;
+    }
 };
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {

@@ -345,4 +345,5 @@ long fy_gpu_table_cache(long *stats, long limit) {
 }
 
 int fy_gpu_fft_planes(int mode) { return fyg_fft_plane_set(mode); }
+int fy_gpu_batch_mix(int mode) { return fyg_mix_set(mode); }
 int fy_gpu_fft_radix(int mode) { return fyg_fft_radix_mode(mode); }

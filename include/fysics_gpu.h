@@ -114,10 +114,26 @@ typedef struct {
 } fy_cube;
 
 /* process n cubes on one stream.  Cubes of one region shape (and stage set)
- * share launches: one per stage per 16 cubes, in order of first appearance.  A
+ * share launches: one per stage per 16 cubes, in order of first appearance;
+ * batches of several shapes run as one launch per stage (fy_gpu_batch_mix).  A
  * cube's output does not depend on what it is submitted with.  No reference
  * counterpart: the batched form of process_tile (pipeline.c:252-297). */
 int fy_gpu_process_cubes(const fy_cube *cubes, int n, void *stream);
+/* how fy_gpu_process_cubes (and its graph, and the served waves) launch a
+ * batch of several shapes.  When every cube runs the u8 chain of
+ * deconvolution and fast guided filter (fp32, s a power of two) at padded
+ * lengths of 64, 96, 128, 192 or 256, each stage can be ONE launch over many
+ * cubes of mixed shapes (at most 128 per launch): the grid is cut into one
+ * segment per cube, largest first, and each segment runs the body its length
+ * selects.  mode 2: one such launch per stage for the whole batch; mode 1
+ * (default): the batch is packed into parts whose half spectra fit 96 MB and
+ * each part is one launch per stage, the parts alternating between two
+ * streams (the 256 MB last-level cache then keeps the spectra of the parts in
+ * flight between their passes); mode 0: one launch per stage per shape group
+ * and 16 cubes, the groups forked over helper streams.  Served waves use
+ * mode 0.  All modes give the same bits.  Returns the previous mode.  No
+ * reference counterpart. */
+int fy_gpu_batch_mix(int mode);
 /* the batched form captured: fy_gpu_graph_cubes runs the cubes once, then
  * records the same launches (one per stage per shape group and 16 cubes) into a
  * HIP graph; fy_graph_launch replays them on `stream` with whatever the cubes'

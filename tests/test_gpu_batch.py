@@ -230,3 +230,75 @@ def test_odd_inner_origin(orc, halo):
         assert diff.max() <= 1 and (diff > 0).mean() < 2e-3, f"cube {i}"
         _, f64 = fc.oracle64(orc)
         assert rel_l2(gf, f64) <= 1e-5, f"cube {i}"
+
+
+def make_mix_cases(rng, rings):
+    """Eight cubes of five shapes whose padded lengths the single-launch form
+    covers: 64^3 and 96^3 (fused plane passes), 64x128x128, 192x64x96 and
+    128^3 with mixed axes (row passes); one cube all air.  rings: every cube
+    with ring removal (the mixed launches then carry ring images)."""
+    ring = None
+    if rings:
+        ring, _ = ring_model((24, 48, 56), np.random.default_rng(7))
+        assert ring.have_rings
+    dims = [(56, 60, 64), (80, 90, 96), (60, 100, 120), (130, 64, 70), (56, 60, 64), (100, 110, 128),
+            (80, 90, 96), (56, 60, 64)]
+    cases = []
+    for i, shp in enumerate(dims):
+        vol = phantom.sheets(shp, rng, n_sheets=6)
+        u8 = phantom.noisy_u8(phantom.forward(vol, **phantom.CONFIG1_PHYS), rng)
+        if i == 4:
+            u8[:] = 0
+        P = cube_params(rng, ring)
+        origin = (int(rng.integers(0, 8)), int(rng.integers(0, 12)), int(rng.integers(0, 16)))
+        cases.append(Case(u8, P, origin))
+    cases[0].ring = ring
+    return cases
+
+
+def kernels_run(fn):
+    """names of the kernels (launch sites) that fn() ran"""
+    lib = F.lib()
+    torch.cuda.synchronize()
+    lib.fy_gpu_profile_reset()
+    lib.fy_gpu_profile(1)
+    try:
+        out = fn()
+    finally:
+        lib.fy_gpu_profile(0)
+    prof = F.profile_read()
+    lib.fy_gpu_profile_reset()
+    return out, set(prof)
+
+
+@pytest.mark.parametrize("rings", [False, True])
+def test_mixed_launch_equals_grouped(orc, rings):
+    """A batch of several shapes in one launch per stage (the default) against
+    the same batch in grouped launches and each cube alone: the same bits.  The
+    u8 output is within +-1 of the oracle's on < 0.2 % of voxels."""
+    rng = np.random.default_rng(41 + rings)
+    cases = make_mix_cases(rng, rings)
+    lib = F.lib()
+    print(f"mixed launches (rings {rings}): one launch per stage", flush=True)
+    run(cases)                  # decides the radix plans of new lengths (timed with the per-shape kernels)
+    mixed, names = kernels_run(lambda: run(cases))
+    assert {"mix_xfwd", "mix_z", "mix_xinv", "mix_coef", "mix_up"} <= names, sorted(names)
+    assert not any(n.startswith("rfft_x") or n == "zfused" or n == "guided_coef" for n in names), sorted(names)
+    old = lib.fy_gpu_batch_mix(0)
+    try:
+        print("mixed launches: grouped", flush=True)
+        grouped, gnames = kernels_run(lambda: run(cases))
+        assert not any(n.startswith("mix_") for n in gnames)
+    finally:
+        lib.fy_gpu_batch_mix(old)
+    for i, c in enumerate(cases):
+        assert np.array_equal(mixed[i], grouped[i]), f"cube {i} {c.u8.shape}: mixed and grouped differ"
+        assert np.array_equal(mixed[i], run([c])[0]), f"cube {i}: mixed and alone differ"
+        if i == 4:
+            assert not mixed[i].any()
+            continue
+        assert mixed[i].any()
+        if i in (0, 3, 5):
+            want, _, air = c.oracle(orc)
+            diff = np.abs(mixed[i].astype(int) - want.astype(int))
+            assert diff.max() <= 1 and (diff > 0).mean() < 2e-3, f"cube {i}"

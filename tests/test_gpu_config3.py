@@ -4,8 +4,10 @@ and physics (delta/beta U[100,1000], pixel {2.4,4.3,7.9} um, sigma_z/sigma_xy
 U[1.0,1.3]), u8 -> dewindow -> Wiener deconvolution -> guided r=2 s=2 ->
 dithered u8, through fy_gpu_process_cubes and through its captured graph.
 
-Bars: every cube's u8 output is bit-identical alone, in the 96-cube mix and in
-the graph replay; for one cube of each size class the float tile after the
+The 96 cubes run as one launch per stage for the whole batch, as one launch
+per stage per cache-sized part (the default), as grouped launches per shape,
+alone and as a captured graph.  Bars: every cube's u8 output is bit-identical
+in all of them; for one cube of each size class the float tile after the
 guided filter is within relative L2 1e-5 of the fp64 oracle
 (orc_chain_cube_f64: deconvolution and guided filter in double, same LUT) and
 the u8 tile within +-1 LSB of the oracle's on < 0.1 % of voxels.
@@ -58,23 +60,68 @@ def _cube(b, out, d_float=None):
                   d_float.data_ptr() if d_float is not None else None, C.pointer(b["P"]))
 
 
+def _launches(lib, fn):
+    """kernel launch sites and counts of fn()"""
+    torch.cuda.synchronize()
+    lib.fy_gpu_profile_reset()
+    lib.fy_gpu_profile(1)
+    try:
+        fn()
+    finally:
+        lib.fy_gpu_profile(0)
+    out = F.profile_read()
+    lib.fy_gpu_profile_reset()
+    return out
+
+
 def test_config3_mix_alone_and_graph(batch):
     lib = F.lib()
     assert len(batch) == 96 and sum(int(np.prod(b["shape"])) for b in batch) == 169738240
     stream = torch.cuda.Stream()
     sh = C.c_void_p(stream.cuda_stream)
     arr = (F.Cube * 96)(*[_cube(b, b["mix"]) for b in batch])
-    print("config 3: 96 cubes in one call", flush=True)
-    assert lib.fy_gpu_process_cubes(arr, 96, sh) == 0, lib.fy_gpu_last_error()
-    stream.synchronize()
+    whole = [torch.zeros_like(b["d_in"]) for b in batch]
+    grouped = [torch.zeros_like(b["d_in"]) for b in batch]
+    warr = (F.Cube * 96)(*[_cube(b, w) for b, w in zip(batch, whole)])
+    garr = (F.Cube * 96)(*[_cube(b, g) for b, g in zip(batch, grouped)])
+
+    def call(a):
+        assert lib.fy_gpu_process_cubes(a, 96, sh) == 0, lib.fy_gpu_last_error()
+        stream.synchronize()
+
+    old = lib.fy_gpu_batch_mix(2)
+    try:
+        # a first call decides the radix plans of lengths new to the process
+        # (timed with the per-shape kernels); the profiled call then runs only
+        # its own launches
+        print("config 3: 96 cubes, one launch per stage for the whole batch", flush=True)
+        call(warr)
+        launches = _launches(lib, lambda: call(warr))
+        print("  launches " + ", ".join(f"{k} x{v[1]}" for k, v in sorted(launches.items())), flush=True)
+        # X forward, Y forward, Z, Y inverse, X inverse, coefficient march,
+        # upsample, and the generation step
+        for k in ("mix_xfwd", "mix_z", "mix_xinv", "mix_coef", "mix_up"):
+            assert launches.get(k, (0, 0))[1] == 1, launches
+        assert launches.get("mix_cols", (0, 0))[1] == 2, launches
+        assert sum(v[1] for v in launches.values()) <= 8, launches
+        print("config 3: 96 cubes, grouped launches", flush=True)
+        lib.fy_gpu_batch_mix(0)
+        call(garr)
+        lib.fy_gpu_batch_mix(1)
+        print("config 3: 96 cubes in parts that fit the cache (the default)", flush=True)
+        launches = _launches(lib, lambda: call(arr))
+        print("  launches " + ", ".join(f"{k} x{v[1]}" for k, v in sorted(launches.items())), flush=True)
+        assert all(k.startswith("mix_") or k == "gen_step" for k in launches), launches
+    finally:
+        lib.fy_gpu_batch_mix(old)
     print("config 3: each cube alone", flush=True)
     for b in batch:
         c = _cube(b, b["alone"])
         assert lib.fy_gpu_process_cubes(C.byref(c), 1, sh) == 0, lib.fy_gpu_last_error()
     stream.synchronize()
     print("config 3: captured graph, two replays", flush=True)
-    garr = (F.Cube * 96)(*[_cube(b, b["graph"]) for b in batch])
-    g = lib.fy_gpu_graph_cubes(garr, 96, sh)
+    garr2 = (F.Cube * 96)(*[_cube(b, b["graph"]) for b in batch])
+    g = lib.fy_gpu_graph_cubes(garr2, 96, sh)
     assert g, lib.fy_gpu_last_error()
     try:
         for b in batch:
@@ -89,6 +136,8 @@ def test_config3_mix_alone_and_graph(batch):
         mix = b["mix"].cpu().numpy()
         assert mix.any(), f"cube {k}"
         assert np.array_equal(mix, b["alone"].cpu().numpy()), f"cube {k} {b['shape']}: mix != alone"
+        assert np.array_equal(mix, whole[k].cpu().numpy()), f"cube {k} {b['shape']}: parts != whole batch"
+        assert np.array_equal(mix, grouped[k].cpu().numpy()), f"cube {k} {b['shape']}: mixed != grouped"
         assert np.array_equal(mix, b["graph"].cpu().numpy()), f"cube {k} {b['shape']}: graph != mix"
 
 
