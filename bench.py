@@ -161,6 +161,8 @@ def main():
     ap.add_argument("--served-rate", type=float, default=400.0)
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the 96-cube batched measurement and print its JSON (for profiling)")
+    ap.add_argument("--batched-form", default="",
+                    help="with --batched-only: time only this launch form (mixed_parts, mixed_whole, grouped, ...)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference point that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters into HBM bytes)")
@@ -170,7 +172,7 @@ def main():
         return
     if args.batched_only:
         import fysics_amd as F
-        out = batched_bench(F, reps=args.steps)
+        out = batched_bench(F, reps=args.steps, only=args.batched_form)
         out["fft_radix_plans"] = [l for l in F.tuning_report().splitlines() if l.startswith("fft ")]
         print(json.dumps(out))
         return
@@ -343,7 +345,7 @@ def _cube_params(F, phys, z_ratio, eps):
     return P
 
 
-def batched_bench(F, reps=20, seed=100):
+def batched_bench(F, reps=20, seed=100, only=""):
     """The batched workload: 96 cubes (32x64^3, 32x96^3, 16x128^3, 12x160^3 -> 192^3
     FFT, 4x192x256x256; 169.7 M voxels), each with its own phantom and physics,
     u8 in HBM -> deconv + guided -> u8 in HBM.  Timed as one fy_gpu_process_cubes
@@ -391,6 +393,10 @@ def batched_bench(F, reps=20, seed=100):
 
     def captured(mode):
         def fn():
+            if mode not in graphs:                 # recorded under the form's mode on first use
+                graphs[mode] = lib.fy_gpu_graph_cubes(arr, len(cubes), sh)
+                if not graphs[mode]:
+                    raise RuntimeError(lib.fy_gpu_last_error().decode())
             if lib.fy_graph_launch(C.c_void_p(graphs[mode])):
                 raise RuntimeError(lib.fy_gpu_last_error().decode())
         return fn
@@ -399,12 +405,9 @@ def batched_bench(F, reps=20, seed=100):
             ("grouped", 0, batch), ("grouped_graph", 0, captured(0)), ("one_call_per_cube", 1, one_by_one))
     old = lib.fy_gpu_batch_mix(1)
     try:
-        for mode in (1, 0):
-            lib.fy_gpu_batch_mix(mode)
-            graphs[mode] = lib.fy_gpu_graph_cubes(arr, len(cubes), sh)
-            if not graphs[mode]:
-                raise RuntimeError(lib.fy_gpu_last_error().decode())
         for name, mode, fn in runs:
+            if only and name != only:
+                continue
             lib.fy_gpu_batch_mix(mode)
             for _ in range(3):
                 fn()
