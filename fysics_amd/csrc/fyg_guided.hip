@@ -8,9 +8,10 @@
 // GPU computes them on LDS tiles: a workgroup loads its 32x8x4 output tile plus
 // an r-voxel halo once, then sums along x, y and z inside LDS, so each input
 // voxel is read from HBM about once per kernel instead of once per axis pass.
-// The moment kernel subtracts a per-tile constant c0 from I before forming
-// box(I-c0) and box((I-c0)^2): var = E[(I-c0)^2] - E[I-c0]^2 is the same
-// quantity with far less cancellation than E[I^2] - E[I]^2 in fp32.
+// The moment kernels centre the second moment on a per-tile constant c0:
+// var = E[(I-c0)^2] - (E[I]-c0)^2 is the same quantity with far less
+// cancellation than E[I^2] - E[I]^2 in fp32.  E[I] itself is summed from the
+// samples, as the reference sums it, and forms b = E[I](1-a).
 //
 // Exact path (guided.c:113-137):   k_moments(I) -> a,b ; k_box2(a,b) -> out = ma*I + mb
 // Fast path  (guided.c:152-231):   k_moments(I decimated by s) -> a,b (low-res);
@@ -244,7 +245,11 @@ struct MomArgs {
     float *a, *b;           // grid-sized outputs
 };
 
-// box(I-c0), box((I-c0)^2) -> a = var/(var+eps), b = mean*(1-a)   (guided.c:122-132)
+// box(I), box((I-c0)^2) -> var = box((I-c0)^2) - (box(I)-c0)^2, a = var/(var+eps),
+// b = box(I)*(1-a)   (guided.c:122-132).  The mean that forms b is summed from
+// the samples themselves, as the reference sums it: of samples >= 0 it is > 0
+// unless every one is 0, which the recenter's "air stays 0" test
+// (pipeline.c:417) reads; only the second moment is centred.
 template <int RHO>
 __global__ __launch_bounds__(NT) void k_moments(MomArgs p) {
     extern __shared__ float sm[];
@@ -253,19 +258,21 @@ __global__ __launch_bounds__(NT) void k_moments(MomArgs p) {
     const int bx = blockIdx.x * TX, by = blockIdx.y * TY, bz = blockIdx.z * TZ;
     const float c0 = p.in[((size_t)bz * g.s * g.fny + (size_t)by * g.s) * g.fnx + (size_t)bx * g.s];
     auto load = [&](int z, int y, int x, float *v) {
-        float t = p.in[((size_t)z * g.s * g.fny + (size_t)y * g.s) * g.fnx + (size_t)x * g.s] - c0;
-        v[0] = t;
+        float u = p.in[((size_t)z * g.s * g.fny + (size_t)y * g.s) * g.fnx + (size_t)x * g.s];
+        float t = u - c0;
+        v[0] = u;
         v[1] = t * t;
     };
     const float epsf = (float)p.eps;
     auto out = [&](int z, int y, int x, const float *s, float rc) {
-        float m = s[0] * rc, q = s[1] * rc;
+        float mean = s[0] * rc, q = s[1] * rc;
+        float m = mean - c0;
         float var = q - m * m;
         if (var < 0.0f) var = 0.0f;
         float ai = var / (var + epsf);
         size_t o = ((size_t)z * g.ny + y) * g.nx + x;
         p.a[o] = ai;
-        p.b[o] = (m + c0) * (1.0f - ai);
+        p.b[o] = mean * (1.0f - ai);
     };
     box_tile<RHO, 2>(g, sm, load, out);
 }
@@ -305,10 +312,12 @@ __global__ __launch_bounds__(NT) void k_box2(Box2Args p) {
 // A workgroup owns a 16x16 column block of the decimated grid and marches
 // along z through ZC output planes (plus a 2*rho halo on each side).  Per
 // sample plane it
-//   1. stages the centred samples v = I - c0 of an (16+2rho)^2 patch in LDS,
-//   2. forms each column's x-then-y window sums of v and v^2 and keeps the last
-//      2*rho+1 of them in registers, so their sum is box(v), box(v^2) centred
-//      rho planes back -> a = var/(var+eps), b = (m+c0)(1-a) for that plane,
+//   1. stages the samples I and the centred samples v = I - c0 of an
+//      (16+2rho)^2 patch in LDS,
+//   2. forms each column's x-then-y window sums of I and v^2 and keeps the last
+//      2*rho+1 of them in registers, so their sum is box(I), box(v^2) centred
+//      rho planes back -> var = box(v^2) - (box(I) - c0)^2, a = var/(var+eps),
+//      b = box(I)(1-a) for that plane,
 //   3. stages a, b in LDS and sums them the same way for the (16-2*rho)^2 inner
 //      columns -> mean_a, mean_b one more rho plane back.
 // The halo costs 1.3x in samples instead of the 3.4x of a 3-D tile and a and b
@@ -356,7 +365,8 @@ __device__ __forceinline__ int xcd_tile(int h, int T) {
 template <int RHO, class P, int ZC>
 struct CoefLds {
     using M = March<RHO, ZC>;
-    P sv[M::SW][M::SP];
+    P sv[M::SW][M::SP];                 // centred samples I - c0
+    P sr[M::SW][M::SP];                 // the samples I
     P sa[M::AW][M::AP], sb[M::AW][M::AP];
 };
 
@@ -367,6 +377,7 @@ __device__ __forceinline__ void coef_body(const CoefArgs<P> &p, int tx, int ty, 
     static_assert(M::AW * M::AW == NT, "one thread per a, b column");
     static_assert(M::SW <= M::SP, "sample row fits its pitch");
     auto &sv = sm.sv;
+    auto &sr = sm.sr;
     auto &sa = sm.sa;
     auto &sb = sm.sb;
     const Grid &g = p.g;
@@ -416,7 +427,10 @@ __device__ __forceinline__ void coef_body(const CoefArgs<P> &p, int tx, int ty, 
         const bool zin = zp >= 0 && zp < g.nz;
 #pragma unroll
         for (int k = 0; k < M::KS; k++)
-            if (scell[k] >= 0) (&sv[0][0])[scell[k]] = (zin && sok[k]) ? cur[k] - c0 : 0.0f;
+            if (scell[k] >= 0) {
+                (&sv[0][0])[scell[k]] = (zin && sok[k]) ? cur[k] - c0 : 0.0f;
+                (&sr[0][0])[scell[k]] = (zin && sok[k]) ? cur[k] : 0.0f;
+            }
         __syncthreads();
         // x-then-y window sums of v and v^2 at plane zp
         P s1 = 0.0f, s2 = 0.0f;
@@ -426,7 +440,7 @@ __device__ __forceinline__ void coef_body(const CoefArgs<P> &p, int tx, int ty, 
 #pragma unroll
             for (int dx = 0; dx < M::Q; dx++) {
                 P v = sv[ay + dy][ax + dx];
-                r1 += v;
+                r1 += sr[ay + dy][ax + dx];
                 r2 += v * v;
             }
             s1 += r1;
@@ -442,11 +456,12 @@ __device__ __forceinline__ void coef_body(const CoefArgs<P> &p, int tx, int ty, 
 #pragma unroll
                 for (int j = 1; j <= M::Q; j++) { t1 += q1[(slot + j) % M::Q]; t2 += q2[(slot + j) % M::Q]; }
                 P rc = rcxy * count_recip_tab<P>(za, g.nz, RHO);
-                P m = t1 * rc, q = t2 * rc;
+                P mean = t1 * rc, q = t2 * rc;
+                P m = mean - c0;
                 P var = q - m * m;
                 if (var < 0.0f) var = 0.0f;
                 av = ratio(var, epsf);
-                bv = (m + c0) * (1.0f - av);
+                bv = mean * (1.0f - av);
             }
             sa[ay][ax] = av;
             sb[ay][ax] = bv;
@@ -794,7 +809,7 @@ __global__ void k_box_pass(PassArgs p) {
 }
 
 struct PointArgs {
-    int mode;               // 0: (I-c0),(I-c0)^2 from decimated source; 1: moments -> a,b; 2: apply
+    int mode;               // 0: I,(I-c0)^2 from decimated source; 1: moments -> a,b; 2: apply
     Grid g;
     const float *in;        // the guide; c0 = in[0], the centring constant, is read here on the device
     double eps;
@@ -813,16 +828,18 @@ __global__ void k_point(PointArgs p) {
         size_t t = i / g.nx;
         int y = (int)(t % g.ny), z = (int)(t / g.ny);
         if (p.mode == 0) {
-            float v = p.in[((size_t)z * g.s * g.fny + (size_t)y * g.s) * g.fnx + (size_t)x * g.s] - c0;
-            p.o0[i] = v;
+            float u = p.in[((size_t)z * g.s * g.fny + (size_t)y * g.s) * g.fnx + (size_t)x * g.s];
+            float v = u - c0;
+            p.o0[i] = u;
             p.o1[i] = v * v;
         } else if (p.mode == 1) {
-            float m = p.m0[i], q = p.m1[i];
+            float mean = p.m0[i], q = p.m1[i];
+            float m = mean - c0;
             float var = q - m * m;
             if (var < 0.0f) var = 0.0f;
             float ai = var / (var + (float)p.eps);
             p.o0[i] = ai;
-            p.o1[i] = (m + c0) * (1.0f - ai);
+            p.o1[i] = mean * (1.0f - ai);
         } else {
             emit(p.em, g.ny, g.nx, z, y, x, p.m0[i] * p.in[i] + p.m1[i]);
         }
@@ -1230,8 +1247,9 @@ namespace {
 // kind: the window radius on the decimated grid (1 or 2); nbx, nby: the
 // cube's x and y column blocks, nbt its workgroups (a segment starts on a
 // multiple of 8, so xcd_tile's XCD assignment holds within the cube)
-__global__ __launch_bounds__(NT) void k_mix_coef(MixBatch<CoefArgs<float>> mb) {
-    const MixSegs &t = mb.t;
+template <int C>
+__global__ __launch_bounds__(NT) void k_mix_coef(MixBatch<CoefArgs<float>, C> mb) {
+    const MixSegs<C> &t = mb.t;
     const int sg = mix_seg(t, blockIdx.x);
     const int l = mix_uniform((int)blockIdx.x - t.start[sg]), T = mix_uniform(t.nbt[sg]);
     if (l >= T) return;
@@ -1249,9 +1267,10 @@ __global__ __launch_bounds__(NT) void k_mix_coef(MixBatch<CoefArgs<float>> mb) {
 }
 
 // kind: lanes per row chunk (64, 32 or 16); nbx, nby: x chunks and y blocks
-__global__ __launch_bounds__(NT) void k_mix_up(MixBatch<UpArgs<float>> mb) {
+template <int C>
+__global__ __launch_bounds__(NT) void k_mix_up(MixBatch<UpArgs<float>, C> mb) {
     constexpr int URZ = kUpRZ[0];
-    const MixSegs &t = mb.t;
+    const MixSegs<C> &t = mb.t;
     const int sg = mix_seg(t, blockIdx.x);
     const int l = mix_uniform((int)blockIdx.x - t.start[sg]);
     const int gx = mix_uniform(t.nbx[sg]), gy = mix_uniform(t.nby[sg]);
@@ -1315,18 +1334,22 @@ extern "C" int fyg_mix_guided_run(const fyg_guided_args *a, int n, fyg_stream *s
         su.add(i, lpr, (long)gx * gy * gz, ul, gx, gy);
     }
     if (sc.total > 0x7fffffff || su.total > 0x7fffffff) return 1;
-    if (sc.total) {
-        auto mb = std::make_unique<MixBatch<CoefArgs<float>>>();
-        mb->t = sc.t;
-        for (int i = 0; i < n; i++) mb->a[i] = c[i];
-        fyg_launch(s, "mix_coef", k_mix_coef, dim3((unsigned)sc.total), dim3(NT), sc.lds, *mb);
-    }
-    if (su.total) {
-        auto mb = std::make_unique<MixBatch<UpArgs<float>>>();
-        mb->t = su.t;
-        for (int i = 0; i < n; i++) mb->a[i] = u[i];
-        fyg_launch(s, "mix_up", k_mix_up, dim3((unsigned)su.total), dim3(NT), su.lds, *mb);
-    }
+    // launches of capacity C (the kernel-argument bytes follow the capacity)
+    auto run = [&](auto cap) {
+        constexpr int C = decltype(cap)::value;
+        if (sc.total) {
+            auto mb = std::make_unique<MixBatch<CoefArgs<float>, C>>();
+            sc.fill(*mb, c.data(), n);
+            fyg_launch(s, "mix_coef", k_mix_coef<C>, dim3((unsigned)sc.total), dim3(NT), sc.lds, *mb);
+        }
+        if (su.total) {
+            auto mb = std::make_unique<MixBatch<UpArgs<float>, C>>();
+            su.fill(*mb, u.data(), n);
+            fyg_launch(s, "mix_up", k_mix_up<C>, dim3((unsigned)su.total), dim3(NT), su.lds, *mb);
+        }
+    };
+    if (n <= FYG_MIX_SMALL) run(std::integral_constant<int, FYG_MIX_SMALL>{});
+    else run(std::integral_constant<int, FYG_MIX_MAX>{});
     return fyg_last_error();
 }
 

@@ -62,20 +62,23 @@ struct Batch {
     T a[FYG_BATCH];
 };
 
-// segment table of a launch over cubes of mixed shapes: segment i covers
-// blocks [start[i], start[i+1]) and runs body kind[i] on descriptor idx[i];
-// nbx, nby, nbt carry the cube's grid shape where the body needs it
+// segment table of a launch over cubes of mixed shapes, for up to C cubes:
+// segment i covers blocks [start[i], start[i+1]) and runs body kind[i] on
+// descriptor idx[i]; nbx, nby, nbt carry the cube's grid shape where the body
+// needs it
+template <int C>
 struct MixSegs {
     int n;
-    int start[FYG_MIX_MAX + 1];
-    unsigned short kind[FYG_MIX_MAX];
-    unsigned short idx[FYG_MIX_MAX];
-    int nbx[FYG_MIX_MAX], nby[FYG_MIX_MAX], nbt[FYG_MIX_MAX];
+    int start[C + 1];
+    unsigned short kind[C];
+    unsigned short idx[C];
+    int nbx[C], nby[C], nbt[C];
 };
 
 // the segment of block b: the last one that starts at or before it (the
 // starts sit in the kernel arguments; the search is uniform across the block)
-__device__ __forceinline__ int mix_seg(const MixSegs &t, int b) {
+template <int C>
+__device__ __forceinline__ int mix_seg(const MixSegs<C> &t, int b) {
     int lo = 0, hi = t.n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -99,19 +102,22 @@ __device__ __forceinline__ T &mix_lds_as() {
 }
 
 // the arguments of a mixed launch: the segment table and every cube's
-// descriptor, by value (up to ~30 KB of kernel arguments), so the bodies see
-// the descriptors' pointers as global ones exactly as in the per-shape
-// kernels and nothing is uploaded before the launch
-template <class T>
+// descriptor, by value, so the bodies see the descriptors' pointers as global
+// ones exactly as in the per-shape kernels and nothing is uploaded before the
+// launch.  Two capacities: launches of up to FYG_MIX_SMALL cubes carry a
+// quarter of the bytes (the runtime copies every launch's arguments, ~30 KB
+// at FYG_MIX_MAX cubes).
+#define FYG_MIX_SMALL 32
+template <class T, int C>
 struct MixBatch {
-    MixSegs t;
-    T a[FYG_MIX_MAX];
+    MixSegs<C> t;
+    T a[C];
 };
 
-// host side: appends segments in launch order; the largest dynamic LDS any
-// body of the launch needs
+// host side: appends segments in launch order (up to FYG_MIX_MAX); the
+// largest dynamic LDS any body of the launch needs
 struct MixStage {
-    MixSegs t;
+    MixSegs<FYG_MIX_MAX> t;
     size_t lds = 0;
     long total = 0;
     MixStage() { memset(&t, 0, sizeof t); }
@@ -130,6 +136,21 @@ struct MixStage {
         total += blocks;
         t.start[t.n] = (int)total;
         if (lds_b > lds) lds = lds_b;
+    }
+    // the kernel arguments of capacity C for descriptors d[0..nd)
+    template <int C, class T>
+    void fill(MixBatch<T, C> &mb, const T *d, int nd) const {
+        memset(&mb.t, 0, sizeof mb.t);
+        mb.t.n = t.n;
+        for (int i = 0; i <= t.n; i++) mb.t.start[i] = t.start[i];
+        for (int i = 0; i < t.n; i++) {
+            mb.t.kind[i] = t.kind[i];
+            mb.t.idx[i] = t.idx[i];
+            mb.t.nbx[i] = t.nbx[i];
+            mb.t.nby[i] = t.nby[i];
+            mb.t.nbt[i] = t.nbt[i];
+        }
+        for (int i = 0; i < nd; i++) mb.a[i] = d[i];
     }
 };
 

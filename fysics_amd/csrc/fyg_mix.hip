@@ -59,9 +59,9 @@ constexpr int kPlaneKind = 2 * kNMixLen;
         break;
 
 // X forward: rows (u8 gather + prologue, R2C) or fused X+Y planes
-template <int MODE>
-__global__ __launch_bounds__(256) FYG_MIX_OCC void k_mix_xfwd(MixBatch<XFwd<float>> mb) {
-    const MixSegs &t = mb.t;
+template <int MODE, int C>
+__global__ __launch_bounds__(256) FYG_MIX_OCC void k_mix_xfwd(MixBatch<XFwd<float>, C> mb) {
+    const MixSegs<C> &t = mb.t;
     const int sg = mix_seg(t, blockIdx.x);
     const long b = mix_uniform((int)blockIdx.x - t.start[sg]);
     const XFwd<float> a = mb.a[t.idx[sg]];          // into registers once
@@ -76,8 +76,9 @@ __global__ __launch_bounds__(256) FYG_MIX_OCC void k_mix_xfwd(MixBatch<XFwd<floa
 }
 
 // Y forward or inverse: column groups of every batch (z plane) of a cube
-__global__ __launch_bounds__(256) FYG_MIX_OCC void k_mix_cols(MixBatch<Cols<float>> mb) {
-    const MixSegs &t = mb.t;
+template <int C>
+__global__ __launch_bounds__(256) FYG_MIX_OCC void k_mix_cols(MixBatch<Cols<float>, C> mb) {
+    const MixSegs<C> &t = mb.t;
     const int sg = mix_seg(t, blockIdx.x);
     const int b = mix_uniform((int)blockIdx.x - t.start[sg]), g = mix_uniform(t.nbx[sg]);
     const long bx = mix_uniform(b % g), by = mix_uniform(b / g);
@@ -88,8 +89,9 @@ __global__ __launch_bounds__(256) FYG_MIX_OCC void k_mix_cols(MixBatch<Cols<floa
 }
 
 // Z forward, weight, Z inverse
-__global__ __launch_bounds__(256) FYG_MIX_OCC void k_mix_z(MixBatch<ZFused<float>> mb) {
-    const MixSegs &t = mb.t;
+template <int C>
+__global__ __launch_bounds__(256) FYG_MIX_OCC void k_mix_z(MixBatch<ZFused<float>, C> mb) {
+    const MixSegs<C> &t = mb.t;
     const int sg = mix_seg(t, blockIdx.x);
     const long b = mix_uniform((int)blockIdx.x - t.start[sg]);
     const ZFused<float> &a = mb.a[t.idx[sg]];
@@ -99,8 +101,9 @@ __global__ __launch_bounds__(256) FYG_MIX_OCC void k_mix_z(MixBatch<ZFused<float
 }
 
 // X inverse: rows (C2R + epilogue) or fused Y+X planes
-__global__ __launch_bounds__(256) FYG_MIX_OCC void k_mix_xinv(MixBatch<XInv<float>> mb) {
-    const MixSegs &t = mb.t;
+template <int C>
+__global__ __launch_bounds__(256) FYG_MIX_OCC void k_mix_xinv(MixBatch<XInv<float>, C> mb) {
+    const MixSegs<C> &t = mb.t;
     const int sg = mix_seg(t, blockIdx.x);
     const long b = mix_uniform((int)blockIdx.x - t.start[sg]);
     const XInv<float> a = mb.a[t.idx[sg]];          // into registers once
@@ -120,9 +123,10 @@ struct MixRing {
     RingImg r;
     int nxb;
 };
-__global__ __launch_bounds__(256) void k_mix_ring(MixBatch<MixRing> mb, int *step, int nflag) {
+template <int C>
+__global__ __launch_bounds__(256) void k_mix_ring(MixBatch<MixRing, C> mb, int *step, int nflag) {
     if (step && blockIdx.x == 0) fyg_gen_step_block(step, nflag);
-    const MixSegs &t = mb.t;
+    const MixSegs<C> &t = mb.t;
     const int sg = mix_seg(t, blockIdx.x);
     const int b = mix_uniform((int)blockIdx.x - t.start[sg]);
     const MixRing &m = mb.a[t.idx[sg]];
@@ -253,46 +257,49 @@ extern "C" int fyg_mix_deconv_run(const fyg_deconv_args *a, int n, fyg_stream *s
     for (const MixStage *st : {&sx, &syf, &sz, &syi, &sxi, &sr})
         if (st->total > 0x7fffffff) return 1;
 
-    // the stages' arguments: segments and every cube's descriptor, by value
-    auto args = [](const MixStage &st, auto &mb, const auto &desc) {
-        mb.t = st.t;
-        for (size_t i = 0; i < desc.size(); i++) mb.a[i] = desc[i];
+    // launches of capacity C (the kernel-argument bytes follow the capacity)
+    auto run = [&](auto cap) -> int {
+        constexpr int C = decltype(cap)::value;
+        const fyg_deconv_args &h = a[0];
+        if (mode == 2) {
+            auto mr = std::make_unique<MixBatch<MixRing, C>>();
+            sr.fill(*mr, rings.data(), (int)rings.size());
+            fyg_launch(s, "mix_ring", k_mix_ring<C>, dim3((unsigned)sr.total), dim3(256), 0, *mr, h.step_gen,
+                       h.step_nflag);
+        } else if (h.step_gen && fyg_gen_step(h.step_gen, h.step_nflag, s)) {
+            return 1;
+        }
+        {
+            auto mx = std::make_unique<MixBatch<XFwd<float>, C>>();
+            sx.fill(*mx, xf.data(), n);
+            if (mode == 2)
+                fyg_launch(s, "mix_xfwd", k_mix_xfwd<2, C>, dim3((unsigned)sx.total), dim3(256), sx.lds, *mx);
+            else
+                fyg_launch(s, "mix_xfwd", k_mix_xfwd<1, C>, dim3((unsigned)sx.total), dim3(256), sx.lds, *mx);
+        }
+        auto mc = std::make_unique<MixBatch<Cols<float>, C>>();
+        if (syf.total) {
+            syf.fill(*mc, yf.data(), n);
+            fyg_launch(s, "mix_cols", k_mix_cols<C>, dim3((unsigned)syf.total), dim3(256), syf.lds, *mc);
+        }
+        {
+            auto mz = std::make_unique<MixBatch<ZFused<float>, C>>();
+            sz.fill(*mz, zf.data(), n);
+            fyg_launch(s, "mix_z", k_mix_z<C>, dim3((unsigned)sz.total), dim3(256), sz.lds, *mz);
+        }
+        if (syi.total) {
+            syi.fill(*mc, yi.data(), n);
+            fyg_launch(s, "mix_cols", k_mix_cols<C>, dim3((unsigned)syi.total), dim3(256), syi.lds, *mc);
+        }
+        {
+            auto mi = std::make_unique<MixBatch<XInv<float>, C>>();
+            sxi.fill(*mi, xi.data(), n);
+            fyg_launch(s, "mix_xinv", k_mix_xinv<C>, dim3((unsigned)sxi.total), dim3(256), sxi.lds, *mi);
+        }
+        return 0;
     };
-    const fyg_deconv_args &h = a[0];
-    if (mode == 2) {
-        auto mr = std::make_unique<MixBatch<MixRing>>();
-        args(sr, *mr, rings);
-        fyg_launch(s, "mix_ring", k_mix_ring, dim3((unsigned)sr.total), dim3(256), 0, *mr, h.step_gen,
-                   h.step_nflag);
-    } else if (h.step_gen && fyg_gen_step(h.step_gen, h.step_nflag, s)) {
-        return 1;
-    }
-    {
-        auto mx = std::make_unique<MixBatch<XFwd<float>>>();
-        args(sx, *mx, xf);
-        if (mode == 2)
-            fyg_launch(s, "mix_xfwd", k_mix_xfwd<2>, dim3((unsigned)sx.total), dim3(256), sx.lds, *mx);
-        else
-            fyg_launch(s, "mix_xfwd", k_mix_xfwd<1>, dim3((unsigned)sx.total), dim3(256), sx.lds, *mx);
-    }
-    auto mc = std::make_unique<MixBatch<Cols<float>>>();
-    if (syf.total) {
-        args(syf, *mc, yf);
-        fyg_launch(s, "mix_cols", k_mix_cols, dim3((unsigned)syf.total), dim3(256), syf.lds, *mc);
-    }
-    {
-        auto mz = std::make_unique<MixBatch<ZFused<float>>>();
-        args(sz, *mz, zf);
-        fyg_launch(s, "mix_z", k_mix_z, dim3((unsigned)sz.total), dim3(256), sz.lds, *mz);
-    }
-    if (syi.total) {
-        args(syi, *mc, yi);
-        fyg_launch(s, "mix_cols", k_mix_cols, dim3((unsigned)syi.total), dim3(256), syi.lds, *mc);
-    }
-    {
-        auto mi = std::make_unique<MixBatch<XInv<float>>>();
-        args(sxi, *mi, xi);
-        fyg_launch(s, "mix_xinv", k_mix_xinv, dim3((unsigned)sxi.total), dim3(256), sxi.lds, *mi);
-    }
+    const int rc = n <= FYG_MIX_SMALL ? run(std::integral_constant<int, FYG_MIX_SMALL>{})
+                                      : run(std::integral_constant<int, FYG_MIX_MAX>{});
+    if (rc) return 1;
     return fyg_last_error();
 }

@@ -21,6 +21,11 @@ pytestmark = pytest.mark.gpu
 
 HALO = 4
 
+# a filtered value within this distance of 0 may round to either side of the
+# recenter's air test between two correct fp32 implementations (the float
+# tiles agree to ~2e-6 in absolute terms)
+AIR_EDGE = 8e-6
+
 
 def rel_l2(a, b):
     a = np.asarray(a, np.float64)
@@ -302,3 +307,58 @@ def test_mixed_launch_equals_grouped(orc, rings):
             want, _, air = c.oracle(orc)
             diff = np.abs(mixed[i].astype(int) - want.astype(int))
             assert diff.max() <= 1 and (diff > 0).mean() < 2e-3, f"cube {i}"
+
+
+def test_mixed_launch_options(orc):
+    """The mixed launches with the chain's other per-cube options: z-drift
+    factors (origin z0 != 0), recenter, no dither, decimation s = 4, odd inner
+    tile origins in x.  Bars: the same bits as the grouped launches and as each
+    cube alone; u8 within +-1 of the oracle on < 0.2 % of voxels."""
+    rng = np.random.default_rng(57)
+    vol_z = 400
+    factor = rng.uniform(0.85, 1.15, vol_z).astype(np.float32)
+    dims = [(56, 60, 64), (80, 90, 96), (60, 100, 120), (130, 64, 70), (64, 64, 63)]
+    halos = [(4, 4, 3), (3, 5, 5), (4, 4, 4), (2, 2, 1), (4, 4, 7)]
+    cases = []
+    for i, (shp, halo) in enumerate(zip(dims, halos)):
+        vol = phantom.sheets(shp, rng, n_sheets=6)
+        u8 = phantom.noisy_u8(phantom.forward(vol, **phantom.CONFIG1_PHYS), rng)
+        P = cube_params(rng)
+        P.zdrift_factor = factor.ctypes.data_as(F.c_float_p)
+        P.vol_z = vol_z
+        if i % 2 == 0:
+            P.recenter, P.norm_lo, P.norm_hi = 1, 20, 230
+        if i == 1:
+            P.no_dither = 1
+        if i == 2:
+            P.guided_s = 4
+        origin = (int(rng.integers(1, 200)), int(rng.integers(0, 12)), int(rng.integers(0, 16)))
+        cases.append(Case(u8, P, origin, halo=halo))
+    cases[0].factor = factor
+    lib = F.lib()
+    print("mixed launches with options", flush=True)
+    run(cases)
+    mixed, names = kernels_run(lambda: run(cases))
+    assert {"mix_xfwd", "mix_z", "mix_xinv", "mix_coef", "mix_up"} <= names, sorted(names)
+    old = lib.fy_gpu_batch_mix(0)
+    try:
+        grouped = run(cases)
+    finally:
+        lib.fy_gpu_batch_mix(old)
+    for i, c in enumerate(cases):
+        assert mixed[i].any(), f"cube {i}"
+        assert np.array_equal(mixed[i], grouped[i]), f"cube {i} {c.u8.shape}: mixed and grouped differ"
+        assert np.array_equal(mixed[i], run([c])[0]), f"cube {i}: mixed and alone differ"
+        want, ftile, _ = c.oracle(orc)
+        diff = np.abs(mixed[i].astype(int) - want.astype(int))
+        # recenter keeps a voxel at 0 when its filtered value is <= 0 ("air stays
+        # 0", pipeline.c:417) and lifts any positive value to ~0.05 (u8 ~13): where
+        # the oracle's value lies within rounding of 0 either side may be air
+        h = [o - r for o, r in zip(c.inner_o, c.origin)]
+        pre = ftile[h[0]:h[0] + c.inner[0], h[1]:h[1] + c.inner[1], h[2]:h[2] + c.inner[2]]
+        edge = (np.abs(pre) < AIR_EDGE) & ((mixed[i] == 0) | (want == 0))
+        far = diff[~edge]
+        print(f"  cube {i} {c.u8.shape}: u8 off by one {(far > 0).mean():.2e}, at the air edge "
+              f"{int((edge & (diff > 1)).sum())}", flush=True)
+        assert far.max() <= 1 and (far > 0).mean() < 2e-3, f"cube {i}"
+        assert (edge & (diff > 1)).mean() < 1e-3, f"cube {i}"
