@@ -214,7 +214,9 @@ int radix_plan(int kind, int L, fyg_stream *s, bool may_tune) {
     if (!may_tune) return 0;
     RadixRec r;
     if (time_plans(kind, L, s, r.us) == 0)
-        r.pv = r.us[1] < 0.98f * r.us[0] ? 1 : 0;      // plan 1 only when clearly faster
+        // plan 1 only when clearly faster: plans within a few percent of each other
+        // would otherwise flip with the timing noise from one process to the next
+        r.pv = r.us[1] < 0.95f * r.us[0] ? 1 : 0;
     else
         r.us[0] = r.us[1] = -1.0f;                      // not timed: plan 0
     g_rad[{kind, L}] = r;

@@ -32,8 +32,9 @@ int mix_len_index(int L) {
 
 // The stage kernels keep their LDS in dynamic memory, so the compiler cannot
 // see that the bodies' ~37 KB per workgroup already limits a CU to four
-// workgroups (four waves per SIMD); left alone it schedules for twice that
-// occupancy and keeps fewer loads in flight.  FYG_MIX_OCC states the limit.
+// workgroups (four waves per SIMD).  FYG_MIX_OCC states that limit, so the
+// register allocation of the bodies is planned for the occupancy the launch
+// can reach, as it is in the per-shape kernels with their static LDS.
 #define FYG_MIX_OCC __attribute__((amdgpu_waves_per_eu(1, 4)))
 
 // body kinds: 2*li + pv for the row, column and Z bodies of length kMixLen[li];

@@ -318,7 +318,7 @@ int fy_gpu_fft_planes(int mode);
  * synthetic problem of about 16 M voxels (a one-time ~200 MB allocation and a
  * few ms; never under graph capture or in fy_gpu_deconvolve_noalloc, which
  * take plan 0 for lengths not decided yet) and keeps plan 1 only when it is at
- * least 2 % faster.  The plans round differently (relative L2 ~1e-7 between
+ * least 5 % faster.  The plans round differently (relative L2 ~1e-7 between
  * them), so a length keeps its plan for the process; the choices are saved and
  * loaded with the tile shapes (fy_gpu_tuning_save / _load: "fft<TAB>kind<TAB>
  * length<TAB>plan" lines) and listed by fy_gpu_tuning_report, and
