@@ -362,3 +362,42 @@ def test_mixed_launch_options(orc):
               f"{int((edge & (diff > 1)).sum())}", flush=True)
         assert far.max() <= 1 and (far > 0).mean() < 2e-3, f"cube {i}"
         assert (edge & (diff > 1)).mean() < 1e-3, f"cube {i}"
+
+
+def test_mixed_launch_many_cubes_and_fallback():
+    """140 cubes of three shapes (more than one planning chunk of 128) in every
+    launch form, and a batch where one cube's padded length (48) is outside
+    the mixed kernels' cover, so the whole batch takes the grouped launches:
+    every cube equals its run alone."""
+    rng = np.random.default_rng(71)
+    lib = F.lib()
+    pool = []
+    for shp in [(56, 60, 64), (60, 64, 60), (80, 90, 96)]:
+        vol = phantom.sheets(shp, rng, n_sheets=4)
+        pool.append(phantom.noisy_u8(phantom.forward(vol, **phantom.CONFIG1_PHYS), rng))
+    cases = []
+    for i in range(140):
+        u8 = pool[i % 3]
+        P = cube_params(rng)
+        origin = (int(rng.integers(0, 8)), int(rng.integers(0, 12)), int(rng.integers(0, 16)))
+        cases.append(Case(u8, P, origin))
+    alone = [run([c])[0] for c in cases]
+    old = lib.fy_gpu_batch_mix(1)
+    try:
+        for mode in (1, 2, 0):
+            lib.fy_gpu_batch_mix(mode)
+            print(f"140 cubes, launch form {mode}", flush=True)
+            got, names = kernels_run(lambda: run(cases))
+            assert any(n.startswith("mix_") for n in names) == (mode != 0), sorted(names)
+            for i in range(len(cases)):
+                assert np.array_equal(got[i], alone[i]), f"mode {mode} cube {i}"
+    finally:
+        lib.fy_gpu_batch_mix(old)
+    # one cube padded to 48 in z: the batch runs grouped
+    vol = phantom.sheets((44, 60, 64), rng, n_sheets=4)
+    odd = Case(phantom.noisy_u8(phantom.forward(vol, **phantom.CONFIG1_PHYS), rng), cube_params(rng), (0, 0, 0))
+    batch = cases[:5] + [odd]
+    got, names = kernels_run(lambda: run(batch))
+    assert not any(n.startswith("mix_") for n in names), sorted(names)
+    for i, c in enumerate(batch):
+        assert np.array_equal(got[i], run([c])[0]), f"fallback cube {i}"
