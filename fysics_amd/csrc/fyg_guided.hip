@@ -1289,6 +1289,9 @@ __global__ __launch_bounds__(NT) void k_mix_up(MixBatch<UpArgs<float>, C> mb) {
 
 // the cube's fast filter can run in a mixed launch: fp32, s a power of two,
 // a decimated window radius of 1 or 2, a decimated grid of at least 2 per axis
+static_assert(sizeof(MixBatch<CoefArgs<float>, FYG_MIX_MAX>) <= kMixArgMax, "mixed-launch arguments too large");
+static_assert(sizeof(MixBatch<UpArgs<float>, FYG_MIX_MAX>) <= kMixArgMax, "mixed-launch arguments too large");
+
 extern "C" int fyg_mix_guided_ok(const fyg_guided_args *a) {
     const int sd = a->s, r = a->r < 1 ? 2 : a->r;
     if (a->f64 || sd < 2 || (sd & (sd - 1)) || r / sd > 2) return 0;

@@ -113,6 +113,8 @@ struct MixBatch {
     MixSegs<C> t;
     T a[C];
 };
+// the largest kernel arguments a launch on this runtime was measured to take
+constexpr size_t kMixArgMax = (size_t)48 << 10;
 
 // host side: appends segments in launch order (up to FYG_MIX_MAX); the
 // largest dynamic LDS any body of the launch needs

@@ -172,6 +172,12 @@ size_t body_lds(int what, int L) {
 
 }  // namespace
 
+static_assert(sizeof(MixBatch<XFwd<float>, FYG_MIX_MAX>) <= kMixArgMax, "mixed-launch arguments too large");
+static_assert(sizeof(MixBatch<Cols<float>, FYG_MIX_MAX>) <= kMixArgMax, "mixed-launch arguments too large");
+static_assert(sizeof(MixBatch<ZFused<float>, FYG_MIX_MAX>) <= kMixArgMax, "mixed-launch arguments too large");
+static_assert(sizeof(MixBatch<XInv<float>, FYG_MIX_MAX>) <= kMixArgMax, "mixed-launch arguments too large");
+static_assert(sizeof(MixBatch<MixRing, FYG_MIX_MAX>) <= kMixArgMax, "mixed-launch arguments too large");
+
 extern "C" int fyg_mix_len_ok(int L) { return mix_len_index(L) >= 0; }
 
 // n cubes' deconvolutions (fp32, u8 input, one ring mode), one launch per stage
