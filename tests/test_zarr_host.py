@@ -183,3 +183,30 @@ def test_shared_chunk_mappings_under_threads(tmp_path):
         t.join()
     lib.fy_gpu_zarr_cache_free(cache)
     assert not errors, errors[:3]
+
+
+def test_zarr_cache_threads_under_tsan(tmp_path):
+    """The shared chunk mappings (fy_gpu_zarr_cache) under ThreadSanitizer:
+    fy_zarr.c and the driver tests/tsan/zarr_cache_threads.c built with
+    -fsanitize=thread; eight threads read overlapping regions through a cache
+    that evicts and remaps chunks while others hold them.  Bars: no race
+    reported, every cached region equals the uncached read."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    exe = str(tmp_path / "tsan_zarr")
+    cmd = ["gcc", "-std=gnu11", "-g", "-O1", "-fsanitize=thread", "-I", os.path.join(root, "include"),
+           "-I", os.path.join(root, "fysics_amd", "csrc"), "-I", os.path.join(root, "fysics_amd", "csrc", "host"),
+           os.path.join(here, "tsan", "zarr_cache_threads.c"),
+           os.path.join(root, "fysics_amd", "csrc", "host", "fy_zarr.c"), "-lpthread", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0 and "sanitize" in r.stderr:
+        pytest.skip("ThreadSanitizer not available: " + r.stderr[:200])
+    assert r.returncode == 0, r.stderr
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 exitcode=66")
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, env=env, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr[-2000:]
+    assert r.stdout.startswith("ok"), r.stdout
