@@ -35,7 +35,8 @@ int   fyg_stream_persistent(fyg_stream *s);
 int   fyg_stream_trim(fyg_stream *s);
 /* Fork / join: helper i (< FYG_SUBS) of s is a stream object of its own
  * (workspaces, default priority; persistent when s is), created on first use
- * and freed with s.  fyg_fork makes helpers 0..n-1 wait for the work queued
+ * and freed with s; an eager helper has a hardware queue of its own (a
+ * full-CU-mask stream, up to 12 per process) so it never queues behind s.  fyg_fork makes helpers 0..n-1 wait for the work queued
  * on s so far; fyg_join makes s wait for everything queued on them.  Both are
  * event dependencies, so they also hold inside a graph capture of s. */
 #define FYG_SUBS 3

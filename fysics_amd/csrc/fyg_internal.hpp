@@ -19,6 +19,7 @@ struct fyg_stream {
     int nflag = 0;
     int persist = 0;         // tables from the process-wide cache (stable addresses for graphs)
     int capturing = 0;       // launches are being recorded into a graph: no timing events
+    int own_queue = 0;       // created with a hardware queue of its own (a fork's eager helper)
     struct Arena *arena = nullptr;   // per-stream table slots (fyg_stream_table)
     fyg_stream *sub[FYG_SUBS] = {};  // helper streams of a fork (own workspaces), created on first use
     hipEvent_t fork_ev = nullptr;
