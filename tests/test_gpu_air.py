@@ -161,3 +161,24 @@ def test_despeckle_large_components_256(orc):
     want_q, want_f = run_oracle(orc, u8, P)
     assert np.array_equal(got_f, want_f)
     assert np.array_equal(got_q, want_q)
+
+
+@pytest.mark.parametrize("shape,min_comp", [((26, 70, 150), 6), ((19, 33, 129), 40)])
+def test_despeckle_many_islands(orc, shape, min_comp):
+    """Oblique sheets in air plus speckle islands of one to a few voxels (about
+    2000-7000 components, most of them small and interior), on rows longer than
+    one wave and of odd lengths: the union-find over runs joined along y and z
+    gives the oracle's zero mask bit for bit."""
+    rng = np.random.default_rng(77)
+    z, y, x = np.indices(shape)
+    u8 = np.zeros(shape, np.uint8)
+    sheet = ((z + 0.3 * y + 0.1 * x) % 12) < 3
+    u8[sheet] = rng.integers(190, 220, size=int(sheet.sum()))
+    speck = rng.random(shape) < 0.06
+    u8[speck] = rng.integers(150, 256, size=int(speck.sum()))
+    print(f"despeckle shape={shape} min_comp={min_comp}", flush=True)
+    P = air_params(cut_u8=-1, min_comp=min_comp)
+    got_q, got_f = run_cube(u8, P)
+    want_q, want_f = run_oracle(orc, u8, P)
+    assert np.array_equal(got_f, want_f)
+    assert np.array_equal(got_q, want_q)
