@@ -246,6 +246,7 @@ SIGNATURES = {
     "fy_gpu_tuning_load": (C.c_int, [C.c_char_p]),
     "fy_gpu_tuning_clear": (None, []),
     "fy_gpu_sync": (C.c_int, [_VP]),
+    "fy_gpu_stream_release": (C.c_int, [_VP]),
     "fy_gpu_table_cache": (C.c_long, [C.POINTER(C.c_long), C.c_long]),
     "fy_gpu_fft_planes": (C.c_int, [C.c_int]),
     "fy_gpu_batch_mix": (C.c_int, [C.c_int]),

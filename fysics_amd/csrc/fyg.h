@@ -24,6 +24,9 @@ typedef struct fyg_stream fyg_stream;
 fyg_stream *fyg_thread_stream(void);
 /* wrap an external hipStream_t (NULL -> the thread stream) */
 fyg_stream *fyg_stream_wrap(void *hip_stream);
+/* frees the stream object of hip_stream (NULL: the calling thread's own
+ * stream) after waiting for it; the next call makes a new one */
+int   fyg_stream_release(void *hip_stream);
 void *fyg_stream_handle(fyg_stream *s);          /* the hipStream_t */
 /* a private stream object on hip_stream (NULL: the thread stream's), with its
  * own workspaces and flags and tables from the process-wide cache, for work

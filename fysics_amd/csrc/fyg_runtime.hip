@@ -67,6 +67,28 @@ extern "C" fyg_stream *fyg_stream_wrap(void *hip_stream) {
 
 extern "C" void *fyg_stream_handle(fyg_stream *s) { return s ? (void *)s->s : nullptr; }
 
+// forget the stream object of a caller's stream (NULL: the calling thread's
+// own stream, which is destroyed): waits for its work, then frees its
+// workspaces, table slots and helper streams; the next call starts afresh
+extern "C" int fyg_stream_release(void *hip_stream) {
+    if (!hip_stream) {
+        fyg_stream *st = tls_stream;
+        tls_stream = nullptr;
+        if (st) fyg_stream_free(st);
+        return 0;
+    }
+    fyg_stream *st = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_wrap_mu);
+        auto it = g_wrapped.find(hip_stream);
+        if (it == g_wrapped.end()) return 0;
+        st = it->second;
+        g_wrapped.erase(it);
+    }
+    fyg_stream_private_free(st);
+    return 0;
+}
+
 // a stream object of its own (workspaces, flags) on an existing hipStream_t, for
 // work recorded into a graph: nothing else can grow or free the buffers the
 // graph's kernels address.  Tables come from the process-wide cache.

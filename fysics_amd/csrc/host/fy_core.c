@@ -331,6 +331,7 @@ int fy_gpu_sync(void *stream) {
     fyg_stream *s = fyg_stream_wrap(stream);
     return s ? fyg_sync(s) : 1;
 }
+int fy_gpu_stream_release(void *stream) { return fyg_stream_release(stream); }
 void fy_gpu_profile(int on) { fyg_prof_enable(on); }
 int fy_gpu_profile_read(char *names, int names_len, double *ms, long *count, int max) {
     return fyg_prof_collect(names, names_len, ms, count, max);

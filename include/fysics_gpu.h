@@ -300,6 +300,17 @@ void fy_gpu_tuning_clear(void);
 
 int fy_gpu_sync(void *stream);
 
+/* forget the library's state for `stream` (NULL: the calling thread's own
+ * stream, which is then destroyed): waits for the stream, then frees its
+ * device workspaces, pinned staging, table slots and helper streams (each
+ * helper holds a hardware queue of its own).  The next call on the stream
+ * starts afresh.  No call on the stream may run meanwhile on another thread,
+ * and a server or calibration sweep created on it must be destroyed first.
+ * Returns 0.  No
+ * reference counterpart: the reference's thread-local scratch is never freed
+ * (fft.c:45,109-112; guided.c:49,83). */
+int fy_gpu_stream_release(void *stream);
+
 /* pass structure of the deconvolution for cubes whose padded X and Y axes
  * are one length of 64 or 96 (fp32): mode 1 (default) transforms whole z
  * planes along X and Y in one kernel each way, so the half spectrum makes one

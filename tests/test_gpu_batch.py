@@ -401,3 +401,53 @@ def test_mixed_launch_many_cubes_and_fallback():
     assert not any(n.startswith("mix_") for n in names), sorted(names)
     for i, c in enumerate(batch):
         assert np.array_equal(got[i], run([c])[0]), f"fallback cube {i}"
+
+
+def test_stream_release_and_thread_stream():
+    """fy_gpu_stream_release drops the library's state of a caller's stream (its
+    workspaces and the helper streams of the mixed launches) and of a thread's
+    own stream (NULL); the calls after it rebuild that state and give the same
+    bits."""
+    import threading
+    rng = np.random.default_rng(43)
+    cases = make_mix_cases(rng, False)
+    lib = F.lib()
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+    arr = (F.Cube * len(cases))(*[c.cube() for c in cases])
+
+    def call(h):
+        for c in cases:
+            c.d_out.zero_()
+        torch.cuda.synchronize()
+        assert lib.fy_gpu_process_cubes(arr, len(cases), h) == 0, lib.fy_gpu_last_error()
+        assert lib.fy_gpu_sync(h) == 0
+        return [c.d_out.cpu().numpy().copy() for c in cases]
+
+    print("stream release: a caller's stream", flush=True)
+    first = call(sh)
+    assert sum(int(o.any()) for o in first) == len(cases) - 1        # cube 4 is all air
+    for _ in range(3):
+        assert lib.fy_gpu_stream_release(sh) == 0
+        again = call(sh)
+        assert all(np.array_equal(a, b) for a, b in zip(first, again))
+    assert lib.fy_gpu_stream_release(sh) == 0
+    assert lib.fy_gpu_stream_release(sh) == 0                        # nothing left to release
+    print("stream release: a worker thread's own stream", flush=True)
+    out = {}
+
+    def worker():
+        try:
+            out["a"] = call(None)
+            assert lib.fy_gpu_stream_release(None) == 0
+            out["b"] = call(None)
+            assert lib.fy_gpu_stream_release(None) == 0
+        except BaseException as e:                                    # reported by the main thread
+            out["err"] = e
+
+    t = threading.Thread(target=worker)
+    t.start()
+    t.join()
+    assert "err" not in out, out.get("err")
+    for k in ("a", "b"):
+        assert all(np.array_equal(a, b) for a, b in zip(first, out[k])), k
