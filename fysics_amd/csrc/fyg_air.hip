@@ -10,6 +10,7 @@
 //   k_air_cut        smoothed histogram in LDS, then one thread: valley depth
 //                    (stream.c:138-170) -> cut
 //   k_air_apply      trilinear upsample of the scratch, texture-split threshold
+//                    (without the despeckle; with it, k_cc_init applies the rule)
 //   k_cc_*           6-connected components of f > 0: x runs labelled by a wave
 //                    scan, runs joined along y and z by union-find (parents only
 //                    ever decrease: L[x] <= x, a root is its component's smallest
@@ -409,38 +410,62 @@ __global__ __launch_bounds__(256) void k_air_cut(const unsigned long long *hist,
     out->tex = ct / 255.0f;
 }
 
-// zero f where the (upsampled) scratch is below the cut of its texture class;
-// grid (rows, x blocks): the row's (z, y) blend weights are uniform
-__global__ __launch_bounds__(NT) void k_air_apply(float *f, const float *orig, const float *sm, int nz, int ny,
-                                                   int nx, int lz, int ly, int lx, int lowres, const AirCut *cutp) {
+// The air rule: the scratch at the voxel (a trilinear upsample of the
+// decimated grid when lowres) below the cut of the voxel's texture class.
+struct AirGrid {
+    const float *orig, *sm;
+    int ny, nx, lz, ly, lx, lowres;
+};
+
+// the scratch at voxels x = xc + lane of row (z, y), one wave (xc a multiple of
+// 64; every lane must call it).  At half resolution each lane forms the blend
+// of the four low-res rows at one column and the voxels take the two columns
+// around them by shuffles: 4 loads per lane instead of 8 per voxel, the same
+// expressions in the same order as a voxel's own blend.
+__device__ __forceinline__ float air_scratch_row(const AirGrid &g, int z, int y, int xc, int lane) {
     #pragma clang fp contract(off)
-    const AirCut cut = *cutp;
-    const int row = blockIdx.x, z = row / ny, y = row - z * ny;
-    const int x = blockIdx.y * NT + threadIdx.x;
-    if (x >= nx) return;
-    const size_t i = (size_t)row * nx + x;
-    float v;
-    if (lowres) {
-        int z0 = z >> 1, y0 = y >> 1, x0 = x >> 1;
-        z0 = z0 > lz - 2 ? lz - 2 : z0;
-        y0 = y0 > ly - 2 ? ly - 2 : y0;
-        x0 = x0 > lx - 2 ? lx - 2 : x0;
-        float zf = 0.5f * (float)z - (float)z0, yf = 0.5f * (float)y - (float)y0, fx = 0.5f * (float)x - (float)x0;
-        zf = zf > 1.0f ? 1.0f : zf;
-        yf = yf > 1.0f ? 1.0f : yf;
-        fx = fx > 1.0f ? 1.0f : fx;
-        const size_t r00 = ((size_t)z0 * ly + y0) * lx, r01 = r00 + lx, r10 = r00 + (size_t)ly * lx, r11 = r10 + lx;
-        const float w00 = (1 - zf) * (1 - yf), w01 = (1 - zf) * yf, w10 = zf * (1 - yf), w11 = zf * yf;
-        const float a = w00 * sm[r00 + x0] + w01 * sm[r01 + x0] + w10 * sm[r10 + x0] + w11 * sm[r11 + x0];
-        const float b = w00 * sm[r00 + x0 + 1] + w01 * sm[r01 + x0 + 1] + w10 * sm[r10 + x0 + 1] +
-                        w11 * sm[r11 + x0 + 1];
-        v = a + (b - a) * fx;
-    } else {
-        v = sm[i];
-    }
-    float tex = orig[i] - v;
+    const int x = xc + lane;
+    if (!g.lowres) return x < g.nx ? g.sm[((size_t)z * g.ny + y) * g.nx + x] : 0.0f;
+    int z0 = z >> 1, y0 = y >> 1;
+    z0 = z0 > g.lz - 2 ? g.lz - 2 : z0;
+    y0 = y0 > g.ly - 2 ? g.ly - 2 : y0;
+    float zf = 0.5f * (float)z - (float)z0, yf = 0.5f * (float)y - (float)y0;
+    zf = zf > 1.0f ? 1.0f : zf;
+    yf = yf > 1.0f ? 1.0f : yf;
+    const size_t r00 = ((size_t)z0 * g.ly + y0) * g.lx, r01 = r00 + g.lx, r10 = r00 + (size_t)g.ly * g.lx,
+                 r11 = r10 + g.lx;
+    const float w00 = (1 - zf) * (1 - yf), w01 = (1 - zf) * yf, w10 = zf * (1 - yf), w11 = zf * yf;
+    // columns base .. base+32 cover the knots of the wave's voxels
+    const int base = (xc >> 1) < g.lx - 2 ? (xc >> 1) : g.lx - 2;
+    const int q = base + lane < g.lx - 1 ? base + lane : g.lx - 1;
+    const float *sm = g.sm;
+    const float bl = w00 * sm[r00 + q] + w01 * sm[r01 + q] + w10 * sm[r10 + q] + w11 * sm[r11 + q];
+    int x0 = x >> 1;
+    x0 = x0 > g.lx - 2 ? g.lx - 2 : x0;
+    float fx = 0.5f * (float)x - (float)x0;
+    fx = fx > 1.0f ? 1.0f : fx;
+    const float a = __shfl(bl, x0 - base), b = __shfl(bl, x0 + 1 - base);
+    return a + (b - a) * fx;
+}
+
+__device__ __forceinline__ bool air_zeroes(const AirGrid &g, const AirCut &cut, float v, size_t i) {
+    #pragma clang fp contract(off)
+    float tex = g.orig[i] - v;
     tex = tex < 0 ? -tex : tex;
-    if (v < (tex < 0.03f ? cut.uni : cut.tex)) f[i] = 0.0f;
+    return v < (tex < 0.03f ? cut.uni : cut.tex);
+}
+
+// zero f where the air rule holds; grid (rows, x blocks of NT)
+__global__ __launch_bounds__(NT) void k_air_apply(float *f, AirGrid g, const AirCut *cutp) {
+    const AirCut cut = *cutp;
+    const int row = blockIdx.x, z = row / g.ny, y = row - z * g.ny;
+    const int lane = threadIdx.x & 63, xc = blockIdx.y * NT + (threadIdx.x & ~63);
+    if (xc >= g.nx) return;                             // whole waves
+    const float v = air_scratch_row(g, z, y, xc, lane);
+    const int x = xc + lane;
+    if (x >= g.nx) return;
+    const size_t i = (size_t)row * g.nx + x;
+    if (air_zeroes(g, cut, v, i)) f[i] = 0.0f;
 }
 
 // ---- connected components (6-connectivity) of f > 0
@@ -489,26 +514,51 @@ __device__ void cc_unite(int *L, int a, int b, int *err) {
 // row chunk of 64 voxels; the last gap before each voxel comes from a max-scan
 // across the wave, carried along the row.  Also clears the counts of the run
 // starts: only run starts are ever roots, and counts are kept per root.
-__global__ __launch_bounds__(NT) void k_cc_init(const float *f, int *L, int *cnt, long nrows, int nx) {
+// The air rule is applied first, in the same pass (k_air_apply's zeros,
+// written where they change f's bits), so f is read once for both.
+__global__ __launch_bounds__(NT) void k_cc_init(float *f, int *L, int *cnt, long nrows, int nx, AirGrid ag,
+                                                const AirCut *cutp) {
     const int lane = threadIdx.x & 63;
     const long row = (long)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
     if (row >= nrows) return;                           // whole waves leave together
     const size_t base = (size_t)row * nx;
+    const AirCut cut = *cutp;
+    const int z = (int)(row / ag.ny), y = (int)(row - (long)z * ag.ny);
     int carry = -1;                                     // last gap (x) before this chunk
-    for (int x0 = 0; x0 < nx; x0 += 64) {
-        const int x = x0 + lane;
-        const bool in = x < nx;
-        const bool m = in && f[base + x] > 0.0f;
-        int g = m ? carry : x;                          // gap positions; carry for material
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(g, o);
-            if (lane >= o) g = g > t ? g : t;
+    // four chunks at a time: their loads and air decisions are independent,
+    // so all are issued before the scans that carry the gap along the row
+    for (int xg = 0; xg < nx; xg += 4 * 64) {
+        bool mm[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int xc = xg + 64 * c, x = xc + lane;
+            mm[c] = false;
+            if (xc >= nx) continue;                     // whole waves
+            const float v = air_scratch_row(ag, z, y, xc, lane);
+            if (x < nx) {
+                const float fv = f[base + x];
+                mm[c] = fv > 0.0f;
+                if (air_zeroes(ag, cut, v, base + x)) {
+                    if (__float_as_uint(fv) != 0u) f[base + x] = 0.0f;
+                    mm[c] = false;
+                }
+            }
         }
-        if (in) {
-            L[base + x] = m ? (int)(base + g + 1) : -1;
-            if (m && g + 1 == x) cnt[base + x] = 0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int x = xg + 64 * c + lane;
+            const bool in = x < nx, m = mm[c];
+            int g = m ? carry : x;                      // gap positions; carry for material
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(g, o);
+                if (lane >= o) g = g > t ? g : t;
+            }
+            if (in) {
+                L[base + x] = m ? (int)(base + g + 1) : -1;
+                if (m && g + 1 == x) cnt[base + x] = 0;
+            }
+            carry = __shfl(g, 63);
         }
-        carry = __shfl(g, 63);
     }
 }
 
@@ -778,14 +828,17 @@ extern "C" int fyg_air_zero_run(float *f, const float *orig, int nz, int ny, int
     fyg_launch(s, "air_cut", k_air_cut, dim3(1), dim3(256), 0, (const unsigned long long *)hist, cp, cut);
     if ((long)nz * ny > 0x7fffffffL || (nx + NT - 1) / NT > 65535) return 1;
     dim3 ag((unsigned)((long)nz * ny), (unsigned)((nx + NT - 1) / NT));
-    fyg_launch(s, "air_apply", k_air_apply, ag, dim3(NT), 0, f, orig, (const float *)fin, nz, ny, nx, lz, ly, lx,
-               lowres, (const AirCut *)cut);
-    if (a->min_component > 0) {                           // pipeline.c:396-398
+    const AirGrid agr{orig, (const float *)fin, ny, nx, lz, ly, lx, lowres};
+    if (a->min_component <= 0) {
+        fyg_launch(s, "air_apply", k_air_apply, ag, dim3(NT), 0, f, agr, (const AirCut *)cut);
+    } else {                                              // pipeline.c:396-398
         const unsigned g = grid_for(n, 16384);
         if (fyg_zero(err, sizeof(int), s)) return 1;
         const long nrows = (long)nz * ny;
         const dim3 wrows((unsigned)((nrows + NT / 64 - 1) / (NT / 64)));     // one wave per row
-        fyg_launch(s, "cc_init", k_cc_init, wrows, dim3(NT), 0, (const float *)f, L, cnt, nrows, nx);
+        // the air rule and the x runs of the components in one pass over f
+        fyg_launch(s, "air_cc_init", k_cc_init, wrows, dim3(NT), 0, f, L, cnt, nrows, nx, agr,
+                   (const AirCut *)cut);
         fyg_launch(s, "cc_union", k_cc_union, ag, dim3(NT), 0, L, ny, nx, err);
         fyg_launch(s, "cc_flatten", k_cc_flatten_starts, ag, dim3(NT), 0, L, ny, nx);
         const Faces fc = faces_of(nz, ny, nx);
