@@ -451,3 +451,32 @@ def test_stream_release_and_thread_stream():
     assert "err" not in out, out.get("err")
     for k in ("a", "b"):
         assert all(np.array_equal(a, b) for a, b in zip(first, out[k])), k
+
+
+def test_helper_streams_past_the_queue_limit():
+    """Six caller streams each running a grouped batch of five shapes (three
+    helper streams apiece): the first twelve helpers get hardware queues of
+    their own, the rest are ordinary streams; every stream's results are the
+    same bits, and releasing the streams returns their queues."""
+    rng = np.random.default_rng(44)
+    cases = make_mix_cases(rng, False)
+    lib = F.lib()
+    streams = [torch.cuda.Stream() for _ in range(6)]
+    outs = [[torch.zeros_like(c.d_out) for c in cases] for _ in streams]
+    old = lib.fy_gpu_batch_mix(0)
+    try:
+        for k, (st, out) in enumerate(zip(streams, outs)):
+            print(f"helper streams: caller stream {k}", flush=True)
+            arr = (F.Cube * len(cases))(*[F.Cube(c.d_in.data_ptr(), *c.origin, *c.u8.shape, *c.inner_o, *c.inner,
+                                                 o.data_ptr(), None, C.pointer(c.P)) for c, o in zip(cases, out)])
+            assert lib.fy_gpu_process_cubes(arr, len(cases), C.c_void_p(st.cuda_stream)) == 0, \
+                lib.fy_gpu_last_error()
+        torch.cuda.synchronize()
+    finally:
+        lib.fy_gpu_batch_mix(old)
+        for st in streams:
+            assert lib.fy_gpu_stream_release(C.c_void_p(st.cuda_stream)) == 0
+    first = [o.cpu().numpy() for o in outs[0]]
+    assert sum(int(o.any()) for o in first) == len(cases) - 1
+    for k, out in enumerate(outs[1:], 1):
+        assert all(np.array_equal(a, o.cpu().numpy()) for a, o in zip(first, out)), f"stream {k}"
